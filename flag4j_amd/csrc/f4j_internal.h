@@ -1,0 +1,48 @@
+// Internal declarations shared by the flag4j_amd translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+namespace f4j {
+
+// Operand access kinds in the "real-expanded" view of a product.
+// A complex128 product C = op(A) op(B) (M x N, inner K) is computed as the
+// real product  C~ (M x 2N) = A~ (M x 2K) * B~ (2K x 2N)  where
+//   A~[m][2k+c]      = component c of op(A)(m,k)
+//   B~[2k][2n+c]     = component c of op(B)(k,n)
+//   B~[2k+1][2n]     = -Im op(B)(k,n),   B~[2k+1][2n+1] = Re op(B)(k,n)
+// so that C~ is exactly the interleaved (re, im) row-major output.
+enum OpKind : int {
+    R_N = 0,  // real, not transposed
+    R_T = 1,  // real, transposed
+    Z_N = 2,  // complex, not transposed
+    Z_T = 3,  // complex, transposed
+    Z_C = 4,  // complex, conjugate transposed
+};
+
+struct GemmProblem {
+    int64_t M, N, K;          // real-expanded sizes: for complex N and K are doubled
+    const double *A;
+    int64_t lda;              // leading dimension of A's storage, in doubles
+    const double *B;
+    int64_t ldb;
+    double *C;
+    int64_t ldc;
+};
+
+// Launch the FP64 MFMA GEMM for one problem (kernel-level, device pointers).
+hipError_t launch_gemm_f64(OpKind ka, OpKind kb, const GemmProblem &p, hipStream_t stream, int tile_cfg);
+
+// Flag4j-order (bitwise) GEMM; sizes and leading dimensions in element units.
+hipError_t launch_gemm_strict(bool cplx, int opA, int opB, int64_t M, int64_t N, int64_t K, const double *A,
+                              int64_t lda, const double *B, int64_t ldb, double *C, int64_t ldc,
+                              hipStream_t stream);
+
+// Fill rows x cols (leading dimension ld, in doubles) with zeros.
+hipError_t launch_zero_fill(double *C, int64_t rows, int64_t cols, int64_t ld, hipStream_t stream);
+
+void set_error(const std::string &msg);
+
+}  // namespace f4j

@@ -1,0 +1,490 @@
+// C ABI of flag4j_amd: lifecycle, device buffers, dispatcher and GEMM entries.
+//
+// The dispatcher here stands where Flag4j's ReDeMatMultDispatcher /
+// Cm128DeMatMultDispatcher stand (M:linalg/ops/dispatch/ReDeMatMultDispatcher.java:157-237,
+// Cm128DeMatMultDispatcher.java:155-226; M: = src/main/java/org/flag4j/): it
+// takes the shapes of a product and picks the kernel.  Products whose operands
+// hold fewer than `smallThreshold` entries in total (the reference's small
+// path, ReDeMatMultDispatcher.java:159-168) run the bitwise-strict kernel;
+// larger ones run the configured precision on the matrix cores with a tile
+// shape chosen per shape bucket.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <mutex>
+#include <sstream>
+#include <string>
+
+#include "../../include/flag4j_amd.h"
+#include "f4j_internal.h"
+
+namespace f4j {
+
+namespace {
+
+thread_local std::string t_err;
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+};
+
+struct Config {
+    int64_t small_threshold = 2450;   // ReDeMatMultDispatchConfigs.java:63 default
+    int default_precision = F4J_PREC_FP64;
+    int default_slices = 0;
+};
+
+struct State {
+    std::mutex mu;                    // guards init state and the staging buffers
+    std::mutex cfg_mu;                // guards cfg
+    bool inited = false;
+    int device = -1;
+    hipStream_t stream = nullptr;
+    hipDeviceProp_t prop{};
+    Config cfg;
+    DevBuf bufA, bufB, bufC;
+};
+
+State &S() {
+    static State s;
+    return s;
+}
+
+int fail(int code, const std::string &msg) {
+    t_err = msg;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char *what) {
+    const int code = (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) ? F4J_ERR_OUT_OF_MEMORY : F4J_ERR_HIP;
+    return fail(code, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define F4J_HIP(call, what)                        \
+    do {                                           \
+        hipError_t e_ = (call);                    \
+        if (e_ != hipSuccess) return hip_fail(e_, what); \
+    } while (0)
+
+int init_locked(State &s, int device) {
+    if (s.inited) {
+        if (device == s.device) return F4J_OK;
+        return fail(F4J_ERR_INVALID_ARG, "f4j_init: already initialised on device " + std::to_string(s.device) +
+                                             "; call f4j_shutdown first");
+    }
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return fail(F4J_ERR_NO_DEVICE, "no HIP device visible");
+    if (device < 0 || device >= n)
+        return fail(F4J_ERR_INVALID_ARG, "device " + std::to_string(device) + " out of range [0," + std::to_string(n) + ")");
+    F4J_HIP(hipSetDevice(device), "hipSetDevice");
+    F4J_HIP(hipGetDeviceProperties(&s.prop, device), "hipGetDeviceProperties");
+    F4J_HIP(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking), "hipStreamCreate");
+    s.device = device;
+    s.inited = true;
+    return F4J_OK;
+}
+
+// Make sure the library is initialised and the calling thread's current
+// device is the library's device.
+int ensure_ready() {
+    State &s = S();
+    {
+        std::lock_guard<std::mutex> g(s.mu);
+        if (!s.inited) {
+            const int rc = init_locked(s, 0);
+            if (rc != F4J_OK) return rc;
+        }
+    }
+    F4J_HIP(hipSetDevice(s.device), "hipSetDevice");
+    return F4J_OK;
+}
+
+hipStream_t pick_stream(void *stream) { return stream ? (hipStream_t)stream : S().stream; }
+
+int grow(DevBuf &b, size_t bytes) {
+    if (bytes <= b.cap) return F4J_OK;
+    size_t cap = std::max<size_t>(bytes, b.cap + b.cap / 2);
+    cap = (cap + 4095) & ~size_t(4095);
+    if (b.p) {
+        F4J_HIP(hipFree(b.p), "hipFree");
+        b.p = nullptr;
+        b.cap = 0;
+    }
+    void *p = nullptr;
+    hipError_t e = hipMalloc(&p, cap);
+    if (e != hipSuccess && cap > bytes) {
+        (void)hipGetLastError();
+        cap = bytes;
+        e = hipMalloc(&p, cap);
+    }
+    if (e != hipSuccess) return hip_fail(e, "hipMalloc (staging buffer)");
+    b.p = p;
+    b.cap = cap;
+    return F4J_OK;
+}
+
+bool valid_op(int op, bool cplx) { return op == F4J_OP_N || op == F4J_OP_T || (cplx && op == F4J_OP_C); }
+
+// Shape/argument validation shared by every GEMM entry.  Sizes in elements.
+int validate(bool cplx, int opA, int opB, int64_t M, int64_t N, int64_t K, const void *A, int64_t lda, const void *B,
+             int64_t ldb, const void *C, int64_t ldc, int precision) {
+    if (!valid_op(opA, cplx) || !valid_op(opB, cplx))
+        return fail(F4J_ERR_INVALID_ARG, "invalid op code (N=0, T=1" + std::string(cplx ? ", C=2)" : ")"));
+    if (M < 0 || N < 0 || K < 0) return fail(F4J_ERR_INVALID_ARG, "negative matrix dimension");
+    if (precision < F4J_PREC_AUTO || precision > F4J_PREC_SPLIT_INT8)
+        return fail(F4J_ERR_INVALID_ARG, "invalid precision code " + std::to_string(precision));
+    const int64_t rowA = opA == F4J_OP_N ? K : M, rowB = opB == F4J_OP_N ? N : K;
+    const int64_t nrA = opA == F4J_OP_N ? M : K, nrB = opB == F4J_OP_N ? K : N;
+    if (M > 0 && N > 0) {
+        if (!C) return fail(F4J_ERR_INVALID_ARG, "C is null");
+        if (ldc < std::max<int64_t>(N, 1)) return fail(F4J_ERR_INVALID_ARG, "ldc < N");
+    }
+    if (M > 0 && N > 0 && K > 0) {
+        if (!A || !B) return fail(F4J_ERR_INVALID_ARG, "A or B is null");
+        if (lda < std::max<int64_t>(rowA, 1)) return fail(F4J_ERR_INVALID_ARG, "lda smaller than the stored row length of A");
+        if (ldb < std::max<int64_t>(rowB, 1)) return fail(F4J_ERR_INVALID_ARG, "ldb smaller than the stored row length of B");
+    }
+    // Flag4j indexes with 32-bit ints (M:arrays/Shape.java:397-412): larger
+    // matrices cannot come from it, but the library itself is 64-bit clean.
+    (void)nrA;
+    (void)nrB;
+    return F4J_OK;
+}
+
+// Tile configuration per shape bucket: the 128x128 tile when it yields enough
+// workgroups to fill the 256 CUs, else the 64x64 tile.
+int pick_tile(int64_t M, int64_t Nx) {
+    const int64_t t128 = ((M + 127) / 128) * ((Nx + 127) / 128);
+    return t128 >= 512 ? 0 : 1;
+}
+
+const char *tile_name(int cfg) {
+    switch (cfg) {
+        case 1: return "MFMA_F64_64x64x16";
+        case 2: return "MFMA_F64_128x128x32";
+        default: return "MFMA_F64_128x128x16";
+    }
+}
+
+int resolve_precision(int precision, int64_t M, int64_t N, int64_t K, int opA, int opB) {
+    if (precision != F4J_PREC_AUTO) return precision;
+    // Entries held by A and B (ReDeMatMultDispatcher.java:158: a.dataLength() + b.dataLength()).
+    (void)opA;
+    (void)opB;
+    const int64_t total = M * K + K * N;
+    std::lock_guard<std::mutex> g(S().cfg_mu);
+    if (total < S().cfg.small_threshold) return F4J_PREC_STRICT;
+    return S().cfg.default_precision;
+}
+
+OpKind kind_of(int op, bool cplx) {
+    if (!cplx) return op == F4J_OP_N ? R_N : R_T;
+    return op == F4J_OP_N ? Z_N : (op == F4J_OP_T ? Z_T : Z_C);
+}
+
+// Device-level GEMM, all pointers on the device, async on `stream`.
+int gemm_dev(bool cplx, int opA, int opB, int64_t M, int64_t N, int64_t K, const double *A, int64_t lda,
+             const double *B, int64_t ldb, double *C, int64_t ldc, int precision, int slices, int use3m,
+             hipStream_t stream) {
+    (void)slices;
+    (void)use3m;
+    if (M == 0 || N == 0) return F4J_OK;
+    const int w = cplx ? 2 : 1;
+    if (K == 0) {
+        F4J_HIP(launch_zero_fill(C, M, N * w, ldc * w, stream), "zero fill");
+        return F4J_OK;
+    }
+    const int prec = resolve_precision(precision, M, N, K, opA, opB);
+    if (prec == F4J_PREC_STRICT) {
+        F4J_HIP(launch_gemm_strict(cplx, opA, opB, M, N, K, A, lda, B, ldb, C, ldc, stream), "strict gemm launch");
+        return F4J_OK;
+    }
+    if (prec != F4J_PREC_FP64)
+        return fail(F4J_ERR_UNSUPPORTED, "precision " + std::to_string(prec) + " is not available in this build");
+    GemmProblem p;
+    p.M = M;
+    p.N = N * w;
+    p.K = K * w;
+    p.A = A;
+    p.lda = lda * w;
+    p.B = B;
+    p.ldb = ldb * w;
+    p.C = C;
+    p.ldc = ldc * w;
+    F4J_HIP(launch_gemm_f64(kind_of(opA, cplx), kind_of(opB, cplx), p, stream, pick_tile(p.M, p.N)), "fp64 gemm launch");
+    return F4J_OK;
+}
+
+// Host-array GEMM: stage operands into the pooled device buffers, run, copy back.
+int gemm_host(bool cplx, int opA, int opB, int64_t M, int64_t N, int64_t K, const double *A, int64_t lda,
+              const double *B, int64_t ldb, double *C, int64_t ldc, int precision, int slices, int use3m) {
+    int rc = validate(cplx, opA, opB, M, N, K, A, lda, B, ldb, C, ldc, precision);
+    if (rc != F4J_OK) return rc;
+    if (M == 0 || N == 0) return F4J_OK;
+    rc = ensure_ready();
+    if (rc != F4J_OK) return rc;
+    State &s = S();
+    std::lock_guard<std::mutex> g(s.mu);
+    const size_t es = cplx ? 16 : 8;
+    const int64_t rA = opA == F4J_OP_N ? M : K, cA = opA == F4J_OP_N ? K : M;
+    const int64_t rB = opB == F4J_OP_N ? K : N, cB = opB == F4J_OP_N ? N : K;
+    if (K > 0) {
+        if ((rc = grow(s.bufA, (size_t)(rA * cA) * es)) != F4J_OK) return rc;
+        if ((rc = grow(s.bufB, (size_t)(rB * cB) * es)) != F4J_OK) return rc;
+    }
+    if ((rc = grow(s.bufC, (size_t)(M * N) * es)) != F4J_OK) return rc;
+    if (K > 0) {
+        F4J_HIP(hipMemcpy2DAsync(s.bufA.p, cA * es, A, lda * es, cA * es, rA, hipMemcpyHostToDevice, s.stream), "copy A");
+        F4J_HIP(hipMemcpy2DAsync(s.bufB.p, cB * es, B, ldb * es, cB * es, rB, hipMemcpyHostToDevice, s.stream), "copy B");
+    }
+    rc = gemm_dev(cplx, opA, opB, M, N, K, (const double *)s.bufA.p, cA, (const double *)s.bufB.p, cB,
+                  (double *)s.bufC.p, N, precision, slices, use3m, s.stream);
+    if (rc != F4J_OK) {
+        (void)hipStreamSynchronize(s.stream);
+        return rc;
+    }
+    F4J_HIP(hipMemcpy2DAsync(C, ldc * es, s.bufC.p, N * es, N * es, M, hipMemcpyDeviceToHost, s.stream), "copy C");
+    F4J_HIP(hipStreamSynchronize(s.stream), "stream synchronize");
+    return F4J_OK;
+}
+
+const char *prec_names[] = {"fp64", "strict", "split_bf16", "split_fp16", "split_int8"};
+
+int parse_precision(const std::string &v) {
+    for (int i = 0; i < 5; ++i)
+        if (v == prec_names[i]) return i;
+    return -100;
+}
+
+std::string trim(const std::string &x) {
+    size_t a = x.find_first_not_of(" \t\r\n"), b = x.find_last_not_of(" \t\r\n");
+    return a == std::string::npos ? std::string() : x.substr(a, b - a + 1);
+}
+
+int config_set_locked(Config &c, const std::string &key, const std::string &value) {
+    char *end = nullptr;
+    if (key == "smallThreshold") {
+        const long long v = std::strtoll(value.c_str(), &end, 10);
+        if (end == value.c_str() || *end || v < 0) return fail(F4J_ERR_INVALID_ARG, "smallThreshold: not a non-negative integer");
+        c.small_threshold = v;
+    } else if (key == "defaultPrecision") {
+        const int p = parse_precision(value);
+        if (p < 0) return fail(F4J_ERR_INVALID_ARG, "defaultPrecision: unknown value '" + value + "'");
+        c.default_precision = p;
+    } else if (key == "defaultSlices") {
+        const long v = std::strtol(value.c_str(), &end, 10);
+        if (end == value.c_str() || *end || v < 0 || v > 16) return fail(F4J_ERR_INVALID_ARG, "defaultSlices: expected 0..16");
+        c.default_slices = (int)v;
+    } else {
+        return fail(F4J_ERR_INVALID_ARG, "unknown configuration key '" + key + "'");
+    }
+    return F4J_OK;
+}
+
+}  // namespace
+
+void set_error(const std::string &msg) { t_err = msg; }
+
+}  // namespace f4j
+
+using namespace f4j;
+
+extern "C" {
+
+int f4j_abi_version(void) { return F4J_ABI_VERSION; }
+
+int f4j_init(int device) {
+    State &s = S();
+    std::lock_guard<std::mutex> g(s.mu);
+    return init_locked(s, device);
+}
+
+int f4j_shutdown(void) {
+    State &s = S();
+    std::lock_guard<std::mutex> g(s.mu);
+    if (!s.inited) return F4J_OK;
+    F4J_HIP(hipSetDevice(s.device), "hipSetDevice");
+    F4J_HIP(hipStreamSynchronize(s.stream), "stream synchronize");
+    for (DevBuf *b : {&s.bufA, &s.bufB, &s.bufC})
+        if (b->p) {
+            F4J_HIP(hipFree(b->p), "hipFree");
+            b->p = nullptr;
+            b->cap = 0;
+        }
+    F4J_HIP(hipStreamDestroy(s.stream), "hipStreamDestroy");
+    s.stream = nullptr;
+    s.inited = false;
+    s.device = -1;
+    return F4J_OK;
+}
+
+const char *f4j_last_error(void) { return t_err.c_str(); }
+
+int f4j_device_info_get(f4j_device_info *out) {
+    if (!out) return fail(F4J_ERR_INVALID_ARG, "out is null");
+    int rc = ensure_ready();
+    if (rc != F4J_OK) return rc;
+    const hipDeviceProp_t &p = S().prop;
+    std::memset(out, 0, sizeof(*out));
+    out->device = S().device;
+    out->compute_units = p.multiProcessorCount;
+    out->clock_khz = p.clockRate;
+    int lds = 0;
+    if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, S().device) != hipSuccess) lds = 0;
+    out->lds_bytes_per_cu = lds;
+    out->xcds = 8;
+    out->hbm_bytes = (int64_t)p.totalGlobalMem;
+    // MI355X published dense peaks (FP64 matrix = FP64 vector on CDNA4).
+    out->peak_fp64_tflops = 78.6;
+    out->peak_bf16_tflops = 2500.0;
+    out->peak_hbm_gbs = 8000.0;
+    std::snprintf(out->name, sizeof(out->name), "%s", p.name);
+    std::snprintf(out->arch, sizeof(out->arch), "%s", p.gcnArchName);
+    return F4J_OK;
+}
+
+int f4j_dev_alloc(size_t bytes, void **dptr) {
+    if (!dptr) return fail(F4J_ERR_INVALID_ARG, "dptr is null");
+    int rc = ensure_ready();
+    if (rc != F4J_OK) return rc;
+    *dptr = nullptr;
+    F4J_HIP(hipMalloc(dptr, bytes ? bytes : 1), "hipMalloc");
+    return F4J_OK;
+}
+
+int f4j_dev_free(void *dptr) {
+    if (!dptr) return F4J_OK;
+    int rc = ensure_ready();
+    if (rc != F4J_OK) return rc;
+    F4J_HIP(hipFree(dptr), "hipFree");
+    return F4J_OK;
+}
+
+int f4j_memcpy_h2d(void *dst, const void *src, size_t bytes, void *stream) {
+    int rc = ensure_ready();
+    if (rc != F4J_OK) return rc;
+    if (bytes == 0) return F4J_OK;
+    if (!dst || !src) return fail(F4J_ERR_INVALID_ARG, "null pointer");
+    F4J_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, pick_stream(stream)), "hipMemcpyAsync H2D");
+    return F4J_OK;
+}
+
+int f4j_memcpy_d2h(void *dst, const void *src, size_t bytes, void *stream) {
+    int rc = ensure_ready();
+    if (rc != F4J_OK) return rc;
+    if (bytes == 0) return F4J_OK;
+    if (!dst || !src) return fail(F4J_ERR_INVALID_ARG, "null pointer");
+    hipStream_t st = pick_stream(stream);
+    F4J_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st), "hipMemcpyAsync D2H");
+    F4J_HIP(hipStreamSynchronize(st), "stream synchronize");
+    return F4J_OK;
+}
+
+int f4j_stream_sync(void *stream) {
+    int rc = ensure_ready();
+    if (rc != F4J_OK) return rc;
+    F4J_HIP(hipStreamSynchronize(pick_stream(stream)), "stream synchronize");
+    return F4J_OK;
+}
+
+int f4j_dgemm(int opA, int opB, int64_t M, int64_t N, int64_t K, const double *A, int64_t lda, const double *B,
+              int64_t ldb, double *C, int64_t ldc, int precision, int slices) {
+    return gemm_host(false, opA, opB, M, N, K, A, lda, B, ldb, C, ldc, precision, slices, 0);
+}
+
+int f4j_dgemm_dev(int opA, int opB, int64_t M, int64_t N, int64_t K, const double *A, int64_t lda, const double *B,
+                  int64_t ldb, double *C, int64_t ldc, int precision, int slices, void *stream) {
+    int rc = validate(false, opA, opB, M, N, K, A, lda, B, ldb, C, ldc, precision);
+    if (rc != F4J_OK) return rc;
+    if ((rc = ensure_ready()) != F4J_OK) return rc;
+    return gemm_dev(false, opA, opB, M, N, K, A, lda, B, ldb, C, ldc, precision, slices, 0, pick_stream(stream));
+}
+
+int f4j_zgemm(int opA, int opB, int64_t M, int64_t N, int64_t K, const double *A, int64_t lda, const double *B,
+              int64_t ldb, double *C, int64_t ldc, int precision, int slices, int use_3m) {
+    return gemm_host(true, opA, opB, M, N, K, A, lda, B, ldb, C, ldc, precision, slices, use_3m);
+}
+
+int f4j_zgemm_dev(int opA, int opB, int64_t M, int64_t N, int64_t K, const double *A, int64_t lda, const double *B,
+                  int64_t ldb, double *C, int64_t ldc, int precision, int slices, int use_3m, void *stream) {
+    int rc = validate(true, opA, opB, M, N, K, A, lda, B, ldb, C, ldc, precision);
+    if (rc != F4J_OK) return rc;
+    if ((rc = ensure_ready()) != F4J_OK) return rc;
+    return gemm_dev(true, opA, opB, M, N, K, A, lda, B, ldb, C, ldc, precision, slices, use_3m, pick_stream(stream));
+}
+
+int f4j_config_set(const char *key, const char *value) {
+    if (!key || !value) return fail(F4J_ERR_INVALID_ARG, "null key or value");
+    State &s = S();
+    std::lock_guard<std::mutex> g(s.cfg_mu);
+    return config_set_locked(s.cfg, trim(key), trim(value));
+}
+
+int f4j_config_get(const char *key, char *buf, size_t buflen) {
+    if (!key || !buf || buflen == 0) return fail(F4J_ERR_INVALID_ARG, "null key or buffer");
+    State &s = S();
+    std::string v;
+    {
+        std::lock_guard<std::mutex> g(s.cfg_mu);
+        const std::string k = key;
+        if (k == "smallThreshold") v = std::to_string(s.cfg.small_threshold);
+        else if (k == "defaultPrecision") v = prec_names[s.cfg.default_precision];
+        else if (k == "defaultSlices") v = std::to_string(s.cfg.default_slices);
+        else return fail(F4J_ERR_INVALID_ARG, "unknown configuration key '" + k + "'");
+    }
+    std::snprintf(buf, buflen, "%s", v.c_str());
+    return F4J_OK;
+}
+
+int f4j_config_load(const char *path) {
+    if (!path) return fail(F4J_ERR_INVALID_ARG, "null path");
+    std::ifstream in(path);
+    if (!in) return fail(F4J_ERR_INVALID_ARG, std::string("cannot open ") + path);
+    State &s = S();
+    std::lock_guard<std::mutex> g(s.cfg_mu);
+    Config c = s.cfg;       // all-or-nothing: apply only if every line parses
+    std::string line;
+    int lineno = 0;
+    while (std::getline(in, line)) {
+        ++lineno;
+        const std::string t = trim(line);
+        if (t.empty() || t[0] == '#' || t[0] == '!') continue;
+        const size_t eq = t.find_first_of("=:");
+        if (eq == std::string::npos)
+            return fail(F4J_ERR_INVALID_ARG, std::string(path) + ":" + std::to_string(lineno) + ": expected key=value");
+        const int rc = config_set_locked(c, trim(t.substr(0, eq)), trim(t.substr(eq + 1)));
+        if (rc != F4J_OK) {
+            t_err = std::string(path) + ":" + std::to_string(lineno) + ": " + t_err;
+            return rc;
+        }
+    }
+    s.cfg = c;
+    return F4J_OK;
+}
+
+int f4j_dispatch_name(int is_complex, int opA, int opB, int64_t M, int64_t N, int64_t K, int precision, char *buf,
+                      size_t buflen) {
+    if (!buf || buflen == 0) return fail(F4J_ERR_INVALID_ARG, "null buffer");
+    const bool cplx = is_complex != 0;
+    if (!valid_op(opA, cplx) || !valid_op(opB, cplx) || M < 0 || N < 0 || K < 0)
+        return fail(F4J_ERR_INVALID_ARG, "invalid op or size");
+    if (precision < F4J_PREC_AUTO || precision > F4J_PREC_SPLIT_INT8)
+        return fail(F4J_ERR_INVALID_ARG, "invalid precision code");
+    std::string name;
+    const int prec = resolve_precision(precision, M, N, K, opA, opB);
+    if (M == 0 || N == 0) name = "EMPTY";
+    else if (K == 0) name = "ZERO_FILL";
+    else if (prec == F4J_PREC_STRICT) name = "STRICT";
+    else if (prec == F4J_PREC_FP64) name = tile_name(pick_tile(M, N * (cplx ? 2 : 1)));
+    else name = std::string("SPLIT_") + prec_names[prec];
+    std::snprintf(buf, buflen, "%s", name.c_str());
+    return F4J_OK;
+}
+
+}  // extern "C"

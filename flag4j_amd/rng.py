@@ -1,0 +1,90 @@
+"""Seeded synthetic inputs with java.util.Random semantics, vectorised.
+
+java.util.Random is the 48-bit LCG  s' = (0x5DEECE66D * s + 0xB) mod 2^48,
+seeded with s0 = seed ^ 0x5DEECE66D, next(bits) = s >> (48 - bits), and
+nextDouble() = ((next(26) << 27) + next(27)) * 2^-53 (Java SE API
+specification of java.util.Random).  Because the step is affine, state n is
+s_n = A_n s0 + C_n (mod 2^48) with A_n = a^n, C_n = c (a^n - 1)/(a - 1); this
+module evaluates that in blocks with numpy uint64 arithmetic (wrapping mod
+2^64 keeps the low 48 bits exact), so large inputs are generated without a
+Python-level loop per element.  Flag4j draws its random matrices from
+java.util.Random (M:rng/RandomComplex.java:69, M:rng/RandomDenseTensor.java:94-97),
+so a Java harness seeded the same way produces the same matrices.
+"""
+import numpy as np
+
+_A = 0x5DEECE66D
+_C = 0xB
+_MASK = (1 << 48) - 1
+_BLOCK = 1 << 14
+
+
+def _affine_pow(n):
+    """(A_n, C_n) such that s_n = A_n * s_0 + C_n mod 2^48."""
+    a, c = 1, 0
+    pa, pc = _A, _C
+    while n:
+        if n & 1:
+            a, c = (pa * a) & _MASK, (pa * c + pc) & _MASK
+        pa, pc = (pa * pa) & _MASK, (pa * pc + pc) & _MASK
+        n >>= 1
+    return a, c
+
+
+def _states(s0, start, count):
+    """LCG states s_{start+1} .. s_{start+count} (the values next() shifts from)."""
+    L = _BLOCK
+    nblk = (count + L - 1) // L
+    # states within one block, relative to its origin: s_{k} for k = 1..L
+    a1 = np.empty(L, dtype=np.uint64)
+    c1 = np.empty(L, dtype=np.uint64)
+    a, c = 1, 0
+    for k in range(L):
+        a, c = (_A * a) & _MASK, (_A * c + _C) & _MASK
+        a1[k], c1[k] = a, c
+    out = np.empty(nblk * L, dtype=np.uint64)
+    mask = np.uint64(_MASK)
+    ab, cb = _affine_pow(start)
+    aL, cL = _affine_pow(L)
+    origin = (ab * s0 + cb) & _MASK              # s_start
+    for b in range(nblk):
+        o = np.uint64(origin)
+        with np.errstate(over="ignore"):
+            out[b * L:(b + 1) * L] = (a1 * o + c1) & mask
+        origin = (aL * origin + cL) & _MASK
+    return out[:count]
+
+
+def _next_doubles(seed, start_double, n):
+    s0 = (seed ^ _A) & _MASK
+    st = _states(s0, 2 * start_double, 2 * n)
+    hi = (st[0::2] >> np.uint64(48 - 26)).astype(np.int64)
+    lo = (st[1::2] >> np.uint64(48 - 27)).astype(np.int64)
+    return ((hi << 27) + lo).astype(np.float64) * 2.0**-53
+
+
+def next_doubles(seed, n, chunk=1 << 22):
+    """The first n values of new java.util.Random(seed).nextDouble()."""
+    out = np.empty(n, dtype=np.float64)
+    for s in range(0, n, chunk):
+        e = min(n, s + chunk)
+        out[s:e] = _next_doubles(seed, s, e - s)
+    return out
+
+
+def uniform(seed, rows, cols):
+    """rows x cols matrix of 2r - 1 (uniform on [-1, 1)), row-major."""
+    return (2.0 * next_doubles(seed, rows * cols) - 1.0).reshape(rows, cols)
+
+
+def uniform_complex(seed, rows, cols):
+    """Complex matrix whose real and imaginary parts are 2r - 1, drawn re then im per entry."""
+    v = 2.0 * next_doubles(seed, 2 * rows * cols) - 1.0
+    return v.view(np.complex128).reshape(rows, cols)
+
+
+def scaled_uniform(seed, rows, cols, phi):
+    """(2r - 1) * 10^(phi (2r' - 1)): entries spread over 2 phi decades (r, r' consecutive draws)."""
+    v = next_doubles(seed, 2 * rows * cols)
+    r, rp = v[0::2], v[1::2]
+    return ((2.0 * r - 1.0) * 10.0 ** (phi * (2.0 * rp - 1.0))).reshape(rows, cols)

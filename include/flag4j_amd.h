@@ -1,0 +1,138 @@
+/*
+ * flag4j_amd: MI355X-native dense matrix multiply behind a C ABI.
+ *
+ * Drop-in backend for Flag4j's dense matmul path.  Citation prefix
+ * M: = src/main/java/org/flag4j/ in the Flag4j sources.  Each entry point
+ * names the reference interface it replaces.
+ *
+ * Conventions
+ *  - All matrices are row-major.  Leading dimensions (lda, ldb, ldc) are in
+ *    elements (complex elements for the z* calls) and must be >= the stored
+ *    row length.  Complex data is interleaved (re, im) binary64 pairs.
+ *  - op(A) is M x K and op(B) is K x N.  opA == F4J_OP_N means A is stored
+ *    M x K; F4J_OP_T / F4J_OP_C mean A is stored K x M (and, for C, its
+ *    entries are conjugated).  Same for B (stored K x N, or N x K).
+ *  - The library never throws or aborts: every call returns an f4j_status;
+ *    f4j_last_error() gives the calling thread's last message.  On failure no
+ *    partial result is promised.
+ *  - Host-array calls are synchronous and thread-safe; they stage through
+ *    library-owned device buffers that are grown geometrically and reused.
+ *  - *_dev calls take device pointers and run asynchronously on `stream`
+ *    (a hipStream_t; NULL selects the library's own non-blocking stream).
+ *  - The first call initialises device 0 if f4j_init was not called.
+ */
+#ifndef FLAG4J_AMD_H
+#define FLAG4J_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define F4J_ABI_VERSION 1
+
+typedef enum f4j_status {
+    F4J_OK = 0,
+    F4J_ERR_INVALID_ARG = -1,     /* bad op/precision code, null pointer, negative size, ld too small */
+    F4J_ERR_SHAPE = -2,           /* inner dimensions differ (Flag4j: LinearAlgebraException) */
+    F4J_ERR_OUT_OF_MEMORY = -3,
+    F4J_ERR_HIP = -4,             /* a HIP runtime call failed */
+    F4J_ERR_NO_DEVICE = -5,
+    F4J_ERR_UNSUPPORTED = -6,     /* valid request this build does not implement */
+    F4J_ERR_ACCURACY = -7         /* split-precision residual check failed */
+} f4j_status;
+
+typedef enum f4j_op { F4J_OP_N = 0, F4J_OP_T = 1, F4J_OP_C = 2 } f4j_op;
+
+typedef enum f4j_precision {
+    F4J_PREC_AUTO = -1,           /* dispatcher decides (small -> STRICT, else the configured default) */
+    F4J_PREC_FP64 = 0,            /* FP64 matrix cores, FMA accumulation */
+    F4J_PREC_STRICT = 1,          /* Flag4j's arithmetic: per element, k ascending, separate mul and add:
+                                     bitwise equal to the reference CPU kernels */
+    F4J_PREC_SPLIT_BF16 = 2,      /* FP64 split into BF16 slices, FP32-accumulated MFMA, FP64 recombination */
+    F4J_PREC_SPLIT_FP16 = 3,      /* same with FP16 slices */
+    F4J_PREC_SPLIT_INT8 = 4       /* same with INT8 slices and exact INT32 accumulation */
+} f4j_precision;
+
+typedef struct f4j_device_info {
+    int device;
+    int compute_units;
+    int clock_khz;
+    int lds_bytes_per_cu;
+    int xcds;
+    int64_t hbm_bytes;
+    double peak_fp64_tflops;      /* dense FP64 matrix peak used for roofline fractions */
+    double peak_bf16_tflops;
+    double peak_hbm_gbs;
+    char name[64];
+    char arch[32];
+} f4j_device_info;
+
+/* ---- lifecycle (no reference counterpart: Flag4j has no native layer; the
+ *      closest is its lazy static initialisation, M:concurrency/ThreadManager.java:66-88) ---- */
+int         f4j_abi_version(void);
+int         f4j_init(int device);
+int         f4j_shutdown(void);
+const char *f4j_last_error(void);
+int         f4j_device_info_get(f4j_device_info *out);
+
+/* ---- device memory (device-resident operands, pinned and reused by callers) ---- */
+int f4j_dev_alloc(size_t bytes, void **dptr);
+int f4j_dev_free(void *dptr);
+int f4j_memcpy_h2d(void *dst, const void *src, size_t bytes, void *stream);
+int f4j_memcpy_d2h(void *dst, const void *src, size_t bytes, void *stream);
+int f4j_stream_sync(void *stream);
+
+/* ---- real FP64 GEMM ----
+ * Replaces, by (opA, opB):
+ *   (N, N): Matrix.mult(Matrix)           M:arrays/dense/Matrix.java:638-640 ->
+ *           ReDeMatMultDispatcher.dispatch M:linalg/ops/dispatch/ReDeMatMultDispatcher.java:157-237 ->
+ *           RealDenseMatMult.*             M:linalg/ops/dense/real/RealDenseMatMult.java:50-415
+ *   (N, T): Matrix.multTranspose(Matrix)  M:arrays/dense/Matrix.java:655-663 ->
+ *           RealDenseMatrixMultiplyDispatcher.dispatchTranspose M:linalg/ops/RealDenseMatrixMultiplyDispatcher.java:148-154 ->
+ *           RealDenseMatMultTranspose.*   M:linalg/ops/dense/real/RealDenseMatMultTranspose.java:53-233
+ *   (T, *): A.T().mult(B) without materialising A.T() (M:linalg/ops/TransposeDispatcher.java:77-99).
+ * C (M x N) = op(A) op(B).  `slices` is used by the SPLIT precisions (0 = choose). */
+int f4j_dgemm(int opA, int opB, int64_t M, int64_t N, int64_t K,
+              const double *A, int64_t lda, const double *B, int64_t ldb,
+              double *C, int64_t ldc, int precision, int slices);
+int f4j_dgemm_dev(int opA, int opB, int64_t M, int64_t N, int64_t K,
+                  const double *A, int64_t lda, const double *B, int64_t ldb,
+                  double *C, int64_t ldc, int precision, int slices, void *stream);
+
+/* ---- complex128 GEMM ----
+ * Replaces, by (opA, opB):
+ *   (N, N): CMatrix.mult(CMatrix)         M:arrays/dense/CMatrix.java:463-465 ->
+ *           Cm128DeMatMultDispatcher       M:linalg/ops/dispatch/Cm128DeMatMultDispatcher.java:155-226 ->
+ *           DenseSemiringMatMult.*         M:linalg/ops/dense/semiring_ops/DenseSemiringMatMult.java:54-424
+ *   (N, T): CMatrix.multTranspose(CMatrix) M:arrays/backend/semiring_arrays/AbstractDenseSemiringMatrix.java:330-334 ->
+ *           DenseSemiringMatMultTranspose.* M:linalg/ops/dense/semiring_ops/DenseSemiringMatMultTranspose.java:54-178
+ *   (C, N): A.H().mult(B)                 M:arrays/backend/field_arrays/AbstractDenseFieldMatrix.java:98-102
+ *           with the conjugate transpose fused into the operand load.
+ * `use_3m` selects the 3-multiplication (Gauss) form in the SPLIT precisions. */
+int f4j_zgemm(int opA, int opB, int64_t M, int64_t N, int64_t K,
+              const double *A, int64_t lda, const double *B, int64_t ldb,
+              double *C, int64_t ldc, int precision, int slices, int use_3m);
+int f4j_zgemm_dev(int opA, int opB, int64_t M, int64_t N, int64_t K,
+                  const double *A, int64_t lda, const double *B, int64_t ldb,
+                  double *C, int64_t ldc, int precision, int slices, int use_3m, void *stream);
+
+/* ---- dispatcher configuration ----
+ * Mirrors M:linalg/ops/dispatch/configs/ReDeMatMultDispatchConfigs.java:52-117 and
+ * Cm128DeMatMultDispatchConfigs.java:52-111 (key=value .properties files); unknown keys are
+ * rejected with F4J_ERR_INVALID_ARG instead of being ignored.  Keys: smallThreshold,
+ * defaultPrecision (fp64|strict|split_bf16|split_fp16|split_int8), defaultSlices. */
+int f4j_config_set(const char *key, const char *value);
+int f4j_config_get(const char *key, char *buf, size_t buflen);
+int f4j_config_load(const char *path);
+/* The kernel the dispatcher would run for a real (is_complex = 0) or complex product of
+ * op(A) (M x K) times op(B) (K x N): writes its name (e.g. "STRICT", "MFMA_F64_128x128x16"). */
+int f4j_dispatch_name(int is_complex, int opA, int opB, int64_t M, int64_t N, int64_t K,
+                      int precision, char *buf, size_t buflen);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,208 @@
+// Host-side emulator of the FP64 MFMA GEMM kernel's index math.
+//
+// Runs the same tile mapping (tile_of), stored-block geometry, LDS placement
+// and per-lane fragment addressing (Opnd<> from gemm_tile.h) as the device
+// kernel, sequentially on the CPU, with the documented operand/accumulator
+// layouts of v_mfma_f64_16x16x4f64.  Every LDS write and read is bounds
+// checked, every read must hit a location written in the same stage, and
+// every C element must be written exactly once.  Inputs are small integers,
+// so the product is exact and compared bit for bit with a direct product.
+//
+// usage: gemm_emu <ka> <kb> <M> <N> <K> <cfg>   (M,N,K in element units;
+//        kinds 0..4 = R_N R_T Z_N Z_T Z_C)
+#include <cstdio>
+#include <cstdlib>
+#include <random>
+#include <vector>
+
+#include "../../flag4j_amd/csrc/gemm_tile.h"
+
+using namespace f4j;
+
+static int g_fail = 0;
+#define CHECK(c, ...)                          \
+    do {                                       \
+        if (!(c)) {                            \
+            if (g_fail++ < 10) {               \
+                std::printf("FAIL: " __VA_ARGS__); \
+                std::printf("\n");             \
+            }                                  \
+        }                                      \
+    } while (0)
+
+template <int BM, int BN, int BK, int WGM, int WGN, int KA, int KB>
+void emulate(const GemmProblem &p, std::vector<int> &written) {
+    constexpr int NT = WGM * WGN * 64;
+    constexpr int WTM = BM / WGM, WTN = BN / WGN;
+    constexpr int FM = WTM / 16, FN = WTN / 16;
+    using OA = Opnd<KA, true, BM, BK>;
+    using OB = Opnd<KB, false, BN, BK>;
+    static_assert((OA::SR * OA::SC) % NT == 0 && (OB::SR * OB::SC) % NT == 0, "");
+    constexpr int EA = OA::SR * OA::SC / NT, EB = OB::SR * OB::SC / NT;
+    const int64_t tiles_m = (p.M + BM - 1) / BM, tiles_n = (p.N + BN - 1) / BN;
+    std::vector<int> tile_seen(tiles_m * tiles_n, 0);
+    for (int64_t bid = 0; bid < tiles_m * tiles_n; ++bid) {
+        int64_t tm, tn;
+        tile_of(bid, tiles_m, tiles_n, tm, tn);
+        CHECK(tm >= 0 && tm < tiles_m && tn >= 0 && tn < tiles_n, "tile out of range bid=%ld", (long)bid);
+        if (g_fail) return;
+        tile_seen[tm * tiles_n + tn]++;
+        const int64_t m0 = tm * BM, n0 = tn * BN;
+        std::vector<double> acc((size_t)WGM * WGN * FM * FN * 64 * 4, 0.0);
+        const int64_t nk = (p.K + BK - 1) / BK;
+        for (int64_t kt = 0; kt < nk; ++kt) {
+            const int64_t k0 = kt * BK;
+            std::vector<double> la(OA::LDS_DOUBLES, 0.0), lb(OB::LDS_DOUBLES, 0.0);
+            std::vector<int> wa(OA::LDS_DOUBLES, 0), wb(OB::LDS_DOUBLES, 0);
+            int64_t r0, c0, rows, cols;
+            OA::block(m0, k0, p.M, p.K, r0, c0, rows, cols);
+            for (int tid = 0; tid < NT; ++tid)
+                for (int i = 0; i < EA; ++i) {
+                    const int e = tid + i * NT;
+                    const int64_t r = r0 + OA::e_row(e), c = c0 + OA::e_col(e);
+                    const int li = OA::e_lds(e);
+                    CHECK(li >= 0 && li < OA::LDS_DOUBLES, "A lds write oob");
+                    if (g_fail) return;
+                    la[li] = (r < rows && c < cols) ? p.A[r * p.lda + c] : 0.0;
+                    wa[li]++;
+                }
+            OB::block(n0, k0, p.N, p.K, r0, c0, rows, cols);
+            for (int tid = 0; tid < NT; ++tid)
+                for (int i = 0; i < EB; ++i) {
+                    const int e = tid + i * NT;
+                    const int64_t r = r0 + OB::e_row(e), c = c0 + OB::e_col(e);
+                    const int li = OB::e_lds(e);
+                    CHECK(li >= 0 && li < OB::LDS_DOUBLES, "B lds write oob");
+                    if (g_fail) return;
+                    lb[li] = (r < rows && c < cols) ? p.B[r * p.ldb + c] : 0.0;
+                    wb[li]++;
+                }
+            for (int li = 0; li < OA::LDS_DOUBLES; ++li) CHECK(wa[li] <= 1, "A lds double write");
+            for (int li = 0; li < OB::LDS_DOUBLES; ++li) CHECK(wb[li] <= 1, "B lds double write");
+            for (int wave = 0; wave < WGM * WGN; ++wave) {
+                const int wm = (wave / WGN) * WTM, wn = (wave % WGN) * WTN;
+                for (int ks = 0; ks < BK / 4; ++ks) {
+                    double a[FM][64], b[FN][64];
+                    for (int lane = 0; lane < 64; ++lane) {
+                        for (int i = 0; i < FM; ++i) {
+                            const int li = OA::lane_base(lane, wm) + ks * OA::KS + i * OA::FS;
+                            CHECK(li >= 0 && li < OA::LDS_DOUBLES && wa[li] == 1, "A frag read oob/unwritten");
+                            if (g_fail) return;
+                            a[i][lane] = la[li] * OA::lane_sign(lane);
+                        }
+                        for (int j = 0; j < FN; ++j) {
+                            const int li = OB::lane_base(lane, wn) + ks * OB::KS + j * OB::FS;
+                            CHECK(li >= 0 && li < OB::LDS_DOUBLES && wb[li] == 1, "B frag read oob/unwritten");
+                            if (g_fail) return;
+                            b[j][lane] = lb[li] * OB::lane_sign(lane);
+                        }
+                    }
+                    // v_mfma_f64_16x16x4f64: A[i][k] in lane i+16k, B[k][j] in lane j+16k,
+                    // D[i][j] in lane j+16*(i/4), item i%4.
+                    for (int i = 0; i < FM; ++i)
+                        for (int j = 0; j < FN; ++j)
+                            for (int di = 0; di < 16; ++di)
+                                for (int dj = 0; dj < 16; ++dj) {
+                                    double s = 0;
+                                    for (int k = 0; k < 4; ++k) s += a[i][di + 16 * k] * b[j][dj + 16 * k];
+                                    const int lane = dj + 16 * (di / 4), item = di % 4;
+                                    acc[((((size_t)wave * FM + i) * FN + j) * 64 + lane) * 4 + item] += s;
+                                }
+                }
+            }
+        }
+        for (int wave = 0; wave < WGM * WGN; ++wave) {
+            const int wm = (wave / WGN) * WTM, wn = (wave % WGN) * WTN;
+            for (int lane = 0; lane < 64; ++lane)
+                for (int i = 0; i < FM; ++i)
+                    for (int j = 0; j < FN; ++j) {
+                        const int64_t col = n0 + wn + (lane & 15) + j * 16;
+                        if (col >= p.N) continue;
+                        for (int r = 0; r < 4; ++r) {
+                            const int64_t row = m0 + wm + 4 * (lane >> 4) + i * 16 + r;
+                            if (row < p.M) {
+                                p.C[row * p.ldc + col] = acc[((((size_t)wave * FM + i) * FN + j) * 64 + lane) * 4 + r];
+                                written[row * p.N + col]++;
+                            }
+                        }
+                    }
+        }
+    }
+    for (size_t t = 0; t < tile_seen.size(); ++t) CHECK(tile_seen[t] == 1, "tile %zu seen %d times", t, tile_seen[t]);
+}
+
+template <int KA, int KB>
+void run_cfg(int cfg, const GemmProblem &p, std::vector<int> &w) {
+    switch (cfg) {
+        case 1: emulate<64, 64, 16, 2, 2, KA, KB>(p, w); break;
+        case 2: emulate<128, 128, 32, 2, 2, KA, KB>(p, w); break;
+        default: emulate<128, 128, 16, 2, 2, KA, KB>(p, w); break;
+    }
+}
+
+// element (r,c) of op(X) where op(X) is R x Ccols; X complex interleaved or real
+static void get(const std::vector<double> &X, int kind, int64_t R, int64_t Cc, int64_t r, int64_t c, double &re,
+                double &im) {
+    const bool cplx = kind >= Z_N, trans = (kind == R_T || kind == Z_T || kind == Z_C);
+    const int64_t idx = trans ? c * R + r : r * Cc + c;
+    if (!cplx) {
+        re = X[idx];
+        im = 0;
+        return;
+    }
+    re = X[2 * idx];
+    im = (kind == Z_C) ? -X[2 * idx + 1] : X[2 * idx + 1];
+}
+
+int main(int argc, char **argv) {
+    if (argc < 7) return 2;
+    const int ka = std::atoi(argv[1]), kb = std::atoi(argv[2]);
+    const int64_t M = std::atoll(argv[3]), N = std::atoll(argv[4]), K = std::atoll(argv[5]);
+    const int cfg = std::atoi(argv[6]);
+    const bool cplx = ka >= Z_N;
+    const int w = cplx ? 2 : 1;
+    std::mt19937_64 rng(1234 + M * 7 + N * 13 + K * 17 + ka * 3 + kb);
+    std::uniform_int_distribution<int> d(-8, 8);
+    std::vector<double> A(M * K * w), B(K * N * w);
+    for (auto &x : A) x = d(rng);
+    for (auto &x : B) x = d(rng);
+    const bool ta = (ka == R_T || ka == Z_T || ka == Z_C), tb = (kb == R_T || kb == Z_T || kb == Z_C);
+    GemmProblem p;
+    p.M = M;
+    p.N = N * w;
+    p.K = K * w;
+    p.A = A.data();
+    p.lda = (ta ? M : K) * w;
+    p.B = B.data();
+    p.ldb = (tb ? K : N) * w;
+    std::vector<double> C(M * N * w, -777.0);
+    p.C = C.data();
+    p.ldc = N * w;
+    std::vector<int> written(M * N * w, 0);
+#define CASE(A_, B_) \
+    if (ka == A_ && kb == B_) run_cfg<A_, B_>(cfg, p, written);
+    CASE(R_N, R_N) CASE(R_N, R_T) CASE(R_T, R_N) CASE(R_T, R_T)
+    CASE(Z_N, Z_N) CASE(Z_N, Z_T) CASE(Z_N, Z_C) CASE(Z_T, Z_N) CASE(Z_T, Z_T) CASE(Z_T, Z_C)
+    CASE(Z_C, Z_N) CASE(Z_C, Z_T) CASE(Z_C, Z_C)
+#undef CASE
+    for (size_t i = 0; i < written.size(); ++i) CHECK(written[i] == 1, "C[%zu] written %d times", i, written[i]);
+    for (int64_t i = 0; i < M && !g_fail; ++i)
+        for (int64_t j = 0; j < N; ++j) {
+            double sr = 0, si = 0;
+            for (int64_t k = 0; k < K; ++k) {
+                double ar, ai, br, bi;
+                get(A, ka, M, K, i, k, ar, ai);
+                get(B, kb, K, N, k, j, br, bi);
+                sr += ar * br - ai * bi;
+                si += ar * bi + ai * br;
+            }
+            const double gr = C[(i * N + j) * w], gi = cplx ? C[(i * N + j) * w + 1] : 0.0;
+            CHECK(gr == sr && gi == si, "C(%ld,%ld) = (%g,%g) want (%g,%g)", (long)i, (long)j, gr, gi, sr, si);
+        }
+    if (g_fail) {
+        std::printf("FAILED %d\n", g_fail);
+        return 1;
+    }
+    std::printf("OK\n");
+    return 0;
+}

@@ -1,0 +1,132 @@
+"""The C ABI without a GPU: exported symbols, argument validation, configuration,
+and the dispatcher's decisions (none of these touch the device)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import flag4j_amd as f4j
+from flag4j_amd import _native
+from conftest import ROOT
+
+HEADER = os.path.join(ROOT, "include", "flag4j_amd.h")
+
+
+def declared_symbols():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(f4j_\w+)\s*\(", text)))
+
+
+def test_exports_every_declared_symbol():
+    lib = _native.load()
+    syms = declared_symbols()
+    assert len(syms) >= 18
+    for s in syms:
+        assert hasattr(lib, s), s
+    assert set(syms) == set(_native.SIGNATURES)
+
+
+def test_abi_version():
+    assert _native.load().f4j_abi_version() == 1
+
+
+def _dgemm(*args):
+    return _native.load().f4j_dgemm(*args)
+
+
+def test_invalid_arguments_rejected_before_device_use():
+    a = np.zeros(4)
+    p = ctypes.c_void_p(a.ctypes.data)
+    L = _native
+    assert _dgemm(7, 0, 2, 2, 2, p, 2, p, 2, p, 2, 0, 0) == L.ERR_INVALID_ARG      # bad op
+    assert _dgemm(2, 0, 2, 2, 2, p, 2, p, 2, p, 2, 0, 0) == L.ERR_INVALID_ARG      # op C on real
+    assert _dgemm(0, 0, -1, 2, 2, p, 2, p, 2, p, 2, 0, 0) == L.ERR_INVALID_ARG     # negative size
+    assert _dgemm(0, 0, 2, 2, 2, None, 2, p, 2, p, 2, 0, 0) == L.ERR_INVALID_ARG   # null A
+    assert _dgemm(0, 0, 2, 2, 3, p, 2, p, 2, p, 2, 0, 0) == L.ERR_INVALID_ARG      # lda < K
+    assert _dgemm(0, 0, 2, 3, 2, p, 2, p, 3, p, 2, 0, 0) == L.ERR_INVALID_ARG      # ldc < N
+    assert b"ldc" in _native.load().f4j_last_error()
+    assert _dgemm(0, 0, 2, 2, 2, p, 2, p, 2, p, 2, 9, 0) == L.ERR_INVALID_ARG      # precision
+    assert _native.load().f4j_zgemm(0, 3, 2, 2, 2, p, 2, p, 2, p, 2, 0, 0, 0) == L.ERR_INVALID_ARG
+
+
+def test_empty_products_need_no_device():
+    a = np.zeros(1)
+    p = ctypes.c_void_p(a.ctypes.data)
+    assert _dgemm(0, 0, 0, 5, 5, None, 5, p, 5, p, 5, 0, 0) == 0
+    assert _dgemm(0, 0, 5, 0, 5, p, 5, None, 1, None, 1, 0, 0) == 0
+
+
+def test_shape_errors_match_reference_exceptions():
+    """Matrix.mult -> LinearAlgebraException (MatrixMultTests.java:61-62);
+    multTranspose -> IllegalArgumentException (MatrixMultTests.java:281-282);
+    CMatrix.mult -> LinearAlgebraException (CMatrixMultTests.java:100-101)."""
+    A = f4j.Matrix(np.ones((4, 3)))
+    with pytest.raises(f4j.LinearAlgebraException):
+        A.mult(f4j.Matrix(np.ones((4, 2))))
+    with pytest.raises(f4j.IllegalArgumentException):
+        A.multTranspose(f4j.Matrix(np.ones((2, 4))))
+    C = f4j.CMatrix(np.ones((4, 3), dtype=complex))
+    with pytest.raises(f4j.LinearAlgebraException):
+        C.mult(f4j.CMatrix(np.ones((2, 2), dtype=complex)))
+    with pytest.raises(f4j.LinearAlgebraException):
+        f4j.Matrix(np.ones((3, 2))).pow(2)
+    with pytest.raises(f4j.IllegalArgumentException):
+        f4j.Matrix(np.ones((3, 3))).pow(-1)
+    with pytest.raises(f4j.IllegalArgumentException):
+        f4j.CMatrix(np.ones((2, 3), dtype=complex)).pow(2)
+    with pytest.raises(f4j.IllegalArgumentException):
+        f4j.Matrix(f4j.Shape(2, 3), np.ones(5))
+    with pytest.raises(ArithmeticError):
+        f4j.Shape(70000, 70000)
+
+
+def _name(cplx, opA, opB, M, N, K, prec="auto"):
+    buf = ctypes.create_string_buffer(64)
+    _native.check(_native.load().f4j_dispatch_name(int(cplx), opA, opB, M, N, K, _native.PREC[prec], buf, 64))
+    return buf.value.decode()
+
+
+def test_dispatcher_small_path_threshold():
+    """Products with A.len + B.len < smallThreshold take the small path
+    (ReDeMatMultDispatcher.java:159-168); here that is the bitwise-strict kernel."""
+    assert _name(0, 0, 0, 4, 2, 3) == "STRICT"
+    assert _name(0, 0, 0, 34, 35, 35) == "STRICT"             # 34*35 + 35*35 = 2415 < 2450
+    assert _name(0, 0, 0, 35, 35, 35).startswith("MFMA")      # 2450 entries: not small
+    assert _name(0, 0, 0, 512, 512, 512).startswith("MFMA_F64")
+    assert _name(1, 2, 0, 300, 300, 300).startswith("MFMA_F64")
+    assert _name(0, 0, 0, 8192, 8192, 8192) == "MFMA_F64_128x128x16"
+    assert _name(0, 0, 0, 0, 3, 3) == "EMPTY"
+    assert _name(0, 0, 0, 3, 3, 0) == "ZERO_FILL"
+    assert _name(0, 0, 0, 4, 2, 3, "fp64").startswith("MFMA_F64")
+
+
+def test_config_keys_validated():
+    L = _native.load()
+    buf = ctypes.create_string_buffer(32)
+    assert L.f4j_config_set(b"noSuchKey", b"1") == _native.ERR_INVALID_ARG
+    assert L.f4j_config_set(b"smallThreshold", b"abc") == _native.ERR_INVALID_ARG
+    assert L.f4j_config_set(b"defaultPrecision", b"fp128") == _native.ERR_INVALID_ARG
+    _native.check(L.f4j_config_get(b"smallThreshold", buf, 32))
+    old = buf.value
+    try:
+        _native.check(L.f4j_config_set(b"smallThreshold", b"0"))
+        assert _name(0, 0, 0, 4, 2, 3).startswith("MFMA")
+    finally:
+        _native.check(L.f4j_config_set(b"smallThreshold", old))
+
+
+def test_config_file_all_or_nothing(tmp_path):
+    L = _native.load()
+    good = tmp_path / "GpuMatMult.properties"
+    good.write_text("# comment\nsmallThreshold = 2450\ndefaultPrecision=fp64\n")
+    _native.check(L.f4j_config_load(str(good).encode()))
+    bad = tmp_path / "bad.properties"
+    bad.write_text("smallThreshold = 10\nsquareStandardVecThreshold = 256\n")
+    assert L.f4j_config_load(str(bad).encode()) == _native.ERR_INVALID_ARG
+    assert b"bad.properties:2" in L.f4j_last_error()
+    buf = ctypes.create_string_buffer(32)
+    _native.check(L.f4j_config_get(b"smallThreshold", buf, 32))
+    assert buf.value == b"2450"

@@ -1,0 +1,236 @@
+"""Pins the CPU oracle against the reference's own golden vectors and invariants,
+and against exact arithmetic where no reference vector exists.
+
+T: = src/test/java/org/flag4j/ of the reference; the fixtures were transcribed
+from those files by tests/golden/transcribe_fixtures.py.
+"""
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+from conftest import bits, cplx_mat, real_mat
+from oracle import oracle as o
+
+REAL_NN = {
+    "standard": o.real_standard,
+    "reordered": o.real_reordered,
+    "blocked_reordered_bs64": lambda A, B: o.real_blocked_reordered(A, B, 64),
+    "blocked_reordered_bs2": lambda A, B: o.real_blocked_reordered(A, B, 2),
+    "mt_reordered_T1": lambda A, B: o.real_mt_reordered(A, B, threads=1),
+    "mt_reordered_T3": lambda A, B: o.real_mt_reordered(A, B, threads=3),
+    "mt_blocked_reordered_T3_bs2": lambda A, B: o.real_mt_blocked_reordered(A, B, threads=3, bs=2),
+}
+
+
+@pytest.mark.parametrize("case", ["real_square", "real_rect"])
+@pytest.mark.parametrize("kernel", sorted(REAL_NN))
+def test_real_kernels_bitwise_golden(golden, case, kernel):
+    """T:linalg/ops/dense/real/MatrixMultiplyTests.java:14-107 (assertArrayEquals is bitwise)."""
+    g = golden[case]
+    A, B, exp = real_mat(g["a"]), real_mat(g["b"]), real_mat(g["c"])
+    act = REAL_NN[kernel](A, B).reshape(-1)
+    assert np.array_equal(bits(act), bits(exp))
+
+
+def test_golden_vectors_reject_fma(golden):
+    """A fused multiply-add accumulation is a plausible mistake; the rectangle
+    vector must catch it (6 of 24 entries differ)."""
+    g = golden["real_rect"]
+    A, B, exp = real_mat(g["a"]), real_mat(g["b"]), real_mat(g["c"])
+    fma = o.real_reordered_fma(A, B).reshape(-1)
+    assert (bits(fma) != bits(exp)).sum() == 6
+
+
+def test_golden_vectors_reject_transposed_operand(golden):
+    g = golden["real_square"]
+    A, B, exp = real_mat(g["a"]), real_mat(g["b"]), real_mat(g["c"])
+    assert not np.array_equal(o.real_reordered(A, B.T.copy()).reshape(-1), exp)
+    assert not np.array_equal(o.real_reordered(B, A).reshape(-1), exp)
+
+
+def test_real_column_vector_golden(golden):
+    """T:linalg/ops/dense/real/MatrixMultiplyTests.java:111-135 (N = 1 kernels)."""
+    g = golden["real_colvec"]
+    A, x, exp = real_mat(g["a"]), real_mat(g["b"]).reshape(-1), real_mat(g["c"])
+    assert np.array_equal(bits(o.real_blocked_vector(A, x, 64)), bits(exp))
+    assert np.array_equal(bits(o.real_standard(A, x.reshape(-1, 1)).reshape(-1)), bits(exp))
+
+
+def test_blocked_vector_partial_sums():
+    """The N = 1 blocked kernel (RealDenseMatMult.java:458-491) sums each 64-k
+    block from +0.0 and adds the block sums: check against a literal
+    restatement of that rule at K > 64, and that it differs from the k-ordered
+    form on an input built to separate them."""
+    rng = np.random.default_rng(3)
+    A = rng.standard_normal((5, 200)) * 10.0 ** rng.integers(-8, 8, (5, 200))
+    x = rng.standard_normal(200)
+    y = o.real_blocked_vector(A, x, 64)
+    for i in range(5):
+        tot = 0.0
+        for kk in range(0, 200, 64):
+            s = 0.0
+            for k in range(kk, min(kk + 64, 200)):
+                s += A[i, k] * x[k]
+            tot += s
+        assert y[i] == tot
+    assert not np.array_equal(y, o.real_standard(A, x.reshape(-1, 1)).reshape(-1))
+
+
+def test_real_api_mult_and_pow(golden):
+    """T:arrays/dense/matrix/MatrixMultTests.java:26-47 and :193-225 (pow(3) = A*(A*A),
+    Matrix.java:1932-1950)."""
+    g = golden["real_api_mult"]
+    assert np.array_equal(bits(o.real_reordered(real_mat(g["a"]), real_mat(g["b"]))), bits(real_mat(g["c"])))
+    g = golden["real_api_pow"]
+    A = real_mat(g["a"])
+    I = np.eye(3)
+    p3 = o.real_reordered(o.real_reordered(I, A), o.real_reordered(A, A))
+    assert np.array_equal(bits(p3), bits(real_mat(g["c3"])))
+    assert np.array_equal(real_mat(g["c0"]), I)
+    assert np.array_equal(real_mat(g["c1"]), A)
+
+
+@pytest.mark.parametrize("case", ["real_T_square", "real_T_rect", "real_api_multT"])
+def test_real_mult_transpose_invariant(golden, case):
+    """multTranspose(B) == mult(B.T()) exactly: RealDenseMatMultTransposeTests.java:13-69,
+    MatrixMultTests.java:250-266.  The fixture's B literal is the K x N factor;
+    multTranspose receives its transpose (N x K)."""
+    g = golden[case]
+    A, B = real_mat(g["a"]), real_mat(g["b"])
+    nn = o.real_reordered(A, B)
+    nt = o.real_mult_transpose(A, B.T.copy())
+    assert np.array_equal(bits(nn), bits(nt))
+
+
+@pytest.mark.parametrize("case", ["cplx_square", "cplx_rect", "cplx_api_mult"])
+def test_complex_bitwise_golden(golden, case):
+    """T:linalg/ops/dense/complex/CMatrixMultiplyTests.java:17-131,
+    T:arrays/dense/complex_matrix/CMatrixMultTests.java:64-87."""
+    g = golden[case]
+    A, B, exp = cplx_mat(g["a"]), cplx_mat(g["b"]), cplx_mat(g["c"])
+    act = o.cplx_gemm(A, B).reshape(exp.shape)
+    assert np.array_equal(act.real, exp.real) and np.array_equal(act.imag, exp.imag)
+    mt = o.cplx_mt_reordered(A, B, threads=2).reshape(exp.shape)
+    assert np.array_equal(mt.view(np.float64), act.view(np.float64))
+
+
+def test_complex_column_vector_golden(golden):
+    g = golden["cplx_colvec"]
+    A, B, exp = cplx_mat(g["a"]), cplx_mat(g["b"]), cplx_mat(g["c"])
+    assert np.array_equal(o.cplx_gemm(A, B).reshape(-1), exp)
+
+
+def test_complex_pow_golden(golden):
+    """T:arrays/dense/complex_matrix/CMatrixMultTests.java:198-219."""
+    g = golden["cplx_api_pow"]
+    A = cplx_mat(g["a"])
+    I = np.eye(3, dtype=np.complex128)
+    p3 = o.cplx_gemm(o.cplx_gemm(I, A), o.cplx_gemm(A, A))
+    assert np.array_equal(p3, cplx_mat(g["c3"]))
+
+
+def test_complex_golden_rejects_swapped_product_terms(golden):
+    """A plausible mistake, ar*bi - ai*br in the real part, must fail the vector."""
+    g = golden["cplx_square"]
+    A, B, exp = cplx_mat(g["a"]), cplx_mat(g["b"]), cplx_mat(g["c"])
+    wrong = (A.real @ B.real + A.imag @ B.imag) + 1j * (A.real @ B.imag - A.imag @ B.real)
+    assert not np.allclose(wrong.reshape(-1), exp)
+
+
+@pytest.mark.parametrize("case", ["cplx_T_square", "cplx_T_rect"])
+def test_complex_mult_transpose_invariant(golden, case):
+    """ComplexDenseMatMultTransposeTests.java:40-107: multTranspose(B) == mult(B.T())."""
+    g = golden[case]
+    A, B = cplx_mat(g["a"]), cplx_mat(g["b"])
+    nn = o.cplx_gemm(A, B)
+    nt = o.cplx_gemm(A, B.T.copy(), "N", "T")
+    assert np.array_equal(nn.view(np.float64), nt.view(np.float64))
+
+
+def test_complex_conjugate_transpose_operand():
+    """op(A) = A^H read on the fly equals multiplying the materialised A.H()
+    (conjugation is exact, Complex128.java:513-516)."""
+    rng = np.random.default_rng(5)
+    A = rng.standard_normal((7, 5)) + 1j * rng.standard_normal((7, 5))
+    B = rng.standard_normal((7, 4)) + 1j * rng.standard_normal((7, 4))
+    fused = o.cplx_gemm(A, B, "C", "N")
+    mat = o.cplx_gemm(A.conj().T.copy(), B)
+    assert np.array_equal(fused.view(np.float64), mat.view(np.float64))
+
+
+def test_integer_inputs_exact():
+    """Integer-valued inputs have exact products and sums: every kernel must give
+    the exact integer product."""
+    rng = np.random.default_rng(11)
+    A = rng.integers(-50, 50, (33, 47)).astype(np.float64)
+    B = rng.integers(-50, 50, (47, 29)).astype(np.float64)
+    exact = (A.astype(np.int64) @ B.astype(np.int64)).astype(np.float64)
+    for f in REAL_NN.values():
+        assert np.array_equal(f(A, B), exact)
+    assert np.array_equal(o.real_mult_transpose(A, B.T.copy()), exact)
+
+
+def test_dot2_against_exact_rationals():
+    """The accuracy oracle (compensated Dot2) against exact rational dot products."""
+    rng = np.random.default_rng(7)
+    M, N, K = 6, 5, 300
+    A = rng.standard_normal((M, K)) * 10.0 ** rng.uniform(-8, 8, (M, K))
+    B = rng.standard_normal((K, N)) * 10.0 ** rng.uniform(-8, 8, (K, N))
+    I, J = np.repeat(np.arange(M), N), np.tile(np.arange(N), M)
+    hi, lo, ab = o.real_dot2(A, B, I, J)
+    for s in range(len(I)):
+        ex = sum(Fraction(A[I[s], k]) * Fraction(B[k, J[s]]) for k in range(K))
+        err = abs(Fraction(hi[s]) + Fraction(lo[s]) - ex)
+        assert float(err) <= 2.0**-100 * ab[s] + 2.0**-1074
+    # transposed B
+    hi2, lo2, _ = o.real_dot2(A, B.T.copy(), I, J, transB=True)
+    assert np.array_equal(hi, hi2) and np.array_equal(lo, lo2)
+
+
+def test_cplx_dot2_against_exact_rationals():
+    rng = np.random.default_rng(8)
+    M, N, K = 4, 3, 50
+    A = rng.standard_normal((K, M)) + 1j * rng.standard_normal((K, M))
+    B = rng.standard_normal((K, N)) + 1j * rng.standard_normal((K, N))
+    I, J = np.repeat(np.arange(M), N), np.tile(np.arange(N), M)
+    rh, rl, ih, il, ab = o.cplx_dot2(A, B, I, J, "C", "N")
+    for s in range(len(I)):
+        i, j = I[s], J[s]
+        er = sum(Fraction(A[k, i].real) * Fraction(B[k, j].real) + Fraction(A[k, i].imag) * Fraction(B[k, j].imag)
+                 for k in range(K))
+        ei = sum(Fraction(A[k, i].real) * Fraction(B[k, j].imag) - Fraction(A[k, i].imag) * Fraction(B[k, j].real)
+                 for k in range(K))
+        assert float(abs(Fraction(rh[s]) + Fraction(rl[s]) - er)) <= 2.0**-100 * ab[s]
+        assert float(abs(Fraction(ih[s]) + Fraction(il[s]) - ei)) <= 2.0**-100 * ab[s]
+
+
+def test_transpose_selector_reference_table():
+    """T:linalg/ops/RealDenseMatrixMultiplyDispatcherTests.java:56-64 (rows list :19)."""
+    rows = [5, 25, 50, 150, 250, 500, 1024, 2048, 4096]
+    exp = ["MULT_T", "MULT_T", "MULT_T_BLOCKED", "MULT_T_CONCURRENT", "MULT_T_CONCURRENT", "MULT_T_CONCURRENT",
+           "MULT_T_CONCURRENT", "MULT_T_BLOCKED_CONCURRENT", "MULT_T_BLOCKED_CONCURRENT"]
+    assert [o.select_transpose(r) for r in rows] == exp
+
+
+def test_real_dispatch_selections_for_the_workloads():
+    """ReDeMatMultDispatcher.getFunc (ReDeMatMultDispatcher.java:183-237) on the
+    benchmark shapes, as derived by hand from the source: (m, n=inner, k=B.cols)."""
+    assert o.select_real(512, 512, 512) == "MT_REORD"
+    assert o.select_real(8192, 8192, 8192) == "MT_BLK_REORD"
+    assert o.select_real(16384, 4096, 16384) == "MT_REORD"
+    assert o.select_real(4, 3, 2) == "SMALL_STANDARD"
+    assert o.select_real(40, 40, 40) == "MT_STRD"
+    assert o.select_real(1000, 5, 1000) == "STRD"       # m == maxDim, minDim < 10
+    assert o.select_real(20, 1000, 600) == "MT_STRD"    # n == maxDim, m < 25
+    assert o.select_real(30, 600, 1000) == "MT_REORD"   # k == maxDim
+    assert o.select_real(300, 300, 1) == "MT_BLK_VEC"
+    assert o.select_real(100, 100, 1) == "BLK_VEC"
+    assert o.select_cplx(300, 300, 1) == "MT_STRD_VEC"
+    assert o.select_cplx(2048, 1536, 2048) == "MT_REORD"
+
+
+def test_java_random_known_answers():
+    """Known outputs of java.util.Random (Java SE specification of the LCG)."""
+    assert o.JavaRandom(42).next_int() == -1170105035
+    assert o.JavaRandom(0).next_double() == 0.730967787376657
