@@ -47,7 +47,7 @@ def cpu_baseline(A, B, n, threads):
     linearly to the full product."""
     import numpy as np
     from oracle import oracle as o
-    rows = min(n, threads * 64 * 2)
+    rows = min(n, threads * 64 * 8)
     C = np.empty((rows, n))
     t0 = time.perf_counter()
     o.real_mt_blocked_reordered(A[:rows], B, C=C, rows=(0, rows), threads=threads, bs=64)

@@ -116,10 +116,8 @@ __global__ __launch_bounds__(WGM *WGN *WAVE) void gemm_f64_kernel(GemmProblem p)
         __syncthreads();
     }
 
-    // Accumulator layout of v_mfma_f64_16x16x4f64: lane l, item r holds
-    // D[4*(l/16) + r][l % 16].
-    const int64_t col_l = n0 + wn + (lane & 15);
-    const int64_t row_l = m0 + wm + 4 * (lane >> 4);
+    const int64_t col_l = n0 + wn + mfma_f64_acc_col(lane);
+    const int64_t row_l = m0 + wm;
 #pragma unroll
     for (int i = 0; i < FM; ++i)
 #pragma unroll
@@ -128,7 +126,7 @@ __global__ __launch_bounds__(WGM *WGN *WAVE) void gemm_f64_kernel(GemmProblem p)
             if (col >= p.N) continue;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int64_t row = row_l + i * 16 + r;
+                const int64_t row = row_l + i * 16 + mfma_f64_acc_row(lane, r);
                 if (row < p.M) p.C[row * p.ldc + col] = acc[i][j][r];
             }
         }

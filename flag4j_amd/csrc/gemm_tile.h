@@ -85,6 +85,12 @@ struct Opnd {
     F4J_HD static int e_lds(int e) { return (e / SC) * LD + e % SC; }
 };
 
+// Result layout of v_mfma_f64_16x16x4f64: lane l, item r of the 4-double
+// accumulator holds D[l/16 + 4r][l%16] (rows interleave across the 4 items;
+// the FP32/BF16 16x16 forms instead hold 4 consecutive rows per lane).
+F4J_HD int mfma_f64_acc_row(int lane, int item) { return (lane >> 4) + 4 * item; }
+F4J_HD int mfma_f64_acc_col(int lane) { return lane & 15; }
+
 // Workgroup id -> (tile row, tile col).  The hardware deals consecutive
 // workgroup ids round-robin over the 8 XCDs; remap so each XCD walks a
 // contiguous band of tiles, then order the band in groups of GROUP tile-rows

@@ -98,14 +98,14 @@ void emulate(const GemmProblem &p, std::vector<int> &written) {
                         }
                     }
                     // v_mfma_f64_16x16x4f64: A[i][k] in lane i+16k, B[k][j] in lane j+16k,
-                    // D[i][j] in lane j+16*(i/4), item i%4.
+                    // D[i][j] in lane j+16*(i%4), item i/4.
                     for (int i = 0; i < FM; ++i)
                         for (int j = 0; j < FN; ++j)
                             for (int di = 0; di < 16; ++di)
                                 for (int dj = 0; dj < 16; ++dj) {
                                     double s = 0;
                                     for (int k = 0; k < 4; ++k) s += a[i][di + 16 * k] * b[j][dj + 16 * k];
-                                    const int lane = dj + 16 * (di / 4), item = di % 4;
+                                    const int lane = dj + 16 * (di % 4), item = di / 4;
                                     acc[((((size_t)wave * FM + i) * FN + j) * 64 + lane) * 4 + item] += s;
                                 }
                 }
@@ -119,7 +119,7 @@ void emulate(const GemmProblem &p, std::vector<int> &written) {
                         const int64_t col = n0 + wn + (lane & 15) + j * 16;
                         if (col >= p.N) continue;
                         for (int r = 0; r < 4; ++r) {
-                            const int64_t row = m0 + wm + 4 * (lane >> 4) + i * 16 + r;
+                            const int64_t row = m0 + wm + i * 16 + mfma_f64_acc_row(lane, r);
                             if (row < p.M) {
                                 p.C[row * p.ldc + col] = acc[((((size_t)wave * FM + i) * FN + j) * 64 + lane) * 4 + r];
                                 written[row * p.N + col]++;

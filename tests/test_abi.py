@@ -130,3 +130,13 @@ def test_config_file_all_or_nothing(tmp_path):
     buf = ctypes.create_string_buffer(32)
     _native.check(L.f4j_config_get(b"smallThreshold", buf, 32))
     assert buf.value == b"2450"
+
+
+def test_strict_kernel_has_no_fused_multiply_add():
+    """The strict kernel must round every product and sum on its own (Java
+    semantics): its gfx950 assembly may hold no v_fma_f64."""
+    import subprocess
+    subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "flag4j_amd", "csrc"), "strict-asm"], check=True)
+    asm = open(os.path.join(ROOT, "_build", "obj", "gemm_strict.s")).read()
+    assert "v_mul_f64" in asm and "v_add_f64" in asm
+    assert "v_fma_f64" not in asm

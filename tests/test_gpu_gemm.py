@@ -70,7 +70,8 @@ def test_golden_real_mult_transpose_invariant(golden, case):
     A = f4j.Matrix(real_mat(g["a"]))
     Bt = f4j.Matrix(real_mat(g["b"])).T()          # stored N x K
     assert A.multTranspose(Bt).equals(A.mult(Bt.T()))
-    assert np.array_equal(bits(A.multTranspose(Bt).data), bits(o.real_reordered(A.numpy(), real_mat(g["b"]))))
+    assert np.array_equal(bits(A.multTranspose(Bt).data),
+                          bits(o.real_reordered(A.numpy(), real_mat(g["b"])).reshape(-1)))
 
 
 @pytest.mark.parametrize("case", ["cplx_square", "cplx_rect", "cplx_api_mult", "cplx_colvec"])
