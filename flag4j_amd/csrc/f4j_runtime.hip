@@ -38,6 +38,7 @@ struct Config {
     int64_t small_threshold = 2450;   // ReDeMatMultDispatchConfigs.java:63 default
     int default_precision = F4J_PREC_FP64;
     int default_slices = 0;
+    int tile_config = -1;             // -1: chosen per shape bucket; else a fixed FP64 tile id
 };
 
 struct State {
@@ -159,7 +160,13 @@ int validate(bool cplx, int opA, int opB, int64_t M, int64_t N, int64_t K, const
 
 // Tile configuration per shape bucket: the 128x128 tile when it yields enough
 // workgroups to fill the 256 CUs, else the 64x64 tile.
+constexpr int NUM_TILE_CFGS = 4;
+
 int pick_tile(int64_t M, int64_t Nx) {
+    {
+        std::lock_guard<std::mutex> g(S().cfg_mu);
+        if (S().cfg.tile_config >= 0) return S().cfg.tile_config;
+    }
     const int64_t t128 = ((M + 127) / 128) * ((Nx + 127) / 128);
     return t128 >= 512 ? 0 : 1;
 }
@@ -167,7 +174,8 @@ int pick_tile(int64_t M, int64_t Nx) {
 const char *tile_name(int cfg) {
     switch (cfg) {
         case 1: return "MFMA_F64_64x64x16";
-        case 2: return "MFMA_F64_128x128x32";
+        case 2: return "MFMA_F64_128x128x32_W1";
+        case 3: return "MFMA_F64_128x128x16_W1";
         default: return "MFMA_F64_128x128x16";
     }
 }
@@ -207,6 +215,10 @@ int gemm_dev(bool cplx, int opA, int opB, int64_t M, int64_t N, int64_t K, const
     }
     if (prec != F4J_PREC_FP64)
         return fail(F4J_ERR_UNSUPPORTED, "precision " + std::to_string(prec) + " is not available in this build");
+    // Tile loads address a block with 32-bit buffer offsets (at most 256 stored
+    // rows of a tile): leading dimensions beyond ~1M doubles are not supported.
+    if ((lda * w) > (int64_t)(0x7FFFFFF0 / (256 * 8)) || (ldb * w) > (int64_t)(0x7FFFFFF0 / (256 * 8)))
+        return fail(F4J_ERR_UNSUPPORTED, "leading dimension too large for the MFMA kernels (limit 1048575 doubles)");
     GemmProblem p;
     p.M = M;
     p.N = N * w;
@@ -277,6 +289,11 @@ int config_set_locked(Config &c, const std::string &key, const std::string &valu
         const int p = parse_precision(value);
         if (p < 0) return fail(F4J_ERR_INVALID_ARG, "defaultPrecision: unknown value '" + value + "'");
         c.default_precision = p;
+    } else if (key == "tileConfig") {
+        const long v = std::strtol(value.c_str(), &end, 10);
+        if (end == value.c_str() || *end || v < -1 || v >= NUM_TILE_CFGS)
+            return fail(F4J_ERR_INVALID_ARG, "tileConfig: expected -1.." + std::to_string(NUM_TILE_CFGS - 1));
+        c.tile_config = (int)v;
     } else if (key == "defaultSlices") {
         const long v = std::strtol(value.c_str(), &end, 10);
         if (end == value.c_str() || *end || v < 0 || v > 16) return fail(F4J_ERR_INVALID_ARG, "defaultSlices: expected 0..16");
@@ -436,6 +453,7 @@ int f4j_config_get(const char *key, char *buf, size_t buflen) {
         if (k == "smallThreshold") v = std::to_string(s.cfg.small_threshold);
         else if (k == "defaultPrecision") v = prec_names[s.cfg.default_precision];
         else if (k == "defaultSlices") v = std::to_string(s.cfg.default_slices);
+        else if (k == "tileConfig") v = std::to_string(s.cfg.tile_config);
         else return fail(F4J_ERR_INVALID_ARG, "unknown configuration key '" + k + "'");
     }
     std::snprintf(buf, buflen, "%s", v.c_str());

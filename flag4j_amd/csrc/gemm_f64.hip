@@ -18,6 +18,8 @@
 // feeds the MFMAs.  Each wave owns a (BM/WGM) x (BN/WGN) sub-tile held as
 // 16x16 MFMA accumulators.  Loads are bounds-checked with zero fill, so any
 // M, N, K (including ragged edges) is valid.
+#include <type_traits>
+
 #include "gemm_tile.h"
 
 namespace f4j {
@@ -27,15 +29,48 @@ typedef double dbl4 __attribute__((ext_vector_type(4)));
 
 constexpr int WAVE = 64;
 
-template <int BM, int BN, int BK, int WGM, int WGN, int KA, int KB>
-__global__ __launch_bounds__(WGM *WGN *WAVE) void gemm_f64_kernel(GemmProblem p) {
+template <class O, int NT, int V>
+struct Loader {
+    using TC = TileCopy<O, NT, V>;
+    typedef double dbl2 __attribute__((ext_vector_type(2)));
+    using vec_t = typename std::conditional<V == 2, dbl2, double>::type;
+    vec_t r[TC::PASSES];
+
+    __device__ void load(const double *G, int64_t ld, int64_t x0, int64_t k0, int64_t X, int64_t K, int tid) {
+        int64_t r0, c0, rows, cols;
+        O::block(x0, k0, X, K, r0, c0, rows, cols);
+        const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+            (void *)(G + r0 * ld + c0), (short)0, (int)TC::num_records(r0, c0, rows, cols, ld), 0x00020000);
+#pragma unroll
+        for (int i = 0; i < TC::PASSES; ++i) {
+            const uint32_t off = TC::offset(tid, i, c0, cols, ld);
+            if constexpr (V == 2)
+                r[i] = __builtin_bit_cast(vec_t, __builtin_amdgcn_raw_buffer_load_b128(rs, (int)off, 0, 0));
+            else
+                r[i] = __builtin_bit_cast(vec_t, __builtin_amdgcn_raw_buffer_load_b64(rs, (int)off, 0, 0));
+        }
+    }
+    __device__ void store(double *lds, int tid) const {
+#pragma unroll
+        for (int i = 0; i < TC::PASSES; ++i) {
+            double *d = lds + TC::lds_index(tid, i);
+            if constexpr (V == 2) {
+                d[0] = r[i][0];
+                d[1] = r[i][1];
+            } else {
+                d[0] = r[i];
+            }
+        }
+    }
+};
+
+template <int BM, int BN, int BK, int WGM, int WGN, int MINW, int V, int KA, int KB>
+__global__ __launch_bounds__(WGM *WGN *WAVE, MINW) void gemm_f64_kernel(GemmProblem p) {
     constexpr int NT = WGM * WGN * WAVE;
     constexpr int WTM = BM / WGM, WTN = BN / WGN;
     constexpr int FM = WTM / 16, FN = WTN / 16;
     using OA = Opnd<KA, true, BM, BK>;
     using OB = Opnd<KB, false, BN, BK>;
-    static_assert((OA::SR * OA::SC) % NT == 0 && (OB::SR * OB::SC) % NT == 0, "tile/threads mismatch");
-    constexpr int EA = OA::SR * OA::SC / NT, EB = OB::SR * OB::SC / NT;
     constexpr int LDS_STAGE = OA::LDS_DOUBLES + OB::LDS_DOUBLES;
 
     __shared__ double lds[2 * LDS_STAGE];
@@ -57,41 +92,22 @@ __global__ __launch_bounds__(WGM *WGN *WAVE) void gemm_f64_kernel(GemmProblem p)
 #pragma unroll
         for (int j = 0; j < FN; ++j) acc[i][j] = dbl4{0.0, 0.0, 0.0, 0.0};
 
-    double ra[EA], rb[EB];
-
-    auto gload = [&](int64_t k0) {
-        int64_t r0, c0, rows, cols;
-        OA::block(m0, k0, p.M, p.K, r0, c0, rows, cols);
-#pragma unroll
-        for (int i = 0; i < EA; ++i) {
-            const int e = tid + i * NT;
-            const int64_t r = r0 + OA::e_row(e), c = c0 + OA::e_col(e);
-            ra[i] = (r < rows && c < cols) ? p.A[r * p.lda + c] : 0.0;
-        }
-        OB::block(n0, k0, p.N, p.K, r0, c0, rows, cols);
-#pragma unroll
-        for (int i = 0; i < EB; ++i) {
-            const int e = tid + i * NT;
-            const int64_t r = r0 + OB::e_row(e), c = c0 + OB::e_col(e);
-            rb[i] = (r < rows && c < cols) ? p.B[r * p.ldb + c] : 0.0;
-        }
-    };
-    auto lstore = [&](double *stage) {
-#pragma unroll
-        for (int i = 0; i < EA; ++i) stage[OA::e_lds(tid + i * NT)] = ra[i];
-        double *sb = stage + OA::LDS_DOUBLES;
-#pragma unroll
-        for (int i = 0; i < EB; ++i) sb[OB::e_lds(tid + i * NT)] = rb[i];
-    };
+    Loader<OA, NT, V> la;
+    Loader<OB, NT, V> lb;
 
     const int64_t nk = (p.K + BK - 1) / BK;
-    gload(0);
-    lstore(lds);
+    la.load(p.A, p.lda, m0, 0, p.M, p.K, tid);
+    lb.load(p.B, p.ldb, n0, 0, p.N, p.K, tid);
+    la.store(lds, tid);
+    lb.store(lds + OA::LDS_DOUBLES, tid);
     __syncthreads();
 
     for (int64_t kt = 0; kt < nk; ++kt) {
         const double *cur = lds + (kt & 1) * LDS_STAGE;
-        if (kt + 1 < nk) gload((kt + 1) * BK);
+        if (kt + 1 < nk) {
+            la.load(p.A, p.lda, m0, (kt + 1) * BK, p.M, p.K, tid);
+            lb.load(p.B, p.ldb, n0, (kt + 1) * BK, p.N, p.K, tid);
+        }
         const double *sa = cur, *sb = cur + OA::LDS_DOUBLES;
 #pragma unroll
         for (int ks = 0; ks < BK / 4; ++ks) {
@@ -112,7 +128,11 @@ __global__ __launch_bounds__(WGM *WGN *WAVE) void gemm_f64_kernel(GemmProblem p)
                 for (int j = 0; j < FN; ++j)
                     acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
         }
-        if (kt + 1 < nk) lstore(lds + ((kt + 1) & 1) * LDS_STAGE);
+        if (kt + 1 < nk) {
+            double *nxt = lds + ((kt + 1) & 1) * LDS_STAGE;
+            la.store(nxt, tid);
+            lb.store(nxt + OA::LDS_DOUBLES, tid);
+        }
         __syncthreads();
     }
 
@@ -132,29 +152,36 @@ __global__ __launch_bounds__(WGM *WGN *WAVE) void gemm_f64_kernel(GemmProblem p)
         }
 }
 
-template <int BM, int BN, int BK, int WGM, int WGN, int KA, int KB>
-hipError_t launch_cfg(const GemmProblem &p, hipStream_t stream) {
+template <int BM, int BN, int BK, int WGM, int WGN, int MINW, int KA, int KB>
+hipError_t launch_cfg(const GemmProblem &p, bool vec2, hipStream_t stream) {
     const int64_t tiles = ((p.M + BM - 1) / BM) * ((p.N + BN - 1) / BN);
     if (tiles == 0) return hipSuccess;
-    hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, BK, WGM, WGN, KA, KB>), dim3((unsigned)tiles),
-                       dim3(WGM * WGN * WAVE), 0, stream, p);
+    if (vec2)
+        hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, BK, WGM, WGN, MINW, 2, KA, KB>), dim3((unsigned)tiles),
+                           dim3(WGM * WGN * WAVE), 0, stream, p);
+    else
+        hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, BK, WGM, WGN, MINW, 1, KA, KB>), dim3((unsigned)tiles),
+                           dim3(WGM * WGN * WAVE), 0, stream, p);
     return hipGetLastError();
 }
 
 template <int KA, int KB>
-hipError_t launch_kinds(const GemmProblem &p, hipStream_t stream, int cfg) {
+hipError_t launch_kinds(const GemmProblem &p, bool vec2, hipStream_t stream, int cfg) {
     switch (cfg) {
-        case 1: return launch_cfg<64, 64, 16, 2, 2, KA, KB>(p, stream);
-        case 2: return launch_cfg<128, 128, 32, 2, 2, KA, KB>(p, stream);
-        default: return launch_cfg<128, 128, 16, 2, 2, KA, KB>(p, stream);
+        case 1: return launch_cfg<64, 64, 16, 2, 2, 2, KA, KB>(p, vec2, stream);
+        case 2: return launch_cfg<128, 128, 32, 2, 2, 1, KA, KB>(p, vec2, stream);
+        case 3: return launch_cfg<128, 128, 16, 2, 2, 1, KA, KB>(p, vec2, stream);
+        default: return launch_cfg<128, 128, 16, 2, 2, 2, KA, KB>(p, vec2, stream);
     }
 }
 
 }  // namespace
 
 hipError_t launch_gemm_f64(OpKind ka, OpKind kb, const GemmProblem &p, hipStream_t stream, int cfg) {
+    const bool vec2 = vec2_ok(p.A, p.lda, stored_cols(ka, true, p.M, p.N, p.K)) &&
+                      vec2_ok(p.B, p.ldb, stored_cols(kb, false, p.M, p.N, p.K));
 #define F4J_CASE(A_, B_) \
-    if (ka == A_ && kb == B_) return launch_kinds<A_, B_>(p, stream, cfg);
+    if (ka == A_ && kb == B_) return launch_kinds<A_, B_>(p, vec2, stream, cfg);
     F4J_CASE(R_N, R_N) F4J_CASE(R_N, R_T) F4J_CASE(R_T, R_N) F4J_CASE(R_T, R_T)
     F4J_CASE(Z_N, Z_N) F4J_CASE(Z_N, Z_T) F4J_CASE(Z_N, Z_C)
     F4J_CASE(Z_T, Z_N) F4J_CASE(Z_T, Z_T) F4J_CASE(Z_T, Z_C)

@@ -85,6 +85,50 @@ struct Opnd {
     F4J_HD static int e_lds(int e) { return (e / SC) * LD + e % SC; }
 };
 
+// Global -> LDS copy of one operand's stored block by NT threads with V-double
+// vectors (V = 1 or 2), through a buffer resource whose base is the block's
+// origin and whose num_records ends at the stored matrix's last element.
+// Rows past the matrix end therefore fall outside num_records; columns past
+// the matrix edge (which alias the next row in memory) get an offset beyond
+// any num_records.  The hardware returns 0 for both, so edge tiles need no
+// branches.  Every lane keeps the same column for all its passes.
+template <class O, int NT, int V>
+struct TileCopy {
+    static constexpr int SCV = O::SC / V;             // vectors per stored row
+    static constexpr int ROWS_PER_PASS = NT / SCV;
+    static constexpr int PASSES = O::SR * SCV / NT;
+    static_assert(NT % SCV == 0 && (O::SR * SCV) % NT == 0, "tile copy does not divide evenly");
+    static constexpr uint32_t OOB = 0x80000000u;      // beyond every num_records used
+
+    F4J_HD static int lane_col(int tid) { return V * (tid % SCV); }
+    F4J_HD static int lane_row(int tid) { return tid / SCV; }
+    F4J_HD static int lds_index(int tid, int pass) { return (lane_row(tid) + pass * ROWS_PER_PASS) * O::LD + lane_col(tid); }
+    // num_records (bytes) for a block at (r0, c0) of a rows x cols matrix.
+    F4J_HD static uint32_t num_records(int64_t r0, int64_t c0, int64_t rows, int64_t cols, int64_t ld) {
+        const int64_t n = ((rows - 1 - r0) * ld + (cols - c0)) * 8;
+        return n > (int64_t)0x7FFFFFF0 ? 0x7FFFFFF0u : (uint32_t)n;
+    }
+    // Byte offset of pass `pass` for lane `tid`, or OOB if its columns are past the edge.
+    F4J_HD static uint32_t offset(int tid, int pass, int64_t c0, int64_t cols, int64_t ld) {
+        const int c = lane_col(tid);
+        if (c0 + c + V > cols) return OOB;
+        return (uint32_t)(((int64_t)(lane_row(tid) + pass * ROWS_PER_PASS) * ld + c) * 8);
+    }
+};
+
+// Row length of an operand's stored real view (real-expanded sizes M, N, K).
+F4J_HD int64_t stored_cols(int kind, bool is_a, int64_t M, int64_t N, int64_t K) {
+    const bool trans = (kind == R_T || kind == Z_T || kind == Z_C);
+    if (is_a) return trans ? (kind >= Z_N ? 2 * M : M) : K;
+    return trans ? K : N;
+}
+
+// 16-byte vector loads need 16-byte aligned rows and an even stored row
+// length, so that a vector never straddles the matrix edge.
+F4J_HD bool vec2_ok(const double *ptr, int64_t ld, int64_t cols) {
+    return ((uintptr_t)ptr % 16 == 0) && ld % 2 == 0 && cols % 2 == 0;
+}
+
 // Result layout of v_mfma_f64_16x16x4f64: lane l, item r of the 4-double
 // accumulator holds D[l/16 + 4r][l%16] (rows interleave across the 4 items;
 // the FP32/BF16 16x16 forms instead hold 4 consecutive rows per lane).

@@ -123,7 +123,8 @@ int f4j_zgemm_dev(int opA, int opB, int64_t M, int64_t N, int64_t K,
  * Mirrors M:linalg/ops/dispatch/configs/ReDeMatMultDispatchConfigs.java:52-117 and
  * Cm128DeMatMultDispatchConfigs.java:52-111 (key=value .properties files); unknown keys are
  * rejected with F4J_ERR_INVALID_ARG instead of being ignored.  Keys: smallThreshold,
- * defaultPrecision (fp64|strict|split_bf16|split_fp16|split_int8), defaultSlices. */
+ * defaultPrecision (fp64|strict|split_bf16|split_fp16|split_int8), defaultSlices,
+ * tileConfig (-1 = per shape bucket, else a fixed FP64 tile id). */
 int f4j_config_set(const char *key, const char *value);
 int f4j_config_get(const char *key, char *buf, size_t buflen);
 int f4j_config_load(const char *path);

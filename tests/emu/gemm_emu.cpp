@@ -8,8 +8,8 @@
 // every C element must be written exactly once.  Inputs are small integers,
 // so the product is exact and compared bit for bit with a direct product.
 //
-// usage: gemm_emu <ka> <kb> <M> <N> <K> <cfg>   (M,N,K in element units;
-//        kinds 0..4 = R_N R_T Z_N Z_T Z_C)
+// usage: gemm_emu <ka> <kb> <M> <N> <K> <cfg> [vec]   (M,N,K in element units;
+//        kinds 0..4 = R_N R_T Z_N Z_T Z_C; vec = 1 or 2 doubles per load)
 #include <cstdio>
 #include <cstdlib>
 #include <random>
@@ -30,15 +30,44 @@ static int g_fail = 0;
         }                                      \
     } while (0)
 
-template <int BM, int BN, int BK, int WGM, int WGN, int KA, int KB>
+// Buffer-load semantics of TileCopy: a lane's V doubles come from
+// base + offset unless offset >= num_records, which reads as 0.  Each value
+// must equal the bounds-checked stored element it stands for.
+template <class O, int NT, int V>
+void copy_tile(const double *G, int64_t ld, int64_t x0, int64_t k0, int64_t X, int64_t K, std::vector<double> &lds,
+               std::vector<int> &w) {
+    using TC = TileCopy<O, NT, V>;
+    int64_t r0, c0, rows, cols;
+    O::block(x0, k0, X, K, r0, c0, rows, cols);
+    CHECK(r0 < rows && c0 < cols, "block origin outside the matrix");
+    const uint32_t nr = TC::num_records(r0, c0, rows, cols, ld);
+    const double *base = G + r0 * ld + c0;
+    for (int tid = 0; tid < NT; ++tid)
+        for (int i = 0; i < TC::PASSES; ++i) {
+            const uint32_t off = TC::offset(tid, i, c0, cols, ld);
+            CHECK(off == TC::OOB || off % (8 * V) == 0 || V == 1, "misaligned vector offset");
+            for (int v = 0; v < V; ++v) {
+                const uint64_t o = (uint64_t)off + 8 * v;
+                const double got = o + 8 <= nr ? base[o / 8] : 0.0;
+                const int64_t r = r0 + TC::lane_row(tid) + i * TC::ROWS_PER_PASS, c = c0 + TC::lane_col(tid) + v;
+                const double want = (r < rows && c < cols) ? G[r * ld + c] : 0.0;
+                CHECK(got == want, "buffer load (%ld,%ld) got %g want %g", (long)r, (long)c, got, want);
+                const int li = TC::lds_index(tid, i) + v;
+                CHECK(li >= 0 && li < O::LDS_DOUBLES, "lds write oob");
+                if (g_fail) return;
+                lds[li] = got;
+                w[li]++;
+            }
+        }
+}
+
+template <int BM, int BN, int BK, int WGM, int WGN, int V, int KA, int KB>
 void emulate(const GemmProblem &p, std::vector<int> &written) {
     constexpr int NT = WGM * WGN * 64;
     constexpr int WTM = BM / WGM, WTN = BN / WGN;
     constexpr int FM = WTM / 16, FN = WTN / 16;
     using OA = Opnd<KA, true, BM, BK>;
     using OB = Opnd<KB, false, BN, BK>;
-    static_assert((OA::SR * OA::SC) % NT == 0 && (OB::SR * OB::SC) % NT == 0, "");
-    constexpr int EA = OA::SR * OA::SC / NT, EB = OB::SR * OB::SC / NT;
     const int64_t tiles_m = (p.M + BM - 1) / BM, tiles_n = (p.N + BN - 1) / BN;
     std::vector<int> tile_seen(tiles_m * tiles_n, 0);
     for (int64_t bid = 0; bid < tiles_m * tiles_n; ++bid) {
@@ -54,29 +83,9 @@ void emulate(const GemmProblem &p, std::vector<int> &written) {
             const int64_t k0 = kt * BK;
             std::vector<double> la(OA::LDS_DOUBLES, 0.0), lb(OB::LDS_DOUBLES, 0.0);
             std::vector<int> wa(OA::LDS_DOUBLES, 0), wb(OB::LDS_DOUBLES, 0);
-            int64_t r0, c0, rows, cols;
-            OA::block(m0, k0, p.M, p.K, r0, c0, rows, cols);
-            for (int tid = 0; tid < NT; ++tid)
-                for (int i = 0; i < EA; ++i) {
-                    const int e = tid + i * NT;
-                    const int64_t r = r0 + OA::e_row(e), c = c0 + OA::e_col(e);
-                    const int li = OA::e_lds(e);
-                    CHECK(li >= 0 && li < OA::LDS_DOUBLES, "A lds write oob");
-                    if (g_fail) return;
-                    la[li] = (r < rows && c < cols) ? p.A[r * p.lda + c] : 0.0;
-                    wa[li]++;
-                }
-            OB::block(n0, k0, p.N, p.K, r0, c0, rows, cols);
-            for (int tid = 0; tid < NT; ++tid)
-                for (int i = 0; i < EB; ++i) {
-                    const int e = tid + i * NT;
-                    const int64_t r = r0 + OB::e_row(e), c = c0 + OB::e_col(e);
-                    const int li = OB::e_lds(e);
-                    CHECK(li >= 0 && li < OB::LDS_DOUBLES, "B lds write oob");
-                    if (g_fail) return;
-                    lb[li] = (r < rows && c < cols) ? p.B[r * p.ldb + c] : 0.0;
-                    wb[li]++;
-                }
+            copy_tile<OA, NT, V>(p.A, p.lda, m0, k0, p.M, p.K, la, wa);
+            copy_tile<OB, NT, V>(p.B, p.ldb, n0, k0, p.N, p.K, lb, wb);
+            if (g_fail) return;
             for (int li = 0; li < OA::LDS_DOUBLES; ++li) CHECK(wa[li] <= 1, "A lds double write");
             for (int li = 0; li < OB::LDS_DOUBLES; ++li) CHECK(wb[li] <= 1, "B lds double write");
             for (int wave = 0; wave < WGM * WGN; ++wave) {
@@ -131,13 +140,19 @@ void emulate(const GemmProblem &p, std::vector<int> &written) {
     for (size_t t = 0; t < tile_seen.size(); ++t) CHECK(tile_seen[t] == 1, "tile %zu seen %d times", t, tile_seen[t]);
 }
 
-template <int KA, int KB>
-void run_cfg(int cfg, const GemmProblem &p, std::vector<int> &w) {
+template <int V, int KA, int KB>
+void run_v(int cfg, const GemmProblem &p, std::vector<int> &w) {
     switch (cfg) {
-        case 1: emulate<64, 64, 16, 2, 2, KA, KB>(p, w); break;
-        case 2: emulate<128, 128, 32, 2, 2, KA, KB>(p, w); break;
-        default: emulate<128, 128, 16, 2, 2, KA, KB>(p, w); break;
+        case 1: emulate<64, 64, 16, 2, 2, V, KA, KB>(p, w); break;
+        case 2: emulate<128, 128, 32, 2, 2, V, KA, KB>(p, w); break;
+        default: emulate<128, 128, 16, 2, 2, V, KA, KB>(p, w); break;
     }
+}
+
+template <int KA, int KB>
+void run_cfg(int cfg, int vec, const GemmProblem &p, std::vector<int> &w) {
+    if (vec == 2) run_v<2, KA, KB>(cfg, p, w);
+    else run_v<1, KA, KB>(cfg, p, w);
 }
 
 // element (r,c) of op(X) where op(X) is R x Ccols; X complex interleaved or real
@@ -159,6 +174,7 @@ int main(int argc, char **argv) {
     const int ka = std::atoi(argv[1]), kb = std::atoi(argv[2]);
     const int64_t M = std::atoll(argv[3]), N = std::atoll(argv[4]), K = std::atoll(argv[5]);
     const int cfg = std::atoi(argv[6]);
+    const int vec = argc > 7 ? std::atoi(argv[7]) : 1;
     const bool cplx = ka >= Z_N;
     const int w = cplx ? 2 : 1;
     std::mt19937_64 rng(1234 + M * 7 + N * 13 + K * 17 + ka * 3 + kb);
@@ -179,8 +195,11 @@ int main(int argc, char **argv) {
     p.C = C.data();
     p.ldc = N * w;
     std::vector<int> written(M * N * w, 0);
+    // 16-byte loads only where the library would use them.
+    const int vec_used = (vec == 2 && vec2_ok(p.A, p.lda, stored_cols(ka, true, p.M, p.N, p.K)) &&
+                          vec2_ok(p.B, p.ldb, stored_cols(kb, false, p.M, p.N, p.K))) ? 2 : 1;
 #define CASE(A_, B_) \
-    if (ka == A_ && kb == B_) run_cfg<A_, B_>(cfg, p, written);
+    if (ka == A_ && kb == B_) run_cfg<A_, B_>(cfg, vec_used, p, written);
     CASE(R_N, R_N) CASE(R_N, R_T) CASE(R_T, R_N) CASE(R_T, R_T)
     CASE(Z_N, Z_N) CASE(Z_N, Z_T) CASE(Z_N, Z_C) CASE(Z_T, Z_N) CASE(Z_T, Z_T) CASE(Z_T, Z_C)
     CASE(Z_C, Z_N) CASE(Z_C, Z_T) CASE(Z_C, Z_C)
@@ -203,6 +222,6 @@ int main(int argc, char **argv) {
         std::printf("FAILED %d\n", g_fail);
         return 1;
     }
-    std::printf("OK\n");
+    std::printf("OK vec=%d\n", vec_used);
     return 0;
 }

@@ -28,9 +28,11 @@ PAIRS = [(a, b) for a in ("R_N", "R_T") for b in ("R_N", "R_T")] + \
 
 
 @pytest.mark.parametrize("ka,kb", PAIRS)
-@pytest.mark.parametrize("shape,cfg", [((37, 45, 29), 1), ((130, 70, 35), 0), ((20, 129, 17), 2), ((1, 1, 1), 1)])
-def test_kernel_index_math(emu, ka, kb, shape, cfg):
+@pytest.mark.parametrize("shape,cfg", [((37, 45, 29), 1), ((130, 70, 35), 0), ((20, 129, 17), 2), ((1, 1, 1), 1),
+                                       ((64, 64, 32), 1), ((130, 140, 36), 0)])
+@pytest.mark.parametrize("vec", [1, 2])
+def test_kernel_index_math(emu, ka, kb, shape, cfg, vec):
     M, N, K = shape
-    r = subprocess.run([emu, str(KINDS[ka]), str(KINDS[kb]), str(M), str(N), str(K), str(cfg)],
+    r = subprocess.run([emu, str(KINDS[ka]), str(KINDS[kb]), str(M), str(N), str(K), str(cfg), str(vec)],
                        capture_output=True, text=True, timeout=120)
-    assert r.returncode == 0 and r.stdout.strip().endswith("OK"), r.stdout
+    assert r.returncode == 0 and r.stdout.strip().startswith("OK"), r.stdout
