@@ -34,6 +34,11 @@ class DeviceInfo(ctypes.Structure):
                 ("peak_hbm_gbs", ctypes.c_double), ("name", ctypes.c_char * 64), ("arch", ctypes.c_char * 32)]
 
 
+class SplitReport(ctypes.Structure):
+    _fields_ = [("precision", ctypes.c_int), ("slices", ctypes.c_int), ("products", ctypes.c_int),
+                ("launches", ctypes.c_int), ("checked_rows", ctypes.c_int), ("max_rel_err", ctypes.c_double)]
+
+
 SIGNATURES = {
     "f4j_abi_version": ([], _i),
     "f4j_init": ([_i], _i),
@@ -49,6 +54,7 @@ SIGNATURES = {
     "f4j_dgemm_dev": ([_i, _i, _i64, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _i, _i, _vp], _i),
     "f4j_zgemm": ([_i, _i, _i64, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _i, _i, _i], _i),
     "f4j_zgemm_dev": ([_i, _i, _i64, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _i, _i, _i, _vp], _i),
+    "f4j_last_split_report": ([ctypes.POINTER(SplitReport)], _i),
     "f4j_config_set": ([_cp, _cp], _i),
     "f4j_config_get": ([_cp, _cp, _sz], _i),
     "f4j_config_load": ([_cp], _i),
@@ -87,6 +93,22 @@ def check(status):
         msg = load().f4j_last_error()
         raise NativeError(status, msg.decode() if msg else "")
     return status
+
+
+def last_split_report():
+    rep = SplitReport()
+    check(load().f4j_last_split_report(ctypes.byref(rep)))
+    return rep
+
+
+def config_set(key, value):
+    check(load().f4j_config_set(key.encode(), str(value).encode()))
+
+
+def config_get(key):
+    buf = ctypes.create_string_buffer(64)
+    check(load().f4j_config_get(key.encode(), buf, 64))
+    return buf.value.decode()
 
 
 def device_info():

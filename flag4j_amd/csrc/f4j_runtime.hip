@@ -22,12 +22,15 @@
 
 #include "../../include/flag4j_amd.h"
 #include "f4j_internal.h"
+#include "gemm_tile.h"
+#include "split_gemm.h"
 
 namespace f4j {
 
 namespace {
 
 thread_local std::string t_err;
+thread_local f4j_split_report t_split{};
 
 struct DevBuf {
     void *p = nullptr;
@@ -39,6 +42,9 @@ struct Config {
     int default_precision = F4J_PREC_FP64;
     int default_slices = 0;
     int tile_config = -1;             // -1: chosen per shape bucket; else a fixed FP64 tile id
+    int split_check_rows = 4;         // rows of C checked in FP64 after a split product (0 = off)
+    double split_tolerance = 1e-15;   // max |C - AB| / (|A||B|) accepted on the checked rows
+    int split_fallback = 1;           // 1: recompute in FP64 when the check fails; 0: return F4J_ERR_ACCURACY
 };
 
 struct State {
@@ -50,6 +56,8 @@ struct State {
     hipDeviceProp_t prop{};
     Config cfg;
     DevBuf bufA, bufB, bufC;
+    std::mutex ws_mu;                 // guards ws (split-precision workspace)
+    DevBuf ws;
 };
 
 State &S() {
@@ -160,23 +168,27 @@ int validate(bool cplx, int opA, int opB, int64_t M, int64_t N, int64_t K, const
 
 // Tile configuration per shape bucket: the 128x128 tile when it yields enough
 // workgroups to fill the 256 CUs, else the 64x64 tile.
-constexpr int NUM_TILE_CFGS = 4;
+constexpr int NUM_TILE_CFGS = NUM_FP64_TILES;
 
 int pick_tile(int64_t M, int64_t Nx) {
     {
         std::lock_guard<std::mutex> g(S().cfg_mu);
         if (S().cfg.tile_config >= 0) return S().cfg.tile_config;
     }
+    // 128x128 tiles with 8 waves (4 per SIMD) once they cover the CUs;
+    // 64x64 tiles below that.
     const int64_t t128 = ((M + 127) / 128) * ((Nx + 127) / 128);
-    return t128 >= 512 ? 0 : 1;
+    return t128 >= 256 ? 4 : 1;
 }
 
-const char *tile_name(int cfg) {
+std::string tile_name(int cfg) {
     switch (cfg) {
-        case 1: return "MFMA_F64_64x64x16";
-        case 2: return "MFMA_F64_128x128x32_W1";
-        case 3: return "MFMA_F64_128x128x16_W1";
-        default: return "MFMA_F64_128x128x16";
+#define F4J_TILE_NAME(id, BM, BN, BK, WGM, WGN, MINW) \
+    case id:                                        \
+        return "MFMA_F64_" #BM "x" #BN "x" #BK "_w" #WGM "x" #WGN "_o" #MINW;
+        F4J_FP64_TILES(F4J_TILE_NAME)
+#undef F4J_TILE_NAME
+        default: return "MFMA_F64_?";
     }
 }
 
@@ -196,6 +208,62 @@ OpKind kind_of(int op, bool cplx) {
     return op == F4J_OP_N ? Z_N : (op == F4J_OP_T ? Z_T : Z_C);
 }
 
+int default_slices_for(int prec) {
+    std::lock_guard<std::mutex> g(S().cfg_mu);
+    if (S().cfg.default_slices > 0) return S().cfg.default_slices;
+    return prec == F4J_PREC_SPLIT_FP16 ? 5 : 6;
+}
+
+// Split-precision real GEMM (synchronous: its workspace is shared and its
+// residual check reads back one number).
+int run_split(int opA, int opB, int64_t M, int64_t N, int64_t K, const double *A, int64_t lda, const double *B,
+              int64_t ldb, double *C, int64_t ldc, int prec, int slices, hipStream_t stream) {
+    const int carrier = prec == F4J_PREC_SPLIT_FP16 ? SPLIT_FP16 : SPLIT_BF16;
+    const int S_ = slices > 0 ? slices : default_slices_for(prec);
+    if (S_ < 1 || S_ > SPLIT_MAX_SLICES)
+        return fail(F4J_ERR_INVALID_ARG, "slices must be in 1.." + std::to_string(SPLIT_MAX_SLICES));
+    Config c;
+    {
+        std::lock_guard<std::mutex> g(S().cfg_mu);
+        c = S().cfg;
+    }
+    const SplitPlan pl = plan_split(M, N, K, S_, carrier, c.split_check_rows);
+    State &s = S();
+    std::lock_guard<std::mutex> g(s.ws_mu);
+    int rc = grow(s.ws, pl.bytes);
+    if (rc != F4J_OK) return rc;
+    F4J_HIP(launch_split_gemm(pl, opA, opB, M, N, K, A, lda, B, ldb, C, ldc, s.ws.p, stream), "split gemm launch");
+    f4j_split_report rep{};
+    rep.precision = prec;
+    rep.slices = S_;
+    rep.products = pl.products();
+    rep.launches = (int)pl.groups.size();
+    rep.checked_rows = (int)std::min<int64_t>(c.split_check_rows, M);
+    rep.max_rel_err = -1.0;
+    if (rep.checked_rows > 0) {
+        F4J_HIP(launch_split_residual(pl, opA, opB, M, N, K, A, lda, B, ldb, C, ldc, rep.checked_rows, s.ws.p, stream),
+                "split residual launch");
+        double err = 0.0;
+        F4J_HIP(hipMemcpyAsync(&err, (char *)s.ws.p + pl.off_err, sizeof(double), hipMemcpyDeviceToHost, stream),
+                "copy residual");
+        F4J_HIP(hipStreamSynchronize(stream), "stream synchronize");
+        rep.max_rel_err = err;
+        if (!(err <= c.split_tolerance)) {
+            if (!c.split_fallback) {
+                t_split = rep;
+                return fail(F4J_ERR_ACCURACY, "split product error " + std::to_string(err) + " above tolerance " +
+                                                  std::to_string(c.split_tolerance));
+            }
+            GemmProblem p{M, N, K, A, lda, B, ldb, C, ldc};
+            F4J_HIP(launch_gemm_f64(opA ? R_T : R_N, opB ? R_T : R_N, p, stream, pick_tile(M, N)), "fp64 fallback");
+            rep.precision = F4J_PREC_FP64;
+        }
+    }
+    F4J_HIP(hipStreamSynchronize(stream), "stream synchronize");
+    t_split = rep;
+    return F4J_OK;
+}
+
 // Device-level GEMM, all pointers on the device, async on `stream`.
 int gemm_dev(bool cplx, int opA, int opB, int64_t M, int64_t N, int64_t K, const double *A, int64_t lda,
              const double *B, int64_t ldb, double *C, int64_t ldc, int precision, int slices, int use3m,
@@ -213,8 +281,11 @@ int gemm_dev(bool cplx, int opA, int opB, int64_t M, int64_t N, int64_t K, const
         F4J_HIP(launch_gemm_strict(cplx, opA, opB, M, N, K, A, lda, B, ldb, C, ldc, stream), "strict gemm launch");
         return F4J_OK;
     }
+    if ((prec == F4J_PREC_SPLIT_BF16 || prec == F4J_PREC_SPLIT_FP16) && !cplx)
+        return run_split(opA, opB, M, N, K, A, lda, B, ldb, C, ldc, prec, slices, stream);
     if (prec != F4J_PREC_FP64)
-        return fail(F4J_ERR_UNSUPPORTED, "precision " + std::to_string(prec) + " is not available in this build");
+        return fail(F4J_ERR_UNSUPPORTED, "precision " + std::to_string(prec) + " is not available for " +
+                                             (cplx ? "complex" : "real") + " products in this build");
     // Tile loads address a block with 32-bit buffer offsets (at most 256 stored
     // rows of a tile): leading dimensions beyond ~1M doubles are not supported.
     if ((lda * w) > (int64_t)(0x7FFFFFF0 / (256 * 8)) || (ldb * w) > (int64_t)(0x7FFFFFF0 / (256 * 8)))
@@ -294,9 +365,21 @@ int config_set_locked(Config &c, const std::string &key, const std::string &valu
         if (end == value.c_str() || *end || v < -1 || v >= NUM_TILE_CFGS)
             return fail(F4J_ERR_INVALID_ARG, "tileConfig: expected -1.." + std::to_string(NUM_TILE_CFGS - 1));
         c.tile_config = (int)v;
+    } else if (key == "splitCheckRows") {
+        const long v = std::strtol(value.c_str(), &end, 10);
+        if (end == value.c_str() || *end || v < 0 || v > 65536) return fail(F4J_ERR_INVALID_ARG, "splitCheckRows: expected 0..65536");
+        c.split_check_rows = (int)v;
+    } else if (key == "splitTolerance") {
+        const double v = std::strtod(value.c_str(), &end);
+        if (end == value.c_str() || *end || !(v >= 0)) return fail(F4J_ERR_INVALID_ARG, "splitTolerance: expected a non-negative number");
+        c.split_tolerance = v;
+    } else if (key == "splitFallback") {
+        if (value != "0" && value != "1") return fail(F4J_ERR_INVALID_ARG, "splitFallback: expected 0 or 1");
+        c.split_fallback = value == "1";
     } else if (key == "defaultSlices") {
         const long v = std::strtol(value.c_str(), &end, 10);
-        if (end == value.c_str() || *end || v < 0 || v > 16) return fail(F4J_ERR_INVALID_ARG, "defaultSlices: expected 0..16");
+        if (end == value.c_str() || *end || v < 0 || v > SPLIT_MAX_SLICES)
+            return fail(F4J_ERR_INVALID_ARG, "defaultSlices: expected 0.." + std::to_string(SPLIT_MAX_SLICES));
         c.default_slices = (int)v;
     } else {
         return fail(F4J_ERR_INVALID_ARG, "unknown configuration key '" + key + "'");
@@ -328,7 +411,7 @@ int f4j_shutdown(void) {
     if (!s.inited) return F4J_OK;
     F4J_HIP(hipSetDevice(s.device), "hipSetDevice");
     F4J_HIP(hipStreamSynchronize(s.stream), "stream synchronize");
-    for (DevBuf *b : {&s.bufA, &s.bufB, &s.bufC})
+    for (DevBuf *b : {&s.bufA, &s.bufB, &s.bufC, &s.ws})
         if (b->p) {
             F4J_HIP(hipFree(b->p), "hipFree");
             b->p = nullptr;
@@ -436,6 +519,12 @@ int f4j_zgemm_dev(int opA, int opB, int64_t M, int64_t N, int64_t K, const doubl
     return gemm_dev(true, opA, opB, M, N, K, A, lda, B, ldb, C, ldc, precision, slices, use_3m, pick_stream(stream));
 }
 
+int f4j_last_split_report(f4j_split_report *out) {
+    if (!out) return fail(F4J_ERR_INVALID_ARG, "out is null");
+    *out = t_split;
+    return F4J_OK;
+}
+
 int f4j_config_set(const char *key, const char *value) {
     if (!key || !value) return fail(F4J_ERR_INVALID_ARG, "null key or value");
     State &s = S();
@@ -454,6 +543,12 @@ int f4j_config_get(const char *key, char *buf, size_t buflen) {
         else if (k == "defaultPrecision") v = prec_names[s.cfg.default_precision];
         else if (k == "defaultSlices") v = std::to_string(s.cfg.default_slices);
         else if (k == "tileConfig") v = std::to_string(s.cfg.tile_config);
+        else if (k == "splitCheckRows") v = std::to_string(s.cfg.split_check_rows);
+        else if (k == "splitTolerance") {
+            char b[32];
+            std::snprintf(b, sizeof(b), "%.17g", s.cfg.split_tolerance);
+            v = b;
+        } else if (k == "splitFallback") v = std::to_string(s.cfg.split_fallback);
         else return fail(F4J_ERR_INVALID_ARG, "unknown configuration key '" + k + "'");
     }
     std::snprintf(buf, buflen, "%s", v.c_str());

@@ -168,10 +168,11 @@ hipError_t launch_cfg(const GemmProblem &p, bool vec2, hipStream_t stream) {
 template <int KA, int KB>
 hipError_t launch_kinds(const GemmProblem &p, bool vec2, hipStream_t stream, int cfg) {
     switch (cfg) {
-        case 1: return launch_cfg<64, 64, 16, 2, 2, 2, KA, KB>(p, vec2, stream);
-        case 2: return launch_cfg<128, 128, 32, 2, 2, 1, KA, KB>(p, vec2, stream);
-        case 3: return launch_cfg<128, 128, 16, 2, 2, 1, KA, KB>(p, vec2, stream);
-        default: return launch_cfg<128, 128, 16, 2, 2, 2, KA, KB>(p, vec2, stream);
+#define F4J_TILE_CASE(id, BM, BN, BK, WGM, WGN, MINW) \
+    case id: return launch_cfg<BM, BN, BK, WGM, WGN, MINW, KA, KB>(p, vec2, stream);
+        F4J_FP64_TILES(F4J_TILE_CASE)
+#undef F4J_TILE_CASE
+        default: return hipErrorInvalidValue;
     }
 }
 

@@ -135,6 +135,25 @@ F4J_HD bool vec2_ok(const double *ptr, int64_t ld, int64_t cols) {
 F4J_HD int mfma_f64_acc_row(int lane, int item) { return (lane >> 4) + 4 * item; }
 F4J_HD int mfma_f64_acc_col(int lane) { return lane & 15; }
 
+// FP64 MFMA tile configurations: X(id, BM, BN, BK, WGM, WGN, MINW) with a
+// BM x BN workgroup tile walked in BK slices by WGM x WGN waves and at least
+// MINW waves resident per SIMD.  Shared by the launcher, the dispatcher's
+// names and the host emulator.
+#define F4J_FP64_TILES(X)            \
+    X(0, 128, 128, 16, 2, 2, 2)      \
+    X(1, 64, 64, 16, 2, 2, 2)        \
+    X(2, 128, 128, 32, 2, 2, 1)      \
+    X(3, 128, 128, 16, 2, 2, 1)      \
+    X(4, 128, 128, 16, 4, 2, 4)      \
+    X(5, 128, 128, 32, 4, 2, 2)      \
+    X(6, 256, 128, 16, 4, 2, 2)      \
+    X(7, 64, 128, 16, 2, 2, 4)       \
+    X(8, 128, 64, 16, 2, 2, 4)
+
+#define F4J_COUNT_TILE(id, BM, BN, BK, WGM, WGN, MINW) +1
+constexpr int NUM_FP64_TILES = 0 F4J_FP64_TILES(F4J_COUNT_TILE);
+#undef F4J_COUNT_TILE
+
 // Workgroup id -> (tile row, tile col).  The hardware deals consecutive
 // workgroup ids round-robin over the 8 XCDs; remap so each XCD walks a
 // contiguous band of tiles, then order the band in groups of GROUP tile-rows

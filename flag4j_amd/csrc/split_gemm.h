@@ -1,0 +1,62 @@
+// Split-precision GEMM plan and launch interface (kernels in split_gemm.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <vector>
+
+namespace f4j {
+
+constexpr int SPLIT_MAX_SLICES = 8;
+constexpr int SPLIT_MAX_SEGS = SPLIT_MAX_SLICES * (SPLIT_MAX_SLICES + 1) / 2;
+
+enum SplitCarrier { SPLIT_BF16 = 0, SPLIT_FP16 = 1 };
+
+// One launch of the slice-product GEMM: digit pairs whose levels lie in
+// [Llo, Lhi], summed exactly in FP64.
+struct SplitGroup {
+    int Llo, Lhi;
+};
+
+struct SplitPlan {
+    int S = 0, carrier = SPLIT_BF16;
+    int64_t Mp = 0, Np = 0, Kp = 0;     // padded sizes of the slice arrays
+    size_t off_slices_a = 0, off_slices_b = 0, off_exp_a = 0, off_exp_b = 0, off_err = 0, off_residual = 0, bytes = 0;
+    std::vector<SplitGroup> groups;     // in launch order (smallest weights first)
+    int products() const { return S * (S + 1) / 2; }
+};
+
+struct SliceGemmArgs {
+    const void *A, *B;                  // slice arrays [S][Xp][Kp], k contiguous
+    int64_t Mp, Np, Kp, M, N;
+    const int *eA, *eB;                 // row / column exponents
+    double *C;
+    int64_t ldc;
+    int beta;                           // 0: C = result, 1: C += result
+    int base_exp;
+    int nseg;
+    int seg_a[SPLIT_MAX_SEGS], seg_b[SPLIT_MAX_SEGS], seg_shift[SPLIT_MAX_SEGS];
+};
+
+int split_radix(int carrier);
+int split_flush(int carrier);
+
+// Sizes, workspace layout and launch groups for C (M x N) = op(A) op(B),
+// with room for a residual check of `check_rows` rows.
+SplitPlan plan_split(int64_t M, int64_t N, int64_t K, int slices, int carrier, int check_rows);
+size_t split_residual_bytes(int64_t N, int64_t K, int rows);
+
+// Enqueue exponents, digit slices and slice products; C is overwritten.
+// `ws` must hold plan.bytes bytes.  op codes: 0 = N, 1 = T.
+hipError_t launch_split_gemm(const SplitPlan &pl, int opA, int opB, int64_t M, int64_t N, int64_t K, const double *A,
+                             int64_t lda, const double *B, int64_t ldb, double *C, int64_t ldc, void *ws,
+                             hipStream_t stream);
+
+// Enqueue the FP64 residual check of `rows` evenly spaced rows of C; the
+// maximum relative error (as double bits) is left at ws + plan.off_err.
+hipError_t launch_split_residual(const SplitPlan &pl, int opA, int opB, int64_t M, int64_t N, int64_t K,
+                                 const double *A, int64_t lda, const double *B, int64_t ldb, const double *C,
+                                 int64_t ldc, int rows, void *ws, hipStream_t stream);
+
+}  // namespace f4j

@@ -1,0 +1,543 @@
+// Split-precision FP64 GEMM: FP64 operands sliced into BF16 or FP16 integer
+// digits, slice products on the BF16/FP16 matrix cores with FP32
+// accumulation, recombined exactly in FP64 (an Ozaki-style scheme).
+//
+// No reference counterpart: Flag4j is FP64-only.  This is an alternative
+// arithmetic for Matrix.mult (M:arrays/dense/Matrix.java:638-640,
+// M: = src/main/java/org/flag4j/) selected by precision F4J_PREC_SPLIT_*.
+//
+// Representation.  Row i of A is scaled by 2^-eA[i] with eA[i] the binary
+// exponent of its largest entry (|a| < 2^eA), so every scaled entry x lies in
+// (-1, 1).  x is written with signed digits in radix 2^R:
+//     x = 2^-(R-1) (d_1 + 2^-R (d_2 + 2^-R (d_3 + ...)))
+//     d_1 = rint(x 2^(R-1)),  r_1 = x 2^(R-1) - d_1,  d_s = rint(r_{s-1} 2^R), ...
+// so |d_s| <= 2^(R-1) and the remainder after S digits is at most
+// 2^-(R-1) 2^-(R(S-1)) / 2 in units of 2^eA.  All steps are exact in FP64.
+//   BF16 (8 significant bits) holds every integer |d| <= 256: R = 9.
+//   FP16 (11 significant bits) holds every integer |d| <= 2048: R = 10.
+// Columns of B are treated alike with exponents eB[j].
+//
+// Exactness of the products.  A digit product is at most 2^(2R-2) (2^16 for
+// BF16, 2^18 for FP16).  FP32 represents every integer up to 2^24, so an
+// FP32 MFMA accumulator is exact over FLUSH = 2^(24-2R+2) consecutive k
+// (256 for BF16, 64 for FP16).  Every FLUSH k the FP32 accumulator is added,
+// times a power of two, to an FP64 accumulator; the host groups the digit
+// pairs so that this sum stays below 2^53, hence exact.  The only
+// approximation left is the truncation: the pairs (s, t) with s + t > S + 1
+// and the digits beyond S are dropped.
+//
+// Levels.  Pair (s, t) carries weight 2^(eA+eB) 2^-(2R-2) 2^-R(L) with level
+// L = s + t - 2.  A launch sums a group of levels [Llo, Lhi] as
+//     acc = sum_L 2^(R (Lhi - L)) (sum over pairs at level L, over k)
+// and its epilogue adds 2^(eA+eB-(2R-2)-R Lhi) acc to C in FP64.  Groups run
+// from the smallest weights to the largest, so C is rounded once per group.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <type_traits>
+#include <cmath>
+#include <cstdint>
+#include <vector>
+
+#include "f4j_internal.h"
+#include "split_gemm.h"
+
+namespace f4j {
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int WAVE = 64;
+
+// ------------------------------------------------------------------------
+// Exponents: eX[x] = max over k of the frexp exponent of X(x, k), i.e. the
+// smallest E with |X(x, k)| < 2^E for all k (INT_MIN for an all-zero line).
+// ------------------------------------------------------------------------
+
+__global__ __launch_bounds__(256) void fill_int_kernel(int *p, int64_t n, int v) {
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) p[i] = v;
+}
+
+__device__ inline int exp_of(double v) {
+    if (v == 0.0) return INT_MIN;
+    int e;
+    (void)frexp(v, &e);
+    return e;
+}
+
+// Lines contiguous in memory (stored row x of length K): one block per line.
+__global__ __launch_bounds__(256) void exp_lines_kernel(const double *X, int64_t ld, int64_t K, int *e) {
+    const int64_t x = blockIdx.x;
+    const double *row = X + x * ld;
+    int m = INT_MIN;
+    for (int64_t k = threadIdx.x; k < K; k += 256) m = max(m, exp_of(row[k]));
+    for (int off = 32; off > 0; off >>= 1) m = max(m, __shfl_xor(m, off));
+    __shared__ int red[4];
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) e[x] = max(max(red[0], red[1]), max(red[2], red[3]));
+}
+
+// Lines strided in memory (stored K x X, line x is column x): each thread
+// scans a k-chunk of one column and merges with atomicMax.
+__global__ __launch_bounds__(256) void exp_cols_kernel(const double *X, int64_t ld, int64_t K, int64_t ncols,
+                                                       int64_t kchunk, int *e) {
+    const int64_t x = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (x >= ncols) return;
+    const int64_t k0 = (int64_t)blockIdx.y * kchunk, k1 = min(K, k0 + kchunk);
+    int m = INT_MIN;
+    for (int64_t k = k0; k < k1; ++k) m = max(m, exp_of(X[k * ld + x]));
+    if (m != INT_MIN) atomicMax(e + x, m);
+}
+
+// ------------------------------------------------------------------------
+// Digit split.  Output: S slices, slice s at out + s*Xp*Kp, row-major Xp x Kp
+// (k contiguous), zero outside the X x K matrix.
+// ------------------------------------------------------------------------
+
+template <int R, int S, class T>
+__device__ inline void digits(double v, int e, T *d) {
+    // v * 2^-e in (-1, 1); all steps exact.
+    double y = (e == INT_MIN) ? 0.0 : ldexp(v, (R - 1) - e);
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        const double q = rint(y);
+        d[s] = (T)(float)q;
+        y = (y - q) * (double)(1 << R);
+    }
+}
+
+// Lines contiguous: thread handles 8 consecutive k of one line.
+template <int R, int S, class T>
+__global__ __launch_bounds__(256) void split_lines_kernel(const double *X, int64_t ld, int64_t nx, int64_t K,
+                                                          const int *e, T *out, int64_t Xp, int64_t Kp) {
+    const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t k8 = Kp / 8;
+    if (g >= Xp * k8) return;
+    const int64_t x = g / k8, k0 = (g % k8) * 8;
+    T d[8][S];
+    const int ex = x < nx ? e[x] : INT_MIN;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+        const int64_t k = k0 + u;
+        const double v = (x < nx && k < K) ? X[x * ld + k] : 0.0;
+        digits<R, S>(v, ex, d[u]);
+    }
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        T w[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) w[u] = d[u][s];
+        *(u32x4 *)(out + (int64_t)s * Xp * Kp + x * Kp + k0) = __builtin_bit_cast(u32x4, w);
+    }
+}
+
+// Lines strided (stored K x X): a 64 x 64 (k, x) block is transposed through
+// LDS so the slice writes stay k-contiguous.
+template <int R, int S, class T>
+__global__ __launch_bounds__(256) void split_cols_kernel(const double *X, int64_t ld, int64_t nx, int64_t K,
+                                                         const int *e, T *out, int64_t Xp, int64_t Kp) {
+    __shared__ double tile[64][65];
+    const int64_t x0 = (int64_t)blockIdx.x * 64, k0 = (int64_t)blockIdx.y * 64;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    for (int r = ty; r < 64; r += 4) {
+        const int64_t k = k0 + r, x = x0 + tx;
+        tile[tx][r] = (k < K && x < nx) ? X[k * ld + x] : 0.0;
+    }
+    __syncthreads();
+    // thread -> (x = x0 + t/8 ... ) : 64 x-lines x 8 groups of 8 k = 512 items, 2 per thread
+    for (int item = threadIdx.x; item < 512; item += 256) {
+        const int xl = item >> 3, kg = (item & 7) * 8;
+        const int64_t x = x0 + xl;
+        if (x >= Xp) continue;
+        const int ex = x < nx ? e[x] : INT_MIN;
+        T d[8][S];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) digits<R, S>(tile[xl][kg + u], ex, d[u]);
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            T w[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) w[u] = d[u][s];
+            *(u32x4 *)(out + (int64_t)s * Xp * Kp + x * Kp + k0 + kg) = __builtin_bit_cast(u32x4, w);
+        }
+    }
+}
+
+// ------------------------------------------------------------------------
+// Slice-product GEMM: C (+)= 2^(eA[i]+eB[j]+base) * sum_seg 2^shift_seg *
+// sum_k A_{sa}[i][k] B_{sb}[j][k], both slice arrays k-contiguous.
+// v_mfma_f32_32x32x16_{bf16,f16}: A/B lane l holds row l%32, k = 8(l/32)+v;
+// result item r of lane l is row 8(r/4) + 4(l/32) + r%4, column l%32.
+// ------------------------------------------------------------------------
+
+template <class T>
+__device__ inline f32x16 mfma32(const u32x4 &a, const u32x4 &b, const f32x16 &c) {
+    if constexpr (sizeof(T) == 2 && std::is_same<T, __bf16>::value)
+        return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c,
+                                                       0, 0, 0);
+    else
+        return __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(f16x8, a), __builtin_bit_cast(f16x8, b), c, 0,
+                                                      0, 0);
+}
+
+constexpr int SBM = 128, SBN = 128, SBK = 64;     // workgroup tile, k slice (elements)
+constexpr int SLD = SBK + 8;                     // LDS row stride in elements (16-byte pad)
+constexpr int STAGE = (SBM + SBN) * SLD;         // elements per stage
+
+template <class T, int FLUSH>
+__global__ __launch_bounds__(256, 2) void slice_gemm_kernel(SliceGemmArgs a) {
+    __shared__ __attribute__((aligned(16))) T lds[2 * STAGE];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int64_t tiles_n = a.Np / SBN;
+    const int64_t m0 = (blockIdx.x / tiles_n) * SBM, n0 = (blockIdx.x % tiles_n) * SBN;
+    const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
+
+    f32x16 acc[2][2];
+    double acc64[2][2][16];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                acc[i][j][r] = 0.f;
+                acc64[i][j][r] = 0.0;
+            }
+        }
+
+    const int tps = (int)(a.Kp / SBK);                // k-tiles per segment
+    const int ntiles = tps * a.nseg;
+    constexpr int FLUSH_TILES = FLUSH / SBK;          // Kp is a multiple of FLUSH
+    // per-lane load geometry: 4 passes of 32 rows x 8 16-byte columns
+    const int lrow = tid >> 3, lcol = (tid & 7) * 8;
+    int lane_off[4];
+#pragma unroll
+    for (int p = 0; p < 4; ++p) lane_off[p] = (int)(((int64_t)(lrow + 32 * p) * a.Kp + lcol) * (int64_t)sizeof(T));
+    u32x4 ra[4], rb[4];
+
+    // (segment, k offset) of the tile to load next, advanced without division
+    int ld_seg = 0, ld_kt = 0;
+    auto gload = [&]() {
+        const int64_t kk = (int64_t)ld_kt * SBK;
+        const T *pa = (const T *)a.A + (int64_t)a.seg_a[ld_seg] * a.Mp * a.Kp + m0 * a.Kp + kk;
+        const T *pb = (const T *)a.B + (int64_t)a.seg_b[ld_seg] * a.Np * a.Kp + n0 * a.Kp + kk;
+        const __amdgpu_buffer_rsrc_t rsa = __builtin_amdgcn_make_buffer_rsrc((void *)pa, (short)0, 0x7FFFFFF0, 0x00020000);
+        const __amdgpu_buffer_rsrc_t rsb = __builtin_amdgcn_make_buffer_rsrc((void *)pb, (short)0, 0x7FFFFFF0, 0x00020000);
+#pragma unroll
+        for (int p = 0; p < 4; ++p) {
+            ra[p] = __builtin_bit_cast(u32x4, __builtin_amdgcn_raw_buffer_load_b128(rsa, lane_off[p], 0, 0));
+            rb[p] = __builtin_bit_cast(u32x4, __builtin_amdgcn_raw_buffer_load_b128(rsb, lane_off[p], 0, 0));
+        }
+        if (++ld_kt == tps) {
+            ld_kt = 0;
+            ++ld_seg;
+        }
+    };
+    auto lstore = [&](T *st) {
+#pragma unroll
+        for (int p = 0; p < 4; ++p) {
+            *(u32x4 *)(st + (lrow + 32 * p) * SLD + lcol) = ra[p];
+            *(u32x4 *)(st + SBM * SLD + (lrow + 32 * p) * SLD + lcol) = rb[p];
+        }
+    };
+
+    gload();
+    lstore(lds);
+    __syncthreads();
+    const int fr = lane & 31, fk = (lane >> 5) * 8;
+    int seg = 0, kt_in_seg = 0, since_flush = 0;
+    for (int t = 0; t < ntiles; ++t) {
+        const T *cur = lds + (t & 1) * STAGE;
+        if (t + 1 < ntiles) gload();
+#pragma unroll
+        for (int ks = 0; ks < SBK / 16; ++ks) {
+            u32x4 fa[2], fb[2];
+#pragma unroll
+            for (int i = 0; i < 2; ++i) fa[i] = *(const u32x4 *)(cur + (wm + 32 * i + fr) * SLD + ks * 16 + fk);
+#pragma unroll
+            for (int j = 0; j < 2; ++j) fb[j] = *(const u32x4 *)(cur + SBM * SLD + (wn + 32 * j + fr) * SLD + ks * 16 + fk);
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j) acc[i][j] = mfma32<T>(fa[i], fb[j], acc[i][j]);
+        }
+        // Flush the exact FP32 partial sums into the FP64 accumulator every
+        // FLUSH k and at the end of each segment (the scale changes there).
+        ++kt_in_seg;
+        ++since_flush;
+        if (since_flush == FLUSH_TILES || kt_in_seg == tps) {
+            const double sc = __builtin_bit_cast(double, (uint64_t)(1023 + a.seg_shift[seg]) << 52);
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j)
+#pragma unroll
+                    for (int r = 0; r < 16; ++r) {
+                        acc64[i][j][r] = fma((double)acc[i][j][r], sc, acc64[i][j][r]);
+                        acc[i][j][r] = 0.f;
+                    }
+            since_flush = 0;
+            if (kt_in_seg == tps) {
+                kt_in_seg = 0;
+                ++seg;
+            }
+        }
+        if (t + 1 < ntiles) lstore(lds + ((t + 1) & 1) * STAGE);
+        __syncthreads();
+    }
+
+    // Epilogue: C = beta C + 2^(eA+eB+base) acc64.
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int64_t col = n0 + wn + 32 * j + (lane & 31);
+            if (col >= a.N) continue;
+            const int eb = a.eB[col];
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int64_t row = m0 + wm + 32 * i + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+                if (row >= a.M) continue;
+                const int ea = a.eA[row];
+                double v = 0.0;
+                if (ea != INT_MIN && eb != INT_MIN) v = ldexp(acc64[i][j][r], ea + eb + a.base_exp);
+                double *c = a.C + row * a.ldc + col;
+                *c = a.beta ? (*c + v) : v;
+            }
+        }
+}
+
+// ------------------------------------------------------------------------
+// Residual check on sampled rows: max over j of |C_ij - (AB)_ij| / (|A||B|)_ij
+// with (AB)_ij from a compensated dot product (Dot2: TwoProduct by FMA and
+// TwoSum; Ogita, Rump and Oishi 2005), accurate to ~u^2, so the figure is the
+// split result's own error.
+// ------------------------------------------------------------------------
+
+constexpr int RES_KCHUNK = 256;
+
+// Pass 1: block (x, y, z) forms compensated partial dot products of sampled
+// row y over k-chunk z for 256 columns.
+__global__ __launch_bounds__(256) void residual_partial_kernel(const double *A, int64_t lda, int opA, const double *B,
+                                                               int64_t ldb, int opB, int64_t M, int64_t N, int64_t K,
+                                                               int64_t row_step, double *part) {
+    const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t i = min(M - 1, (int64_t)blockIdx.y * row_step);
+    if (j >= N) return;
+    const int64_t k0 = (int64_t)blockIdx.z * RES_KCHUNK, k1 = min(K, k0 + RES_KCHUNK);
+    double p = 0.0, q = 0.0, sa = 0.0;
+    for (int64_t k = k0; k < k1; ++k) {
+        const double x = opA ? A[k * lda + i] : A[i * lda + k];
+        const double y = opB ? B[j * ldb + k] : B[k * ldb + j];
+        const double h = x * y;
+        const double r = fma(x, y, -h);
+        const double s = p + h;
+        const double z = s - p;
+        q += ((p - (s - z)) + (h - z)) + r;
+        p = s;
+        sa = fma(fabs(x), fabs(y), sa);
+    }
+    double *o = part + (((int64_t)blockIdx.y * gridDim.z + blockIdx.z) * N + j) * 3;
+    o[0] = p;
+    o[1] = q;
+    o[2] = sa;
+}
+
+// Pass 2: combine the chunk partials of (row y, column j) with TwoSum and
+// merge |C - AB| / (|A||B|) into the maximum.
+__global__ __launch_bounds__(256) void residual_combine_kernel(const double *C, int64_t ldc, int64_t M, int64_t N,
+                                                               int64_t row_step, int nchunks, const double *part,
+                                                               unsigned long long *max_err_bits) {
+    const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t i = min(M - 1, (int64_t)blockIdx.y * row_step);
+    if (j >= N) return;
+    double p = 0.0, q = 0.0, sa = 0.0;
+    for (int c = 0; c < nchunks; ++c) {
+        const double *o = part + (((int64_t)blockIdx.y * nchunks + c) * N + j) * 3;
+        const double h = o[0];
+        const double s = p + h;
+        const double z = s - p;
+        q += ((p - (s - z)) + (h - z)) + o[1];
+        p = s;
+        sa += o[2];
+    }
+    const double c = C[i * ldc + j];
+    const double err = sa > 0.0 ? fabs((c - p) - q) / sa : (c == 0.0 ? 0.0 : INFINITY);
+    atomicMax(max_err_bits, __builtin_bit_cast(unsigned long long, err));
+}
+
+template <int R, int S, class T>
+void launch_split_one(const double *X, int64_t ld, bool lines, int64_t nx, int64_t K, const int *e, T *out, int64_t Xp,
+                      int64_t Kp, hipStream_t stream) {
+    if (lines) {
+        const int64_t items = Xp * (Kp / 8);
+        hipLaunchKernelGGL((split_lines_kernel<R, S, T>), dim3((unsigned)((items + 255) / 256)), dim3(256), 0, stream, X,
+                           ld, nx, K, e, out, Xp, Kp);
+    } else {
+        hipLaunchKernelGGL((split_cols_kernel<R, S, T>), dim3((unsigned)(Xp / 64), (unsigned)(Kp / 64)), dim3(256), 0,
+                           stream, X, ld, nx, K, e, out, Xp, Kp);
+    }
+}
+
+template <int R, class T>
+void launch_split_s(int S, const double *X, int64_t ld, bool lines, int64_t nx, int64_t K, const int *e, T *out,
+                    int64_t Xp, int64_t Kp, hipStream_t stream) {
+    switch (S) {
+        case 1: launch_split_one<R, 1>(X, ld, lines, nx, K, e, out, Xp, Kp, stream); break;
+        case 2: launch_split_one<R, 2>(X, ld, lines, nx, K, e, out, Xp, Kp, stream); break;
+        case 3: launch_split_one<R, 3>(X, ld, lines, nx, K, e, out, Xp, Kp, stream); break;
+        case 4: launch_split_one<R, 4>(X, ld, lines, nx, K, e, out, Xp, Kp, stream); break;
+        case 5: launch_split_one<R, 5>(X, ld, lines, nx, K, e, out, Xp, Kp, stream); break;
+        case 6: launch_split_one<R, 6>(X, ld, lines, nx, K, e, out, Xp, Kp, stream); break;
+        case 7: launch_split_one<R, 7>(X, ld, lines, nx, K, e, out, Xp, Kp, stream); break;
+        default: launch_split_one<R, 8>(X, ld, lines, nx, K, e, out, Xp, Kp, stream); break;
+    }
+}
+
+}  // namespace
+
+int split_radix(int carrier) { return carrier == SPLIT_FP16 ? 10 : 9; }
+int split_flush(int carrier) { return carrier == SPLIT_FP16 ? 64 : 256; }
+
+SplitPlan plan_split(int64_t M, int64_t N, int64_t K, int slices, int carrier, int check_rows) {
+    SplitPlan pl;
+    pl.S = slices;
+    pl.carrier = carrier;
+    pl.Mp = (M + SBM - 1) / SBM * SBM;
+    pl.Np = (N + SBN - 1) / SBN * SBN;
+    pl.Kp = (K + 255) / 256 * 256;
+    const size_t slice_bytes = (size_t)pl.S * (size_t)(pl.Mp + pl.Np) * (size_t)pl.Kp * 2;
+    pl.off_slices_a = 0;
+    pl.off_slices_b = (size_t)pl.S * pl.Mp * pl.Kp * 2;
+    pl.off_exp_a = (slice_bytes + 255) & ~size_t(255);
+    pl.off_exp_b = pl.off_exp_a + ((size_t)M * 4 + 255) / 256 * 256;
+    pl.off_err = pl.off_exp_b + ((size_t)N * 4 + 255) / 256 * 256;
+    pl.off_residual = pl.off_err + 256;
+    pl.bytes = pl.off_residual + split_residual_bytes(N, K, (int)std::min<int64_t>(check_rows, M));
+
+    // Group the triangular pairs (s + t <= S + 1) by level, from the highest
+    // level (smallest weight) down, keeping each group's FP64 sum exact:
+    // sum_L 2^(R (Lhi - L)) npairs(L) Kp 2^(2R-2) < 2^52.
+    const int R = split_radix(carrier);
+    const double lim = std::ldexp(1.0, 52);
+    int L = pl.S - 1;
+    while (L >= 0) {
+        SplitGroup g;
+        g.Lhi = L;
+        g.Llo = L;
+        double bound = (double)(L + 1) * (double)pl.Kp * std::ldexp(1.0, 2 * R - 2);
+        while (g.Llo > 0) {
+            const int Ln = g.Llo - 1;
+            const double nb = bound + (double)(Ln + 1) * (double)pl.Kp * std::ldexp(1.0, 2 * R - 2 + R * (L - Ln));
+            if (nb >= lim) break;
+            bound = nb;
+            g.Llo = Ln;
+        }
+        pl.groups.push_back(g);
+        L = g.Llo - 1;
+    }
+    return pl;
+}
+
+hipError_t launch_split_gemm(const SplitPlan &pl, int opA, int opB, int64_t M, int64_t N, int64_t K, const double *A,
+                             int64_t lda, const double *B, int64_t ldb, double *C, int64_t ldc, void *ws,
+                             hipStream_t stream) {
+    char *w = (char *)ws;
+    void *sa = w + pl.off_slices_a, *sb = w + pl.off_slices_b;
+    int *ea = (int *)(w + pl.off_exp_a), *eb = (int *)(w + pl.off_exp_b);
+    const int S = pl.S;
+    const bool fp16 = pl.carrier == SPLIT_FP16;
+
+    // exponents
+    hipLaunchKernelGGL(fill_int_kernel, dim3(64), dim3(256), 0, stream, ea, M, INT_MIN);
+    hipLaunchKernelGGL(fill_int_kernel, dim3(64), dim3(256), 0, stream, eb, N, INT_MIN);
+    const int64_t kchunk = 256;
+    if (opA == 0)
+        hipLaunchKernelGGL(exp_lines_kernel, dim3((unsigned)M), dim3(256), 0, stream, A, lda, K, ea);
+    else
+        hipLaunchKernelGGL(exp_cols_kernel, dim3((unsigned)((M + 255) / 256), (unsigned)((K + kchunk - 1) / kchunk)),
+                           dim3(256), 0, stream, A, lda, K, M, kchunk, ea);
+    if (opB == 1)
+        hipLaunchKernelGGL(exp_lines_kernel, dim3((unsigned)N), dim3(256), 0, stream, B, ldb, K, eb);
+    else
+        hipLaunchKernelGGL(exp_cols_kernel, dim3((unsigned)((N + 255) / 256), (unsigned)((K + kchunk - 1) / kchunk)),
+                           dim3(256), 0, stream, B, ldb, K, N, kchunk, eb);
+
+    // digit slices
+    auto split = [&](const double *X, int64_t ld, bool lines, int64_t nx, int64_t Xp, const int *e, void *out) {
+        if (fp16)
+            launch_split_s<10, _Float16>(S, X, ld, lines, nx, K, e, (_Float16 *)out, Xp, pl.Kp, stream);
+        else
+            launch_split_s<9, __bf16>(S, X, ld, lines, nx, K, e, (__bf16 *)out, Xp, pl.Kp, stream);
+    };
+    split(A, lda, opA == 0, M, pl.Mp, ea, sa);
+    split(B, ldb, opB == 1, N, pl.Np, eb, sb);
+    hipError_t err = hipGetLastError();
+    if (err != hipSuccess) return err;
+
+    // slice products, smallest weights first
+    const int R = split_radix(pl.carrier);
+    bool first = true;
+    for (const SplitGroup &g : pl.groups) {
+        SliceGemmArgs a{};
+        a.A = sa;
+        a.B = sb;
+        a.Mp = pl.Mp;
+        a.Np = pl.Np;
+        a.Kp = pl.Kp;
+        a.M = M;
+        a.N = N;
+        a.eA = ea;
+        a.eB = eb;
+        a.C = C;
+        a.ldc = ldc;
+        a.beta = first ? 0 : 1;
+        a.base_exp = -(2 * R - 2) - R * g.Lhi;
+        a.nseg = 0;
+        for (int L = g.Lhi; L >= g.Llo; --L)
+            for (int s = 0; s <= L; ++s) {
+                const int t = L - s;
+                if (s >= S || t >= S) continue;
+                a.seg_a[a.nseg] = s;
+                a.seg_b[a.nseg] = t;
+                a.seg_shift[a.nseg] = R * (g.Lhi - L);
+                ++a.nseg;
+            }
+        const unsigned blocks = (unsigned)((pl.Mp / SBM) * (pl.Np / SBN));
+        if (fp16)
+            hipLaunchKernelGGL((slice_gemm_kernel<_Float16, 64>), dim3(blocks), dim3(256), 0, stream, a);
+        else
+            hipLaunchKernelGGL((slice_gemm_kernel<__bf16, 256>), dim3(blocks), dim3(256), 0, stream, a);
+        first = false;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_split_residual(const SplitPlan &pl, int opA, int opB, int64_t M, int64_t N, int64_t K,
+                                 const double *A, int64_t lda, const double *B, int64_t ldb, const double *C,
+                                 int64_t ldc, int rows, void *ws, hipStream_t stream) {
+    unsigned long long *err = (unsigned long long *)((char *)ws + pl.off_err);
+    double *part = (double *)((char *)ws + pl.off_residual);
+    hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(256), 0, stream, (int *)err, 2, 0);
+    const int r = (int)std::min<int64_t>(rows, M);
+    if (r <= 0) return hipGetLastError();
+    const int64_t step = r > 1 ? (M - 1) / (r - 1) : 0;
+    const int nch = (int)((K + RES_KCHUNK - 1) / RES_KCHUNK);
+    hipLaunchKernelGGL(residual_partial_kernel, dim3((unsigned)((N + 255) / 256), (unsigned)r, (unsigned)nch), dim3(256),
+                       0, stream, A, lda, opA, B, ldb, opB, M, N, K, step, part);
+    hipLaunchKernelGGL(residual_combine_kernel, dim3((unsigned)((N + 255) / 256), (unsigned)r), dim3(256), 0, stream, C,
+                       ldc, M, N, step, nch, part, err);
+    return hipGetLastError();
+}
+
+size_t split_residual_bytes(int64_t N, int64_t K, int rows) {
+    const int64_t nch = (K + RES_KCHUNK - 1) / RES_KCHUNK;
+    return (size_t)rows * (size_t)nch * (size_t)N * 3 * sizeof(double);
+}
+
+}  // namespace f4j

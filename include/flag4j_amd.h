@@ -119,12 +119,29 @@ int f4j_zgemm_dev(int opA, int opB, int64_t M, int64_t N, int64_t K,
                   const double *A, int64_t lda, const double *B, int64_t ldb,
                   double *C, int64_t ldc, int precision, int slices, int use_3m, void *stream);
 
+/* ---- split-precision report (no reference counterpart: Flag4j computes only in FP64) ----
+ * What the calling thread's last SPLIT_* product delivered.  The FP64 residual check
+ * recomputes `checked_rows` evenly spaced rows of C with compensated FP64 dot products on
+ * the GPU and measures max |C - AB| / (|A||B|) over them. */
+typedef struct f4j_split_report {
+    int precision;        /* precision delivered: the SPLIT_* asked for, or FP64 after a fallback */
+    int slices;           /* digit slices per operand */
+    int products;         /* slice products computed (S (S + 1) / 2) */
+    int launches;         /* slice-product kernel launches */
+    int checked_rows;     /* rows verified by the residual check (0 = check off) */
+    double max_rel_err;   /* max over the checked rows of |C - AB| / (|A||B|); -1 if not checked */
+} f4j_split_report;
+int f4j_last_split_report(f4j_split_report *out);
+
 /* ---- dispatcher configuration ----
  * Mirrors M:linalg/ops/dispatch/configs/ReDeMatMultDispatchConfigs.java:52-117 and
  * Cm128DeMatMultDispatchConfigs.java:52-111 (key=value .properties files); unknown keys are
  * rejected with F4J_ERR_INVALID_ARG instead of being ignored.  Keys: smallThreshold,
  * defaultPrecision (fp64|strict|split_bf16|split_fp16|split_int8), defaultSlices,
- * tileConfig (-1 = per shape bucket, else a fixed FP64 tile id). */
+ * tileConfig (-1 = per shape bucket, else a fixed FP64 tile id), splitCheckRows (rows of C the
+ * FP64 residual check verifies after a split product; 0 = off), splitTolerance (largest accepted
+ * |C - AB| / (|A||B|)), splitFallback (1 = recompute in FP64 when the check fails, 0 = return
+ * F4J_ERR_ACCURACY). */
 int f4j_config_set(const char *key, const char *value);
 int f4j_config_get(const char *key, char *buf, size_t buflen);
 int f4j_config_load(const char *path);

@@ -143,9 +143,11 @@ void emulate(const GemmProblem &p, std::vector<int> &written) {
 template <int V, int KA, int KB>
 void run_v(int cfg, const GemmProblem &p, std::vector<int> &w) {
     switch (cfg) {
-        case 1: emulate<64, 64, 16, 2, 2, V, KA, KB>(p, w); break;
-        case 2: emulate<128, 128, 32, 2, 2, V, KA, KB>(p, w); break;
-        default: emulate<128, 128, 16, 2, 2, V, KA, KB>(p, w); break;
+#define F4J_TILE_CASE(id, BM, BN, BK, WGM, WGN, MINW) \
+    case id: emulate<BM, BN, BK, WGM, WGN, V, KA, KB>(p, w); break;
+        F4J_FP64_TILES(F4J_TILE_CASE)
+#undef F4J_TILE_CASE
+        default: CHECK(false, "unknown tile config %d", cfg);
     }
 }
 

@@ -97,7 +97,7 @@ def test_dispatcher_small_path_threshold():
     assert _name(0, 0, 0, 35, 35, 35).startswith("MFMA")      # 2450 entries: not small
     assert _name(0, 0, 0, 512, 512, 512).startswith("MFMA_F64")
     assert _name(1, 2, 0, 300, 300, 300).startswith("MFMA_F64")
-    assert _name(0, 0, 0, 8192, 8192, 8192) == "MFMA_F64_128x128x16"
+    assert _name(0, 0, 0, 8192, 8192, 8192).startswith("MFMA_F64_128x128x16")
     assert _name(0, 0, 0, 0, 3, 3) == "EMPTY"
     assert _name(0, 0, 0, 3, 3, 0) == "ZERO_FILL"
     assert _name(0, 0, 0, 4, 2, 3, "fp64").startswith("MFMA_F64")

@@ -29,7 +29,9 @@ PAIRS = [(a, b) for a in ("R_N", "R_T") for b in ("R_N", "R_T")] + \
 
 @pytest.mark.parametrize("ka,kb", PAIRS)
 @pytest.mark.parametrize("shape,cfg", [((37, 45, 29), 1), ((130, 70, 35), 0), ((20, 129, 17), 2), ((1, 1, 1), 1),
-                                       ((64, 64, 32), 1), ((130, 140, 36), 0)])
+                                       ((64, 64, 32), 1), ((130, 140, 36), 0), ((150, 130, 40), 4),
+                                       ((130, 129, 70), 5), ((260, 140, 20), 6), ((70, 140, 18), 7),
+                                       ((140, 70, 33), 8)])
 @pytest.mark.parametrize("vec", [1, 2])
 def test_kernel_index_math(emu, ka, kb, shape, cfg, vec):
     M, N, K = shape

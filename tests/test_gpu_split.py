@@ -1,0 +1,153 @@
+"""GPU tests of the split-precision GEMM (FP64 sliced into BF16/FP16 digits).
+
+Accuracy is measured against the compensated oracle (Dot2) on sampled
+entries, normalised by (|A||B|)_ij.  Bars stated per test.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+import flag4j_amd as f4j
+from flag4j_amd import _native as L
+from flag4j_amd import rng
+from oracle import oracle as o
+
+pytestmark = pytest.mark.gpu
+
+
+def split_mult(A, B, prec, slices, opA="N", opB="N"):
+    M = A.shape[0] if opA == "N" else A.shape[1]
+    K = A.shape[1] if opA == "N" else A.shape[0]
+    N = B.shape[1] if opB == "N" else B.shape[0]
+    C = np.empty((M, N))
+    L.check(L.load().f4j_dgemm(L.OP[opA], L.OP[opB], M, N, K, A.ctypes.data, A.shape[1], B.ctypes.data,
+                               B.shape[1], C.ctypes.data, N, L.PREC[prec], slices))
+    return C, L.last_split_report()
+
+
+def sampled_err(C, A, B, n=600, seed=0):
+    r = np.random.default_rng(seed)
+    I, J = r.integers(0, A.shape[0], n), r.integers(0, B.shape[1], n)
+    hi, lo, ab = o.real_dot2(A, B, I, J)
+    return float(np.max(np.abs((C[I, J] - hi) - lo) / ab))
+
+
+@pytest.fixture(autouse=True)
+def default_split_config():
+    L.config_set("splitCheckRows", 4)
+    L.config_set("splitTolerance", 1e-15)
+    L.config_set("splitFallback", 1)
+    yield
+    L.config_set("splitCheckRows", 4)
+    L.config_set("splitTolerance", 1e-15)
+    L.config_set("splitFallback", 1)
+
+
+@pytest.mark.parametrize("prec", ["split_bf16", "split_fp16"])
+def test_small_integers_are_exact(prec):
+    """Entries that one digit represents exactly (integers below 2^8 in magnitude
+    relative to the row/column maximum): the slice products are exact integer
+    GEMMs, so C equals the exact product bit for bit, even with S = 1."""
+    r = np.random.default_rng(1)
+    A = r.integers(-127, 128, (200, 300)).astype(np.float64)
+    B = r.integers(-127, 128, (300, 150)).astype(np.float64)
+    A[:, 0] = 127.0                      # pin each row's maximum exponent
+    B[0, :] = 127.0
+    exact = (A.astype(np.int64) @ B.astype(np.int64)).astype(np.float64)
+    for S in (1, 2, 3):
+        C, rep = split_mult(A, B, prec, S)
+        assert np.array_equal(C, exact), S
+        assert rep.slices == S and rep.products == S * (S + 1) // 2
+
+
+@pytest.mark.parametrize("prec,S", [("split_bf16", 6), ("split_fp16", 5)])
+def test_default_slices_reach_fp64_accuracy(prec, S):
+    """Uniform[-1,1) inputs at 1024^3: max error over sampled entries <= 1e-15 of
+    (|A||B|)_ij, and the on-GPU residual check agrees with the oracle."""
+    A, B = rng.uniform(1, 1024, 1024), rng.uniform(2, 1024, 1024)
+    C, rep = split_mult(A, B, prec, S)
+    err = sampled_err(C, A, B)
+    assert err <= 1e-15, err
+    assert rep.precision == L.PREC[prec]
+    assert rep.checked_rows == 4 and 0 <= rep.max_rel_err <= 1e-15
+
+
+@pytest.mark.parametrize("prec", ["split_bf16", "split_fp16"])
+def test_error_falls_with_slices(prec):
+    """Each extra slice adds R bits (9 for BF16, 10 for FP16): the error must fall
+    by a large factor per slice until it reaches FP64 rounding."""
+    A, B = rng.uniform(3, 512, 768), rng.uniform(4, 768, 512)
+    L.config_set("splitCheckRows", 0)
+    errs = []
+    for S in range(2, 7):
+        C, rep = split_mult(A, B, prec, S)
+        errs.append(sampled_err(C, A, B))
+        assert rep.checked_rows == 0 and rep.max_rel_err == -1.0
+    for a, b in zip(errs, errs[1:]):
+        assert b <= max(a / 64, 2e-16), errs
+    assert errs[0] > 1e-9
+
+
+@pytest.mark.parametrize("phi,S,bar", [(0, 6, 1e-15), (4, 7, 1e-15), (8, 8, 1e-15)])
+def test_wide_dynamic_range(phi, S, bar):
+    """Entries (2r-1) 10^(phi (2r'-1)): wider ranges need more slices (the digit
+    grid is set by each row's largest entry).  With enough slices the bar holds."""
+    A = rng.scaled_uniform(5, 512, 512, phi)
+    B = rng.scaled_uniform(6, 512, 512, phi)
+    L.config_set("splitCheckRows", 0)
+    C, _ = split_mult(A, B, "split_bf16", S)
+    assert sampled_err(C, A, B) <= bar
+
+
+def test_residual_check_fallback_and_error():
+    A, B = rng.uniform(7, 256, 256), rng.uniform(8, 256, 256)
+    L.config_set("splitTolerance", 1e-30)                 # nothing passes
+    C, rep = split_mult(A, B, "split_bf16", 2)
+    assert rep.precision == L.PREC["fp64"]                # recomputed in FP64
+    ref = o.real_reordered(A, B)
+    assert np.all(np.abs(C - ref) <= 2 * 256 * 2.0**-53 * (np.abs(A) @ np.abs(B)))
+    L.config_set("splitFallback", 0)
+    C = np.empty((256, 256))
+    st = L.load().f4j_dgemm(0, 0, 256, 256, 256, A.ctypes.data, 256, B.ctypes.data, 256, C.ctypes.data, 256,
+                            L.PREC["split_bf16"], 2)
+    assert st == L.ERR_ACCURACY
+    assert L.last_split_report().max_rel_err > 1e-12
+
+
+@pytest.mark.parametrize("opA,opB", [("N", "T"), ("T", "N"), ("T", "T")])
+def test_transposed_operands(opA, opB):
+    M, N, K = 300, 200, 500
+    A = rng.uniform(9, *((M, K) if opA == "N" else (K, M)))
+    B = rng.uniform(10, *((K, N) if opB == "N" else (N, K)))
+    C, _ = split_mult(A, B, "split_bf16", 6, opA, opB)
+    Aop = np.ascontiguousarray(A if opA == "N" else A.T)
+    Bop = np.ascontiguousarray(B if opB == "N" else B.T)
+    assert sampled_err(C, Aop, Bop) <= 1e-15
+
+
+@pytest.mark.parametrize("M,N,K", [(1, 1, 1), (7, 300, 13), (130, 70, 257), (129, 257, 1000)])
+@pytest.mark.parametrize("prec", ["split_bf16", "split_fp16"])
+def test_ragged_shapes(M, N, K, prec):
+    A, B = rng.uniform(M + 11, M, K), rng.uniform(N + 12, K, N)
+    C, _ = split_mult(A, B, prec, 6)
+    assert sampled_err(C, A, B, n=200) <= 1e-15
+
+
+def test_zero_rows_and_columns():
+    A, B = rng.uniform(13, 100, 120), rng.uniform(14, 120, 90)
+    A[5, :] = 0.0
+    B[:, 7] = 0.0
+    C, _ = split_mult(A, B, "split_bf16", 6)
+    assert np.all(C[5, :] == 0.0) and np.all(C[:, 7] == 0.0)
+    mask = np.ones_like(C, dtype=bool)
+    mask[5, :] = False
+    mask[:, 7] = False
+    ref = o.real_reordered(A, B)
+    assert np.all(np.abs(C - ref)[mask] <= 2 * 120 * 2.0**-53 * (np.abs(A) @ np.abs(B))[mask])
+
+
+def test_matrix_api_split():
+    A, B = rng.uniform(15, 200, 200), rng.uniform(16, 200, 200)
+    C = f4j.Matrix(A).mult(f4j.Matrix(B), precision="split_bf16", slices=6).numpy()
+    assert sampled_err(C, A, B) <= 1e-15
