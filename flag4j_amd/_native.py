@@ -54,6 +54,15 @@ SIGNATURES = {
     "f4j_dgemm_dev": ([_i, _i, _i64, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _i, _i, _vp], _i),
     "f4j_zgemm": ([_i, _i, _i64, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _i, _i, _i], _i),
     "f4j_zgemm_dev": ([_i, _i, _i64, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _i, _i, _i, _vp], _i),
+    "f4j_grouped_plan_create": ([_i, _i, _i, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i,
+                                 ctypes.POINTER(_vp)], _i),
+    "f4j_grouped_plan_run": ([_vp, _vp], _i),
+    "f4j_grouped_plan_destroy": ([_vp], _i),
+    "f4j_grouped_plan_info": ([_vp, ctypes.POINTER(ctypes.c_double), _vp, _vp, _i], _i),
+    "f4j_grouped_plan_tune": ([_vp, _i], _i),
+    "f4j_tuner_save": ([_cp], _i),
+    "f4j_tuner_load": ([_cp], _i),
+    "f4j_tuner_clear": ([], _i),
     "f4j_last_split_report": ([ctypes.POINTER(SplitReport)], _i),
     "f4j_config_set": ([_cp, _cp], _i),
     "f4j_config_get": ([_cp, _cp, _sz], _i),

@@ -30,10 +30,22 @@ struct GemmProblem {
     int64_t ldb;
     double *C;
     int64_t ldc;
+    int flags;                // bit 0: 16-byte loads allowed (used by the grouped kernel)
+};
+
+// One tile of one problem in a grouped launch.
+struct GroupedItem {
+    int prob;                 // index into the problem table
+    int tm, tn;               // tile row / column
 };
 
 // Launch the FP64 MFMA GEMM for one problem (kernel-level, device pointers).
 hipError_t launch_gemm_f64(OpKind ka, OpKind kb, const GemmProblem &p, hipStream_t stream, int tile_cfg);
+
+// Persistent grouped FP64 GEMM over a work-item list; workgroup g runs items
+// [begin[g], begin[g+1]).  All problems share op kinds and tile config.
+hipError_t launch_gemm_f64_grouped(OpKind ka, OpKind kb, const GemmProblem *probs, const GroupedItem *items,
+                                   const int *begin, int wgs, int cfg, hipStream_t stream);
 
 // Flag4j-order (bitwise) GEMM; sizes and leading dimensions in element units.
 hipError_t launch_gemm_strict(bool cplx, int opA, int opB, int64_t M, int64_t N, int64_t K, const double *A,
@@ -44,5 +56,13 @@ hipError_t launch_gemm_strict(bool cplx, int opA, int opB, int64_t M, int64_t N,
 hipError_t launch_zero_fill(double *C, int64_t rows, int64_t cols, int64_t ld, hipStream_t stream);
 
 void set_error(const std::string &msg);
+
+// Initialise the library on first use and make its device current on the
+// calling thread (returns an f4j_status).
+int runtime_ready();
+
+// The stream a NULL stream argument stands for (the library's own
+// non-blocking stream); valid after runtime_ready().
+hipStream_t runtime_stream(void *stream);
 
 }  // namespace f4j

@@ -290,7 +290,7 @@ int gemm_dev(bool cplx, int opA, int opB, int64_t M, int64_t N, int64_t K, const
     // rows of a tile): leading dimensions beyond ~1M doubles are not supported.
     if ((lda * w) > (int64_t)(0x7FFFFFF0 / (256 * 8)) || (ldb * w) > (int64_t)(0x7FFFFFF0 / (256 * 8)))
         return fail(F4J_ERR_UNSUPPORTED, "leading dimension too large for the MFMA kernels (limit 1048575 doubles)");
-    GemmProblem p;
+    GemmProblem p{};
     p.M = M;
     p.N = N * w;
     p.K = K * w;
@@ -390,6 +390,10 @@ int config_set_locked(Config &c, const std::string &key, const std::string &valu
 }  // namespace
 
 void set_error(const std::string &msg) { t_err = msg; }
+
+int runtime_ready() { return ensure_ready(); }
+
+hipStream_t runtime_stream(void *stream) { return pick_stream(stream); }
 
 }  // namespace f4j
 

@@ -64,8 +64,11 @@ struct Loader {
     }
 };
 
-template <int BM, int BN, int BK, int WGM, int WGN, int MINW, int V, int KA, int KB>
-__global__ __launch_bounds__(WGM *WGN *WAVE, MINW) void gemm_f64_kernel(GemmProblem p) {
+// One BM x BN output tile at (m0, n0) of problem p.  `lds` holds two stages.
+// Every loop iteration ends in a barrier, so on return no wave still reads
+// `lds` and the caller may start the next tile at once.
+template <int BM, int BN, int BK, int WGM, int WGN, int V, int KA, int KB>
+__device__ inline void gemm_tile(const GemmProblem &p, int64_t m0, int64_t n0, double *lds) {
     constexpr int NT = WGM * WGN * WAVE;
     constexpr int WTM = BM / WGM, WTN = BN / WGN;
     constexpr int FM = WTM / 16, FN = WTN / 16;
@@ -73,15 +76,7 @@ __global__ __launch_bounds__(WGM *WGN *WAVE, MINW) void gemm_f64_kernel(GemmProb
     using OB = Opnd<KB, false, BN, BK>;
     constexpr int LDS_STAGE = OA::LDS_DOUBLES + OB::LDS_DOUBLES;
 
-    __shared__ double lds[2 * LDS_STAGE];
-
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-
-    const int64_t tiles_m = (p.M + BM - 1) / BM, tiles_n = (p.N + BN - 1) / BN;
-    int64_t tm, tn;
-    tile_of(blockIdx.x, tiles_m, tiles_n, tm, tn);
-    const int64_t m0 = tm * BM, n0 = tn * BN;
-
     const int wm = (wave / WGN) * WTM, wn = (wave % WGN) * WTN;
     const int baseA = OA::lane_base(lane, wm), baseB = OB::lane_base(lane, wn);
     const double sgnA = OA::lane_sign(lane), sgnB = OB::lane_sign(lane);
@@ -152,6 +147,36 @@ __global__ __launch_bounds__(WGM *WGN *WAVE, MINW) void gemm_f64_kernel(GemmProb
         }
 }
 
+template <int BM, int BN, int BK, int WGM, int WGN, int MINW, int V, int KA, int KB>
+__global__ __launch_bounds__(WGM *WGN *WAVE, MINW) void gemm_f64_kernel(GemmProblem p) {
+    constexpr int LDS_STAGE = Opnd<KA, true, BM, BK>::LDS_DOUBLES + Opnd<KB, false, BN, BK>::LDS_DOUBLES;
+    __shared__ double lds[2 * LDS_STAGE];
+    const int64_t tiles_m = (p.M + BM - 1) / BM, tiles_n = (p.N + BN - 1) / BN;
+    int64_t tm, tn;
+    tile_of(blockIdx.x, tiles_m, tiles_n, tm, tn);
+    gemm_tile<BM, BN, BK, WGM, WGN, V, KA, KB>(p, tm * BM, tn * BN, lds);
+}
+
+// Persistent grouped kernel: workgroup g computes the work items
+// items[begin[g] .. begin[g+1]), each one tile of one problem.  Problems whose
+// rows are 16-byte aligned (flags bit 0) use 16-byte loads.
+template <int BM, int BN, int BK, int WGM, int WGN, int MINW, int KA, int KB>
+__global__ __launch_bounds__(WGM *WGN *WAVE, MINW) void gemm_f64_grouped_kernel(const GemmProblem *probs,
+                                                                             const GroupedItem *items,
+                                                                             const int *begin) {
+    constexpr int LDS_STAGE = Opnd<KA, true, BM, BK>::LDS_DOUBLES + Opnd<KB, false, BN, BK>::LDS_DOUBLES;
+    __shared__ double lds[2 * LDS_STAGE];
+    const int b0 = begin[blockIdx.x], b1 = begin[blockIdx.x + 1];
+    for (int it = b0; it < b1; ++it) {
+        const GroupedItem w = items[it];
+        const GemmProblem p = probs[w.prob];
+        if (p.flags & 1)
+            gemm_tile<BM, BN, BK, WGM, WGN, 2, KA, KB>(p, (int64_t)w.tm * BM, (int64_t)w.tn * BN, lds);
+        else
+            gemm_tile<BM, BN, BK, WGM, WGN, 1, KA, KB>(p, (int64_t)w.tm * BM, (int64_t)w.tn * BN, lds);
+    }
+}
+
 template <int BM, int BN, int BK, int WGM, int WGN, int MINW, int KA, int KB>
 hipError_t launch_cfg(const GemmProblem &p, bool vec2, hipStream_t stream) {
     const int64_t tiles = ((p.M + BM - 1) / BM) * ((p.N + BN - 1) / BN);
@@ -176,7 +201,64 @@ hipError_t launch_kinds(const GemmProblem &p, bool vec2, hipStream_t stream, int
     }
 }
 
+template <int KA, int KB>
+hipError_t launch_grouped_kinds(const GemmProblem *probs, const GroupedItem *items, const int *begin, int wgs, int cfg,
+                                hipStream_t stream) {
+    switch (cfg) {
+#define F4J_TILE_CASE(id, BM, BN, BK, WGM, WGN, MINW)                                                        \
+    case id:                                                                                                \
+        hipLaunchKernelGGL((gemm_f64_grouped_kernel<BM, BN, BK, WGM, WGN, MINW, KA, KB>), dim3((unsigned)wgs), \
+                           dim3(WGM * WGN * WAVE), 0, stream, probs, items, begin);                          \
+        return hipGetLastError();
+        F4J_FP64_TILES(F4J_TILE_CASE)
+#undef F4J_TILE_CASE
+        default: return hipErrorInvalidValue;
+    }
+}
+
 }  // namespace
+
+namespace {
+template <int KA, int KB>
+int grouped_occ_kinds(int cfg) {
+    int n = 0;
+    switch (cfg) {
+#define F4J_TILE_CASE(id, BM, BN, BK, WGM, WGN, MINW)                                                           \
+    case id:                                                                                                   \
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, gemm_f64_grouped_kernel<BM, BN, BK, WGM, WGN, MINW, KA, KB>, \
+                                                         WGM * WGN * WAVE, 0) != hipSuccess)                   \
+            n = 1;                                                                                             \
+        return n;
+        F4J_FP64_TILES(F4J_TILE_CASE)
+#undef F4J_TILE_CASE
+        default: return 1;
+    }
+}
+}  // namespace
+
+int grouped_wgs_per_cu(OpKind ka, OpKind kb, int cfg) {
+#define F4J_CASE(A_, B_) \
+    if (ka == A_ && kb == B_) return grouped_occ_kinds<A_, B_>(cfg);
+    F4J_CASE(R_N, R_N) F4J_CASE(R_N, R_T) F4J_CASE(R_T, R_N) F4J_CASE(R_T, R_T)
+    F4J_CASE(Z_N, Z_N) F4J_CASE(Z_N, Z_T) F4J_CASE(Z_N, Z_C)
+    F4J_CASE(Z_T, Z_N) F4J_CASE(Z_T, Z_T) F4J_CASE(Z_T, Z_C)
+    F4J_CASE(Z_C, Z_N) F4J_CASE(Z_C, Z_T) F4J_CASE(Z_C, Z_C)
+#undef F4J_CASE
+    return 1;
+}
+
+hipError_t launch_gemm_f64_grouped(OpKind ka, OpKind kb, const GemmProblem *probs, const GroupedItem *items,
+                                   const int *begin, int wgs, int cfg, hipStream_t stream) {
+    if (wgs <= 0) return hipSuccess;
+#define F4J_CASE(A_, B_) \
+    if (ka == A_ && kb == B_) return launch_grouped_kinds<A_, B_>(probs, items, begin, wgs, cfg, stream);
+    F4J_CASE(R_N, R_N) F4J_CASE(R_N, R_T) F4J_CASE(R_T, R_N) F4J_CASE(R_T, R_T)
+    F4J_CASE(Z_N, Z_N) F4J_CASE(Z_N, Z_T) F4J_CASE(Z_N, Z_C)
+    F4J_CASE(Z_T, Z_N) F4J_CASE(Z_T, Z_T) F4J_CASE(Z_T, Z_C)
+    F4J_CASE(Z_C, Z_N) F4J_CASE(Z_C, Z_T) F4J_CASE(Z_C, Z_C)
+#undef F4J_CASE
+    return hipErrorInvalidValue;
+}
 
 hipError_t launch_gemm_f64(OpKind ka, OpKind kb, const GemmProblem &p, hipStream_t stream, int cfg) {
     const bool vec2 = vec2_ok(p.A, p.lda, stored_cols(ka, true, p.M, p.N, p.K)) &&

@@ -1,0 +1,448 @@
+// Grouped (batched) GEMM: thousands of independent products of uneven shapes
+// in persistent launches captured once into a hipGraph.
+//
+// No reference counterpart: Flag4j multiplies one pair per call (each call
+// pays a dispatcher lookup and a thread-pool fork/join,
+// M:linalg/ops/dispatch/ReDeMatMultDispatcher.java:157-171,
+// M:concurrency/ThreadManager.java:155-183; M: = src/main/java/org/flag4j/).
+// This is the batched entry point that replaces a Java loop of Matrix.mult /
+// A.H().mult(B) calls.
+//
+// Plan.  Problems are split into size buckets (by the larger of M and N~).
+// Each bucket runs one persistent launch of the FP64 MFMA tile kernel with
+// the bucket's tile configuration; its tiles are assigned to workgroups by
+// longest-processing-time-first list scheduling on an estimated cost, so the
+// workgroups finish together.  The launches are captured into one hipGraph at
+// plan creation; running the plan replays the graph.  The autotuner times each
+// candidate tile configuration per bucket on the plan's own problems and
+// records the winner per (device, operand kinds, bucket) in a cache that can
+// be saved to and loaded from a text file.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <mutex>
+#include <numeric>
+#include <queue>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/flag4j_amd.h"
+#include "f4j_internal.h"
+#include "gemm_tile.h"
+
+namespace f4j {
+namespace {
+
+constexpr int NUM_BUCKETS = 4;
+// Upper bound of max(M, N~) per bucket (the last is unbounded).
+constexpr int64_t BUCKET_LIMIT[NUM_BUCKETS - 1] = {64, 160, 320};
+// Tile configurations the autotuner tries (ids of F4J_FP64_TILES).
+constexpr int CANDIDATES[] = {1, 7, 8, 0, 4};
+// Untuned defaults per bucket.
+constexpr int DEFAULT_CFG[NUM_BUCKETS] = {1, 1, 1, 4};
+
+struct TileGeom {
+    int BM, BN, BK, threads;
+};
+
+TileGeom geom_of(int cfg) {
+    switch (cfg) {
+#define F4J_TILE_GEOM(id, BM, BN, BK, WGM, WGN, MINW) \
+    case id: return TileGeom{BM, BN, BK, WGM * WGN * 64};
+        F4J_FP64_TILES(F4J_TILE_GEOM)
+#undef F4J_TILE_GEOM
+        default: return TileGeom{64, 64, 16, 256};
+    }
+}
+
+int bucket_of(int64_t M, int64_t Nx) {
+    const int64_t x = std::max(M, Nx);
+    for (int b = 0; b < NUM_BUCKETS - 1; ++b)
+        if (x <= BUCKET_LIMIT[b]) return b;
+    return NUM_BUCKETS - 1;
+}
+
+// Tuner cache: key "arch kinds bucket" -> cfg.
+std::mutex g_tune_mu;
+std::map<std::string, int> g_tune;
+
+std::string tune_key(const std::string &arch, int ka, int kb, int bucket) {
+    return arch + " " + std::to_string(ka) + "," + std::to_string(kb) + " " + std::to_string(bucket);
+}
+
+std::string device_arch() {
+    int dev = 0;
+    hipDeviceProp_t p{};
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&p, dev) != hipSuccess) return "unknown";
+    std::string a = p.gcnArchName;
+    return a.substr(0, a.find(':'));
+}
+
+}  // namespace
+
+// Occupancy of the grouped kernel for a configuration (workgroups per CU).
+int grouped_wgs_per_cu(OpKind ka, OpKind kb, int cfg);
+
+}  // namespace f4j
+
+using namespace f4j;
+
+struct f4j_grouped_plan {
+    int is_complex = 0, opA = 0, opB = 0, precision = 0;
+    OpKind ka = R_N, kb = R_N;
+    int64_t count = 0;
+    int cus = 0;
+    std::vector<GemmProblem> probs;          // real-expanded problems
+    std::vector<int> prob_bucket;
+    struct Bucket {
+        int cfg = -1, wgs = 0;
+        int64_t items = 0;
+        GroupedItem *d_items = nullptr;
+        int *d_begin = nullptr;
+    };
+    Bucket buckets[NUM_BUCKETS];
+    GemmProblem *d_probs = nullptr;
+    hipStream_t cap = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    double flops = 0;
+};
+
+namespace f4j {
+namespace {
+
+int fail(int code, const std::string &m) {
+    set_error(m);
+    return code;
+}
+
+int hipfail(hipError_t e, const char *what) {
+    return fail((e == hipErrorOutOfMemory) ? F4J_ERR_OUT_OF_MEMORY : F4J_ERR_HIP,
+                std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define G_HIP(call, what)                              \
+    do {                                               \
+        hipError_t e_ = (call);                        \
+        if (e_ != hipSuccess) return hipfail(e_, what); \
+    } while (0)
+
+void free_bucket(f4j_grouped_plan::Bucket &b) {
+    if (b.d_items) (void)hipFree(b.d_items);
+    if (b.d_begin) (void)hipFree(b.d_begin);
+    b.d_items = nullptr;
+    b.d_begin = nullptr;
+    b.items = 0;
+    b.wgs = 0;
+}
+
+// Build bucket b's LPT schedule for tile configuration cfg and upload it.
+int schedule_bucket(f4j_grouped_plan &pl, int b, int cfg) {
+    f4j_grouped_plan::Bucket &bk = pl.buckets[b];
+    free_bucket(bk);
+    bk.cfg = cfg;
+    const TileGeom g = geom_of(cfg);
+    struct Cand {
+        double cost;
+        GroupedItem it;
+    };
+    std::vector<Cand> items;
+    for (int64_t i = 0; i < pl.count; ++i) {
+        if (pl.prob_bucket[i] != b) continue;
+        const GemmProblem &p = pl.probs[i];
+        const int64_t tm = (p.M + g.BM - 1) / g.BM, tn = (p.N + g.BN - 1) / g.BN;
+        // MFMA time of a full tile (ragged tiles run full MFMAs) plus a fixed
+        // prologue/epilogue cost worth about four k-slices.
+        const double cost = (double)g.BM * g.BN * (double)(((p.K + g.BK - 1) / g.BK) + 4) * g.BK;
+        for (int64_t x = 0; x < tm; ++x)
+            for (int64_t y = 0; y < tn; ++y) items.push_back({cost, GroupedItem{(int)i, (int)x, (int)y}});
+    }
+    bk.items = (int64_t)items.size();
+    if (items.empty()) return F4J_OK;
+    const int per_cu = grouped_wgs_per_cu(pl.ka, pl.kb, cfg);
+    const int wgs = (int)std::min<int64_t>((int64_t)items.size(), (int64_t)pl.cus * std::max(per_cu, 1));
+    std::stable_sort(items.begin(), items.end(), [](const Cand &x, const Cand &y) { return x.cost > y.cost; });
+    using Slot = std::pair<double, int>;   // (load, wg)
+    std::priority_queue<Slot, std::vector<Slot>, std::greater<Slot>> heap;
+    for (int w = 0; w < wgs; ++w) heap.push({0.0, w});
+    std::vector<std::vector<GroupedItem>> per(wgs);
+    for (const Cand &c : items) {
+        Slot s = heap.top();
+        heap.pop();
+        per[s.second].push_back(c.it);
+        heap.push({s.first + c.cost, s.second});
+    }
+    std::vector<GroupedItem> flat;
+    std::vector<int> begin(wgs + 1, 0);
+    flat.reserve(items.size());
+    for (int w = 0; w < wgs; ++w) {
+        begin[w] = (int)flat.size();
+        flat.insert(flat.end(), per[w].begin(), per[w].end());
+    }
+    begin[wgs] = (int)flat.size();
+    G_HIP(hipMalloc(&bk.d_items, flat.size() * sizeof(GroupedItem)), "hipMalloc (grouped items)");
+    G_HIP(hipMalloc(&bk.d_begin, begin.size() * sizeof(int)), "hipMalloc (grouped schedule)");
+    G_HIP(hipMemcpy(bk.d_items, flat.data(), flat.size() * sizeof(GroupedItem), hipMemcpyHostToDevice), "copy items");
+    G_HIP(hipMemcpy(bk.d_begin, begin.data(), begin.size() * sizeof(int), hipMemcpyHostToDevice), "copy schedule");
+    bk.wgs = wgs;
+    return F4J_OK;
+}
+
+hipError_t enqueue_bucket(const f4j_grouped_plan &pl, int b, hipStream_t st) {
+    const f4j_grouped_plan::Bucket &bk = pl.buckets[b];
+    if (bk.items == 0) return hipSuccess;
+    return launch_gemm_f64_grouped(pl.ka, pl.kb, pl.d_probs, bk.d_items, bk.d_begin, bk.wgs, bk.cfg, st);
+}
+
+int capture(f4j_grouped_plan &pl) {
+    if (pl.exec) {
+        (void)hipGraphExecDestroy(pl.exec);
+        pl.exec = nullptr;
+    }
+    if (pl.graph) {
+        (void)hipGraphDestroy(pl.graph);
+        pl.graph = nullptr;
+    }
+    G_HIP(hipStreamBeginCapture(pl.cap, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    hipError_t e = hipSuccess;
+    for (int b = 0; b < NUM_BUCKETS && e == hipSuccess; ++b) e = enqueue_bucket(pl, b, pl.cap);
+    hipGraph_t g = nullptr;
+    const hipError_t e2 = hipStreamEndCapture(pl.cap, &g);
+    if (e != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return hipfail(e, "grouped launch during capture");
+    }
+    if (e2 != hipSuccess) return hipfail(e2, "hipStreamEndCapture");
+    pl.graph = g;
+    G_HIP(hipGraphInstantiate(&pl.exec, pl.graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+    return F4J_OK;
+}
+
+void destroy_plan(f4j_grouped_plan *pl) {
+    if (!pl) return;
+    if (pl->exec) (void)hipGraphExecDestroy(pl->exec);
+    if (pl->graph) (void)hipGraphDestroy(pl->graph);
+    for (auto &b : pl->buckets) free_bucket(b);
+    if (pl->d_probs) (void)hipFree(pl->d_probs);
+    if (pl->cap) (void)hipStreamDestroy(pl->cap);
+    delete pl;
+}
+
+OpKind kind_for(int op, bool cplx) {
+    if (!cplx) return op == F4J_OP_N ? R_N : R_T;
+    return op == F4J_OP_N ? Z_N : (op == F4J_OP_T ? Z_T : Z_C);
+}
+
+}  // namespace
+}  // namespace f4j
+
+extern "C" {
+
+int f4j_grouped_plan_create(int is_complex, int opA, int opB, int64_t count, const int64_t *M, const int64_t *N,
+                            const int64_t *K, const double *const *A, const int64_t *lda, const double *const *B,
+                            const int64_t *ldb, double *const *C, const int64_t *ldc, int precision,
+                            f4j_grouped_plan **out) {
+    if (!out) return fail(F4J_ERR_INVALID_ARG, "out is null");
+    *out = nullptr;
+    const bool cplx = is_complex != 0;
+    auto okop = [&](int op) { return op == F4J_OP_N || op == F4J_OP_T || (cplx && op == F4J_OP_C); };
+    if (!okop(opA) || !okop(opB)) return fail(F4J_ERR_INVALID_ARG, "invalid op code");
+    if (count < 0) return fail(F4J_ERR_INVALID_ARG, "negative count");
+    if (count > 0 && (!M || !N || !K || !A || !lda || !B || !ldb || !C || !ldc))
+        return fail(F4J_ERR_INVALID_ARG, "null problem array");
+    if (precision != F4J_PREC_FP64 && precision != F4J_PREC_AUTO)
+        return fail(F4J_ERR_UNSUPPORTED, "grouped products support precision FP64 only in this build");
+    if (count > (int64_t)0x7FFFFFFF) return fail(F4J_ERR_INVALID_ARG, "count too large");
+    if (const int rc = runtime_ready(); rc != F4J_OK) return rc;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return fail(F4J_ERR_NO_DEVICE, "no HIP device");
+    f4j_grouped_plan *pl = new f4j_grouped_plan;
+    pl->is_complex = is_complex;
+    pl->opA = opA;
+    pl->opB = opB;
+    pl->precision = F4J_PREC_FP64;
+    pl->ka = kind_for(opA, cplx);
+    pl->kb = kind_for(opB, cplx);
+    pl->count = count;
+    hipDeviceProp_t prop{};
+    if (hipGetDeviceProperties(&prop, dev) != hipSuccess) {
+        destroy_plan(pl);
+        return fail(F4J_ERR_HIP, "hipGetDeviceProperties failed");
+    }
+    pl->cus = prop.multiProcessorCount;
+    const int w = cplx ? 2 : 1;
+    pl->probs.resize(count);
+    pl->prob_bucket.resize(count);
+    for (int64_t i = 0; i < count; ++i) {
+        if (M[i] < 0 || N[i] < 0 || K[i] < 0) {
+            destroy_plan(pl);
+            return fail(F4J_ERR_INVALID_ARG, "problem " + std::to_string(i) + ": negative dimension");
+        }
+        const int64_t rowA = opA == F4J_OP_N ? K[i] : M[i], rowB = opB == F4J_OP_N ? N[i] : K[i];
+        const bool empty = M[i] == 0 || N[i] == 0;
+        if (!empty && (ldc[i] < std::max<int64_t>(N[i], 1) || !C[i])) {
+            destroy_plan(pl);
+            return fail(F4J_ERR_INVALID_ARG, "problem " + std::to_string(i) + ": bad C or ldc");
+        }
+        if (!empty && K[i] > 0 &&
+            (!A[i] || !B[i] || lda[i] < std::max<int64_t>(rowA, 1) || ldb[i] < std::max<int64_t>(rowB, 1))) {
+            destroy_plan(pl);
+            return fail(F4J_ERR_INVALID_ARG, "problem " + std::to_string(i) + ": bad A/B or leading dimension");
+        }
+        if (lda[i] * w > (int64_t)(0x7FFFFFF0 / (256 * 8)) || ldb[i] * w > (int64_t)(0x7FFFFFF0 / (256 * 8))) {
+            destroy_plan(pl);
+            return fail(F4J_ERR_UNSUPPORTED, "problem " + std::to_string(i) + ": leading dimension too large");
+        }
+        GemmProblem &p = pl->probs[i];
+        p.M = empty ? 0 : M[i];
+        p.N = empty ? 0 : N[i] * w;
+        p.K = K[i] * w;
+        p.A = A[i];
+        p.lda = lda[i] * w;
+        p.B = B[i];
+        p.ldb = ldb[i] * w;
+        p.C = C[i];
+        p.ldc = ldc[i] * w;
+        p.flags = (vec2_ok(p.A, p.lda, stored_cols(pl->ka, true, p.M, p.N, p.K)) &&
+                   vec2_ok(p.B, p.ldb, stored_cols(pl->kb, false, p.M, p.N, p.K)))
+                      ? 1
+                      : 0;
+        pl->prob_bucket[i] = bucket_of(p.M, p.N);
+        pl->flops += 2.0 * (double)p.M * (double)p.N * (double)p.K;
+    }
+    hipError_t e = hipStreamCreateWithFlags(&pl->cap, hipStreamNonBlocking);
+    if (e == hipSuccess && count > 0) e = hipMalloc(&pl->d_probs, count * sizeof(GemmProblem));
+    if (e == hipSuccess && count > 0)
+        e = hipMemcpy(pl->d_probs, pl->probs.data(), count * sizeof(GemmProblem), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        destroy_plan(pl);
+        return hipfail(e, "grouped plan setup");
+    }
+    const std::string arch = device_arch();
+    for (int b = 0; b < NUM_BUCKETS; ++b) {
+        int cfg = DEFAULT_CFG[b];
+        {
+            std::lock_guard<std::mutex> g(g_tune_mu);
+            auto it = g_tune.find(tune_key(arch, pl->ka, pl->kb, b));
+            if (it != g_tune.end()) cfg = it->second;
+        }
+        const int rc = schedule_bucket(*pl, b, cfg);
+        if (rc != F4J_OK) {
+            destroy_plan(pl);
+            return rc;
+        }
+    }
+    const int rc = capture(*pl);
+    if (rc != F4J_OK) {
+        destroy_plan(pl);
+        return rc;
+    }
+    *out = pl;
+    return F4J_OK;
+}
+
+int f4j_grouped_plan_run(f4j_grouped_plan *plan, void *stream) {
+    if (!plan) return fail(F4J_ERR_INVALID_ARG, "plan is null");
+    if (const int rc = runtime_ready(); rc != F4J_OK) return rc;
+    if (!plan->exec) return F4J_OK;
+    G_HIP(hipGraphLaunch(plan->exec, runtime_stream(stream)), "hipGraphLaunch");
+    return F4J_OK;
+}
+
+int f4j_grouped_plan_destroy(f4j_grouped_plan *plan) {
+    if (plan && plan->cap) (void)hipStreamSynchronize(plan->cap);
+    destroy_plan(plan);
+    return F4J_OK;
+}
+
+int f4j_grouped_plan_info(const f4j_grouped_plan *plan, double *flops, int *tile_cfg, int64_t *items, int nb) {
+    if (!plan) return fail(F4J_ERR_INVALID_ARG, "plan is null");
+    if (flops) *flops = plan->flops;
+    for (int b = 0; b < std::min(nb, NUM_BUCKETS); ++b) {
+        if (tile_cfg) tile_cfg[b] = plan->buckets[b].cfg;
+        if (items) items[b] = plan->buckets[b].items;
+    }
+    return F4J_OK;
+}
+
+int f4j_grouped_plan_tune(f4j_grouped_plan *plan, int reps) {
+    if (!plan) return fail(F4J_ERR_INVALID_ARG, "plan is null");
+    if (reps < 1) reps = 3;
+    const std::string arch = device_arch();
+    hipEvent_t e0, e1;
+    G_HIP(hipEventCreate(&e0), "hipEventCreate");
+    G_HIP(hipEventCreate(&e1), "hipEventCreate");
+    for (int b = 0; b < NUM_BUCKETS; ++b) {
+        if (plan->buckets[b].items == 0) continue;
+        float best = 1e30f;
+        int best_cfg = plan->buckets[b].cfg;
+        for (int cfg : CANDIDATES) {
+            int rc = schedule_bucket(*plan, b, cfg);
+            if (rc != F4J_OK) return rc;
+            G_HIP(enqueue_bucket(*plan, b, plan->cap), "tune warmup");
+            G_HIP(hipEventRecord(e0, plan->cap), "hipEventRecord");
+            for (int r = 0; r < reps; ++r) G_HIP(enqueue_bucket(*plan, b, plan->cap), "tune launch");
+            G_HIP(hipEventRecord(e1, plan->cap), "hipEventRecord");
+            G_HIP(hipEventSynchronize(e1), "hipEventSynchronize");
+            float ms = 0;
+            G_HIP(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+            if (ms < best) {
+                best = ms;
+                best_cfg = cfg;
+            }
+        }
+        int rc = schedule_bucket(*plan, b, best_cfg);
+        if (rc != F4J_OK) return rc;
+        std::lock_guard<std::mutex> g(g_tune_mu);
+        g_tune[tune_key(arch, plan->ka, plan->kb, b)] = best_cfg;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return capture(*plan);
+}
+
+int f4j_tuner_save(const char *path) {
+    if (!path) return fail(F4J_ERR_INVALID_ARG, "null path");
+    std::ofstream o(path);
+    if (!o) return fail(F4J_ERR_INVALID_ARG, std::string("cannot write ") + path);
+    o << "# flag4j_amd grouped-GEMM tuning cache: <arch> <kindA>,<kindB> <bucket> <tile config>\n";
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    for (const auto &kv : g_tune) o << kv.first << " " << kv.second << "\n";
+    return o.good() ? F4J_OK : fail(F4J_ERR_INVALID_ARG, std::string("write failed: ") + path);
+}
+
+int f4j_tuner_load(const char *path) {
+    if (!path) return fail(F4J_ERR_INVALID_ARG, "null path");
+    std::ifstream in(path);
+    if (!in) return fail(F4J_ERR_INVALID_ARG, std::string("cannot open ") + path);
+    std::map<std::string, int> add;
+    std::string line;
+    int lineno = 0;
+    while (std::getline(in, line)) {
+        ++lineno;
+        if (line.empty() || line[0] == '#') continue;
+        std::istringstream ss(line);
+        std::string arch, kinds;
+        int bucket = -1, cfg = -1;
+        if (!(ss >> arch >> kinds >> bucket >> cfg) || bucket < 0 || bucket >= NUM_BUCKETS || cfg < 0 ||
+            cfg >= NUM_FP64_TILES)
+            return fail(F4J_ERR_INVALID_ARG, std::string(path) + ":" + std::to_string(lineno) + ": malformed entry");
+        add[arch + " " + kinds + " " + std::to_string(bucket)] = cfg;
+    }
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    for (const auto &kv : add) g_tune[kv.first] = kv.second;
+    return F4J_OK;
+}
+
+int f4j_tuner_clear(void) {
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    g_tune.clear();
+    return F4J_OK;
+}
+
+}  // extern "C"

@@ -51,7 +51,6 @@ typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
-constexpr int WAVE = 64;
 
 // ------------------------------------------------------------------------
 // Exponents: eX[x] = max over k of the frexp exponent of X(x, k), i.e. the

@@ -88,3 +88,35 @@ def scaled_uniform(seed, rows, cols, phi):
     v = next_doubles(seed, 2 * rows * cols)
     r, rp = v[0::2], v[1::2]
     return ((2.0 * r - 1.0) * 10.0 ** (phi * (2.0 * rp - 1.0))).reshape(rows, cols)
+
+
+def next_gaussians(seed, n):
+    """The first n values of new java.util.Random(seed).nextGaussian(): the
+    polar method on consecutive nextDouble() pairs, rejected pairs skipped,
+    each accepted pair giving v1*mul then v2*mul (log/sqrt from numpy)."""
+    out = np.empty(0)
+    start, need = 0, n
+    while need > 0:
+        m = max(64, int(need * 0.7) + 64)            # pairs to draw (acceptance ~ pi/4)
+        u = _next_doubles(seed, start, 2 * m)
+        start += 2 * m
+        v1, v2 = 2 * u[0::2] - 1, 2 * u[1::2] - 1
+        s = v1 * v1 + v2 * v2
+        ok = (s < 1) & (s != 0)
+        mul = np.sqrt(-2 * np.log(s[ok]) / s[ok])
+        g = np.empty(2 * int(ok.sum()))
+        g[0::2], g[1::2] = v1[ok] * mul, v2[ok] * mul
+        out = np.concatenate([out, g])
+        need = n - out.size
+    return out[:n]
+
+
+def log_uniform_ints(seed, n, a, b):
+    """n integers floor(exp(ln a + r (ln(b+1) - ln a))) in [a, b], r = nextDouble()."""
+    r = next_doubles(seed, n)
+    v = np.floor(np.exp(np.log(a) + r * (np.log(b + 1) - np.log(a)))).astype(np.int64)
+    return np.clip(v, a, b)
+
+
+def gaussian(seed, rows, cols):
+    return next_gaussians(seed, rows * cols).reshape(rows, cols)

@@ -119,6 +119,36 @@ int f4j_zgemm_dev(int opA, int opB, int64_t M, int64_t N, int64_t K,
                   const double *A, int64_t lda, const double *B, int64_t ldb,
                   double *C, int64_t ldc, int precision, int slices, int use_3m, void *stream);
 
+/* ---- grouped (batched) products ----
+ * No reference counterpart: Flag4j multiplies one pair per call; a batch is a Java loop of
+ * Matrix.mult (M:arrays/dense/Matrix.java:638-640) or A.H().mult(B) calls, each paying a
+ * dispatcher lookup and a pool fork/join (M:concurrency/ThreadManager.java:155-183).
+ * A plan fixes `count` independent products C_i (M_i x N_i) = op(A_i) op(B_i), all with the
+ * same op codes, on DEVICE pointers (complex: interleaved, sizes in complex elements).  It
+ * schedules their tiles over persistent launches (one per size bucket, tile shape per bucket
+ * from the autotuner cache) and captures them into a hipGraph; f4j_grouped_plan_run replays the
+ * graph on `stream` (asynchronous).  The operand pointers are baked into the plan: rerun it after
+ * refilling the same buffers.  Precision: F4J_PREC_FP64 (or AUTO, which means FP64 here). */
+typedef struct f4j_grouped_plan f4j_grouped_plan;
+int f4j_grouped_plan_create(int is_complex, int opA, int opB, int64_t count,
+                            const int64_t *M, const int64_t *N, const int64_t *K,
+                            const double *const *A, const int64_t *lda,
+                            const double *const *B, const int64_t *ldb,
+                            double *const *C, const int64_t *ldc, int precision,
+                            f4j_grouped_plan **plan);
+int f4j_grouped_plan_run(f4j_grouped_plan *plan, void *stream);
+int f4j_grouped_plan_destroy(f4j_grouped_plan *plan);
+/* Work of the plan (flops, classical count) and, per size bucket (nb <= 4 entries), the tile
+ * configuration id and the number of tiles. */
+int f4j_grouped_plan_info(const f4j_grouped_plan *plan, double *flops, int *tile_cfg, int64_t *tiles, int nb);
+/* Autotuner: time every candidate tile configuration per bucket on this plan's own problems
+ * (overwrites their C), keep the fastest in the plan and in the process-wide cache keyed by
+ * (device arch, op kinds, bucket), and recapture the graph.  The cache persists via save/load. */
+int f4j_grouped_plan_tune(f4j_grouped_plan *plan, int reps);
+int f4j_tuner_save(const char *path);
+int f4j_tuner_load(const char *path);
+int f4j_tuner_clear(void);
+
 /* ---- split-precision report (no reference counterpart: Flag4j computes only in FP64) ----
  * What the calling thread's last SPLIT_* product delivered.  The FP64 residual check
  * recomputes `checked_rows` evenly spaced rows of C with compensated FP64 dot products on

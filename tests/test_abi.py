@@ -140,3 +140,32 @@ def test_strict_kernel_has_no_fused_multiply_add():
     asm = open(os.path.join(ROOT, "_build", "obj", "gemm_strict.s")).read()
     assert "v_mul_f64" in asm and "v_add_f64" in asm
     assert "v_fma_f64" not in asm
+
+
+def test_tuner_cache_file_roundtrip(tmp_path):
+    L = _native.load()
+    good = tmp_path / "tune.txt"
+    good.write_text("# cache\ngfx950 0,0 2 7\ngfx950 3,2 3 4\n")
+    _native.check(L.f4j_tuner_load(str(good).encode()))
+    out = tmp_path / "out.txt"
+    _native.check(L.f4j_tuner_save(str(out).encode()))
+    lines = [l for l in out.read_text().splitlines() if not l.startswith("#")]
+    assert "gfx950 0,0 2 7" in lines and "gfx950 3,2 3 4" in lines
+    bad = tmp_path / "bad.txt"
+    bad.write_text("gfx950 0,0 9 7\n")                   # bucket out of range
+    assert L.f4j_tuner_load(str(bad).encode()) == _native.ERR_INVALID_ARG
+    bad.write_text("gfx950 0,0 1 99\n")                  # unknown tile config
+    assert L.f4j_tuner_load(str(bad).encode()) == _native.ERR_INVALID_ARG
+    _native.check(L.f4j_tuner_clear())
+
+
+def test_grouped_plan_argument_checks():
+    import ctypes
+    L = _native.load()
+    h = ctypes.c_void_p()
+    assert L.f4j_grouped_plan_create(0, 2, 0, 1, None, None, None, None, None, None, None, None, None, 0,
+                                     ctypes.byref(h)) == _native.ERR_INVALID_ARG
+    assert L.f4j_grouped_plan_create(0, 0, 0, -1, None, None, None, None, None, None, None, None, None, 0,
+                                     ctypes.byref(h)) == _native.ERR_INVALID_ARG
+    assert L.f4j_grouped_plan_create(0, 0, 0, 1, None, None, None, None, None, None, None, None, None, 0,
+                                     ctypes.byref(h)) == _native.ERR_INVALID_ARG

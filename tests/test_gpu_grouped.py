@@ -1,0 +1,119 @@
+"""GPU tests of the grouped (batched) products: every product against the
+oracle within 2 K u (|A||B|), and invariants of the plan (replay, tuning,
+agreement with the single-product path)."""
+import numpy as np
+import pytest
+
+import flag4j_amd as f4j
+from flag4j_amd import _native as L
+from flag4j_amd import device, grouped, rng
+from oracle import oracle as o
+
+pytestmark = pytest.mark.gpu
+U = 2.0**-53
+
+
+def rand_problems(seed, count, lo, hi, opA="N", opB="N", cplx=False):
+    dims = rng.log_uniform_ints(seed, 3 * count, lo, hi).reshape(count, 3)
+    g = rng.next_gaussians(seed + 1, int(np.sum(dims[:, 0] * dims[:, 2] + dims[:, 2] * dims[:, 1])) * (2 if cplx else 1))
+    As, Bs, pos = [], [], 0
+    w = 2 if cplx else 1
+    for M, N, K in dims:
+        a = g[pos:pos + M * K * w]
+        pos += M * K * w
+        b = g[pos:pos + K * N * w]
+        pos += K * N * w
+        if cplx:
+            a, b = a.view(np.complex128), b.view(np.complex128)
+        A = a.reshape((M, K) if opA == "N" else (K, M))
+        B = b.reshape((K, N) if opB == "N" else (N, K))
+        As.append(A)
+        Bs.append(B)
+    return As, Bs
+
+
+def op(X, o_):
+    return {"N": X, "T": X.T, "C": X.conj().T}[o_]
+
+
+def check_real(C, A, B):
+    ref = o.real_reordered(np.ascontiguousarray(A), np.ascontiguousarray(B))
+    K = A.shape[1]
+    return bool(np.all(np.abs(C - ref) <= 2 * K * U * (np.abs(A) @ np.abs(B))))
+
+
+@pytest.mark.parametrize("opA,opB", [("N", "N"), ("N", "T"), ("T", "N")])
+def test_grouped_real_against_oracle(opA, opB):
+    As, Bs = rand_problems(100, 400, 1, 300, opA, opB)
+    Cs = grouped.mult_all(As, Bs, opA, opB)
+    for A, B, C in zip(As, Bs, Cs):
+        assert check_real(C, op(A, opA), op(B, opB))
+
+
+def test_grouped_complex_hermitian():
+    """A_i^H B_i, the batched complex form (the reference forms A.H().mult(B))."""
+    As, Bs = rand_problems(200, 120, 1, 200, "C", "N", cplx=True)
+    Cs = grouped.mult_all(As, Bs, "C", "N")
+    for A, B, C in zip(As, Bs, Cs):
+        ref = o.cplx_gemm(A, B, "C", "N")
+        nA = np.abs(A.real) + np.abs(A.imag)
+        nB = np.abs(B.real) + np.abs(B.imag)
+        lim = 2 * 2 * A.shape[0] * U * (nA.T @ nB)
+        assert np.all(np.abs(C.real - ref.real) <= lim) and np.all(np.abs(C.imag - ref.imag) <= lim)
+
+
+def test_grouped_empty_and_degenerate():
+    As = [np.zeros((0, 5)), np.ones((3, 0)), np.ones((2, 3)), np.ones((4, 4))]
+    Bs = [np.ones((5, 2)), np.ones((0, 6)), np.ones((3, 0)), np.ones((4, 1))]
+    Cs = grouped.mult_all(As, Bs)
+    assert Cs[0].shape == (0, 2)
+    assert Cs[1].shape == (3, 6) and np.all(Cs[1] == 0.0)     # K = 0: C is the zero matrix
+    assert Cs[2].shape == (2, 0)
+    assert np.all(Cs[3] == 4.0)
+    assert grouped.mult_all([], []) == []
+
+
+def test_grouped_equals_single_product_bitwise():
+    """The MFMA accumulation order per element does not depend on the tile
+    shape, so a grouped product equals the single-product call exactly."""
+    As, Bs = rand_problems(300, 60, 8, 400)
+    Cs = grouped.mult_all(As, Bs)
+    for A, B, C in zip(As, Bs, Cs):
+        single = f4j.Matrix(A).mult(f4j.Matrix(B), precision="fp64").numpy()
+        assert np.array_equal(C, single)
+
+
+def test_plan_replay_tune_and_refill():
+    As, Bs = rand_problems(400, 300, 16, 512)
+    shapes = [(a.shape[0], b.shape[1], a.shape[1]) for a, b in zip(As, Bs)]
+    pa = np.concatenate([a.reshape(-1) for a in As])
+    pb = np.concatenate([b.reshape(-1) for b in Bs])
+    oa = np.cumsum([0] + [a.size for a in As])
+    ob = np.cumsum([0] + [b.size for b in Bs])
+    oc = np.cumsum([0] + [m * n for m, n, _ in shapes])
+    with device.DeviceBuffer.from_array(pa) as dA, device.DeviceBuffer.from_array(pb) as dB, \
+            device.DeviceBuffer(int(oc[-1]) * 8) as dC:
+        plan = grouped.GroupedPlan(shapes, [dA.ptr + 8 * int(x) for x in oa[:-1]],
+                                   [dB.ptr + 8 * int(x) for x in ob[:-1]], [dC.ptr + 8 * int(x) for x in oc[:-1]],
+                                   [a.shape[1] for a in As], [b.shape[1] for b in Bs], [n for _, n, _ in shapes])
+        info = plan.info()
+        assert sum(info["tiles"]) > 0 and info["flops"] > 0
+        plan.run()
+        device.sync()
+        c1 = dC.download(np.float64, int(oc[-1]))
+        plan.run()
+        device.sync()
+        assert np.array_equal(dC.download(np.float64, int(oc[-1])), c1)
+        plan.tune(reps=1)
+        plan.run()
+        device.sync()
+        assert np.array_equal(dC.download(np.float64, int(oc[-1])), c1)
+        # refill the same buffers: the captured graph reads the new operands
+        dA.upload(-pa)
+        plan.run()
+        device.sync()
+        assert np.array_equal(dC.download(np.float64, int(oc[-1])), -c1)
+        plan.close()
+    for i in (0, 17, 299):
+        C = c1[oc[i]:oc[i + 1]].reshape(shapes[i][0], shapes[i][1])
+        assert check_real(C, As[i], Bs[i])
