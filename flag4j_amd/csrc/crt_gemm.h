@@ -1,0 +1,48 @@
+// Modular (CRT) split-precision GEMM on the INT8 matrix cores: plan and launch.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace f4j {
+
+constexpr int CRT_MAX_MODULI = 20;
+constexpr int CRT_MAX_CHUNKS = 4;
+
+// Reconstruction constants.  With M = prod m_l and W_l = (M/m_l) * ((M/m_l)^-1 mod m_l),
+// X = sum_l r_l W_l = A'B' (mod M).  Every W_l (and M) is cut into P chunks of 40 bits on a
+// common grid: W_l = sum_p w[l][p] scale[p], so S_p = sum_l r_l w[l][p] is exact in FP64.
+struct CrtTables {
+    int n;                               // moduli used
+    int t;                               // bits of the scaled integer operands (|A'|, |B'| <= 2^t)
+    int P;                               // 40-bit chunks
+    int m[CRT_MAX_MODULI];
+    double minv[CRT_MAX_MODULI];         // 1 / m_l rounded
+    double w[CRT_MAX_MODULI][CRT_MAX_CHUNKS];
+    double mu[CRT_MAX_CHUNKS];           // chunks of M
+    double scale[CRT_MAX_CHUNKS];        // 2^(exponent of chunk p)
+    double c[CRT_MAX_CHUNKS];            // scale[p] / M
+};
+
+struct CrtPlan {
+    CrtTables tb{};
+    int64_t Mp = 0, Np = 0, Kp = 0;      // padded sizes of the residue slices
+    size_t off_ra = 0, off_rb = 0, off_res = 0, off_ea = 0, off_eb = 0, off_err = 0, off_part = 0, bytes = 0;
+};
+
+// Smallest number of moduli whose product gives t >= 53 bits for inner size K
+// (every entry that is within a factor 2 of its line's largest keeps all its bits).
+int crt_default_moduli(int64_t K);
+
+// Plan for C (M x N) = op(A) op(B) with n moduli; returns false if n is
+// outside [2, CRT_MAX_MODULI] or gives fewer than 1 bit for this K.
+bool plan_crt(int64_t M, int64_t N, int64_t K, int n, int check_rows, CrtPlan *plan);
+
+// Enqueue exponents, residue slices, the n residue products and the FP64
+// reconstruction into C.  `ws` holds plan.bytes bytes.  op codes: 0 = N, 1 = T.
+hipError_t launch_crt_gemm(const CrtPlan &pl, int opA, int opB, int64_t M, int64_t N, int64_t K, const double *A,
+                           int64_t lda, const double *B, int64_t ldb, double *C, int64_t ldc, void *ws,
+                           hipStream_t stream);
+
+}  // namespace f4j

@@ -1,0 +1,502 @@
+// Modular (CRT) split-precision FP64 GEMM on the INT8 matrix cores.
+//
+// No reference counterpart: Flag4j is FP64-only.  This is an alternative
+// arithmetic for Matrix.mult (M:arrays/dense/Matrix.java:638-640,
+// M: = src/main/java/org/flag4j/) selected by precision F4J_PREC_SPLIT_INT8.
+// The decomposition follows the "Ozaki scheme II" idea (Ozaki, Uchino and
+// Imamura, 2025): FP64 products from integer products modulo small coprime
+// moduli, recombined by the Chinese remainder theorem.
+//
+// 1. Scaling.  Row i of A is scaled to t-bit fixed point relative to its
+//    largest entry: A'_ik = rint(a_ik 2^(t - eA_i)), |A'| <= 2^t (columns of B
+//    alike).  This rounding is the only approximation; t grows by about 4 bits
+//    per extra modulus and may exceed 53, which helps rows whose entries span a
+//    wide range (the small ones keep t - (eA_i - e(a_ik)) bits).
+// 2. Residues.  For coprime moduli m_l <= 256, A'_ik mod m_l is stored as a
+//    signed 8-bit residue (|r| <= 128).
+// 3. Products.  C_l = A_l B_l^T on v_mfma_i32_32x32x32_i8: |C_l| <= K 2^14,
+//    exact in int32 for K < 2^17, then reduced to C_l mod m_l (one byte).
+// 4. Reconstruction.  X = sum_l r_l W_l = A'B' (mod M), M = prod m_l; the
+//    moduli are chosen with K 2^(2t) < M/4, so the representative of X in
+//    (-M/4, M/4) is A'B' exactly.  W_l and M are cut into 40-bit chunks on a
+//    common grid, so every partial sum S_p = sum_l r_l w_lp is an exact FP64
+//    integer; q = rint(sum_p S_p scale_p / M) and D_p = S_p - q mu_p are exact;
+//    C' = sum_p D_p scale_p is formed with the lower chunks in double-double,
+//    so C' is rounded once, relative to itself.
+//    C = C' 2^(eA_i + eB_j - 2t).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstdint>
+#include <vector>
+
+#include "crt_gemm.h"
+#include "f4j_internal.h"
+#include "gemm_tile.h"
+#include "split_gemm.h"
+
+namespace f4j {
+namespace {
+
+// Pairwise coprime moduli, largest first (256 = 2^8; 255 = 3*5*17; 253 = 11*23;
+// 247 = 13*19; 217 = 7*31; the rest are primes).
+constexpr int MODULI[CRT_MAX_MODULI] = {256, 255, 253, 251, 247, 241, 239, 233, 229, 227,
+                                        223, 217, 211, 199, 197, 193, 191, 181, 179, 173};
+
+// ---- small unsigned bignum (base 2^32, little endian) for the tables ----
+using Big = std::vector<uint32_t>;
+
+Big big_from(uint64_t v) {
+    Big b;
+    while (v) {
+        b.push_back((uint32_t)v);
+        v >>= 32;
+    }
+    return b;
+}
+void big_trim(Big &b) {
+    while (!b.empty() && b.back() == 0) b.pop_back();
+}
+Big big_mul_small(const Big &a, uint32_t x) {
+    Big r(a.size() + 1, 0);
+    uint64_t carry = 0;
+    for (size_t i = 0; i < a.size(); ++i) {
+        const uint64_t v = (uint64_t)a[i] * x + carry;
+        r[i] = (uint32_t)v;
+        carry = v >> 32;
+    }
+    r[a.size()] = (uint32_t)carry;
+    big_trim(r);
+    return r;
+}
+Big big_div_small(const Big &a, uint32_t d, uint32_t *rem) {
+    Big q(a.size(), 0);
+    uint64_t r = 0;
+    for (size_t i = a.size(); i-- > 0;) {
+        const uint64_t cur = (r << 32) | a[i];
+        q[i] = (uint32_t)(cur / d);
+        r = cur % d;
+    }
+    big_trim(q);
+    if (rem) *rem = (uint32_t)r;
+    return q;
+}
+int big_bits(const Big &a) {
+    if (a.empty()) return 0;
+    int b = 32 * (int)(a.size() - 1);
+    uint32_t top = a.back();
+    while (top) {
+        ++b;
+        top >>= 1;
+    }
+    return b;
+}
+// bits [lo, lo + 40) of a (lo may be negative: bits below 0 are zero)
+uint64_t big_bits40(const Big &a, int lo) {
+    uint64_t v = 0;
+    for (int k = 39; k >= 0; --k) {
+        const int bit = lo + k;
+        int b = 0;
+        if (bit >= 0 && bit / 32 < (int)a.size()) b = (a[bit / 32] >> (bit % 32)) & 1;
+        v = (v << 1) | (uint64_t)b;
+    }
+    return v;
+}
+double big_to_double(const Big &a) {
+    const int nb = big_bits(a);
+    if (nb <= 53) {
+        double v = 0;
+        for (size_t i = a.size(); i-- > 0;) v = v * 4294967296.0 + a[i];
+        return v;
+    }
+    // top 53 bits, truncated, then rounded by the conversion of the next bits
+    uint64_t top = 0;
+    for (int k = nb - 1; k >= nb - 64; --k) top = (top << 1) | ((a[k / 32] >> (k % 32)) & 1);
+    return std::ldexp((double)top, nb - 64);
+}
+int inv_mod(int a, int m) {
+    int g = m, x = 0, x1 = 1, aa = a % m;
+    while (aa) {
+        const int q = g / aa;
+        int t = g - q * aa;
+        g = aa;
+        aa = t;
+        t = x - q * x1;
+        x = x1;
+        x1 = t;
+    }
+    return ((x % m) + m) % m;
+}
+
+int t_for(int n, int64_t K) {
+    double lg = 0;
+    for (int l = 0; l < n; ++l) lg += std::log2((double)MODULI[l]);
+    const double lk = std::log2((double)std::max<int64_t>(K, 1));
+    return (int)std::floor((lg - lk - 2.0) / 2.0 - 1e-9);
+}
+
+// ------------------------------------------------------------------------
+// Residue slices: out[l][x][k] = A'(x, k) mod m_l as a signed byte, k contiguous.
+// ------------------------------------------------------------------------
+
+// a is an integer with |a| < 2^106.  First round: q = rint(a / m) carries a
+// relative error <= 2^-52, so a - q m is an integer below 2^54 |a| / 2^106 and
+// fma forms it exactly; the second round brings it to |r| <= m.
+__device__ inline signed char residue(double a, int m, double minv) {
+    double r = fma(-rint(a * minv), (double)m, a);
+    r = fma(-rint(r * minv), (double)m, r);
+    const double hi = m == 256 ? 127.0 : (double)((m - 1) / 2), lo = m == 256 ? -128.0 : -hi;
+    if (r > hi) r -= m;
+    if (r < lo) r += m;
+    return (signed char)(int)r;
+}
+
+__device__ inline double scaled_int(double v, int e, int t) { return e == INT_MIN ? 0.0 : rint(ldexp(v, t - e)); }
+
+// Lines contiguous: thread handles 16 consecutive k of one line.
+__global__ __launch_bounds__(256) void crt_split_lines_kernel(const double *X, int64_t ld, int64_t nx, int64_t K,
+                                                              const int *e, signed char *out, int64_t Xp, int64_t Kp,
+                                                              CrtTables tb) {
+    const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t k16 = Kp / 16;
+    if (g >= Xp * k16) return;
+    const int64_t x = g / k16, k0 = (g % k16) * 16;
+    const int ex = x < nx ? e[x] : INT_MIN;
+    double v[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+        const int64_t k = k0 + u;
+        v[u] = scaled_int((x < nx && k < K) ? X[x * ld + k] : 0.0, ex, tb.t);
+    }
+    for (int l = 0; l < tb.n; ++l) {
+        signed char r[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) r[u] = residue(v[u], tb.m[l], tb.minv[l]);
+        *(int4 *)(out + (int64_t)l * Xp * Kp + x * Kp + k0) = __builtin_bit_cast(int4, r);
+    }
+}
+
+// Lines strided (stored K x X): a 128 (k) x 32 (x) block is transposed through LDS.
+__global__ __launch_bounds__(256) void crt_split_cols_kernel(const double *X, int64_t ld, int64_t nx, int64_t K,
+                                                             const int *e, signed char *out, int64_t Xp, int64_t Kp,
+                                                             CrtTables tb) {
+    __shared__ double tile[32][129];
+    const int64_t x0 = (int64_t)blockIdx.x * 32, k0 = (int64_t)blockIdx.y * 128;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    for (int r = ty; r < 128; r += 8) {
+        const int64_t k = k0 + r, x = x0 + tx;
+        tile[tx][r] = (k < K && x < nx) ? X[k * ld + x] : 0.0;
+    }
+    __syncthreads();
+    const int xl = threadIdx.x >> 3, kg = (threadIdx.x & 7) * 16;
+    const int64_t x = x0 + xl;
+    if (x >= Xp) return;
+    const int ex = x < nx ? e[x] : INT_MIN;
+    double v[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) v[u] = scaled_int(tile[xl][kg + u], ex, tb.t);
+    for (int l = 0; l < tb.n; ++l) {
+        signed char r[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) r[u] = residue(v[u], tb.m[l], tb.minv[l]);
+        *(int4 *)(out + (int64_t)l * Xp * Kp + x * Kp + k0 + kg) = __builtin_bit_cast(int4, r);
+    }
+}
+
+// ------------------------------------------------------------------------
+// Residue products: R[l][i][j] = (A_l B_l^T)_ij mod m_l, one byte.
+// v_mfma_i32_32x32x32_i8: A/B lane l holds row l%32, k = 16(l/32)+v; result
+// item r of lane l is row 8(r/4) + 4(l/32) + r%4, column l%32.
+// ------------------------------------------------------------------------
+
+typedef int i32x4 __attribute__((ext_vector_type(4)));
+typedef int i32x16 __attribute__((ext_vector_type(16)));
+
+constexpr int CBM = 256, CBN = 128, CBK = 128;    // tile, k slice in bytes (= int8 elements)
+constexpr int CSTAGES = 3;                        // B slices in flight through LDS
+constexpr int CSTAGE = CBN * CBK;                 // bytes per B stage (rows of 128 bytes, chunk-swizzled)
+
+struct CrtGemmArgs {
+    const signed char *A, *B;                     // [n][Mp][Kp], [n][Np][Kp]
+    int64_t Mp, Np, Kp, M, N;
+    unsigned char *R;                             // [n][M][N]
+};
+
+typedef __attribute__((address_space(3))) void lds_void;
+
+// 16-byte chunk c of B row r sits at LDS chunk c ^ swz(r): 16 consecutive rows
+// read at one chunk index hit 16 distinct 4-bank groups (and 8 consecutive rows
+// 8 distinct ones), so fragment reads are conflict free.
+__device__ __forceinline__ int swz(int r) { return (r & 7) ^ ((r >> 3) & 1); }
+
+// vmcnt(n) only (gfx9 encoding: expcnt and lgkmcnt at their maxima).
+#define F4J_WAIT_VMCNT(n) __builtin_amdgcn_s_waitcnt(((n) & 15) | (7 << 4) | (15 << 8) | (((n) >> 4) << 14))
+
+// Block tile 256 x 128, four waves of 64 x 128 (2 x 4 MFMA blocks, 128 int32
+// accumulators per lane).  Each wave reads its own 64 rows of A straight into
+// registers (no other wave needs them), three k slices deep; B is shared
+// through a three-stage LDS ring filled by buffer loads that write LDS
+// directly.  Loads run two slices ahead of the MFMAs, with one barrier per
+// slice.  MFMA step ks uses bytes 32 ks + 16 (lane / 32) + v of the k slice
+// for A and B alike.
+__global__ __launch_bounds__(256, 2) void crt_gemm_kernel(CrtGemmArgs a, CrtTables tb) {
+    __shared__ __attribute__((aligned(1024))) signed char lds[CSTAGES * CSTAGE];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);   // uniform: keeps descriptors scalar
+    const int l = blockIdx.y;
+    int64_t tm, tn;
+    tile_of(blockIdx.x, a.Mp / CBM, a.Np / CBN, tm, tn);
+    const int64_t m0 = (int64_t)__builtin_amdgcn_readfirstlane((int)tm) * CBM + wave * 64;
+    const int64_t n0 = (int64_t)__builtin_amdgcn_readfirstlane((int)tn) * CBN;
+
+    i32x16 acc[2][4];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[i][j][r] = 0;
+
+    const signed char *pa = a.A + ((int64_t)l * a.Mp + m0) * a.Kp;
+    const signed char *pb = a.B + ((int64_t)l * a.Np + n0) * a.Kp;
+    const int fr = lane & 31, fh = lane >> 5;
+    const int kp = (int)a.Kp;
+    const int aoff = fr * kp + 16 * fh;
+    // B: instruction p of this wave fills LDS rows 8 (4 wave + p) .. +8; lane
+    // -> row + lane / 8, LDS chunk lane % 8, global chunk (lane % 8) ^ swz(row).
+    // swz(row) = ((lane / 8) & 7) ^ (p & 1): two lane offsets; the row step 8 p kp is uniform.
+    int boff[2];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) boff[q] = (32 * wave + (lane >> 3)) * kp + 16 * ((lane & 7) ^ ((lane >> 3) & 7) ^ q);
+    int roff[4];                                   // fragment read offsets per step ks (row fr; + 4096 j)
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks) roff[ks] = fr * CBK + 16 * ((2 * ks + fh) ^ swz(fr));
+    const int nt = (int)(a.Kp / CBK);
+
+    i32x4 ra0[2][4], ra1[2][4];
+    auto issue_a = [&](int t, i32x4 (&dst)[2][4]) {
+        const __amdgpu_buffer_rsrc_t rsa =
+            __builtin_amdgcn_make_buffer_rsrc((void *)(pa + (int64_t)t * CBK), (short)0, 0x7FFFFFF0, 0x00020000);
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int ks = 0; ks < 4; ++ks)
+                dst[i][ks] = __builtin_bit_cast(
+                    i32x4, __builtin_amdgcn_raw_buffer_load_b128(rsa, aoff + 32 * ks, 32 * i * kp, 0));
+    };
+    auto issue_b = [&](int slot, int t) {        // slice t into stage slot % 3
+        const __amdgpu_buffer_rsrc_t rsb =
+            __builtin_amdgcn_make_buffer_rsrc((void *)(pb + (int64_t)t * CBK), (short)0, 0x7FFFFFF0, 0x00020000);
+        signed char *st = lds + (slot % CSTAGES) * CSTAGE + wave * 4096;
+#pragma unroll
+        for (int p = 0; p < 4; ++p)
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(rsb, (lds_void *)(st + 1024 * p), 16, boff[p & 1], 8 * p * kp,
+                                                     0, 0);
+    };
+    // Issue order: B(0), B(1), A(0), then per step t: B(t + 2) (4 loads) and
+    // A(t + 1) (8 loads).  Past the last slice the loads repeat slice nt - 1
+    // into the free stage and the unused register set, so every step issues
+    // the same 12 loads: the compiler's wait counts then stay exact (a load
+    // issued only on some paths makes it wait for the newest loads), and at
+    // the top of step t the 12 most recent loads are B(t + 1) and A(t), so
+    // vmcnt(12) retires B(t) and everything before it.
+    auto step = [&](int t, i32x4 (&cur)[2][4], i32x4 (&nxt)[2][4]) {
+        __builtin_amdgcn_sched_barrier(0);
+        F4J_WAIT_VMCNT(12);
+        __builtin_amdgcn_s_waitcnt(0xC07F);      // lgkmcnt(0)
+        __builtin_amdgcn_s_barrier();
+        __builtin_amdgcn_sched_barrier(0);
+        issue_b(t + 2, min(t + 2, nt - 1));
+        issue_a(min(t + 1, nt - 1), nxt);
+        __builtin_amdgcn_sched_barrier(0);
+        const signed char *st = lds + (t % CSTAGES) * CSTAGE;
+        i32x4 fb[2][4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) fb[0][j] = *(const i32x4 *)(st + 4096 * j + roff[0]);
+#pragma unroll
+        for (int ks = 0; ks < CBK / 32; ++ks) {
+            if (ks + 1 < CBK / 32) {
+#pragma unroll
+                for (int j = 0; j < 4; ++j) fb[(ks + 1) & 1][j] = *(const i32x4 *)(st + 4096 * j + roff[ks + 1]);
+            }
+            __builtin_amdgcn_sched_barrier(0);   // next step's fragment reads ahead of this step's MFMAs
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(cur[i][ks], fb[ks & 1][j], acc[i][j], 0, 0, 0);
+        }
+    };
+
+    issue_b(0, 0);
+    issue_b(1, min(1, nt - 1));
+    issue_a(0, ra0);
+    int t = 0;
+    for (; t + 1 < nt; t += 2) {
+        step(t, ra0, ra1);
+        step(t + 1, ra1, ra0);
+    }
+    if (t < nt) step(t, ra0, ra1);
+    F4J_WAIT_VMCNT(0);                            // no LDS write in flight when the block ends
+
+    // acc mod m in [0, m): q = floor(acc / m) from FP64 (|acc| < 2^31, error far
+    // below 1/m), then one correction.
+    const int m = tb.m[l];
+    const double minv = tb.minv[l];
+    unsigned char *R = a.R + (int64_t)l * a.M * a.N;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int64_t col = n0 + 32 * j + fr;
+            if (col >= a.N) continue;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int64_t row = m0 + 32 * i + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+                if (row >= a.M) continue;
+                const int v = acc[i][j][r];
+                int rr = v - (int)floor((double)v * minv) * m;
+                if (rr < 0) rr += m;
+                if (rr >= m) rr -= m;
+                R[row * a.N + col] = (unsigned char)rr;
+            }
+        }
+}
+
+// ------------------------------------------------------------------------
+// Reconstruction: C_ij = 2^(eA_i + eB_j - 2t) * CRT(R[.][i][j]).
+// ------------------------------------------------------------------------
+
+__global__ __launch_bounds__(256) void crt_reconstruct_kernel(const unsigned char *R, int64_t M, int64_t N,
+                                                              const int *eA, const int *eB, double *C, int64_t ldc,
+                                                              CrtTables tb) {
+    const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (g >= M * N) return;
+    const int64_t i = g / N, j = g % N;
+    const int ea = eA[i], eb = eB[j];
+    double out = 0.0;
+    if (ea != INT_MIN && eb != INT_MIN) {
+        double S[CRT_MAX_CHUNKS] = {0.0, 0.0, 0.0, 0.0};
+        for (int l = 0; l < tb.n; ++l) {
+            const double r = (double)R[(int64_t)l * M * N + g];
+#pragma unroll
+            for (int p = 0; p < CRT_MAX_CHUNKS; ++p) S[p] = fma(r, tb.w[l][p], S[p]);   // exact integers
+        }
+        double approx = 0.0;
+#pragma unroll
+        for (int p = 0; p < CRT_MAX_CHUNKS; ++p) approx = fma(S[p], tb.c[p], approx);
+        const double q = rint(approx);
+        // D_p scale_p are exact; the lower chunks are summed in double-double
+        // (TwoSum) because they cancel against the top chunk, so C' is rounded
+        // once, relative to itself.
+        double hi = 0.0, lo = 0.0;
+#pragma unroll
+        for (int p = CRT_MAX_CHUNKS - 1; p >= 1; --p) {
+            const double v = fma(-q, tb.mu[p], S[p]) * tb.scale[p];
+            const double s = hi + v, z = s - hi;
+            lo += (hi - (s - z)) + (v - z);
+            hi = s;
+        }
+        const double top = fma(-q, tb.mu[0], S[0]) * tb.scale[0];
+        const double s = top + hi, z = s - top;
+        const double acc = s + (((top - (s - z)) + (hi - z)) + lo);
+        out = ldexp(acc, ea + eb - 2 * tb.t);
+    }
+    C[i * ldc + j] = out;
+}
+
+}  // namespace
+
+int crt_default_moduli(int64_t K) {
+    for (int n = 2; n <= CRT_MAX_MODULI; ++n)
+        if (t_for(n, K) >= 53) return n;
+    return CRT_MAX_MODULI;
+}
+
+bool plan_crt(int64_t M, int64_t N, int64_t K, int n, int check_rows, CrtPlan *plan) {
+    if (n < 2 || n > CRT_MAX_MODULI) return false;
+    const int t = t_for(n, K);
+    if (t < 1) return false;
+    CrtPlan pl;
+    CrtTables &tb = pl.tb;
+    tb.n = n;
+    tb.t = t;
+    Big Mb = big_from(1);
+    for (int l = 0; l < n; ++l) Mb = big_mul_small(Mb, (uint32_t)MODULI[l]);
+    const int T = big_bits(Mb);
+    tb.P = (T + 39) / 40;
+    if (tb.P > CRT_MAX_CHUNKS) return false;
+    const int s = 40 * tb.P - T;          // W' = W 2^s fills exactly P chunks of 40 bits
+    const double Md = big_to_double(Mb);
+    for (int p = 0; p < CRT_MAX_CHUNKS; ++p) {
+        const int lo = 40 * (tb.P - 1 - p);
+        if (p < tb.P) {
+            tb.scale[p] = std::ldexp(1.0, lo - s);
+            tb.mu[p] = (double)big_bits40(Mb, lo - s);
+            tb.c[p] = tb.scale[p] / Md;
+        } else {
+            tb.scale[p] = 0.0;
+            tb.mu[p] = 0.0;
+            tb.c[p] = 0.0;
+        }
+    }
+    for (int l = 0; l < CRT_MAX_MODULI; ++l) {
+        for (int p = 0; p < CRT_MAX_CHUNKS; ++p) tb.w[l][p] = 0.0;
+        tb.m[l] = MODULI[l];
+        tb.minv[l] = 1.0 / MODULI[l];
+        if (l >= n) continue;
+        uint32_t rem = 0;
+        const Big Ml = big_div_small(Mb, (uint32_t)MODULI[l], nullptr);
+        (void)big_div_small(Ml, (uint32_t)MODULI[l], &rem);
+        const Big W = big_mul_small(Ml, (uint32_t)inv_mod((int)rem, MODULI[l]));
+        for (int p = 0; p < tb.P; ++p) tb.w[l][p] = (double)big_bits40(W, 40 * (tb.P - 1 - p) - s);
+    }
+    pl.Mp = (M + CBM - 1) / CBM * CBM;
+    pl.Np = (N + CBN - 1) / CBN * CBN;
+    pl.Kp = (K + CBK - 1) / CBK * CBK;
+    auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
+    pl.off_ra = 0;
+    pl.off_rb = al((size_t)n * pl.Mp * pl.Kp);
+    pl.off_res = pl.off_rb + al((size_t)n * pl.Np * pl.Kp);
+    pl.off_ea = pl.off_res + al((size_t)n * M * N);
+    pl.off_eb = pl.off_ea + al((size_t)M * 4);
+    pl.off_err = pl.off_eb + al((size_t)N * 4);
+    pl.off_part = pl.off_err + 256;
+    pl.bytes = pl.off_part + split_residual_bytes(N, K, (int)std::min<int64_t>(check_rows, M));
+    *plan = pl;
+    return true;
+}
+
+hipError_t launch_crt_gemm(const CrtPlan &pl, int opA, int opB, int64_t M, int64_t N, int64_t K, const double *A,
+                           int64_t lda, const double *B, int64_t ldb, double *C, int64_t ldc, void *ws,
+                           hipStream_t stream) {
+    char *w = (char *)ws;
+    signed char *ra = (signed char *)(w + pl.off_ra), *rb = (signed char *)(w + pl.off_rb);
+    unsigned char *res = (unsigned char *)(w + pl.off_res);
+    int *ea = (int *)(w + pl.off_ea), *eb = (int *)(w + pl.off_eb);
+    hipError_t e = launch_exponents(opA, opB, M, N, K, A, lda, B, ldb, ea, eb, stream);
+    if (e != hipSuccess) return e;
+    auto split = [&](const double *X, int64_t ld, bool lines, int64_t nx, int64_t Xp, const int *ex, signed char *out) {
+        if (lines) {
+            const int64_t items = Xp * (pl.Kp / 16);
+            hipLaunchKernelGGL(crt_split_lines_kernel, dim3((unsigned)((items + 255) / 256)), dim3(256), 0, stream, X,
+                               ld, nx, K, ex, out, Xp, pl.Kp, pl.tb);
+        } else {
+            hipLaunchKernelGGL(crt_split_cols_kernel, dim3((unsigned)(Xp / 32), (unsigned)(pl.Kp / 128)), dim3(256), 0,
+                               stream, X, ld, nx, K, ex, out, Xp, pl.Kp, pl.tb);
+        }
+    };
+    split(A, lda, opA == 0, M, pl.Mp, ea, ra);
+    split(B, ldb, opB == 1, N, pl.Np, eb, rb);
+    CrtGemmArgs a{ra, rb, pl.Mp, pl.Np, pl.Kp, M, N, res};
+    hipLaunchKernelGGL(crt_gemm_kernel, dim3((unsigned)((pl.Mp / CBM) * (pl.Np / CBN)), (unsigned)pl.tb.n), dim3(256),
+                       0, stream, a, pl.tb);
+    const int64_t items = M * N;
+    hipLaunchKernelGGL(crt_reconstruct_kernel, dim3((unsigned)((items + 255) / 256)), dim3(256), 0, stream, res, M, N,
+                       ea, eb, C, ldc, pl.tb);
+    return hipGetLastError();
+}
+
+}  // namespace f4j

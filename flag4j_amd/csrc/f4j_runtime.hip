@@ -23,6 +23,7 @@
 #include "../../include/flag4j_amd.h"
 #include "f4j_internal.h"
 #include "gemm_tile.h"
+#include "crt_gemm.h"
 #include "split_gemm.h"
 
 namespace f4j {
@@ -254,7 +255,60 @@ int run_split(int opA, int opB, int64_t M, int64_t N, int64_t K, const double *A
                 return fail(F4J_ERR_ACCURACY, "split product error " + std::to_string(err) + " above tolerance " +
                                                   std::to_string(c.split_tolerance));
             }
-            GemmProblem p{M, N, K, A, lda, B, ldb, C, ldc};
+            GemmProblem p{M, N, K, A, lda, B, ldb, C, ldc, 0};
+            F4J_HIP(launch_gemm_f64(opA ? R_T : R_N, opB ? R_T : R_N, p, stream, pick_tile(M, N)), "fp64 fallback");
+            rep.precision = F4J_PREC_FP64;
+        }
+    }
+    F4J_HIP(hipStreamSynchronize(stream), "stream synchronize");
+    t_split = rep;
+    return F4J_OK;
+}
+
+// Modular (CRT) split-precision real GEMM on the INT8 matrix cores;
+// `slices` = number of moduli (0: enough for FP64 accuracy at this K).
+int run_crt(int opA, int opB, int64_t M, int64_t N, int64_t K, const double *A, int64_t lda, const double *B,
+            int64_t ldb, double *C, int64_t ldc, int slices, hipStream_t stream) {
+    Config c;
+    {
+        std::lock_guard<std::mutex> g(S().cfg_mu);
+        c = S().cfg;
+    }
+    int n = slices > 0 ? slices : (c.default_slices > 0 ? c.default_slices : crt_default_moduli(K));
+    if (n > CRT_MAX_MODULI) n = CRT_MAX_MODULI;
+    CrtPlan pl;
+    if (K >= (int64_t(1) << 17) || !plan_crt(M, N, K, n, c.split_check_rows, &pl))
+        return fail(F4J_ERR_INVALID_ARG, "split_int8: " + std::to_string(n) +
+                                             " moduli cannot represent products with K = " + std::to_string(K) +
+                                             " (need 2..20 moduli and K < 131072)");
+    State &s = S();
+    std::lock_guard<std::mutex> g(s.ws_mu);
+    int rc = grow(s.ws, pl.bytes);
+    if (rc != F4J_OK) return rc;
+    F4J_HIP(launch_crt_gemm(pl, opA, opB, M, N, K, A, lda, B, ldb, C, ldc, s.ws.p, stream), "crt gemm launch");
+    f4j_split_report rep{};
+    rep.precision = F4J_PREC_SPLIT_INT8;
+    rep.slices = n;
+    rep.products = n;
+    rep.launches = 1;
+    rep.checked_rows = (int)std::min<int64_t>(c.split_check_rows, M);
+    rep.max_rel_err = -1.0;
+    if (rep.checked_rows > 0) {
+        char *w = (char *)s.ws.p;
+        F4J_HIP(launch_residual_check(opA, opB, M, N, K, A, lda, B, ldb, C, ldc, rep.checked_rows,
+                                      (double *)(w + pl.off_part), (unsigned long long *)(w + pl.off_err), stream),
+                "crt residual launch");
+        double err = 0.0;
+        F4J_HIP(hipMemcpyAsync(&err, w + pl.off_err, sizeof(double), hipMemcpyDeviceToHost, stream), "copy residual");
+        F4J_HIP(hipStreamSynchronize(stream), "stream synchronize");
+        rep.max_rel_err = err;
+        if (!(err <= c.split_tolerance)) {
+            if (!c.split_fallback) {
+                t_split = rep;
+                return fail(F4J_ERR_ACCURACY, "split_int8 product error " + std::to_string(err) +
+                                                  " above tolerance " + std::to_string(c.split_tolerance));
+            }
+            GemmProblem p{M, N, K, A, lda, B, ldb, C, ldc, 0};
             F4J_HIP(launch_gemm_f64(opA ? R_T : R_N, opB ? R_T : R_N, p, stream, pick_tile(M, N)), "fp64 fallback");
             rep.precision = F4J_PREC_FP64;
         }
@@ -283,6 +337,8 @@ int gemm_dev(bool cplx, int opA, int opB, int64_t M, int64_t N, int64_t K, const
     }
     if ((prec == F4J_PREC_SPLIT_BF16 || prec == F4J_PREC_SPLIT_FP16) && !cplx)
         return run_split(opA, opB, M, N, K, A, lda, B, ldb, C, ldc, prec, slices, stream);
+    if (prec == F4J_PREC_SPLIT_INT8 && !cplx)
+        return run_crt(opA, opB, M, N, K, A, lda, B, ldb, C, ldc, slices, stream);
     if (prec != F4J_PREC_FP64)
         return fail(F4J_ERR_UNSUPPORTED, "precision " + std::to_string(prec) + " is not available for " +
                                              (cplx ? "complex" : "real") + " products in this build");
@@ -378,8 +434,8 @@ int config_set_locked(Config &c, const std::string &key, const std::string &valu
         c.split_fallback = value == "1";
     } else if (key == "defaultSlices") {
         const long v = std::strtol(value.c_str(), &end, 10);
-        if (end == value.c_str() || *end || v < 0 || v > SPLIT_MAX_SLICES)
-            return fail(F4J_ERR_INVALID_ARG, "defaultSlices: expected 0.." + std::to_string(SPLIT_MAX_SLICES));
+        if (end == value.c_str() || *end || v < 0 || v > CRT_MAX_MODULI)
+            return fail(F4J_ERR_INVALID_ARG, "defaultSlices: expected 0.." + std::to_string(CRT_MAX_MODULI));
         c.default_slices = (int)v;
     } else {
         return fail(F4J_ERR_INVALID_ARG, "unknown configuration key '" + key + "'");

@@ -59,4 +59,16 @@ hipError_t launch_split_residual(const SplitPlan &pl, int opA, int opB, int64_t 
                                  const double *A, int64_t lda, const double *B, int64_t ldb, const double *C,
                                  int64_t ldc, int rows, void *ws, hipStream_t stream);
 
+// Row exponents of op(A) (eA[M]) and column exponents of op(B) (eB[N]): the
+// smallest E with |x| < 2^E over the line, INT_MIN for an all-zero line.
+hipError_t launch_exponents(int opA, int opB, int64_t M, int64_t N, int64_t K, const double *A, int64_t lda,
+                            const double *B, int64_t ldb, int *eA, int *eB, hipStream_t stream);
+
+// FP64 residual check of `rows` evenly spaced rows of C against compensated
+// dot products; max |C - AB| / (|A||B|) left as double bits in *err.
+// `part` holds split_residual_bytes(N, K, rows) bytes.
+hipError_t launch_residual_check(int opA, int opB, int64_t M, int64_t N, int64_t K, const double *A, int64_t lda,
+                                 const double *B, int64_t ldb, const double *C, int64_t ldc, int rows, double *part,
+                                 unsigned long long *err, hipStream_t stream);
+
 }  // namespace f4j

@@ -452,20 +452,7 @@ hipError_t launch_split_gemm(const SplitPlan &pl, int opA, int opB, int64_t M, i
     const int S = pl.S;
     const bool fp16 = pl.carrier == SPLIT_FP16;
 
-    // exponents
-    hipLaunchKernelGGL(fill_int_kernel, dim3(64), dim3(256), 0, stream, ea, M, INT_MIN);
-    hipLaunchKernelGGL(fill_int_kernel, dim3(64), dim3(256), 0, stream, eb, N, INT_MIN);
-    const int64_t kchunk = 256;
-    if (opA == 0)
-        hipLaunchKernelGGL(exp_lines_kernel, dim3((unsigned)M), dim3(256), 0, stream, A, lda, K, ea);
-    else
-        hipLaunchKernelGGL(exp_cols_kernel, dim3((unsigned)((M + 255) / 256), (unsigned)((K + kchunk - 1) / kchunk)),
-                           dim3(256), 0, stream, A, lda, K, M, kchunk, ea);
-    if (opB == 1)
-        hipLaunchKernelGGL(exp_lines_kernel, dim3((unsigned)N), dim3(256), 0, stream, B, ldb, K, eb);
-    else
-        hipLaunchKernelGGL(exp_cols_kernel, dim3((unsigned)((N + 255) / 256), (unsigned)((K + kchunk - 1) / kchunk)),
-                           dim3(256), 0, stream, B, ldb, K, N, kchunk, eb);
+    launch_exponents(opA, opB, M, N, K, A, lda, B, ldb, ea, eb, stream);
 
     // digit slices
     auto split = [&](const double *X, int64_t ld, bool lines, int64_t nx, int64_t Xp, const int *e, void *out) {
@@ -517,11 +504,39 @@ hipError_t launch_split_gemm(const SplitPlan &pl, int opA, int opB, int64_t M, i
     return hipGetLastError();
 }
 
+hipError_t launch_exponents(int opA, int opB, int64_t M, int64_t N, int64_t K, const double *A, int64_t lda,
+                            const double *B, int64_t ldb, int *ea, int *eb, hipStream_t stream) {
+    hipLaunchKernelGGL(fill_int_kernel, dim3(64), dim3(256), 0, stream, ea, M, INT_MIN);
+    hipLaunchKernelGGL(fill_int_kernel, dim3(64), dim3(256), 0, stream, eb, N, INT_MIN);
+    const int64_t kchunk = 256;
+    if (M > 0 && K > 0) {
+        if (opA == 0)
+            hipLaunchKernelGGL(exp_lines_kernel, dim3((unsigned)M), dim3(256), 0, stream, A, lda, K, ea);
+        else
+            hipLaunchKernelGGL(exp_cols_kernel, dim3((unsigned)((M + 255) / 256), (unsigned)((K + kchunk - 1) / kchunk)),
+                               dim3(256), 0, stream, A, lda, K, M, kchunk, ea);
+    }
+    if (N > 0 && K > 0) {
+        if (opB == 1)
+            hipLaunchKernelGGL(exp_lines_kernel, dim3((unsigned)N), dim3(256), 0, stream, B, ldb, K, eb);
+        else
+            hipLaunchKernelGGL(exp_cols_kernel, dim3((unsigned)((N + 255) / 256), (unsigned)((K + kchunk - 1) / kchunk)),
+                               dim3(256), 0, stream, B, ldb, K, N, kchunk, eb);
+    }
+    return hipGetLastError();
+}
+
 hipError_t launch_split_residual(const SplitPlan &pl, int opA, int opB, int64_t M, int64_t N, int64_t K,
                                  const double *A, int64_t lda, const double *B, int64_t ldb, const double *C,
                                  int64_t ldc, int rows, void *ws, hipStream_t stream) {
-    unsigned long long *err = (unsigned long long *)((char *)ws + pl.off_err);
-    double *part = (double *)((char *)ws + pl.off_residual);
+    return launch_residual_check(opA, opB, M, N, K, A, lda, B, ldb, C, ldc, rows,
+                                 (double *)((char *)ws + pl.off_residual),
+                                 (unsigned long long *)((char *)ws + pl.off_err), stream);
+}
+
+hipError_t launch_residual_check(int opA, int opB, int64_t M, int64_t N, int64_t K, const double *A, int64_t lda,
+                                 const double *B, int64_t ldb, const double *C, int64_t ldc, int rows, double *part,
+                                 unsigned long long *err, hipStream_t stream) {
     hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(256), 0, stream, (int *)err, 2, 0);
     const int r = (int)std::min<int64_t>(rows, M);
     if (r <= 0) return hipGetLastError();

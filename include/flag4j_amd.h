@@ -53,7 +53,9 @@ typedef enum f4j_precision {
                                      bitwise equal to the reference CPU kernels */
     F4J_PREC_SPLIT_BF16 = 2,      /* FP64 split into BF16 slices, FP32-accumulated MFMA, FP64 recombination */
     F4J_PREC_SPLIT_FP16 = 3,      /* same with FP16 slices */
-    F4J_PREC_SPLIT_INT8 = 4       /* same with INT8 slices and exact INT32 accumulation */
+    F4J_PREC_SPLIT_INT8 = 4       /* FP64 as integers modulo coprime moduli <= 256 (Chinese remainder theorem):
+                                     INT8 residue products with exact INT32 accumulation, FP64 reconstruction;
+                                     `slices` = number of moduli (2..20, 0 = enough for FP64 accuracy) */
 } f4j_precision;
 
 typedef struct f4j_device_info {
@@ -94,7 +96,8 @@ int f4j_stream_sync(void *stream);
  *           RealDenseMatrixMultiplyDispatcher.dispatchTranspose M:linalg/ops/RealDenseMatrixMultiplyDispatcher.java:148-154 ->
  *           RealDenseMatMultTranspose.*   M:linalg/ops/dense/real/RealDenseMatMultTranspose.java:53-233
  *   (T, *): A.T().mult(B) without materialising A.T() (M:linalg/ops/TransposeDispatcher.java:77-99).
- * C (M x N) = op(A) op(B).  `slices` is used by the SPLIT precisions (0 = choose). */
+ * C (M x N) = op(A) op(B).  `slices` is used by the SPLIT precisions (0 = choose): digit slices
+ * per operand for SPLIT_BF16/FP16 (1..8), moduli for SPLIT_INT8 (2..20). */
 int f4j_dgemm(int opA, int opB, int64_t M, int64_t N, int64_t K,
               const double *A, int64_t lda, const double *B, int64_t ldb,
               double *C, int64_t ldc, int precision, int slices);

@@ -151,3 +151,86 @@ def test_matrix_api_split():
     A, B = rng.uniform(15, 200, 200), rng.uniform(16, 200, 200)
     C = f4j.Matrix(A).mult(f4j.Matrix(B), precision="split_bf16", slices=6).numpy()
     assert sampled_err(C, A, B) <= 1e-15
+
+
+# ---------------------------------------------------------------- modular (CRT) form, INT8 matrix cores
+
+def test_crt_integer_inputs_exact():
+    """Integer entries below 2^20: with the default moduli (t = 53 bits) every
+    scaled entry is exact, the residue products are exact and C is the exact
+    integer product."""
+    r = np.random.default_rng(2)
+    A = r.integers(-(1 << 20), 1 << 20, (150, 260)).astype(np.float64)
+    B = r.integers(-(1 << 20), 1 << 20, (260, 170)).astype(np.float64)
+    C, rep = split_mult(A, B, "split_int8", 0)
+    exact = A.astype(object).dot(B.astype(object))
+    assert np.array_equal(C, np.array(exact, dtype=np.float64))
+    assert rep.precision == L.PREC["split_int8"] and rep.slices >= 14
+
+
+def test_crt_default_moduli_fp64_accuracy():
+    A, B = rng.uniform(21, 1024, 1024), rng.uniform(22, 1024, 1024)
+    C, rep = split_mult(A, B, "split_int8", 0)
+    assert sampled_err(C, A, B) <= 1e-15
+    assert rep.precision == L.PREC["split_int8"] and 0 <= rep.max_rel_err <= 1e-15
+
+
+def test_crt_error_falls_with_moduli():
+    """Each extra modulus adds ~8 bits to the product range, i.e. ~4 bits to t."""
+    A, B = rng.uniform(23, 400, 600), rng.uniform(24, 600, 300)
+    L.config_set("splitCheckRows", 0)
+    errs = []
+    for n in (8, 10, 12, 14, 16):
+        C, rep = split_mult(A, B, "split_int8", n)
+        assert rep.slices == n
+        errs.append(sampled_err(C, A, B))
+    for a, b in zip(errs, errs[1:]):
+        assert b <= max(a / 100, 2.5e-16), errs
+    assert errs[0] > 1e-10
+
+
+@pytest.mark.parametrize("opA,opB", [("N", "T"), ("T", "N"), ("T", "T")])
+def test_crt_transposed(opA, opB):
+    M, N, K = 300, 200, 500
+    A = rng.uniform(25, *((M, K) if opA == "N" else (K, M)))
+    B = rng.uniform(26, *((K, N) if opB == "N" else (N, K)))
+    C, _ = split_mult(A, B, "split_int8", 0, opA, opB)
+    Aop = np.ascontiguousarray(A if opA == "N" else A.T)
+    Bop = np.ascontiguousarray(B if opB == "N" else B.T)
+    assert sampled_err(C, Aop, Bop) <= 1e-15
+
+
+@pytest.mark.parametrize("M,N,K", [(1, 1, 1), (7, 300, 13), (130, 70, 257), (129, 257, 1000)])
+def test_crt_ragged(M, N, K):
+    A, B = rng.uniform(M + 31, M, K), rng.uniform(N + 32, K, N)
+    C, _ = split_mult(A, B, "split_int8", 0)
+    assert sampled_err(C, A, B, n=200) <= 1e-15
+
+
+@pytest.mark.parametrize("phi,n", [(4, 18), (8, 20)])
+def test_crt_zero_lines_and_wide_range(phi, n):
+    """Entries spread over 10^(+-phi): the default moduli (t ~ 53) leave the small
+    entries of a row with too few bits; more moduli raise t (65 and 72 bits at
+    K = 256) and restore the bar."""
+    A = rng.scaled_uniform(27, 256, 256, phi)
+    B = rng.scaled_uniform(28, 256, 256, phi)
+    A[3, :] = 0.0
+    B[:, 9] = 0.0
+    L.config_set("splitCheckRows", 0)
+    C, _ = split_mult(A, B, "split_int8", n)
+    assert np.all(C[3, :] == 0.0) and np.all(C[:, 9] == 0.0)
+    mask = np.ones(C.shape, bool)
+    mask[3, :] = False
+    mask[:, 9] = False
+    I, J = np.nonzero(mask)
+    sel = np.random.default_rng(0).choice(len(I), 400, replace=False)
+    hi, lo, ab = o.real_dot2(A, B, I[sel], J[sel])
+    assert np.max(np.abs((C[I[sel], J[sel]] - hi) - lo) / ab) <= 1e-15
+
+
+def test_crt_rejects_unrepresentable_k():
+    A, B = np.ones((1, 1 << 17)), np.ones((1 << 17, 1))
+    C = np.empty((1, 1))
+    st = L.load().f4j_dgemm(0, 0, 1, 1, 1 << 17, A.ctypes.data, 1 << 17, B.ctypes.data, 1, C.ctypes.data, 1,
+                            L.PREC["split_int8"], 0)
+    assert st == L.ERR_INVALID_ARG
