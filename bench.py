@@ -6,19 +6,23 @@ java.util.Random seeds 1 and 2), operands resident in HBM.  A step is one
 product C = A*B.  The metric is FP64-accurate GFLOP/s, crediting 2*M*N*K flops
 per product whatever arithmetic computes it.
 
-Forms measured on the same inputs: plain FP64 MFMA, and the split-precision
-forms (BF16 digits, 6 slices; FP16 digits, 5 slices), each timed over split,
-slice products, recombination and the FP64 residual check, with its error
-measured against the compensated oracle on sampled entries.  `value` is the
-fastest form whose sampled error is <= 1e-15 of (|A||B|)_ij.  The batched
-workload (10,000 products with M, N, K log-uniform on [16, 512], N(0,1)
-entries) is reported beside it.
+Forms measured on the same inputs, each with W warmup steps and K timed steps
+on one stream: plain FP64 MFMA; the Ozaki-I split forms (BF16 digits, 6
+slices; FP16 digits, 5 slices); and the modular form (INT8 residues modulo
+16 coprime moduli, recombined by the Chinese remainder theorem).  A split step
+covers exponents, splitting, slice products, recombination and the FP64
+residual check.  Each form's error is measured against exact sums on sampled
+entries.  `value` is the fastest form whose sampled error is <= 1e-15 of
+(|A||B|)_ij.  The batched workload (10,000 products with M, N, K log-uniform
+on [16, 512], N(0,1) entries) is reported beside it.
 
 Prints one JSON line.  `roofline` is for the dominant kernel of the form that
-gives `value`, timed with HIP events on the stream it is launched on;
-`cpu_baseline` times the oracle (a C restatement of the kernel Flag4j's
-dispatcher selects at this shape, RealDenseMatMult.concurrentBlockedReordered)
-on a bounded sample of the same product on this host's cores.
+gives `value` (the FP64 GEMM, or the slice-product launch of a split form),
+timed with HIP events on the stream it is launched on; `roofline_fp64` is the
+plain FP64 kernel's.  `cpu_baseline` times the oracle (a C restatement of the
+kernel Flag4j's dispatcher selects at this shape,
+RealDenseMatMult.concurrentBlockedReordered) on a bounded sample of the same
+product on this host's cores.
 """
 import argparse
 import json
@@ -30,7 +34,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_FP64_TFLOPS = 78.6     # MI355X dense FP64 matrix peak
+PEAK_BF16_TFLOPS = 2500.0   # dense BF16 / FP16 matrix peak
+PEAK_INT8_TOPS = 5000.0     # dense INT8 matrix peak
 PEAK_HBM_GBS = 8000.0
+FORMS = {"fp64": 0, "split_bf16": 6, "split_fp16": 5, "split_int8": 0}   # form -> slices (0: library default)
 
 
 def parse():
@@ -41,7 +48,8 @@ def parse():
     ap.add_argument("--size", type=int, default=8192)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--no-split", action="store_true")
+    ap.add_argument("--no-split", action="store_true", help="plain FP64 only")
+    ap.add_argument("--forms", default=",".join(FORMS), help="comma-separated subset of " + ",".join(FORMS))
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--profile-copy", action="store_true",
                     help="after the timed region, copy 512 MiB on the device once (calibrates HBM byte counters)")
@@ -169,38 +177,13 @@ def main():
     torch.cuda.synchronize()
     sh = stream.cuda_stream
 
-    def step():
-        f4j.dgemm_dev("N", "N", n, n, n, dA.data_ptr(), n, dB.data_ptr(), n, dC.data_ptr(), n,
-                      precision="fp64", stream=sh)
-
-    log("[bench] warmup")
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-
-    # Timed region: K back-to-back products, events on the launch stream.
-    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for e0, e1 in evs:
-        e0.record(stream)
-        step()
-        e1.record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    kern_ms = [e0.elapsed_time(e1) for e0, e1 in evs]
-    ms_per_step = wall * 1e3 / args.steps
     flops = 2.0 * n * n * n
-    value = flops / (ms_per_step * 1e-3) / 1e9
-    kavg = float(np.mean(kern_ms))
-    achieved = flops / (kavg * 1e-3) / 1e12
-    log(f"[bench] kernel avg {kavg:.3f} ms, {achieved:.2f} TFLOP/s")
-    import ctypes
-    kbuf = ctypes.create_string_buffer(64)
-    f4j._native.check(f4j._native.load().f4j_dispatch_name(0, 0, 0, n, n, n, 0, kbuf, 64))
-    fp64_kernel = kbuf.value.decode()
+    forms_wanted = ["fp64"] if args.no_split else [f for f in args.forms.split(",") if f]
+    for f in forms_wanted:
+        if f not in FORMS:
+            sys.exit(f"unknown form {f}")
 
-    # Accuracy of the measured product on sampled entries, against exact sums.
+    # Accuracy of each form's product on sampled entries, against exact sums.
     r = np.random.default_rng(0)
     I, J = r.integers(0, n, 512), r.integers(0, n, 512)
     exact, ab = exact_entries(np, A, B, I, J)
@@ -209,31 +192,76 @@ def main():
         Ch = dC.cpu().numpy()
         return float(np.max(np.abs(Ch[I, J] - exact) / ab))
 
-    max_err = sampled_err()
-    forms = {"fp64": {"ms": round(ms_per_step, 3), "gflops": round(value, 1), "max_err": max_err}}
-    best = ("fp64", value)
-    if not args.no_split:
-        for prec, S in (("split_bf16", 6), ("split_fp16", 5)):
-            def split_step():
-                f4j.dgemm_dev("N", "N", n, n, n, dA.data_ptr(), n, dB.data_ptr(), n, dC.data_ptr(), n,
-                              precision=prec, slices=S, stream=sh)
-            split_step()
-            torch.cuda.synchronize()
-            reps = max(2, args.steps // 4)
-            t0 = time.perf_counter()
-            for _ in range(reps):
-                split_step()
-            torch.cuda.synchronize()
-            ms = (time.perf_counter() - t0) * 1e3 / reps
+    def time_form(prec, S):
+        """W warmup steps, then exactly K steps between synchronisations, each
+        bracketed by events on the launch stream."""
+        def step():
+            f4j.dgemm_dev("N", "N", n, n, n, dA.data_ptr(), n, dB.data_ptr(), n, dC.data_ptr(), n,
+                          precision=prec, slices=S, stream=sh)
+        for _ in range(args.warmup):
+            step()
+        evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+               for _ in range(args.steps)]
+        product_ms = []
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for e0, e1 in evs:
+            e0.record(stream)
+            step()
+            e1.record(stream)
+            if prec != "fp64":
+                product_ms.append(f4j._native.last_split_report().product_ms)
+        torch.cuda.synchronize()
+        wall_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+        step_ms = float(np.mean([e0.elapsed_time(e1) for e0, e1 in evs]))
+        return wall_ms, step_ms, (float(np.mean(product_ms)) if product_ms else step_ms)
+
+    f4j._native.config_set("splitTiming", 1)
+    forms, timings = {}, {}
+    best = None
+    for prec in forms_wanted:
+        S = FORMS[prec]
+        log(f"[bench] {prec}: {args.warmup} warmup + {args.steps} timed steps")
+        wall_ms, step_ms, kern_ms = time_form(prec, S)
+        err = sampled_err()
+        gf = flops / (wall_ms * 1e-3) / 1e9
+        entry = {"ms": round(wall_ms, 3), "gflops": round(gf, 1), "max_err": err}
+        ok = err <= 1e-15
+        if prec != "fp64":
             rep = f4j._native.last_split_report()
-            err = sampled_err()
-            gf = flops / (ms * 1e-3) / 1e9
-            forms[prec] = {"slices": S, "products": rep.products, "ms": round(ms, 3), "gflops": round(gf, 1),
-                           "max_err": err, "residual_check_err": rep.max_rel_err,
-                           "delivered": "split" if rep.precision != 0 else "fp64 fallback"}
-            log(f"[bench] {prec} S={S}: {ms:.2f} ms, {gf:.0f} GFLOP/s, err {err:.2e}")
-            if err <= 1e-15 and rep.precision != 0 and gf > best[1]:
-                best = (prec, gf)
+            entry.update({"slices": rep.slices, "products": rep.products, "residual_check_err": rep.max_rel_err,
+                          "product_kernel_ms": round(kern_ms, 4),
+                          "delivered": "split" if rep.precision != 0 else "fp64 fallback"})
+            ok = ok and rep.precision != 0
+            timings[prec] = (kern_ms, rep.products)
+        else:
+            timings[prec] = (kern_ms, 1)
+        forms[prec] = entry
+        log(f"[bench] {prec}: {wall_ms:.2f} ms/step, {gf:.0f} GFLOP/s, err {err:.2e}")
+        if ok and (best is None or gf > best[1]):
+            best = (prec, gf, wall_ms)
+    f4j._native.config_set("splitTiming", 0)
+    if best is None:
+        sys.exit("no form reached the 1e-15 accuracy bar")
+
+    import ctypes
+    kbuf = ctypes.create_string_buffer(64)
+    f4j._native.check(f4j._native.load().f4j_dispatch_name(0, 0, 0, n, n, n, 0, kbuf, 64))
+    fp64_kernel = kbuf.value.decode()
+
+    def roofline(prec):
+        kern_ms, products = timings[prec]
+        if prec == "fp64":
+            achieved, peak, unit, prof = flops / (kern_ms * 1e-3) / 1e12, PEAK_FP64_TFLOPS, "TFLOP/s", "dgemm_f64"
+        elif prec == "split_int8":
+            achieved, peak, unit, prof = (products * flops / (kern_ms * 1e-3) / 1e12, PEAK_INT8_TOPS, "TOP/s",
+                                          "crt_gemm_i8")
+        else:
+            achieved, peak, unit, prof = (products * flops / (kern_ms * 1e-3) / 1e12, PEAK_BF16_TFLOPS, "TFLOP/s",
+                                          "slice_gemm_" + prec[6:])
+        return {"bound": "mfma", "achieved": round(achieved, 3), "peak": peak, "unit": unit,
+                "frac": round(achieved / peak, 4), "traffic": load_traffic(f"{prof}_{n}_pmc.txt"),
+                "kernel_ms": round(kern_ms, 4)}
 
     if args.profile_copy:
         src = torch.empty(64 * 1024 * 1024, dtype=torch.float64, device=dev).fill_(1.0)
@@ -261,6 +289,9 @@ def main():
         log(f"[bench] CPU baseline on {threads} threads")
         cpu = cpu_baseline(A, B, n, threads)
 
+    dtypes = {"fp64": "f64", "split_bf16": "bf16 digits, f32 accumulate, f64 recombine",
+              "split_fp16": "f16 digits, f32 accumulate, f64 recombine",
+              "split_int8": "i8 residues, i32 accumulate, f64 recombine"}
     out = {
         "metric": "FP64-accurate GFLOP/s, Matrix.mult 8192^3",
         "value": round(best[1], 2),
@@ -268,20 +299,19 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 4),
+        "ms_per_step": round(best[2], 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": dtypes[best[0]],
         "data": "synthetic: java.util.Random seeds 1 (A) and 2 (B), entries 2r-1 in [-1,1)",
         "config": {"workload": f"Matrix.mult FP64 {n}x{n}x{n} (C = A*B, row-major, device-resident)",
                    "M": n, "N": n, "K": n, "form": best[0], "fp64_kernel": fp64_kernel},
-        "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(achieved / PEAK_FP64_TFLOPS, 4),
-                     "traffic": load_traffic(f"dgemm_f64_{n}_pmc.txt")},
+        "roofline": roofline(best[0]),
+        "roofline_fp64": roofline("fp64") if "fp64" in timings else None,
         "cpu_baseline": cpu,
         "forms": forms,
-        "accuracy": {"max_err_over_absAabsB_sampled": max_err, "samples": 512,
+        "accuracy": {"max_err_over_absAabsB_sampled": forms[best[0]]["max_err"], "samples": 512,
                      "reference": "exact sums (TwoProduct + fsum) on sampled entries"},
         "batched": batched,
         "end_to_end_gflops_host_arrays": round(flops / e2e_s / 1e9, 2),

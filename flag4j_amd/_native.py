@@ -34,9 +34,13 @@ class DeviceInfo(ctypes.Structure):
                 ("peak_hbm_gbs", ctypes.c_double), ("name", ctypes.c_char * 64), ("arch", ctypes.c_char * 32)]
 
 
+ABI_VERSION = 2          # F4J_ABI_VERSION the structures below match
+
+
 class SplitReport(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int), ("slices", ctypes.c_int), ("products", ctypes.c_int),
-                ("launches", ctypes.c_int), ("checked_rows", ctypes.c_int), ("max_rel_err", ctypes.c_double)]
+                ("launches", ctypes.c_int), ("checked_rows", ctypes.c_int), ("max_rel_err", ctypes.c_double),
+                ("product_ms", ctypes.c_double)]
 
 
 SIGNATURES = {
@@ -85,6 +89,9 @@ def load():
             fn = getattr(lib, name)
             fn.argtypes = args
             fn.restype = res
+        if lib.f4j_abi_version() != ABI_VERSION:
+            raise OSError(f"{LIB_PATH} has ABI version {lib.f4j_abi_version()}, these bindings expect "
+                          f"{ABI_VERSION}: rebuild the library")
         _lib = lib
     return _lib
 

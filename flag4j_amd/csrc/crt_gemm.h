@@ -41,8 +41,9 @@ bool plan_crt(int64_t M, int64_t N, int64_t K, int n, int check_rows, CrtPlan *p
 
 // Enqueue exponents, residue slices, the n residue products and the FP64
 // reconstruction into C.  `ws` holds plan.bytes bytes.  op codes: 0 = N, 1 = T.
+// ev_begin / ev_end (optional) are recorded on `stream` around the product launch.
 hipError_t launch_crt_gemm(const CrtPlan &pl, int opA, int opB, int64_t M, int64_t N, int64_t K, const double *A,
                            int64_t lda, const double *B, int64_t ldb, double *C, int64_t ldc, void *ws,
-                           hipStream_t stream);
+                           hipStream_t stream, hipEvent_t ev_begin = nullptr, hipEvent_t ev_end = nullptr);
 
 }  // namespace f4j

@@ -471,7 +471,7 @@ bool plan_crt(int64_t M, int64_t N, int64_t K, int n, int check_rows, CrtPlan *p
 
 hipError_t launch_crt_gemm(const CrtPlan &pl, int opA, int opB, int64_t M, int64_t N, int64_t K, const double *A,
                            int64_t lda, const double *B, int64_t ldb, double *C, int64_t ldc, void *ws,
-                           hipStream_t stream) {
+                           hipStream_t stream, hipEvent_t ev_begin, hipEvent_t ev_end) {
     char *w = (char *)ws;
     signed char *ra = (signed char *)(w + pl.off_ra), *rb = (signed char *)(w + pl.off_rb);
     unsigned char *res = (unsigned char *)(w + pl.off_res);
@@ -491,8 +491,10 @@ hipError_t launch_crt_gemm(const CrtPlan &pl, int opA, int opB, int64_t M, int64
     split(A, lda, opA == 0, M, pl.Mp, ea, ra);
     split(B, ldb, opB == 1, N, pl.Np, eb, rb);
     CrtGemmArgs a{ra, rb, pl.Mp, pl.Np, pl.Kp, M, N, res};
+    if (ev_begin) hipEventRecord(ev_begin, stream);
     hipLaunchKernelGGL(crt_gemm_kernel, dim3((unsigned)((pl.Mp / CBM) * (pl.Np / CBN)), (unsigned)pl.tb.n), dim3(256),
                        0, stream, a, pl.tb);
+    if (ev_end) hipEventRecord(ev_end, stream);
     const int64_t items = M * N;
     hipLaunchKernelGGL(crt_reconstruct_kernel, dim3((unsigned)((items + 255) / 256)), dim3(256), 0, stream, res, M, N,
                        ea, eb, C, ldc, pl.tb);

@@ -46,6 +46,7 @@ struct Config {
     int split_check_rows = 4;         // rows of C checked in FP64 after a split product (0 = off)
     double split_tolerance = 1e-15;   // max |C - AB| / (|A||B|) accepted on the checked rows
     int split_fallback = 1;           // 1: recompute in FP64 when the check fails; 0: return F4J_ERR_ACCURACY
+    int split_timing = 0;             // 1: time the slice-product launches with HIP events (report product_ms)
 };
 
 struct State {
@@ -57,8 +58,9 @@ struct State {
     hipDeviceProp_t prop{};
     Config cfg;
     DevBuf bufA, bufB, bufC;
-    std::mutex ws_mu;                 // guards ws (split-precision workspace)
+    std::mutex ws_mu;                 // guards ws and tev (split-precision workspace, product timing)
     DevBuf ws;
+    hipEvent_t tev[2] = {nullptr, nullptr};
 };
 
 State &S() {
@@ -95,6 +97,7 @@ int init_locked(State &s, int device) {
     F4J_HIP(hipSetDevice(device), "hipSetDevice");
     F4J_HIP(hipGetDeviceProperties(&s.prop, device), "hipGetDeviceProperties");
     F4J_HIP(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking), "hipStreamCreate");
+    for (hipEvent_t &e : s.tev) F4J_HIP(hipEventCreate(&e), "hipEventCreate");
     s.device = device;
     s.inited = true;
     return F4J_OK;
@@ -233,8 +236,12 @@ int run_split(int opA, int opB, int64_t M, int64_t N, int64_t K, const double *A
     std::lock_guard<std::mutex> g(s.ws_mu);
     int rc = grow(s.ws, pl.bytes);
     if (rc != F4J_OK) return rc;
-    F4J_HIP(launch_split_gemm(pl, opA, opB, M, N, K, A, lda, B, ldb, C, ldc, s.ws.p, stream), "split gemm launch");
+    hipEvent_t *ev = c.split_timing ? s.tev : nullptr;
+    F4J_HIP(launch_split_gemm(pl, opA, opB, M, N, K, A, lda, B, ldb, C, ldc, s.ws.p, stream, ev ? ev[0] : nullptr,
+                              ev ? ev[1] : nullptr),
+            "split gemm launch");
     f4j_split_report rep{};
+    rep.product_ms = -1.0;
     rep.precision = prec;
     rep.slices = S_;
     rep.products = pl.products();
@@ -261,6 +268,11 @@ int run_split(int opA, int opB, int64_t M, int64_t N, int64_t K, const double *A
         }
     }
     F4J_HIP(hipStreamSynchronize(stream), "stream synchronize");
+    if (ev) {
+        float ms = 0.f;
+        F4J_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]), "hipEventElapsedTime");
+        rep.product_ms = ms;
+    }
     t_split = rep;
     return F4J_OK;
 }
@@ -285,8 +297,12 @@ int run_crt(int opA, int opB, int64_t M, int64_t N, int64_t K, const double *A, 
     std::lock_guard<std::mutex> g(s.ws_mu);
     int rc = grow(s.ws, pl.bytes);
     if (rc != F4J_OK) return rc;
-    F4J_HIP(launch_crt_gemm(pl, opA, opB, M, N, K, A, lda, B, ldb, C, ldc, s.ws.p, stream), "crt gemm launch");
+    hipEvent_t *ev = c.split_timing ? s.tev : nullptr;
+    F4J_HIP(launch_crt_gemm(pl, opA, opB, M, N, K, A, lda, B, ldb, C, ldc, s.ws.p, stream, ev ? ev[0] : nullptr,
+                            ev ? ev[1] : nullptr),
+            "crt gemm launch");
     f4j_split_report rep{};
+    rep.product_ms = -1.0;
     rep.precision = F4J_PREC_SPLIT_INT8;
     rep.slices = n;
     rep.products = n;
@@ -314,6 +330,11 @@ int run_crt(int opA, int opB, int64_t M, int64_t N, int64_t K, const double *A, 
         }
     }
     F4J_HIP(hipStreamSynchronize(stream), "stream synchronize");
+    if (ev) {
+        float ms = 0.f;
+        F4J_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]), "hipEventElapsedTime");
+        rep.product_ms = ms;
+    }
     t_split = rep;
     return F4J_OK;
 }
@@ -432,6 +453,9 @@ int config_set_locked(Config &c, const std::string &key, const std::string &valu
     } else if (key == "splitFallback") {
         if (value != "0" && value != "1") return fail(F4J_ERR_INVALID_ARG, "splitFallback: expected 0 or 1");
         c.split_fallback = value == "1";
+    } else if (key == "splitTiming") {
+        if (value != "0" && value != "1") return fail(F4J_ERR_INVALID_ARG, "splitTiming: expected 0 or 1");
+        c.split_timing = value == "1";
     } else if (key == "defaultSlices") {
         const long v = std::strtol(value.c_str(), &end, 10);
         if (end == value.c_str() || *end || v < 0 || v > CRT_MAX_MODULI)
@@ -476,6 +500,11 @@ int f4j_shutdown(void) {
             F4J_HIP(hipFree(b->p), "hipFree");
             b->p = nullptr;
             b->cap = 0;
+        }
+    for (hipEvent_t &e : s.tev)
+        if (e) {
+            F4J_HIP(hipEventDestroy(e), "hipEventDestroy");
+            e = nullptr;
         }
     F4J_HIP(hipStreamDestroy(s.stream), "hipStreamDestroy");
     s.stream = nullptr;
@@ -609,6 +638,7 @@ int f4j_config_get(const char *key, char *buf, size_t buflen) {
             std::snprintf(b, sizeof(b), "%.17g", s.cfg.split_tolerance);
             v = b;
         } else if (k == "splitFallback") v = std::to_string(s.cfg.split_fallback);
+        else if (k == "splitTiming") v = std::to_string(s.cfg.split_timing);
         else return fail(F4J_ERR_INVALID_ARG, "unknown configuration key '" + k + "'");
     }
     std::snprintf(buf, buflen, "%s", v.c_str());

@@ -49,9 +49,11 @@ size_t split_residual_bytes(int64_t N, int64_t K, int rows);
 
 // Enqueue exponents, digit slices and slice products; C is overwritten.
 // `ws` must hold plan.bytes bytes.  op codes: 0 = N, 1 = T.
+// ev_begin / ev_end (optional) are recorded on `stream` around the slice products.
 hipError_t launch_split_gemm(const SplitPlan &pl, int opA, int opB, int64_t M, int64_t N, int64_t K, const double *A,
                              int64_t lda, const double *B, int64_t ldb, double *C, int64_t ldc, void *ws,
-                             hipStream_t stream);
+                             hipStream_t stream,
+                             hipEvent_t ev_begin = nullptr, hipEvent_t ev_end = nullptr);
 
 // Enqueue the FP64 residual check of `rows` evenly spaced rows of C; the
 // maximum relative error (as double bits) is left at ws + plan.off_err.

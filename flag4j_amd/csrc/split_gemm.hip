@@ -445,7 +445,7 @@ SplitPlan plan_split(int64_t M, int64_t N, int64_t K, int slices, int carrier, i
 
 hipError_t launch_split_gemm(const SplitPlan &pl, int opA, int opB, int64_t M, int64_t N, int64_t K, const double *A,
                              int64_t lda, const double *B, int64_t ldb, double *C, int64_t ldc, void *ws,
-                             hipStream_t stream) {
+                             hipStream_t stream, hipEvent_t ev_begin, hipEvent_t ev_end) {
     char *w = (char *)ws;
     void *sa = w + pl.off_slices_a, *sb = w + pl.off_slices_b;
     int *ea = (int *)(w + pl.off_exp_a), *eb = (int *)(w + pl.off_exp_b);
@@ -467,6 +467,7 @@ hipError_t launch_split_gemm(const SplitPlan &pl, int opA, int opB, int64_t M, i
     if (err != hipSuccess) return err;
 
     // slice products, smallest weights first
+    if (ev_begin) hipEventRecord(ev_begin, stream);
     const int R = split_radix(pl.carrier);
     bool first = true;
     for (const SplitGroup &g : pl.groups) {
@@ -501,6 +502,7 @@ hipError_t launch_split_gemm(const SplitPlan &pl, int opA, int opB, int64_t M, i
             hipLaunchKernelGGL((slice_gemm_kernel<__bf16, 256>), dim3(blocks), dim3(256), 0, stream, a);
         first = false;
     }
+    if (ev_end) hipEventRecord(ev_end, stream);
     return hipGetLastError();
 }
 

@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define F4J_ABI_VERSION 1
+#define F4J_ABI_VERSION 2
 
 typedef enum f4j_status {
     F4J_OK = 0,
@@ -163,6 +163,8 @@ typedef struct f4j_split_report {
     int launches;         /* slice-product kernel launches */
     int checked_rows;     /* rows verified by the residual check (0 = check off) */
     double max_rel_err;   /* max over the checked rows of |C - AB| / (|A||B|); -1 if not checked */
+    double product_ms;    /* time of the slice-product launches, from HIP events on the call's stream,
+                             when splitTiming = 1; -1 otherwise */
 } f4j_split_report;
 int f4j_last_split_report(f4j_split_report *out);
 
@@ -174,7 +176,7 @@ int f4j_last_split_report(f4j_split_report *out);
  * tileConfig (-1 = per shape bucket, else a fixed FP64 tile id), splitCheckRows (rows of C the
  * FP64 residual check verifies after a split product; 0 = off), splitTolerance (largest accepted
  * |C - AB| / (|A||B|)), splitFallback (1 = recompute in FP64 when the check fails, 0 = return
- * F4J_ERR_ACCURACY). */
+ * F4J_ERR_ACCURACY), splitTiming (1 = report the slice-product time in f4j_split_report). */
 int f4j_config_set(const char *key, const char *value);
 int f4j_config_get(const char *key, char *buf, size_t buflen);
 int f4j_config_load(const char *path);

@@ -30,7 +30,7 @@ def test_exports_every_declared_symbol():
 
 
 def test_abi_version():
-    assert _native.load().f4j_abi_version() == 1
+    assert _native.load().f4j_abi_version() == 2
 
 
 def _dgemm(*args):
@@ -109,6 +109,11 @@ def test_config_keys_validated():
     assert L.f4j_config_set(b"noSuchKey", b"1") == _native.ERR_INVALID_ARG
     assert L.f4j_config_set(b"smallThreshold", b"abc") == _native.ERR_INVALID_ARG
     assert L.f4j_config_set(b"defaultPrecision", b"fp128") == _native.ERR_INVALID_ARG
+    assert L.f4j_config_set(b"splitTiming", b"2") == _native.ERR_INVALID_ARG
+    _native.check(L.f4j_config_set(b"splitTiming", b"1"))
+    _native.check(L.f4j_config_get(b"splitTiming", buf, 32))
+    assert buf.value == b"1"
+    _native.check(L.f4j_config_set(b"splitTiming", b"0"))
     _native.check(L.f4j_config_get(b"smallThreshold", buf, 32))
     old = buf.value
     try:

@@ -234,3 +234,17 @@ def test_crt_rejects_unrepresentable_k():
     st = L.load().f4j_dgemm(0, 0, 1, 1, 1 << 17, A.ctypes.data, 1 << 17, B.ctypes.data, 1, C.ctypes.data, 1,
                             L.PREC["split_int8"], 0)
     assert st == L.ERR_INVALID_ARG
+
+
+@pytest.mark.parametrize("prec,S", [("split_bf16", 3), ("split_int8", 0)])
+def test_product_timing_report(prec, S):
+    """splitTiming = 1 reports the product launches' event time; 0 reports -1."""
+    A, B = rng.uniform(29, 512, 512), rng.uniform(30, 512, 512)
+    L.config_set("splitTiming", 1)
+    try:
+        _, rep = split_mult(A, B, prec, S)
+        assert 0.0 < rep.product_ms < 1000.0
+    finally:
+        L.config_set("splitTiming", 0)
+    _, rep = split_mult(A, B, prec, S)
+    assert rep.product_ms == -1.0
