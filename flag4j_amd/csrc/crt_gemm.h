@@ -28,6 +28,7 @@ struct CrtTables {
 struct CrtPlan {
     CrtTables tb{};
     int64_t Mp = 0, Np = 0, Kp = 0;      // padded sizes of the residue slices
+    int64_t plane = 0;                   // bytes between the product residues of two moduli (M N, 256-aligned)
     size_t off_ra = 0, off_rb = 0, off_res = 0, off_ea = 0, off_eb = 0, off_err = 0, off_part = 0, bytes = 0;
 };
 

@@ -13,7 +13,7 @@
 //    per extra modulus and may exceed 53, which helps rows whose entries span a
 //    wide range (the small ones keep t - (eA_i - e(a_ik)) bits).
 // 2. Residues.  For coprime moduli m_l <= 256, A'_ik mod m_l is stored as a
-//    signed 8-bit residue (|r| <= 128).
+//    signed 8-bit residue (in [-128, 127]).
 // 3. Products.  C_l = A_l B_l^T on v_mfma_i32_32x32x32_i8: |C_l| <= K 2^14,
 //    exact in int32 for K < 2^17, then reduced to C_l mod m_l (one byte).
 // 4. Reconstruction.  X = sum_l r_l W_l = A'B' (mod M), M = prod m_l; the
@@ -141,67 +141,91 @@ int t_for(int n, int64_t K) {
 // Residue slices: out[l][x][k] = A'(x, k) mod m_l as a signed byte, k contiguous.
 // ------------------------------------------------------------------------
 
-// a is an integer with |a| < 2^106.  First round: q = rint(a / m) carries a
-// relative error <= 2^-52, so a - q m is an integer below 2^54 |a| / 2^106 and
-// fma forms it exactly; the second round brings it to |r| <= m.
-__device__ inline signed char residue(double a, int m, double minv) {
-    double r = fma(-rint(a * minv), (double)m, a);
-    r = fma(-rint(r * minv), (double)m, r);
-    const double hi = m == 256 ? 127.0 : (double)((m - 1) / 2), lo = m == 256 ? -128.0 : -hi;
-    if (r > hi) r -= m;
-    if (r < lo) r += m;
-    return (signed char)(int)r;
+// v is an integer.  Signed residue of v modulo m, in [-128, 127] for m = 256
+// and [-(m-1)/2, (m-1)/2] for odd m.
+//   |v| < 2^58: q = rint(v / m) from one fma against 1.5 2^52 (the product
+//   v (1/m) is exact inside the fma; 1/m rounded moves it by < 1/4), so
+//   v - q m is in (-m, m) and formed exactly; one correction each side.
+//   2^58 <= |v| < 2^100 (t > 58; 20 moduli give at most t = 75): a first
+//   round with rint leaves v - q m below m + 2^-52 |v| < 2^49 in magnitude
+//   (an integer, formed exactly), then the step above.
+template <bool WIDE>
+__device__ inline int residue(double v, int m, double minv) {
+    constexpr double MAGIC = 6755399441055744.0;              // 1.5 2^52
+    if (WIDE) v = fma(-rint(v * minv), (double)m, v);
+    const double q = fma(v, minv, MAGIC) - MAGIC;
+    int r = (int)fma(-q, (double)m, v);
+    const int hi = (m - 1) >> 1;                              // 127 for m = 256
+    r = r > hi ? r - m : r;
+    r = r < -(m >> 1) ? r + m : r;
+    return r;
 }
 
 __device__ inline double scaled_int(double v, int e, int t) { return e == INT_MIN ? 0.0 : rint(ldexp(v, t - e)); }
 
-// Lines contiguous: thread handles 16 consecutive k of one line.
+__device__ inline unsigned pack4(int a, int b, int c, int d) {
+    return (unsigned)(a & 255) | ((unsigned)(b & 255) << 8) | ((unsigned)(c & 255) << 16) | ((unsigned)d << 24);
+}
+
+// Lines contiguous (line x at X + x ld, k contiguous): thread -> 4 consecutive
+// k of one line, one 4-byte store per modulus (a wave writes 256 contiguous bytes).
+template <bool WIDE>
 __global__ __launch_bounds__(256) void crt_split_lines_kernel(const double *X, int64_t ld, int64_t nx, int64_t K,
                                                               const int *e, signed char *out, int64_t Xp, int64_t Kp,
                                                               CrtTables tb) {
     const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    const int64_t k16 = Kp / 16;
-    if (g >= Xp * k16) return;
-    const int64_t x = g / k16, k0 = (g % k16) * 16;
+    const int64_t k4 = Kp / 4;
+    if (g >= Xp * k4) return;
+    const int64_t x = g / k4, k0 = (g - x * k4) * 4;
     const int ex = x < nx ? e[x] : INT_MIN;
-    double v[16];
+    double v[4];
 #pragma unroll
-    for (int u = 0; u < 16; ++u) {
+    for (int u = 0; u < 4; ++u) {
         const int64_t k = k0 + u;
         v[u] = scaled_int((x < nx && k < K) ? X[x * ld + k] : 0.0, ex, tb.t);
     }
+    unsigned *dst = (unsigned *)(out + x * Kp + k0);
+    const int64_t lstride = Xp * Kp / 4;
     for (int l = 0; l < tb.n; ++l) {
-        signed char r[16];
-#pragma unroll
-        for (int u = 0; u < 16; ++u) r[u] = residue(v[u], tb.m[l], tb.minv[l]);
-        *(int4 *)(out + (int64_t)l * Xp * Kp + x * Kp + k0) = __builtin_bit_cast(int4, r);
+        const int m = tb.m[l];
+        const double mi = tb.minv[l];
+        dst[l * lstride] = pack4(residue<WIDE>(v[0], m, mi), residue<WIDE>(v[1], m, mi), residue<WIDE>(v[2], m, mi),
+                                 residue<WIDE>(v[3], m, mi));
     }
 }
 
-// Lines strided (stored K x X): a 128 (k) x 32 (x) block is transposed through LDS.
+// Lines strided (stored K x X, line x is column x): a 64 (k) x 64 (x) block is
+// read along x (coalesced), transposed through LDS (row stride 65 doubles:
+// conflict-free writes, 2-way reads = the 64-lane minimum) and written as
+// 4-byte groups of k; thread -> 4 (column, 4-k group) items.
+template <bool WIDE>
 __global__ __launch_bounds__(256) void crt_split_cols_kernel(const double *X, int64_t ld, int64_t nx, int64_t K,
                                                              const int *e, signed char *out, int64_t Xp, int64_t Kp,
                                                              CrtTables tb) {
-    __shared__ double tile[32][129];
-    const int64_t x0 = (int64_t)blockIdx.x * 32, k0 = (int64_t)blockIdx.y * 128;
-    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
-    for (int r = ty; r < 128; r += 8) {
+    __shared__ double tile[64][65];
+    const int64_t x0 = (int64_t)blockIdx.x * 64, k0 = (int64_t)blockIdx.y * 64;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    for (int r = ty; r < 64; r += 4) {
         const int64_t k = k0 + r, x = x0 + tx;
         tile[tx][r] = (k < K && x < nx) ? X[k * ld + x] : 0.0;
     }
     __syncthreads();
-    const int xl = threadIdx.x >> 3, kg = (threadIdx.x & 7) * 16;
-    const int64_t x = x0 + xl;
-    if (x >= Xp) return;
-    const int ex = x < nx ? e[x] : INT_MIN;
-    double v[16];
+    const int64_t lstride = Xp * Kp / 4;
+#pragma unroll 1
+    for (int it = 0; it < 4; ++it) {
+        const int xl = (threadIdx.x >> 4) + 16 * it, c = threadIdx.x & 15;
+        const int64_t x = x0 + xl;
+        const int ex = x < nx ? e[x] : INT_MIN;
+        double v[4];
 #pragma unroll
-    for (int u = 0; u < 16; ++u) v[u] = scaled_int(tile[xl][kg + u], ex, tb.t);
-    for (int l = 0; l < tb.n; ++l) {
-        signed char r[16];
-#pragma unroll
-        for (int u = 0; u < 16; ++u) r[u] = residue(v[u], tb.m[l], tb.minv[l]);
-        *(int4 *)(out + (int64_t)l * Xp * Kp + x * Kp + k0 + kg) = __builtin_bit_cast(int4, r);
+        for (int u = 0; u < 4; ++u) v[u] = scaled_int(tile[xl][4 * c + u], ex, tb.t);
+        unsigned *dst = (unsigned *)(out + x * Kp + k0 + 4 * c);
+        for (int l = 0; l < tb.n; ++l) {
+            const int m = tb.m[l];
+            const double mi = tb.minv[l];
+            dst[l * lstride] = pack4(residue<WIDE>(v[0], m, mi), residue<WIDE>(v[1], m, mi),
+                                     residue<WIDE>(v[2], m, mi), residue<WIDE>(v[3], m, mi));
+        }
     }
 }
 
@@ -221,7 +245,8 @@ constexpr int CSTAGE = CBN * CBK;                 // bytes per B stage (rows of 
 struct CrtGemmArgs {
     const signed char *A, *B;                     // [n][Mp][Kp], [n][Np][Kp]
     int64_t Mp, Np, Kp, M, N;
-    unsigned char *R;                             // [n][M][N]
+    unsigned char *R;                             // [n][plane], each plane M x N
+    int64_t plane;
 };
 
 typedef __attribute__((address_space(3))) void lds_void;
@@ -345,7 +370,7 @@ __global__ __launch_bounds__(256, 2) void crt_gemm_kernel(CrtGemmArgs a, CrtTabl
     // below 1/m), then one correction.
     const int m = tb.m[l];
     const double minv = tb.minv[l];
-    unsigned char *R = a.R + (int64_t)l * a.M * a.N;
+    unsigned char *R = a.R + (int64_t)l * a.plane;
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -369,42 +394,71 @@ __global__ __launch_bounds__(256, 2) void crt_gemm_kernel(CrtGemmArgs a, CrtTabl
 // Reconstruction: C_ij = 2^(eA_i + eB_j - 2t) * CRT(R[.][i][j]).
 // ------------------------------------------------------------------------
 
-__global__ __launch_bounds__(256) void crt_reconstruct_kernel(const unsigned char *R, int64_t M, int64_t N,
-                                                              const int *eA, const int *eB, double *C, int64_t ldc,
-                                                              CrtTables tb) {
-    const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (g >= M * N) return;
-    const int64_t i = g / N, j = g % N;
-    const int ea = eA[i], eb = eB[j];
-    double out = 0.0;
-    if (ea != INT_MIN && eb != INT_MIN) {
-        double S[CRT_MAX_CHUNKS] = {0.0, 0.0, 0.0, 0.0};
-        for (int l = 0; l < tb.n; ++l) {
-            const double r = (double)R[(int64_t)l * M * N + g];
+constexpr int RV = 8;                             // product entries per thread
+
+// Thread -> RV consecutive entries of the flattened M x N product: one 8-byte
+// load per modulus (planes are 256-byte aligned) instead of RV byte loads.
+__global__ __launch_bounds__(256) void crt_reconstruct_kernel(const unsigned char *R, int64_t plane, int64_t M,
+                                                              int64_t N, const int *eA, const int *eB, double *C,
+                                                              int64_t ldc, CrtTables tb) {
+    const int64_t g0 = ((int64_t)blockIdx.x * 256 + threadIdx.x) * RV;
+    const int64_t total = M * N;
+    if (g0 >= total) return;
+    const int cnt = (int)min<int64_t>(RV, total - g0);
+    double S[CRT_MAX_CHUNKS][RV];
 #pragma unroll
-            for (int p = 0; p < CRT_MAX_CHUNKS; ++p) S[p] = fma(r, tb.w[l][p], S[p]);   // exact integers
+    for (int p = 0; p < CRT_MAX_CHUNKS; ++p)
+#pragma unroll
+        for (int e = 0; e < RV; ++e) S[p][e] = 0.0;
+    for (int l = 0; l < tb.n; ++l) {
+        const unsigned char *src = R + (int64_t)l * plane + g0;
+        unsigned char r[RV];
+        if (cnt == RV) {
+            const uint2 v = *(const uint2 *)src;
+            __builtin_memcpy(r, &v, RV);
+        } else {
+#pragma unroll
+            for (int e = 0; e < RV; ++e) r[e] = e < cnt ? src[e] : 0;
         }
-        double approx = 0.0;
 #pragma unroll
-        for (int p = 0; p < CRT_MAX_CHUNKS; ++p) approx = fma(S[p], tb.c[p], approx);
-        const double q = rint(approx);
-        // D_p scale_p are exact; the lower chunks are summed in double-double
-        // (TwoSum) because they cancel against the top chunk, so C' is rounded
-        // once, relative to itself.
-        double hi = 0.0, lo = 0.0;
+        for (int p = 0; p < CRT_MAX_CHUNKS; ++p)
 #pragma unroll
-        for (int p = CRT_MAX_CHUNKS - 1; p >= 1; --p) {
-            const double v = fma(-q, tb.mu[p], S[p]) * tb.scale[p];
-            const double s = hi + v, z = s - hi;
-            lo += (hi - (s - z)) + (v - z);
-            hi = s;
-        }
-        const double top = fma(-q, tb.mu[0], S[0]) * tb.scale[0];
-        const double s = top + hi, z = s - top;
-        const double acc = s + (((top - (s - z)) + (hi - z)) + lo);
-        out = ldexp(acc, ea + eb - 2 * tb.t);
+            for (int e = 0; e < RV; ++e) S[p][e] = fma((double)r[e], tb.w[l][p], S[p][e]);   // exact integers
     }
-    C[i * ldc + j] = out;
+    int64_t i = g0 / N, j = g0 - i * N;
+#pragma unroll
+    for (int e = 0; e < RV; ++e) {
+        if (e < cnt) {
+            const int ea = eA[i], eb = eB[j];
+            double out = 0.0;
+            if (ea != INT_MIN && eb != INT_MIN) {
+                double approx = 0.0;
+#pragma unroll
+                for (int p = 0; p < CRT_MAX_CHUNKS; ++p) approx = fma(S[p][e], tb.c[p], approx);
+                const double q = rint(approx);
+                // D_p scale_p are exact; the lower chunks are summed in
+                // double-double (TwoSum) because they cancel against the top
+                // chunk, so C' is rounded once, relative to itself.
+                double hi = 0.0, lo = 0.0;
+#pragma unroll
+                for (int p = CRT_MAX_CHUNKS - 1; p >= 1; --p) {
+                    const double v = fma(-q, tb.mu[p], S[p][e]) * tb.scale[p];
+                    const double sm = hi + v, z = sm - hi;
+                    lo += (hi - (sm - z)) + (v - z);
+                    hi = sm;
+                }
+                const double top = fma(-q, tb.mu[0], S[0][e]) * tb.scale[0];
+                const double sm = top + hi, z = sm - top;
+                const double acc = sm + (((top - (sm - z)) + (hi - z)) + lo);
+                out = ldexp(acc, ea + eb - 2 * tb.t);
+            }
+            C[i * ldc + j] = out;
+        }
+        if (++j == N) {
+            j = 0;
+            ++i;
+        }
+    }
 }
 
 }  // namespace
@@ -459,8 +513,9 @@ bool plan_crt(int64_t M, int64_t N, int64_t K, int n, int check_rows, CrtPlan *p
     auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
     pl.off_ra = 0;
     pl.off_rb = al((size_t)n * pl.Mp * pl.Kp);
+    pl.plane = (int64_t)al((size_t)M * N);
     pl.off_res = pl.off_rb + al((size_t)n * pl.Np * pl.Kp);
-    pl.off_ea = pl.off_res + al((size_t)n * M * N);
+    pl.off_ea = pl.off_res + (size_t)n * pl.plane;
     pl.off_eb = pl.off_ea + al((size_t)M * 4);
     pl.off_err = pl.off_eb + al((size_t)N * 4);
     pl.off_part = pl.off_err + 256;
@@ -478,26 +533,37 @@ hipError_t launch_crt_gemm(const CrtPlan &pl, int opA, int opB, int64_t M, int64
     int *ea = (int *)(w + pl.off_ea), *eb = (int *)(w + pl.off_eb);
     hipError_t e = launch_exponents(opA, opB, M, N, K, A, lda, B, ldb, ea, eb, stream);
     if (e != hipSuccess) return e;
+    const bool wide = pl.tb.t > 58;
     auto split = [&](const double *X, int64_t ld, bool lines, int64_t nx, int64_t Xp, const int *ex, signed char *out) {
         if (lines) {
-            const int64_t items = Xp * (pl.Kp / 16);
-            hipLaunchKernelGGL(crt_split_lines_kernel, dim3((unsigned)((items + 255) / 256)), dim3(256), 0, stream, X,
-                               ld, nx, K, ex, out, Xp, pl.Kp, pl.tb);
+            const int64_t items = Xp * (pl.Kp / 4);
+            const dim3 grid((unsigned)((items + 255) / 256));
+            if (wide)
+                hipLaunchKernelGGL(crt_split_lines_kernel<true>, grid, dim3(256), 0, stream, X, ld, nx, K, ex, out, Xp,
+                                   pl.Kp, pl.tb);
+            else
+                hipLaunchKernelGGL(crt_split_lines_kernel<false>, grid, dim3(256), 0, stream, X, ld, nx, K, ex, out,
+                                   Xp, pl.Kp, pl.tb);
         } else {
-            hipLaunchKernelGGL(crt_split_cols_kernel, dim3((unsigned)(Xp / 32), (unsigned)(pl.Kp / 128)), dim3(256), 0,
-                               stream, X, ld, nx, K, ex, out, Xp, pl.Kp, pl.tb);
+            const dim3 grid((unsigned)(Xp / 64), (unsigned)(pl.Kp / 64));
+            if (wide)
+                hipLaunchKernelGGL(crt_split_cols_kernel<true>, grid, dim3(256), 0, stream, X, ld, nx, K, ex, out, Xp,
+                                   pl.Kp, pl.tb);
+            else
+                hipLaunchKernelGGL(crt_split_cols_kernel<false>, grid, dim3(256), 0, stream, X, ld, nx, K, ex, out, Xp,
+                                   pl.Kp, pl.tb);
         }
     };
     split(A, lda, opA == 0, M, pl.Mp, ea, ra);
     split(B, ldb, opB == 1, N, pl.Np, eb, rb);
-    CrtGemmArgs a{ra, rb, pl.Mp, pl.Np, pl.Kp, M, N, res};
+    CrtGemmArgs a{ra, rb, pl.Mp, pl.Np, pl.Kp, M, N, res, pl.plane};
     if (ev_begin) hipEventRecord(ev_begin, stream);
     hipLaunchKernelGGL(crt_gemm_kernel, dim3((unsigned)((pl.Mp / CBM) * (pl.Np / CBN)), (unsigned)pl.tb.n), dim3(256),
                        0, stream, a, pl.tb);
     if (ev_end) hipEventRecord(ev_end, stream);
-    const int64_t items = M * N;
-    hipLaunchKernelGGL(crt_reconstruct_kernel, dim3((unsigned)((items + 255) / 256)), dim3(256), 0, stream, res, M, N,
-                       ea, eb, C, ldc, pl.tb);
+    const int64_t threads = (M * N + RV - 1) / RV;
+    hipLaunchKernelGGL(crt_reconstruct_kernel, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, stream, res,
+                       pl.plane, M, N, ea, eb, C, ldc, pl.tb);
     return hipGetLastError();
 }
 
