@@ -33,6 +33,7 @@
 #include <vector>
 
 #include "crt_gemm.h"
+#include "crt_layout.h"
 #include "f4j_internal.h"
 #include "gemm_tile.h"
 #include "split_gemm.h"
@@ -138,7 +139,7 @@ int t_for(int n, int64_t K) {
 }
 
 // ------------------------------------------------------------------------
-// Residue slices: out[l][x][k] = A'(x, k) mod m_l as a signed byte, k contiguous.
+// Residue slices: A'(x, k) mod m_l as signed bytes, fragment-major (below).
 // ------------------------------------------------------------------------
 
 // v is an integer.  Signed residue of v modulo m, in [-128, 127] for m = 256
@@ -167,65 +168,53 @@ __device__ inline unsigned pack4(int a, int b, int c, int d) {
     return (unsigned)(a & 255) | ((unsigned)(b & 255) << 8) | ((unsigned)(c & 255) << 16) | ((unsigned)d << 24);
 }
 
-// Lines contiguous (line x at X + x ld, k contiguous): thread -> 4 consecutive
-// k of one line, one 4-byte store per modulus (a wave writes 256 contiguous bytes).
-template <bool WIDE>
-__global__ __launch_bounds__(256) void crt_split_lines_kernel(const double *X, int64_t ld, int64_t nx, int64_t K,
-                                                              const int *e, signed char *out, int64_t Xp, int64_t Kp,
-                                                              CrtTables tb) {
-    const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    const int64_t k4 = Kp / 4;
-    if (g >= Xp * k4) return;
-    const int64_t x = g / k4, k0 = (g - x * k4) * 4;
-    const int ex = x < nx ? e[x] : INT_MIN;
-    double v[4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-        const int64_t k = k0 + u;
-        v[u] = scaled_int((x < nx && k < K) ? X[x * ld + k] : 0.0, ex, tb.t);
+// One block: 32 lines x 128 k.  The FP64 block is staged through LDS with
+// coalesced reads (along k when lines are contiguous, along x when they are
+// strided); thread t then takes line t % 32, 16 k at 16 ((t / 32) % 2) + 32 (t / 64)
+// and writes one 16-byte fragment piece per modulus (a wave writes 1 KB).
+template <bool WIDE, bool LINES>
+__global__ __launch_bounds__(256) void crt_split_kernel(const double *X, int64_t ld, int64_t nx, int64_t K,
+                                                        const int *e, signed char *out, int64_t Xp, int64_t Kp,
+                                                        CrtTables tb) {
+    __shared__ double tile[32][129];
+    const int64_t k0 = (int64_t)blockIdx.x * 128, x0 = (int64_t)blockIdx.y * 32;
+    const int t = threadIdx.x;
+    if (LINES) {
+        const int c = t & 127;
+        for (int r = t >> 7; r < 32; r += 2) {
+            const int64_t x = x0 + r, k = k0 + c;
+            tile[r][c] = (x < nx && k < K) ? X[x * ld + k] : 0.0;
+        }
+    } else {
+        const int r = t & 31;
+        for (int c = t >> 5; c < 128; c += 8) {
+            const int64_t x = x0 + r, k = k0 + c;
+            tile[r][c] = (x < nx && k < K) ? X[k * ld + x] : 0.0;
+        }
     }
-    unsigned *dst = (unsigned *)(out + x * Kp + k0);
-    const int64_t lstride = Xp * Kp / 4;
+    __syncthreads();
+    const int r = crt::split_line(t), kb = crt::split_k(t);
+    const int64_t x = x0 + r;
+    const int ex = x < nx ? e[x] : INT_MIN;
+    double v[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) v[u] = scaled_int(tile[r][kb + u], ex, tb.t);
+    const int64_t KC = Kp >> 5;
+    int4 *dst = (int4 *)(out + crt::frag_offset(x, k0 + kb, KC));
+    const int64_t lstride = Xp * Kp / 16;
     for (int l = 0; l < tb.n; ++l) {
         const int m = tb.m[l];
         const double mi = tb.minv[l];
-        dst[l * lstride] = pack4(residue<WIDE>(v[0], m, mi), residue<WIDE>(v[1], m, mi), residue<WIDE>(v[2], m, mi),
-                                 residue<WIDE>(v[3], m, mi));
-    }
-}
-
-// Lines strided (stored K x X, line x is column x): a 64 (k) x 64 (x) block is
-// read along x (coalesced), transposed through LDS (row stride 65 doubles:
-// conflict-free writes, 2-way reads = the 64-lane minimum) and written as
-// 4-byte groups of k; thread -> 4 (column, 4-k group) items.
-template <bool WIDE>
-__global__ __launch_bounds__(256) void crt_split_cols_kernel(const double *X, int64_t ld, int64_t nx, int64_t K,
-                                                             const int *e, signed char *out, int64_t Xp, int64_t Kp,
-                                                             CrtTables tb) {
-    __shared__ double tile[64][65];
-    const int64_t x0 = (int64_t)blockIdx.x * 64, k0 = (int64_t)blockIdx.y * 64;
-    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
-    for (int r = ty; r < 64; r += 4) {
-        const int64_t k = k0 + r, x = x0 + tx;
-        tile[tx][r] = (k < K && x < nx) ? X[k * ld + x] : 0.0;
-    }
-    __syncthreads();
-    const int64_t lstride = Xp * Kp / 4;
-#pragma unroll 1
-    for (int it = 0; it < 4; ++it) {
-        const int xl = (threadIdx.x >> 4) + 16 * it, c = threadIdx.x & 15;
-        const int64_t x = x0 + xl;
-        const int ex = x < nx ? e[x] : INT_MIN;
-        double v[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) v[u] = scaled_int(tile[xl][4 * c + u], ex, tb.t);
-        unsigned *dst = (unsigned *)(out + x * Kp + k0 + 4 * c);
-        for (int l = 0; l < tb.n; ++l) {
-            const int m = tb.m[l];
-            const double mi = tb.minv[l];
-            dst[l * lstride] = pack4(residue<WIDE>(v[0], m, mi), residue<WIDE>(v[1], m, mi),
-                                     residue<WIDE>(v[2], m, mi), residue<WIDE>(v[3], m, mi));
-        }
+        int4 w;
+        w.x = (int)pack4(residue<WIDE>(v[0], m, mi), residue<WIDE>(v[1], m, mi), residue<WIDE>(v[2], m, mi),
+                         residue<WIDE>(v[3], m, mi));
+        w.y = (int)pack4(residue<WIDE>(v[4], m, mi), residue<WIDE>(v[5], m, mi), residue<WIDE>(v[6], m, mi),
+                         residue<WIDE>(v[7], m, mi));
+        w.z = (int)pack4(residue<WIDE>(v[8], m, mi), residue<WIDE>(v[9], m, mi), residue<WIDE>(v[10], m, mi),
+                         residue<WIDE>(v[11], m, mi));
+        w.w = (int)pack4(residue<WIDE>(v[12], m, mi), residue<WIDE>(v[13], m, mi), residue<WIDE>(v[14], m, mi),
+                         residue<WIDE>(v[15], m, mi));
+        dst[l * lstride] = w;
     }
 }
 
@@ -238,12 +227,10 @@ __global__ __launch_bounds__(256) void crt_split_cols_kernel(const double *X, in
 typedef int i32x4 __attribute__((ext_vector_type(4)));
 typedef int i32x16 __attribute__((ext_vector_type(16)));
 
-constexpr int CBM = 256, CBN = 128, CBK = 128;    // tile, k slice in bytes (= int8 elements)
-constexpr int CSTAGES = 3;                        // B slices in flight through LDS
-constexpr int CSTAGE = CBN * CBK;                 // bytes per B stage (rows of 128 bytes, chunk-swizzled)
+constexpr int CBM = crt::BM, CBN = crt::BN, CBK = crt::BK;
 
 struct CrtGemmArgs {
-    const signed char *A, *B;                     // [n][Mp][Kp], [n][Np][Kp]
+    const signed char *A, *B;                     // fragment-major residues, n slices of Mp x Kp and Np x Kp
     int64_t Mp, Np, Kp, M, N;
     unsigned char *R;                             // [n][plane], each plane M x N
     int64_t plane;
@@ -251,23 +238,18 @@ struct CrtGemmArgs {
 
 typedef __attribute__((address_space(3))) void lds_void;
 
-// 16-byte chunk c of B row r sits at LDS chunk c ^ swz(r): 16 consecutive rows
-// read at one chunk index hit 16 distinct 4-bank groups (and 8 consecutive rows
-// 8 distinct ones), so fragment reads are conflict free.
-__device__ __forceinline__ int swz(int r) { return (r & 7) ^ ((r >> 3) & 1); }
-
 // vmcnt(n) only (gfx9 encoding: expcnt and lgkmcnt at their maxima).
 #define F4J_WAIT_VMCNT(n) __builtin_amdgcn_s_waitcnt(((n) & 15) | (7 << 4) | (15 << 8) | (((n) >> 4) << 14))
 
 // Block tile 256 x 128, four waves of 64 x 128 (2 x 4 MFMA blocks, 128 int32
 // accumulators per lane).  Each wave reads its own 64 rows of A straight into
-// registers (no other wave needs them), three k slices deep; B is shared
-// through a three-stage LDS ring filled by buffer loads that write LDS
-// directly.  Loads run two slices ahead of the MFMAs, with one barrier per
-// slice.  MFMA step ks uses bytes 32 ks + 16 (lane / 32) + v of the k slice
-// for A and B alike.
+// registers (no other wave needs them); B is shared through a three-stage LDS
+// ring filled by buffer loads that write LDS directly.  B runs two 128-byte k
+// slices ahead of the MFMAs and A one, with one barrier per slice.  With the
+// fragment-major layout every load (and every LDS fragment read) moves 1 KB
+// of contiguous memory.
 __global__ __launch_bounds__(256, 2) void crt_gemm_kernel(CrtGemmArgs a, CrtTables tb) {
-    __shared__ __attribute__((aligned(1024))) signed char lds[CSTAGES * CSTAGE];
+    __shared__ __attribute__((aligned(1024))) signed char lds[crt::STAGES * crt::STAGE];
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);   // uniform: keeps descriptors scalar
     const int l = blockIdx.y;
@@ -284,41 +266,30 @@ __global__ __launch_bounds__(256, 2) void crt_gemm_kernel(CrtGemmArgs a, CrtTabl
 #pragma unroll
             for (int r = 0; r < 16; ++r) acc[i][j][r] = 0;
 
-    const signed char *pa = a.A + ((int64_t)l * a.Mp + m0) * a.Kp;
-    const signed char *pb = a.B + ((int64_t)l * a.Np + n0) * a.Kp;
-    const int fr = lane & 31, fh = lane >> 5;
-    const int kp = (int)a.Kp;
-    const int aoff = fr * kp + 16 * fh;
-    // B: instruction p of this wave fills LDS rows 8 (4 wave + p) .. +8; lane
-    // -> row + lane / 8, LDS chunk lane % 8, global chunk (lane % 8) ^ swz(row).
-    // swz(row) = ((lane / 8) & 7) ^ (p & 1): two lane offsets; the row step 8 p kp is uniform.
-    int boff[2];
-#pragma unroll
-    for (int q = 0; q < 2; ++q) boff[q] = (32 * wave + (lane >> 3)) * kp + 16 * ((lane & 7) ^ ((lane >> 3) & 7) ^ q);
-    int roff[4];                                   // fragment read offsets per step ks (row fr; + 4096 j)
-#pragma unroll
-    for (int ks = 0; ks < 4; ++ks) roff[ks] = fr * CBK + 16 * ((2 * ks + fh) ^ swz(fr));
+    // Offsets from crt_layout.h: slice t is k blocks 4 t .. 4 t + 3.
+    const int KC = (int)(a.Kp >> 5);
+    const signed char *pa = a.A + (int64_t)l * a.Mp * a.Kp;
+    const signed char *pb = a.B + (int64_t)l * a.Np * a.Kp;
     const int nt = (int)(a.Kp / CBK);
 
     i32x4 ra0[2][4], ra1[2][4];
     auto issue_a = [&](int t, i32x4 (&dst)[2][4]) {
-        const __amdgpu_buffer_rsrc_t rsa =
-            __builtin_amdgcn_make_buffer_rsrc((void *)(pa + (int64_t)t * CBK), (short)0, 0x7FFFFFF0, 0x00020000);
+        const __amdgpu_buffer_rsrc_t rsa = __builtin_amdgcn_make_buffer_rsrc(
+            (void *)(pa + crt::a_base(m0, t, KC)), (short)0, 0x7FFFFFF0, 0x00020000);
 #pragma unroll
         for (int i = 0; i < 2; ++i)
 #pragma unroll
             for (int ks = 0; ks < 4; ++ks)
                 dst[i][ks] = __builtin_bit_cast(
-                    i32x4, __builtin_amdgcn_raw_buffer_load_b128(rsa, aoff + 32 * ks, 32 * i * kp, 0));
+                    i32x4, __builtin_amdgcn_raw_buffer_load_b128(rsa, crt::a_voff(ks, lane), crt::a_soff(i, KC), 0));
     };
-    auto issue_b = [&](int slot, int t) {        // slice t into stage slot % 3
-        const __amdgpu_buffer_rsrc_t rsb =
-            __builtin_amdgcn_make_buffer_rsrc((void *)(pb + (int64_t)t * CBK), (short)0, 0x7FFFFFF0, 0x00020000);
-        signed char *st = lds + (slot % CSTAGES) * CSTAGE + wave * 4096;
+    auto issue_b = [&](int slot, int t) {        // slice t into stage slot % 3; wave w loads column block w
+        const __amdgpu_buffer_rsrc_t rsb = __builtin_amdgcn_make_buffer_rsrc(
+            (void *)(pb + crt::b_base(n0, t, KC)), (short)0, 0x7FFFFFF0, 0x00020000);
 #pragma unroll
         for (int p = 0; p < 4; ++p)
-            __builtin_amdgcn_raw_ptr_buffer_load_lds(rsb, (lds_void *)(st + 1024 * p), 16, boff[p & 1], 8 * p * kp,
-                                                     0, 0);
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(rsb, (lds_void *)(lds + crt::b_lds(slot, wave, p)), 16,
+                                                     crt::b_voff(lane), crt::b_soff(wave, p, KC), 0, 0);
     };
     // Issue order: B(0), B(1), A(0), then per step t: B(t + 2) (4 loads) and
     // A(t + 1) (8 loads).  Past the last slice the loads repeat slice nt - 1
@@ -336,15 +307,15 @@ __global__ __launch_bounds__(256, 2) void crt_gemm_kernel(CrtGemmArgs a, CrtTabl
         issue_b(t + 2, min(t + 2, nt - 1));
         issue_a(min(t + 1, nt - 1), nxt);
         __builtin_amdgcn_sched_barrier(0);
-        const signed char *st = lds + (t % CSTAGES) * CSTAGE;
         i32x4 fb[2][4];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) fb[0][j] = *(const i32x4 *)(st + 4096 * j + roff[0]);
+        for (int j = 0; j < 4; ++j) fb[0][j] = *(const i32x4 *)(lds + crt::b_frag(t, j, 0, lane));
 #pragma unroll
         for (int ks = 0; ks < CBK / 32; ++ks) {
             if (ks + 1 < CBK / 32) {
 #pragma unroll
-                for (int j = 0; j < 4; ++j) fb[(ks + 1) & 1][j] = *(const i32x4 *)(st + 4096 * j + roff[ks + 1]);
+                for (int j = 0; j < 4; ++j)
+                    fb[(ks + 1) & 1][j] = *(const i32x4 *)(lds + crt::b_frag(t, j, ks + 1, lane));
             }
             __builtin_amdgcn_sched_barrier(0);   // next step's fragment reads ahead of this step's MFMAs
 #pragma unroll
@@ -375,11 +346,11 @@ __global__ __launch_bounds__(256, 2) void crt_gemm_kernel(CrtGemmArgs a, CrtTabl
     for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            const int64_t col = n0 + 32 * j + fr;
+            const int64_t col = n0 + crt::acc_col(j, lane);
             if (col >= a.N) continue;
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
-                const int64_t row = m0 + 32 * i + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+                const int64_t row = m0 + crt::acc_row(i, r, lane);
                 if (row >= a.M) continue;
                 const int v = acc[i][j][r];
                 int rr = v - (int)floor((double)v * minv) * m;
@@ -535,23 +506,21 @@ hipError_t launch_crt_gemm(const CrtPlan &pl, int opA, int opB, int64_t M, int64
     if (e != hipSuccess) return e;
     const bool wide = pl.tb.t > 58;
     auto split = [&](const double *X, int64_t ld, bool lines, int64_t nx, int64_t Xp, const int *ex, signed char *out) {
+        const dim3 grid((unsigned)(pl.Kp / 128), (unsigned)(Xp / 32));
         if (lines) {
-            const int64_t items = Xp * (pl.Kp / 4);
-            const dim3 grid((unsigned)((items + 255) / 256));
             if (wide)
-                hipLaunchKernelGGL(crt_split_lines_kernel<true>, grid, dim3(256), 0, stream, X, ld, nx, K, ex, out, Xp,
-                                   pl.Kp, pl.tb);
+                hipLaunchKernelGGL((crt_split_kernel<true, true>), grid, dim3(256), 0, stream, X, ld, nx, K, ex, out,
+                                   Xp, pl.Kp, pl.tb);
             else
-                hipLaunchKernelGGL(crt_split_lines_kernel<false>, grid, dim3(256), 0, stream, X, ld, nx, K, ex, out,
+                hipLaunchKernelGGL((crt_split_kernel<false, true>), grid, dim3(256), 0, stream, X, ld, nx, K, ex, out,
                                    Xp, pl.Kp, pl.tb);
         } else {
-            const dim3 grid((unsigned)(Xp / 64), (unsigned)(pl.Kp / 64));
             if (wide)
-                hipLaunchKernelGGL(crt_split_cols_kernel<true>, grid, dim3(256), 0, stream, X, ld, nx, K, ex, out, Xp,
-                                   pl.Kp, pl.tb);
+                hipLaunchKernelGGL((crt_split_kernel<true, false>), grid, dim3(256), 0, stream, X, ld, nx, K, ex, out,
+                                   Xp, pl.Kp, pl.tb);
             else
-                hipLaunchKernelGGL(crt_split_cols_kernel<false>, grid, dim3(256), 0, stream, X, ld, nx, K, ex, out, Xp,
-                                   pl.Kp, pl.tb);
+                hipLaunchKernelGGL((crt_split_kernel<false, false>), grid, dim3(256), 0, stream, X, ld, nx, K, ex,
+                                   out, Xp, pl.Kp, pl.tb);
         }
     };
     split(A, lda, opA == 0, M, pl.Mp, ea, ra);

@@ -38,3 +38,26 @@ def test_kernel_index_math(emu, ka, kb, shape, cfg, vec):
     r = subprocess.run([emu, str(KINDS[ka]), str(KINDS[kb]), str(M), str(N), str(K), str(cfg), str(vec)],
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 0 and r.stdout.strip().startswith("OK"), r.stdout
+
+
+CRT_SRC = os.path.join(ROOT, "tests", "emu", "crt_emu.cpp")
+CRT_BIN = os.path.join(ROOT, "_build", "crt_emu")
+
+
+@pytest.fixture(scope="module")
+def crt_emu():
+    """Host emulation of the modular GEMM (fragment-major residue layout, LDS
+    stages, epilogue) with the kernels' own offset functions."""
+    os.makedirs(os.path.dirname(CRT_BIN), exist_ok=True)
+    deps = [os.path.join(ROOT, "flag4j_amd", "csrc", h) for h in ("gemm_tile.h", "crt_layout.h")] + [CRT_SRC]
+    if not os.path.exists(CRT_BIN) or any(os.path.getmtime(p) > os.path.getmtime(CRT_BIN) for p in deps):
+        subprocess.run(["g++", "-O2", "-std=c++17", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include", "-o", CRT_BIN,
+                        CRT_SRC], check=True)
+    return CRT_BIN
+
+
+@pytest.mark.parametrize("shape", [(1, 1, 1), (7, 300, 13), (130, 70, 257), (300, 200, 500), (257, 129, 384),
+                                   (512, 1024, 128), (33, 260, 640)])
+def test_crt_kernel_index_math(crt_emu, shape):
+    r = subprocess.run([crt_emu, *map(str, shape)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0 and r.stdout.strip().startswith("OK"), r.stdout

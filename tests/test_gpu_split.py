@@ -120,7 +120,8 @@ def test_transposed_operands(opA, opB):
     M, N, K = 300, 200, 500
     A = rng.uniform(9, *((M, K) if opA == "N" else (K, M)))
     B = rng.uniform(10, *((K, N) if opB == "N" else (N, K)))
-    C, _ = split_mult(A, B, "split_bf16", 6, opA, opB)
+    C, rep = split_mult(A, B, "split_bf16", 6, opA, opB)
+    assert rep.precision == L.PREC["split_bf16"]
     Aop = np.ascontiguousarray(A if opA == "N" else A.T)
     Bop = np.ascontiguousarray(B if opB == "N" else B.T)
     assert sampled_err(C, Aop, Bop) <= 1e-15
@@ -130,7 +131,8 @@ def test_transposed_operands(opA, opB):
 @pytest.mark.parametrize("prec", ["split_bf16", "split_fp16"])
 def test_ragged_shapes(M, N, K, prec):
     A, B = rng.uniform(M + 11, M, K), rng.uniform(N + 12, K, N)
-    C, _ = split_mult(A, B, prec, 6)
+    C, rep = split_mult(A, B, prec, 6)
+    assert rep.precision == L.PREC[prec]
     assert sampled_err(C, A, B, n=200) <= 1e-15
 
 
@@ -194,7 +196,8 @@ def test_crt_transposed(opA, opB):
     M, N, K = 300, 200, 500
     A = rng.uniform(25, *((M, K) if opA == "N" else (K, M)))
     B = rng.uniform(26, *((K, N) if opB == "N" else (N, K)))
-    C, _ = split_mult(A, B, "split_int8", 0, opA, opB)
+    C, rep = split_mult(A, B, "split_int8", 0, opA, opB)
+    assert rep.precision == L.PREC["split_int8"]          # delivered without the FP64 fallback
     Aop = np.ascontiguousarray(A if opA == "N" else A.T)
     Bop = np.ascontiguousarray(B if opB == "N" else B.T)
     assert sampled_err(C, Aop, Bop) <= 1e-15
@@ -203,7 +206,8 @@ def test_crt_transposed(opA, opB):
 @pytest.mark.parametrize("M,N,K", [(1, 1, 1), (7, 300, 13), (130, 70, 257), (129, 257, 1000)])
 def test_crt_ragged(M, N, K):
     A, B = rng.uniform(M + 31, M, K), rng.uniform(N + 32, K, N)
-    C, _ = split_mult(A, B, "split_int8", 0)
+    C, rep = split_mult(A, B, "split_int8", 0)
+    assert rep.precision == L.PREC["split_int8"]
     assert sampled_err(C, A, B, n=200) <= 1e-15
 
 
