@@ -60,6 +60,9 @@ SIGNATURES = {
     "f4j_zgemm_dev": ([_i, _i, _i64, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _i, _i, _i, _vp], _i),
     "f4j_grouped_plan_create": ([_i, _i, _i, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i,
                                  ctypes.POINTER(_vp)], _i),
+    "f4j_grouped_plan_create_split": ([_i, _i, _i, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i,
+                                       ctypes.POINTER(_vp)], _i),
+    "f4j_grouped_plan_split_report": ([_vp, _vp, _i64, ctypes.POINTER(ctypes.c_int)], _i),
     "f4j_grouped_plan_run": ([_vp, _vp], _i),
     "f4j_grouped_plan_destroy": ([_vp], _i),
     "f4j_grouped_plan_info": ([_vp, ctypes.POINTER(ctypes.c_double), _vp, _vp, _i], _i),

@@ -17,6 +17,10 @@
 // candidate tile configuration per bucket on the plan's own problems and
 // records the winner per (device, operand kinds, bucket) in a cache that can
 // be saved to and loaded from a text file.
+//
+// Split-precision plans (f4j_grouped_plan_create_split) capture the batched
+// split engine of split_batch.hip instead: exponents, digit slices, one
+// persistent slice-product launch over every output tile, residual check.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -34,6 +38,8 @@
 #include "../../include/flag4j_amd.h"
 #include "f4j_internal.h"
 #include "gemm_tile.h"
+#include "split_batch.h"
+#include "split_gemm.h"
 
 namespace f4j {
 namespace {
@@ -111,6 +117,8 @@ struct f4j_grouped_plan {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     double flops = 0;
+    f4j::SplitBatch *split = nullptr;        // split-precision plan (precision SPLIT_*), else FP64 buckets
+    hipStream_t last = nullptr;              // stream of the last run (the split report waits on it)
 };
 
 namespace f4j {
@@ -210,7 +218,10 @@ int capture(f4j_grouped_plan &pl) {
     }
     G_HIP(hipStreamBeginCapture(pl.cap, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
     hipError_t e = hipSuccess;
-    for (int b = 0; b < NUM_BUCKETS && e == hipSuccess; ++b) e = enqueue_bucket(pl, b, pl.cap);
+    if (pl.split)
+        e = split_batch_enqueue(pl.split, pl.cap);
+    else
+        for (int b = 0; b < NUM_BUCKETS && e == hipSuccess; ++b) e = enqueue_bucket(pl, b, pl.cap);
     hipGraph_t g = nullptr;
     const hipError_t e2 = hipStreamEndCapture(pl.cap, &g);
     if (e != hipSuccess) {
@@ -229,6 +240,7 @@ void destroy_plan(f4j_grouped_plan *pl) {
     if (pl->graph) (void)hipGraphDestroy(pl->graph);
     for (auto &b : pl->buckets) free_bucket(b);
     if (pl->d_probs) (void)hipFree(pl->d_probs);
+    split_batch_destroy(pl->split);
     if (pl->cap) (void)hipStreamDestroy(pl->cap);
     delete pl;
 }
@@ -236,6 +248,33 @@ void destroy_plan(f4j_grouped_plan *pl) {
 OpKind kind_for(int op, bool cplx) {
     if (!cplx) return op == F4J_OP_N ? R_N : R_T;
     return op == F4J_OP_N ? Z_N : (op == F4J_OP_T ? Z_T : Z_C);
+}
+
+// Argument checks shared by the plan constructors.
+int check_problems(bool cplx, int opA, int opB, int64_t count, const int64_t *M, const int64_t *N, const int64_t *K,
+                   const double *const *A, const int64_t *lda, const double *const *B, const int64_t *ldb,
+                   double *const *C, const int64_t *ldc) {
+    auto okop = [&](int op) { return op == F4J_OP_N || op == F4J_OP_T || (cplx && op == F4J_OP_C); };
+    if (!okop(opA) || !okop(opB)) return fail(F4J_ERR_INVALID_ARG, "invalid op code");
+    if (count < 0) return fail(F4J_ERR_INVALID_ARG, "negative count");
+    if (count > 0 && (!M || !N || !K || !A || !lda || !B || !ldb || !C || !ldc))
+        return fail(F4J_ERR_INVALID_ARG, "null problem array");
+    if (count > (int64_t)0x7FFFFFFF) return fail(F4J_ERR_INVALID_ARG, "count too large");
+    const int w = cplx ? 2 : 1;
+    for (int64_t i = 0; i < count; ++i) {
+        if (M[i] < 0 || N[i] < 0 || K[i] < 0)
+            return fail(F4J_ERR_INVALID_ARG, "problem " + std::to_string(i) + ": negative dimension");
+        const int64_t rowA = opA == F4J_OP_N ? K[i] : M[i], rowB = opB == F4J_OP_N ? N[i] : K[i];
+        const bool empty = M[i] == 0 || N[i] == 0;
+        if (!empty && (ldc[i] < std::max<int64_t>(N[i], 1) || !C[i]))
+            return fail(F4J_ERR_INVALID_ARG, "problem " + std::to_string(i) + ": bad C or ldc");
+        if (!empty && K[i] > 0 &&
+            (!A[i] || !B[i] || lda[i] < std::max<int64_t>(rowA, 1) || ldb[i] < std::max<int64_t>(rowB, 1)))
+            return fail(F4J_ERR_INVALID_ARG, "problem " + std::to_string(i) + ": bad A/B or leading dimension");
+        if (lda[i] * w > (int64_t)(0x7FFFFFF0 / (256 * 8)) || ldb[i] * w > (int64_t)(0x7FFFFFF0 / (256 * 8)))
+            return fail(F4J_ERR_UNSUPPORTED, "problem " + std::to_string(i) + ": leading dimension too large");
+    }
+    return F4J_OK;
 }
 
 }  // namespace
@@ -250,14 +289,13 @@ int f4j_grouped_plan_create(int is_complex, int opA, int opB, int64_t count, con
     if (!out) return fail(F4J_ERR_INVALID_ARG, "out is null");
     *out = nullptr;
     const bool cplx = is_complex != 0;
-    auto okop = [&](int op) { return op == F4J_OP_N || op == F4J_OP_T || (cplx && op == F4J_OP_C); };
-    if (!okop(opA) || !okop(opB)) return fail(F4J_ERR_INVALID_ARG, "invalid op code");
-    if (count < 0) return fail(F4J_ERR_INVALID_ARG, "negative count");
-    if (count > 0 && (!M || !N || !K || !A || !lda || !B || !ldb || !C || !ldc))
-        return fail(F4J_ERR_INVALID_ARG, "null problem array");
+    if (precision == F4J_PREC_SPLIT_BF16 || precision == F4J_PREC_SPLIT_FP16)
+        return f4j_grouped_plan_create_split(is_complex, opA, opB, count, M, N, K, A, lda, B, ldb, C, ldc, precision,
+                                             0, 1, out);
     if (precision != F4J_PREC_FP64 && precision != F4J_PREC_AUTO)
-        return fail(F4J_ERR_UNSUPPORTED, "grouped products support precision FP64 only in this build");
-    if (count > (int64_t)0x7FFFFFFF) return fail(F4J_ERR_INVALID_ARG, "count too large");
+        return fail(F4J_ERR_UNSUPPORTED, "grouped products: precision must be FP64, SPLIT_BF16 or SPLIT_FP16");
+    if (const int rc = check_problems(cplx, opA, opB, count, M, N, K, A, lda, B, ldb, C, ldc); rc != F4J_OK)
+        return rc;
     if (const int rc = runtime_ready(); rc != F4J_OK) return rc;
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) return fail(F4J_ERR_NO_DEVICE, "no HIP device");
@@ -279,25 +317,7 @@ int f4j_grouped_plan_create(int is_complex, int opA, int opB, int64_t count, con
     pl->probs.resize(count);
     pl->prob_bucket.resize(count);
     for (int64_t i = 0; i < count; ++i) {
-        if (M[i] < 0 || N[i] < 0 || K[i] < 0) {
-            destroy_plan(pl);
-            return fail(F4J_ERR_INVALID_ARG, "problem " + std::to_string(i) + ": negative dimension");
-        }
-        const int64_t rowA = opA == F4J_OP_N ? K[i] : M[i], rowB = opB == F4J_OP_N ? N[i] : K[i];
         const bool empty = M[i] == 0 || N[i] == 0;
-        if (!empty && (ldc[i] < std::max<int64_t>(N[i], 1) || !C[i])) {
-            destroy_plan(pl);
-            return fail(F4J_ERR_INVALID_ARG, "problem " + std::to_string(i) + ": bad C or ldc");
-        }
-        if (!empty && K[i] > 0 &&
-            (!A[i] || !B[i] || lda[i] < std::max<int64_t>(rowA, 1) || ldb[i] < std::max<int64_t>(rowB, 1))) {
-            destroy_plan(pl);
-            return fail(F4J_ERR_INVALID_ARG, "problem " + std::to_string(i) + ": bad A/B or leading dimension");
-        }
-        if (lda[i] * w > (int64_t)(0x7FFFFFF0 / (256 * 8)) || ldb[i] * w > (int64_t)(0x7FFFFFF0 / (256 * 8))) {
-            destroy_plan(pl);
-            return fail(F4J_ERR_UNSUPPORTED, "problem " + std::to_string(i) + ": leading dimension too large");
-        }
         GemmProblem &p = pl->probs[i];
         p.M = empty ? 0 : M[i];
         p.N = empty ? 0 : N[i] * w;
@@ -350,7 +370,8 @@ int f4j_grouped_plan_run(f4j_grouped_plan *plan, void *stream) {
     if (!plan) return fail(F4J_ERR_INVALID_ARG, "plan is null");
     if (const int rc = runtime_ready(); rc != F4J_OK) return rc;
     if (!plan->exec) return F4J_OK;
-    G_HIP(hipGraphLaunch(plan->exec, runtime_stream(stream)), "hipGraphLaunch");
+    plan->last = runtime_stream(stream);
+    G_HIP(hipGraphLaunch(plan->exec, plan->last), "hipGraphLaunch");
     return F4J_OK;
 }
 
@@ -363,6 +384,14 @@ int f4j_grouped_plan_destroy(f4j_grouped_plan *plan) {
 int f4j_grouped_plan_info(const f4j_grouped_plan *plan, double *flops, int *tile_cfg, int64_t *items, int nb) {
     if (!plan) return fail(F4J_ERR_INVALID_ARG, "plan is null");
     if (flops) *flops = plan->flops;
+    if (plan->split) {                     // one fixed 64 x 64 slice tile; all tiles in "bucket" 0
+        for (int b = 0; b < std::min(nb, NUM_BUCKETS); ++b) {
+            if (tile_cfg) tile_cfg[b] = -1;
+            if (items) items[b] = 0;
+        }
+        if (items && nb > 0) items[0] = plan->buckets[0].items;
+        return F4J_OK;
+    }
     for (int b = 0; b < std::min(nb, NUM_BUCKETS); ++b) {
         if (tile_cfg) tile_cfg[b] = plan->buckets[b].cfg;
         if (items) items[b] = plan->buckets[b].items;
@@ -372,6 +401,7 @@ int f4j_grouped_plan_info(const f4j_grouped_plan *plan, double *flops, int *tile
 
 int f4j_grouped_plan_tune(f4j_grouped_plan *plan, int reps) {
     if (!plan) return fail(F4J_ERR_INVALID_ARG, "plan is null");
+    if (plan->split) return F4J_OK;        // split plans use the one slice tile; nothing to choose
     if (reps < 1) reps = 3;
     const std::string arch = device_arch();
     hipEvent_t e0, e1;
@@ -404,6 +434,72 @@ int f4j_grouped_plan_tune(f4j_grouped_plan *plan, int reps) {
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     return capture(*plan);
+}
+
+int f4j_grouped_plan_create_split(int is_complex, int opA, int opB, int64_t count, const int64_t *M,
+                                  const int64_t *N, const int64_t *K, const double *const *A, const int64_t *lda,
+                                  const double *const *B, const int64_t *ldb, double *const *C, const int64_t *ldc,
+                                  int precision, int slices, int use3m, f4j_grouped_plan **out) {
+    if (!out) return fail(F4J_ERR_INVALID_ARG, "out is null");
+    *out = nullptr;
+    const bool cplx = is_complex != 0;
+    if (precision != F4J_PREC_SPLIT_BF16 && precision != F4J_PREC_SPLIT_FP16)
+        return fail(F4J_ERR_INVALID_ARG, "split plans take precision SPLIT_BF16 or SPLIT_FP16");
+    if (slices < 0 || slices > SPLIT_MAX_SLICES) return fail(F4J_ERR_INVALID_ARG, "slices must be in 0..8");
+    if (const int rc = check_problems(cplx, opA, opB, count, M, N, K, A, lda, B, ldb, C, ldc); rc != F4J_OK)
+        return rc;
+    if (const int rc = runtime_ready(); rc != F4J_OK) return rc;
+    char buf[32];
+    int check_rows = 4;
+    if (f4j_config_get("splitCheckRows", buf, sizeof(buf)) == F4J_OK) check_rows = std::atoi(buf);
+    const int carrier = precision == F4J_PREC_SPLIT_FP16 ? SPLIT_FP16 : SPLIT_BF16;
+    const int S = slices > 0 ? slices : (carrier == SPLIT_FP16 ? 5 : 6);
+    std::vector<BatchProblem> probs(count);
+    f4j_grouped_plan *pl = new f4j_grouped_plan;
+    pl->is_complex = is_complex;
+    pl->opA = opA;
+    pl->opB = opB;
+    pl->precision = precision;
+    pl->count = count;
+    for (int64_t i = 0; i < count; ++i) {
+        probs[i] = BatchProblem{M[i], N[i], K[i], A[i], lda[i], B[i], ldb[i], C[i], ldc[i]};
+        pl->flops += (cplx ? 8.0 : 2.0) * (double)M[i] * (double)N[i] * (double)K[i];
+    }
+    std::string err;
+    int status = F4J_OK;
+    pl->split = split_batch_create(cplx, opA, opB, carrier, S, use3m != 0, check_rows, probs, &err, &status);
+    if (!pl->split) {
+        destroy_plan(pl);
+        return fail(status, err);
+    }
+    int64_t tiles = 0;
+    for (int64_t i = 0; i < count; ++i)
+        if (M[i] > 0 && N[i] > 0 && K[i] > 0) tiles += ((M[i] + 63) / 64) * ((N[i] + 63) / 64);
+    pl->buckets[0].items = tiles;
+    const hipError_t e = hipStreamCreateWithFlags(&pl->cap, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        destroy_plan(pl);
+        return hipfail(e, "grouped plan setup");
+    }
+    const int rc = capture(*pl);
+    if (rc != F4J_OK) {
+        destroy_plan(pl);
+        return rc;
+    }
+    *out = pl;
+    return F4J_OK;
+}
+
+int f4j_grouped_plan_split_report(f4j_grouped_plan *plan, double *err, int64_t count, int *products) {
+    if (!plan) return fail(F4J_ERR_INVALID_ARG, "plan is null");
+    if (!plan->split) return fail(F4J_ERR_INVALID_ARG, "not a split-precision plan");
+    if (count > 0 && !err) return fail(F4J_ERR_INVALID_ARG, "err is null");
+    if (products) *products = split_batch_real_products(plan->split);
+    if (count > 0)
+        G_HIP(split_batch_errors(plan->split, err, std::min<int64_t>(count, plan->count),
+                                 plan->last ? plan->last : plan->cap),
+              "split report");
+    return F4J_OK;
 }
 
 int f4j_tuner_save(const char *path) {

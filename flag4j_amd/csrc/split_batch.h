@@ -1,0 +1,42 @@
+// Batched split-precision GEMM: many independent real or complex products,
+// each C_p = op(A_p) op(B_p), in a fixed sequence of launches (exponents,
+// digit slices, one persistent slice-product kernel over every output tile of
+// every product, residual check) that can be captured into a hipGraph.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace f4j {
+
+struct BatchProblem {
+    int64_t M, N, K;
+    const double *A;      // leading dimensions in elements (complex: interleaved re, im pairs)
+    int64_t lda;
+    const double *B;
+    int64_t ldb;
+    double *C;
+    int64_t ldc;
+};
+
+struct SplitBatch;
+
+// Plan a batch on `carrier` (SPLIT_BF16 / SPLIT_FP16) with `slices` digits per
+// operand.  Complex products use the 3M form (three real products) when
+// use3m, else four.  `check_rows` rows of every product are verified by the
+// residual check (0 = off).  On failure returns nullptr with *status and *err set.
+SplitBatch *split_batch_create(bool cplx, int opA, int opB, int carrier, int slices, bool use3m, int check_rows,
+                               const std::vector<BatchProblem> &probs, std::string *err, int *status);
+// Enqueue the whole batch on `stream` (no allocation, no memset: capturable).
+hipError_t split_batch_enqueue(const SplitBatch *b, hipStream_t stream);
+// Per-problem max |C - AB| / (|A||B|) over the checked rows of the last run
+// (-1 where nothing was checked); synchronises `stream`.
+hipError_t split_batch_errors(const SplitBatch *b, double *out, int64_t count, hipStream_t stream);
+int split_batch_real_products(const SplitBatch *b);     // real slice products per run (jobs x pairs)
+int split_batch_checked_rows(const SplitBatch *b);
+size_t split_batch_bytes(const SplitBatch *b);          // device memory held
+void split_batch_destroy(SplitBatch *b);
+
+}  // namespace f4j

@@ -1,0 +1,611 @@
+// Batched split-precision GEMM (see split_batch.h).
+//
+// No reference counterpart: Flag4j multiplies one pair of matrices at a time
+// in FP64 (M:arrays/dense/Matrix.java:638-640, M:arrays/dense/CMatrix.java:
+// 650-652; M: = src/main/java/org/flag4j/).  The arithmetic is the one of the
+// single-product split form (split_gemm.hip): per-line power-of-two scaling,
+// signed radix-2^R digits on BF16/FP16, exact FP32 MFMA accumulation flushed
+// into FP64, level groups summed exactly, recombined in FP64.
+//
+// Jobs.  A real product is one job.  A complex product op(A) op(B) with
+// op(A) = a + i b and op(B) = c + i d (b, d carry the conjugation sign) is,
+// in the 3M (Gauss) form, three real jobs
+//     T1 = a c,  T2 = b d,  T3 = (a + b)(c + d),
+//     Re C = T1 - T2,  Im C = T3 - T1 - T2,
+// and in the 4M form four: a c, b d, a d, b c.  A job's operands ("sides")
+// are read from the complex storage as ca re + cb im, so a + b is formed on
+// the fly (one FP64 rounding, 2^-53 relative, before the exact split).
+//
+// Kernels, each one launch for the whole batch:
+//   bexp_kernel    exponent of every line of every side
+//   bsplit_kernel  digit slices [S][Xp][Kp] of every side
+//   bgemm_kernel   persistent: every 64 x 64 output tile of every product,
+//                  in a longest-first schedule; a tile runs all jobs of its
+//                  product and all their level groups in sequence and adds
+//                  each into C with the job's coefficients, so no two
+//                  workgroups write the same element
+//   bres_kernel    compensated (Dot2) check of sampled rows, per product
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <map>
+#include <queue>
+#include <vector>
+
+#include "../../include/flag4j_amd.h"
+#include "f4j_internal.h"
+#include "split_batch.h"
+#include "split_gemm.h"
+#include "split_tile.h"
+
+namespace f4j {
+namespace {
+
+// Batched products use a 64 x 64 tile (2 x 2 waves of 32 x 32): small
+// problems waste less padding, and the FP64 accumulators take 32 registers.
+using BTile = slice::Tile<1, 1>;
+
+struct Side {                 // one operand of one real job
+    const double *X;          // real data, or interleaved complex (re at +0, im at +1)
+    int64_t sx, sk;           // doubles between consecutive lines / consecutive k
+    double ca, cb;            // value = ca re + cb im
+    int cplx, contig;         // contig: consecutive k are adjacent elements
+    int64_t nx, K, Xp, Kp;    // lines, inner size, padded to the tile / the k step
+    int64_t dig;              // offset (carrier elements) of the digit slices [S][Xp][Kp]
+    int64_t eoff;             // offset of the line exponents [Xp]
+};
+
+struct Job {
+    int sa, sb;               // sides of op(A) and op(B)
+    int grp0, ngrp;           // level groups of this job's Kp
+    double cre, cim;          // C += cre T (real part), cim T (imaginary part)
+};
+
+struct Prob {
+    int64_t M, N, K;
+    const double *A, *B;
+    double *C;
+    int64_t lda, ldb, ldc;
+    int job0, njobs;
+};
+
+struct Item {                 // one output tile
+    int prob, tm, tn, cost;
+};
+
+__device__ inline double side_value(const Side &s, int64_t x, int64_t k) {
+    const double *p = s.X + x * s.sx + k * s.sk;
+    return s.cplx ? fma(s.ca, p[0], s.cb * p[1]) : p[0];
+}
+
+__device__ inline int exp_of(double v) {
+    if (v == 0.0) return INT_MIN;
+    int e;
+    (void)frexp(v, &e);
+    return e;
+}
+
+// Work item (side, first line): contiguous sides take 4 lines per block (a
+// wave per line, lanes along k); strided sides 256 lines (a thread per line,
+// lanes along the lines).  Lines past nx (padding) get INT_MIN.
+__global__ __launch_bounds__(256) void bexp_kernel(const Side *sides, const int2 *work, int *epool) {
+    const int2 w = work[blockIdx.x];
+    const Side s = sides[w.x];
+    if (s.contig) {
+        const int64_t x = w.y + (threadIdx.x >> 6);
+        const int lane = threadIdx.x & 63;
+        int m = INT_MIN;
+        if (x < s.nx)
+            for (int64_t k = lane; k < s.K; k += 64) m = max(m, exp_of(side_value(s, x, k)));
+        for (int off = 32; off > 0; off >>= 1) m = max(m, __shfl_xor(m, off));
+        if (lane == 0 && x < s.Xp) epool[s.eoff + x] = m;
+    } else {
+        const int64_t x = w.y + threadIdx.x;
+        if (x >= s.Xp) return;
+        int m = INT_MIN;
+        if (x < s.nx)
+            for (int64_t k = 0; k < s.K; ++k) m = max(m, exp_of(side_value(s, x, k)));
+        epool[s.eoff + x] = m;
+    }
+}
+
+// Work item (side, x0, k0): 32 lines x 64 k staged through LDS with reads
+// along k (contiguous sides) or along the lines (strided sides); thread t then
+// splits line t / 8, 8 consecutive k, and writes 16 bytes per digit slice.
+template <int R, int S, class T>
+__global__ __launch_bounds__(256) void bsplit_kernel(const Side *sides, const int4 *work, const int *epool, T *dpool) {
+    __shared__ double tile[32][65];
+    const int4 w = work[blockIdx.x];
+    const Side s = sides[w.x];
+    const int64_t x0 = w.y, k0 = w.z;
+    const int t = threadIdx.x;
+    if (s.contig) {
+        const int c = t & 63;
+        for (int r = t >> 6; r < 32; r += 4) {
+            const int64_t x = x0 + r, k = k0 + c;
+            tile[r][c] = (x < s.nx && k < s.K) ? side_value(s, x, k) : 0.0;
+        }
+    } else {
+        const int r = t & 31;
+        for (int c = t >> 5; c < 64; c += 8) {
+            const int64_t x = x0 + r, k = k0 + c;
+            tile[r][c] = (x < s.nx && k < s.K) ? side_value(s, x, k) : 0.0;
+        }
+    }
+    __syncthreads();
+    const int r = t >> 3, kq = (t & 7) * 8;
+    const int64_t x = x0 + r;
+    const int e = epool[s.eoff + x];
+    T d[8][S];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) slice::digits<R, S>(tile[r][kq + u], e, d[u]);
+    T *out = dpool + s.dig + x * s.Kp + k0 + kq;
+#pragma unroll
+    for (int q = 0; q < S; ++q) {
+        T v8[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v8[u] = d[u][q];
+        *(slice::u32x4 *)(out + (int64_t)q * s.Xp * s.Kp) = __builtin_bit_cast(slice::u32x4, v8);
+    }
+}
+
+struct GroupSegs {
+    const SplitGroup *g;
+    __device__ int a(int q) const { return g->seg_a[q]; }
+    __device__ int b(int q) const { return g->seg_b[q]; }
+    __device__ int shift(int q) const { return g->seg_shift[q]; }
+};
+
+// Persistent slice-product kernel: workgroup w runs items begin[w] .. begin[w+1].
+template <class T, int FLUSH>
+__global__ __launch_bounds__(256, 2) void bgemm_kernel(const Item *items, const int *begin, const Prob *probs,
+                                                       const Job *jobs, const Side *sides, const SplitGroup *groups,
+                                                       const T *dpool, const int *epool, int cplx) {
+    __shared__ __attribute__((aligned(16))) T lds[2 * BTile::STAGE];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+    const int it1 = begin[blockIdx.x + 1];
+    for (int it = begin[blockIdx.x]; it < it1; ++it) {
+        const Item item = items[it];
+        const Prob &p = probs[item.prob];
+        const int64_t M = p.M, N = p.N, ldc = p.ldc;
+        double *C = p.C;
+        const int64_t m0 = (int64_t)item.tm * BTile::BM, n0 = (int64_t)item.tn * BTile::BN;
+        bool first = true;
+        for (int jb = 0; jb < p.njobs; ++jb) {
+            const Job jo = jobs[p.job0 + jb];
+            const Side &sa = sides[jo.sa];
+            const Side &sb = sides[jo.sb];
+            const T *da = dpool + sa.dig, *db = dpool + sb.dig;
+            const int *ea = epool + sa.eoff, *eb = epool + sb.eoff;
+            const int64_t Mp = sa.Xp, Np = sb.Xp, Kp = sa.Kp;
+            for (int gi = 0; gi < jo.ngrp; ++gi) {
+                const SplitGroup *g = groups + jo.grp0 + gi;
+                double acc64[1][1][16];
+                slice::tile_loop<T, FLUSH, 1, 1>(da, db, Mp, Np, Kp, g->nseg, GroupSegs{g}, m0, n0, lds, acc64);
+                const int be = g->base_exp;
+#pragma unroll
+                for (int i = 0; i < 1; ++i)
+#pragma unroll
+                    for (int j = 0; j < 1; ++j) {
+                        const int64_t col = n0 + slice::acc_col(wn, j, lane);
+                        if (col >= N) continue;
+                        const int ebv = eb[col];
+#pragma unroll
+                        for (int r = 0; r < 16; ++r) {
+                            const int64_t row = m0 + slice::acc_row(wm, i, r, lane);
+                            if (row >= M) continue;
+                            const int eav = ea[row];
+                            double v = 0.0;
+                            if (eav != INT_MIN && ebv != INT_MIN) v = ldexp(acc64[i][j][r], eav + ebv + be);
+                            if (cplx) {
+                                double *c = C + 2 * (row * ldc + col);
+                                c[0] = first ? jo.cre * v : c[0] + jo.cre * v;
+                                c[1] = first ? jo.cim * v : c[1] + jo.cim * v;
+                            } else {
+                                double *c = C + row * ldc + col;
+                                *c = first ? v : *c + v;
+                            }
+                        }
+                    }
+                first = false;
+            }
+        }
+    }
+}
+
+__device__ inline void dot2_add(double &s, double &c, double a, double b) {
+    const double h = a * b;
+    const double r = fma(a, b, -h);
+    const double t = s + h;
+    const double z = t - s;
+    c += ((s - (t - z)) + (h - z)) + r;
+    s = t;
+}
+
+__global__ __launch_bounds__(256) void fill_u64_kernel(unsigned long long *p, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) p[i] = 0ull;
+}
+
+// Work item (problem, checked row, first column; w = rows checked in this
+// problem): compensated dot products (Dot2; Ogita, Rump and Oishi 2005) of
+// row i = y M / w against every column; max |C - AB| / (|A||B|) per problem,
+// |z| = |re| + |im| for complex entries.
+__global__ __launch_bounds__(256) void bres_kernel(const Prob *probs, const int4 *work, int cplx, int opA, int opB,
+                                                   unsigned long long *err) {
+    const int4 w = work[blockIdx.x];
+    const Prob &p = probs[w.x];
+    const int64_t i = (int64_t)w.y * p.M / w.w;
+    const int64_t j = w.z + threadIdx.x;
+    if (j >= p.N) return;
+    double e = 0.0;
+    if (!cplx) {
+        double s = 0.0, c = 0.0, sa = 0.0;
+        for (int64_t k = 0; k < p.K; ++k) {
+            const double a = opA ? p.A[k * p.lda + i] : p.A[i * p.lda + k];
+            const double b = opB ? p.B[j * p.ldb + k] : p.B[k * p.ldb + j];
+            dot2_add(s, c, a, b);
+            sa = fma(fabs(a), fabs(b), sa);
+        }
+        const double cv = p.C[i * p.ldc + j];
+        e = sa > 0.0 ? fabs((cv - s) - c) / sa : (cv == 0.0 ? 0.0 : INFINITY);
+    } else {
+        const double sga = opA == 2 ? -1.0 : 1.0, sgb = opB == 2 ? -1.0 : 1.0;
+        double rs = 0.0, rc = 0.0, is = 0.0, ic = 0.0, sa = 0.0;
+        for (int64_t k = 0; k < p.K; ++k) {
+            const double *pa = opA ? p.A + 2 * (k * p.lda + i) : p.A + 2 * (i * p.lda + k);
+            const double *pb = opB ? p.B + 2 * (j * p.ldb + k) : p.B + 2 * (k * p.ldb + j);
+            const double ar = pa[0], ai = sga * pa[1], br = pb[0], bi = sgb * pb[1];
+            dot2_add(rs, rc, ar, br);
+            dot2_add(rs, rc, -ai, bi);
+            dot2_add(is, ic, ar, bi);
+            dot2_add(is, ic, ai, br);
+            sa = fma(fabs(ar) + fabs(ai), fabs(br) + fabs(bi), sa);
+        }
+        const double *cv = p.C + 2 * (i * p.ldc + j);
+        const double er = fabs((cv[0] - rs) - rc), ei = fabs((cv[1] - is) - ic);
+        e = sa > 0.0 ? fmax(er, ei) / sa : ((cv[0] == 0.0 && cv[1] == 0.0) ? 0.0 : INFINITY);
+    }
+    atomicMax(err + w.x, __builtin_bit_cast(unsigned long long, e));
+}
+
+template <int R, class T>
+void launch_bsplit(int S, dim3 grid, hipStream_t st, const Side *sides, const int4 *work, const int *epool, T *dpool) {
+    switch (S) {
+        case 1: hipLaunchKernelGGL((bsplit_kernel<R, 1, T>), grid, dim3(256), 0, st, sides, work, epool, dpool); break;
+        case 2: hipLaunchKernelGGL((bsplit_kernel<R, 2, T>), grid, dim3(256), 0, st, sides, work, epool, dpool); break;
+        case 3: hipLaunchKernelGGL((bsplit_kernel<R, 3, T>), grid, dim3(256), 0, st, sides, work, epool, dpool); break;
+        case 4: hipLaunchKernelGGL((bsplit_kernel<R, 4, T>), grid, dim3(256), 0, st, sides, work, epool, dpool); break;
+        case 5: hipLaunchKernelGGL((bsplit_kernel<R, 5, T>), grid, dim3(256), 0, st, sides, work, epool, dpool); break;
+        case 6: hipLaunchKernelGGL((bsplit_kernel<R, 6, T>), grid, dim3(256), 0, st, sides, work, epool, dpool); break;
+        case 7: hipLaunchKernelGGL((bsplit_kernel<R, 7, T>), grid, dim3(256), 0, st, sides, work, epool, dpool); break;
+        default: hipLaunchKernelGGL((bsplit_kernel<R, 8, T>), grid, dim3(256), 0, st, sides, work, epool, dpool); break;
+    }
+}
+
+}  // namespace
+
+struct SplitBatch {
+    bool cplx = false;
+    int opA = 0, opB = 0, carrier = SPLIT_BF16, S = 0, check_rows = 0;
+    int64_t count = 0;
+    int real_products = 0;
+    int workers = 0;
+    int64_t n_exp = 0, n_split = 0, n_items = 0, n_res = 0;
+    std::vector<BatchProblem> zero_k;      // products with K = 0: C is zero-filled
+    // device tables
+    Side *d_sides = nullptr;
+    Job *d_jobs = nullptr;
+    Prob *d_probs = nullptr;
+    SplitGroup *d_groups = nullptr;
+    Item *d_items = nullptr;
+    int *d_begin = nullptr;
+    int2 *d_exp_work = nullptr;
+    int4 *d_split_work = nullptr, *d_res_work = nullptr;
+    void *d_digits = nullptr;
+    int *d_exps = nullptr;
+    unsigned long long *d_err = nullptr;
+    std::vector<int> prob_index;           // problem -> index in d_probs (-1: not in the GEMM)
+    size_t bytes = 0;
+};
+
+namespace {
+
+template <class X>
+hipError_t upload(X **dst, const std::vector<X> &v, size_t *bytes) {
+    if (v.empty()) return hipSuccess;
+    hipError_t e = hipMalloc((void **)dst, v.size() * sizeof(X));
+    if (e != hipSuccess) return e;
+    *bytes += v.size() * sizeof(X);
+    return hipMemcpy(*dst, v.data(), v.size() * sizeof(X), hipMemcpyHostToDevice);
+}
+
+}  // namespace
+
+SplitBatch *split_batch_create(bool cplx, int opA, int opB, int carrier, int slices, bool use3m, int check_rows,
+                               const std::vector<BatchProblem> &probs, std::string *err, int *status) {
+    auto bad = [&](int code, const std::string &m) -> SplitBatch * {
+        *status = code;
+        *err = m;
+        return nullptr;
+    };
+    if (slices < 1 || slices > SPLIT_MAX_SLICES) return bad(F4J_ERR_INVALID_ARG, "slices must be in 1..8");
+    const int R = split_radix(carrier);
+    const int w = cplx ? 2 : 1;
+    const double sga = (opA == F4J_OP_C) ? -1.0 : 1.0, sgb = (opB == F4J_OP_C) ? -1.0 : 1.0;
+
+    SplitBatch *b = new SplitBatch;
+    b->cplx = cplx;
+    b->opA = opA;
+    b->opB = opB;
+    b->carrier = carrier;
+    b->S = slices;
+    b->check_rows = check_rows;
+    b->count = (int64_t)probs.size();
+    b->prob_index.assign(probs.size(), -1);
+
+    std::vector<Side> sides;
+    std::vector<Job> jobs;
+    std::vector<Prob> dprobs;
+    std::vector<SplitGroup> groups;
+    std::map<int64_t, std::pair<int, int>> group_of_kp;     // Kp -> (first group, count)
+    int64_t dig = 0, eoff = 0;
+
+    // (ca_a, cb_a, ca_b, cb_b, cre, cim) per job of one product
+    struct Form {
+        double ca_a, cb_a, ca_b, cb_b, cre, cim;
+    };
+    std::vector<Form> forms;
+    if (!cplx) forms = {{1, 0, 1, 0, 1, 0}};
+    else if (use3m) forms = {{1, 0, 1, 0, 1, -1}, {0, sga, 0, sgb, -1, -1}, {1, sga, 1, sgb, 0, 1}};
+    else forms = {{1, 0, 1, 0, 1, 0}, {0, sga, 0, sgb, -1, 0}, {1, 0, 0, sgb, 0, 1}, {0, sga, 1, 0, 0, 1}};
+
+    for (size_t pi = 0; pi < probs.size(); ++pi) {
+        const BatchProblem &q = probs[pi];
+        if (q.M == 0 || q.N == 0) continue;
+        if (q.K == 0) {
+            b->zero_k.push_back(q);
+            continue;
+        }
+        const int64_t Mp = (q.M + BTile::BM - 1) / BTile::BM * BTile::BM;
+        const int64_t Np = (q.N + BTile::BN - 1) / BTile::BN * BTile::BN;
+        const int64_t Kp = (q.K + slice::BK - 1) / slice::BK * slice::BK;
+        auto git = group_of_kp.find(Kp);
+        if (git == group_of_kp.end()) {
+            const std::vector<SplitGroup> g = split_groups(slices, carrier, Kp);
+            git = group_of_kp.emplace(Kp, std::make_pair((int)groups.size(), (int)g.size())).first;
+            groups.insert(groups.end(), g.begin(), g.end());
+        }
+        Prob pr{};
+        pr.M = q.M;
+        pr.N = q.N;
+        pr.K = q.K;
+        pr.A = q.A;
+        pr.B = q.B;
+        pr.C = q.C;
+        pr.lda = q.lda;
+        pr.ldb = q.ldb;
+        pr.ldc = q.ldc;
+        pr.job0 = (int)jobs.size();
+        pr.njobs = (int)forms.size();
+        for (const Form &f : forms) {
+            Side a{};
+            a.X = q.A;
+            a.cplx = cplx;
+            a.ca = f.ca_a;
+            a.cb = f.cb_a;
+            a.nx = q.M;
+            a.K = q.K;
+            a.Xp = Mp;
+            a.Kp = Kp;
+            a.contig = opA == F4J_OP_N;
+            a.sx = (a.contig ? q.lda : 1) * w;
+            a.sk = (a.contig ? 1 : q.lda) * w;
+            Side c{};
+            c.X = q.B;
+            c.cplx = cplx;
+            c.ca = f.ca_b;
+            c.cb = f.cb_b;
+            c.nx = q.N;
+            c.K = q.K;
+            c.Xp = Np;
+            c.Kp = Kp;
+            c.contig = opB != F4J_OP_N;
+            c.sx = (c.contig ? q.ldb : 1) * w;
+            c.sk = (c.contig ? 1 : q.ldb) * w;
+            for (Side *s : {&a, &c}) {
+                s->dig = dig;
+                dig += (int64_t)slices * s->Xp * s->Kp;
+                s->eoff = eoff;
+                eoff += s->Xp;
+            }
+            Job j{};
+            j.sa = (int)sides.size();
+            sides.push_back(a);
+            j.sb = (int)sides.size();
+            sides.push_back(c);
+            j.grp0 = git->second.first;
+            j.ngrp = git->second.second;
+            j.cre = f.cre;
+            j.cim = f.cim;
+            jobs.push_back(j);
+        }
+        b->prob_index[pi] = (int)dprobs.size();
+        dprobs.push_back(pr);
+    }
+    b->real_products = (int)jobs.size() * slices * (slices + 1) / 2;
+
+    // Work lists.
+    std::vector<int2> exp_work;
+    std::vector<int4> split_work;
+    for (size_t si = 0; si < sides.size(); ++si) {
+        const Side &s = sides[si];
+        const int step = s.contig ? 4 : 256;
+        for (int64_t x = 0; x < s.Xp; x += step) exp_work.push_back(make_int2((int)si, (int)x));
+        for (int64_t x = 0; x < s.Xp; x += 32)
+            for (int64_t k = 0; k < s.Kp; k += slice::BK) split_work.push_back(make_int4((int)si, (int)x, (int)k, 0));
+    }
+    std::vector<int4> res_work;
+    if (check_rows > 0)
+        for (size_t pi = 0; pi < dprobs.size(); ++pi) {
+            const Prob &p = dprobs[pi];
+            const int rows = (int)std::min<int64_t>(check_rows, p.M);
+            for (int y = 0; y < rows; ++y)
+                for (int64_t c = 0; c < p.N; c += 256) res_work.push_back(make_int4((int)pi, y, (int)c, rows));
+        }
+
+    // Output tiles, longest first onto the least-loaded workgroup.
+    std::vector<Item> items;
+    for (size_t pi = 0; pi < dprobs.size(); ++pi) {
+        const Prob &p = dprobs[pi];
+        const Side &sa = sides[jobs[p.job0].sa];
+        const int ngrp = jobs[p.job0].ngrp;
+        int64_t cost = 0;
+        for (int gi = 0; gi < ngrp; ++gi) cost += (int64_t)groups[jobs[p.job0].grp0 + gi].nseg * (sa.Kp / slice::BK) + 4;
+        cost *= p.njobs;
+        const int tmn = (int)((p.M + BTile::BM - 1) / BTile::BM), tnn = (int)((p.N + BTile::BN - 1) / BTile::BN);
+        for (int tm = 0; tm < tmn; ++tm)
+            for (int tn = 0; tn < tnn; ++tn) items.push_back(Item{(int)pi, tm, tn, (int)std::min<int64_t>(cost, INT_MAX)});
+    }
+    int dev = 0;
+    hipDeviceProp_t prop{};
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) {
+        split_batch_destroy(b);
+        return bad(F4J_ERR_HIP, "device query failed");
+    }
+    int occ = 0;
+    const hipError_t oe = carrier == SPLIT_FP16
+                              ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, bgemm_kernel<_Float16, 64>, 256, 0)
+                              : hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, bgemm_kernel<__bf16, 256>, 256, 0);
+    if (oe != hipSuccess || occ < 1) occ = 1;
+    b->workers = (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)prop.multiProcessorCount * occ,
+                                                             (int64_t)items.size()));
+    std::vector<std::vector<Item>> per(b->workers);
+    {
+        std::sort(items.begin(), items.end(), [](const Item &x, const Item &y) { return x.cost > y.cost; });
+        using Load = std::pair<int64_t, int>;
+        std::priority_queue<Load, std::vector<Load>, std::greater<Load>> heap;
+        for (int wk = 0; wk < b->workers; ++wk) heap.push({0, wk});
+        for (const Item &it : items) {
+            Load l = heap.top();
+            heap.pop();
+            per[l.second].push_back(it);
+            l.first += it.cost;
+            heap.push(l);
+        }
+    }
+    std::vector<Item> sched;
+    std::vector<int> begin(b->workers + 1, 0);
+    for (int wk = 0; wk < b->workers; ++wk) {
+        begin[wk] = (int)sched.size();
+        sched.insert(sched.end(), per[wk].begin(), per[wk].end());
+    }
+    begin[b->workers] = (int)sched.size();
+
+    b->n_exp = (int64_t)exp_work.size();
+    b->n_split = (int64_t)split_work.size();
+    b->n_items = (int64_t)sched.size();
+    b->n_res = (int64_t)res_work.size();
+    if (b->n_exp > INT_MAX || b->n_split > INT_MAX || b->n_res > INT_MAX) {
+        split_batch_destroy(b);
+        return bad(F4J_ERR_UNSUPPORTED, "batch too large for one plan");
+    }
+
+    hipError_t e = hipSuccess;
+    size_t &by = b->bytes;
+    if (e == hipSuccess) e = upload(&b->d_sides, sides, &by);
+    if (e == hipSuccess) e = upload(&b->d_jobs, jobs, &by);
+    if (e == hipSuccess) e = upload(&b->d_probs, dprobs, &by);
+    if (e == hipSuccess) e = upload(&b->d_groups, groups, &by);
+    if (e == hipSuccess) e = upload(&b->d_items, sched, &by);
+    if (e == hipSuccess) e = upload(&b->d_begin, begin, &by);
+    if (e == hipSuccess) e = upload(&b->d_exp_work, exp_work, &by);
+    if (e == hipSuccess) e = upload(&b->d_split_work, split_work, &by);
+    if (e == hipSuccess) e = upload(&b->d_res_work, res_work, &by);
+    if (e == hipSuccess && dig > 0) {
+        e = hipMalloc(&b->d_digits, (size_t)dig * 2);
+        by += (size_t)dig * 2;
+    }
+    if (e == hipSuccess && eoff > 0) {
+        e = hipMalloc((void **)&b->d_exps, (size_t)eoff * sizeof(int));
+        by += (size_t)eoff * sizeof(int);
+    }
+    if (e == hipSuccess && !dprobs.empty()) {
+        e = hipMalloc((void **)&b->d_err, dprobs.size() * sizeof(unsigned long long));
+        by += dprobs.size() * sizeof(unsigned long long);
+    }
+    if (e == hipSuccess && b->d_err) e = hipMemset(b->d_err, 0, dprobs.size() * sizeof(unsigned long long));
+    if (e != hipSuccess) {
+        split_batch_destroy(b);
+        return bad((e == hipErrorOutOfMemory) ? F4J_ERR_OUT_OF_MEMORY : F4J_ERR_HIP,
+                   std::string("batched split plan: ") + hipGetErrorString(e));
+    }
+    (void)R;
+    return b;
+}
+
+hipError_t split_batch_enqueue(const SplitBatch *b, hipStream_t st) {
+    for (const BatchProblem &q : b->zero_k) {
+        const int w = b->cplx ? 2 : 1;
+        const hipError_t e = launch_zero_fill(q.C, q.M, q.N * w, q.ldc * w, st);
+        if (e != hipSuccess) return e;
+    }
+    if (b->n_items == 0) return hipGetLastError();
+    const bool fp16 = b->carrier == SPLIT_FP16;
+    hipLaunchKernelGGL(bexp_kernel, dim3((unsigned)b->n_exp), dim3(256), 0, st, b->d_sides, b->d_exp_work, b->d_exps);
+    if (fp16)
+        launch_bsplit<10, _Float16>(b->S, dim3((unsigned)b->n_split), st, b->d_sides, b->d_split_work, b->d_exps,
+                                    (_Float16 *)b->d_digits);
+    else
+        launch_bsplit<9, __bf16>(b->S, dim3((unsigned)b->n_split), st, b->d_sides, b->d_split_work, b->d_exps,
+                                 (__bf16 *)b->d_digits);
+    if (fp16)
+        hipLaunchKernelGGL((bgemm_kernel<_Float16, 64>), dim3((unsigned)b->workers), dim3(256), 0, st, b->d_items,
+                           b->d_begin, b->d_probs, b->d_jobs, b->d_sides, b->d_groups, (const _Float16 *)b->d_digits,
+                           b->d_exps, (int)b->cplx);
+    else
+        hipLaunchKernelGGL((bgemm_kernel<__bf16, 256>), dim3((unsigned)b->workers), dim3(256), 0, st, b->d_items,
+                           b->d_begin, b->d_probs, b->d_jobs, b->d_sides, b->d_groups, (const __bf16 *)b->d_digits,
+                           b->d_exps, (int)b->cplx);
+    if (b->n_res > 0) {
+        const int64_t np = (int64_t)b->prob_index.size();
+        (void)np;
+        hipLaunchKernelGGL(fill_u64_kernel, dim3(64), dim3(256), 0, st, b->d_err, (int64_t)std::count_if(
+                               b->prob_index.begin(), b->prob_index.end(), [](int v) { return v >= 0; }));
+        hipLaunchKernelGGL(bres_kernel, dim3((unsigned)b->n_res), dim3(256), 0, st, b->d_probs, b->d_res_work,
+                           (int)b->cplx, b->opA, b->opB, b->d_err);
+    }
+    return hipGetLastError();
+}
+
+hipError_t split_batch_errors(const SplitBatch *b, double *out, int64_t count, hipStream_t st) {
+    const int64_t np = (int64_t)std::count_if(b->prob_index.begin(), b->prob_index.end(), [](int v) { return v >= 0; });
+    std::vector<unsigned long long> bits(np, 0);
+    if (np > 0 && b->check_rows > 0) {
+        hipError_t e = hipMemcpyAsync(bits.data(), b->d_err, np * sizeof(unsigned long long), hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        if (e != hipSuccess) return e;
+    }
+    for (int64_t i = 0; i < std::min<int64_t>(count, (int64_t)b->prob_index.size()); ++i) {
+        const int k = b->prob_index[i];
+        out[i] = (k < 0 || b->check_rows <= 0) ? -1.0 : __builtin_bit_cast(double, bits[k]);
+    }
+    return hipSuccess;
+}
+
+int split_batch_real_products(const SplitBatch *b) { return b->real_products; }
+int split_batch_checked_rows(const SplitBatch *b) { return b->check_rows; }
+size_t split_batch_bytes(const SplitBatch *b) { return b->bytes; }
+
+void split_batch_destroy(SplitBatch *b) {
+    if (!b) return;
+    for (void *p : {(void *)b->d_sides, (void *)b->d_jobs, (void *)b->d_probs, (void *)b->d_groups, (void *)b->d_items,
+                    (void *)b->d_begin, (void *)b->d_exp_work, (void *)b->d_split_work, (void *)b->d_res_work,
+                    b->d_digits, (void *)b->d_exps, (void *)b->d_err})
+        if (p) (void)hipFree(p);
+    delete b;
+}
+
+}  // namespace f4j

@@ -13,10 +13,13 @@ constexpr int SPLIT_MAX_SEGS = SPLIT_MAX_SLICES * (SPLIT_MAX_SLICES + 1) / 2;
 
 enum SplitCarrier { SPLIT_BF16 = 0, SPLIT_FP16 = 1 };
 
-// One launch of the slice-product GEMM: digit pairs whose levels lie in
-// [Llo, Lhi], summed exactly in FP64.
+// One launch of the slice-product GEMM: the digit pairs (s, t) whose levels
+// L = s + t lie in [Llo, Lhi], summed exactly in FP64 with weights
+// 2^shift = 2^(R (Lhi - L)); the sum is scaled by 2^(eA + eB + base_exp).
 struct SplitGroup {
     int Llo, Lhi;
+    int base_exp, nseg;
+    int seg_a[SPLIT_MAX_SEGS], seg_b[SPLIT_MAX_SEGS], seg_shift[SPLIT_MAX_SEGS];
 };
 
 struct SplitPlan {
@@ -41,6 +44,10 @@ struct SliceGemmArgs {
 
 int split_radix(int carrier);
 int split_flush(int carrier);
+
+// Level groups for S digits on `carrier` with padded inner size Kp, smallest
+// weights first: each group's FP64 sum stays below 2^52, hence exact.
+std::vector<SplitGroup> split_groups(int S, int carrier, int64_t Kp);
 
 // Sizes, workspace layout and launch groups for C (M x N) = op(A) op(B),
 // with room for a residual check of `check_rows` rows.

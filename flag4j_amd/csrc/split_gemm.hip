@@ -42,14 +42,13 @@
 
 #include "f4j_internal.h"
 #include "split_gemm.h"
+#include "split_tile.h"
 
 namespace f4j {
 namespace {
 
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
-typedef float f32x16 __attribute__((ext_vector_type(16)));
-typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+using slice::digits;
+using slice::u32x4;
 
 
 // ------------------------------------------------------------------------
@@ -97,18 +96,6 @@ __global__ __launch_bounds__(256) void exp_cols_kernel(const double *X, int64_t 
 // Digit split.  Output: S slices, slice s at out + s*Xp*Kp, row-major Xp x Kp
 // (k contiguous), zero outside the X x K matrix.
 // ------------------------------------------------------------------------
-
-template <int R, int S, class T>
-__device__ inline void digits(double v, int e, T *d) {
-    // v * 2^-e in (-1, 1); all steps exact.
-    double y = (e == INT_MIN) ? 0.0 : ldexp(v, (R - 1) - e);
-#pragma unroll
-    for (int s = 0; s < S; ++s) {
-        const double q = rint(y);
-        d[s] = (T)(float)q;
-        y = (y - q) * (double)(1 << R);
-    }
-}
 
 // Lines contiguous: thread handles 8 consecutive k of one line.
 template <int R, int S, class T>
@@ -174,133 +161,37 @@ __global__ __launch_bounds__(256) void split_cols_kernel(const double *X, int64_
 // result item r of lane l is row 8(r/4) + 4(l/32) + r%4, column l%32.
 // ------------------------------------------------------------------------
 
-template <class T>
-__device__ inline f32x16 mfma32(const u32x4 &a, const u32x4 &b, const f32x16 &c) {
-    if constexpr (sizeof(T) == 2 && std::is_same<T, __bf16>::value)
-        return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c,
-                                                       0, 0, 0);
-    else
-        return __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(f16x8, a), __builtin_bit_cast(f16x8, b), c, 0,
-                                                      0, 0);
-}
+constexpr int SBM = slice::BM, SBN = slice::BN;
 
-constexpr int SBM = 128, SBN = 128, SBK = 64;     // workgroup tile, k slice (elements)
-constexpr int SLD = SBK + 8;                     // LDS row stride in elements (16-byte pad)
-constexpr int STAGE = (SBM + SBN) * SLD;         // elements per stage
+// Digit pairs of one launch, read from the kernel arguments.
+struct ArgSegs {
+    const SliceGemmArgs &args;
+    __device__ int a(int g) const { return args.seg_a[g]; }
+    __device__ int b(int g) const { return args.seg_b[g]; }
+    __device__ int shift(int g) const { return args.seg_shift[g]; }
+};
 
 template <class T, int FLUSH>
 __global__ __launch_bounds__(256, 2) void slice_gemm_kernel(SliceGemmArgs a) {
-    __shared__ __attribute__((aligned(16))) T lds[2 * STAGE];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    __shared__ __attribute__((aligned(16))) T lds[2 * slice::STAGE];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int64_t tiles_n = a.Np / SBN;
     const int64_t m0 = (blockIdx.x / tiles_n) * SBM, n0 = (blockIdx.x % tiles_n) * SBN;
     const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
-
-    f32x16 acc[2][2];
     double acc64[2][2][16];
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j) {
-#pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                acc[i][j][r] = 0.f;
-                acc64[i][j][r] = 0.0;
-            }
-        }
-
-    const int tps = (int)(a.Kp / SBK);                // k-tiles per segment
-    const int ntiles = tps * a.nseg;
-    constexpr int FLUSH_TILES = FLUSH / SBK;          // Kp is a multiple of FLUSH
-    // per-lane load geometry: 4 passes of 32 rows x 8 16-byte columns
-    const int lrow = tid >> 3, lcol = (tid & 7) * 8;
-    int lane_off[4];
-#pragma unroll
-    for (int p = 0; p < 4; ++p) lane_off[p] = (int)(((int64_t)(lrow + 32 * p) * a.Kp + lcol) * (int64_t)sizeof(T));
-    u32x4 ra[4], rb[4];
-
-    // (segment, k offset) of the tile to load next, advanced without division
-    int ld_seg = 0, ld_kt = 0;
-    auto gload = [&]() {
-        const int64_t kk = (int64_t)ld_kt * SBK;
-        const T *pa = (const T *)a.A + (int64_t)a.seg_a[ld_seg] * a.Mp * a.Kp + m0 * a.Kp + kk;
-        const T *pb = (const T *)a.B + (int64_t)a.seg_b[ld_seg] * a.Np * a.Kp + n0 * a.Kp + kk;
-        const __amdgpu_buffer_rsrc_t rsa = __builtin_amdgcn_make_buffer_rsrc((void *)pa, (short)0, 0x7FFFFFF0, 0x00020000);
-        const __amdgpu_buffer_rsrc_t rsb = __builtin_amdgcn_make_buffer_rsrc((void *)pb, (short)0, 0x7FFFFFF0, 0x00020000);
-#pragma unroll
-        for (int p = 0; p < 4; ++p) {
-            ra[p] = __builtin_bit_cast(u32x4, __builtin_amdgcn_raw_buffer_load_b128(rsa, lane_off[p], 0, 0));
-            rb[p] = __builtin_bit_cast(u32x4, __builtin_amdgcn_raw_buffer_load_b128(rsb, lane_off[p], 0, 0));
-        }
-        if (++ld_kt == tps) {
-            ld_kt = 0;
-            ++ld_seg;
-        }
-    };
-    auto lstore = [&](T *st) {
-#pragma unroll
-        for (int p = 0; p < 4; ++p) {
-            *(u32x4 *)(st + (lrow + 32 * p) * SLD + lcol) = ra[p];
-            *(u32x4 *)(st + SBM * SLD + (lrow + 32 * p) * SLD + lcol) = rb[p];
-        }
-    };
-
-    gload();
-    lstore(lds);
-    __syncthreads();
-    const int fr = lane & 31, fk = (lane >> 5) * 8;
-    int seg = 0, kt_in_seg = 0, since_flush = 0;
-    for (int t = 0; t < ntiles; ++t) {
-        const T *cur = lds + (t & 1) * STAGE;
-        if (t + 1 < ntiles) gload();
-#pragma unroll
-        for (int ks = 0; ks < SBK / 16; ++ks) {
-            u32x4 fa[2], fb[2];
-#pragma unroll
-            for (int i = 0; i < 2; ++i) fa[i] = *(const u32x4 *)(cur + (wm + 32 * i + fr) * SLD + ks * 16 + fk);
-#pragma unroll
-            for (int j = 0; j < 2; ++j) fb[j] = *(const u32x4 *)(cur + SBM * SLD + (wn + 32 * j + fr) * SLD + ks * 16 + fk);
-#pragma unroll
-            for (int i = 0; i < 2; ++i)
-#pragma unroll
-                for (int j = 0; j < 2; ++j) acc[i][j] = mfma32<T>(fa[i], fb[j], acc[i][j]);
-        }
-        // Flush the exact FP32 partial sums into the FP64 accumulator every
-        // FLUSH k and at the end of each segment (the scale changes there).
-        ++kt_in_seg;
-        ++since_flush;
-        if (since_flush == FLUSH_TILES || kt_in_seg == tps) {
-            const double sc = __builtin_bit_cast(double, (uint64_t)(1023 + a.seg_shift[seg]) << 52);
-#pragma unroll
-            for (int i = 0; i < 2; ++i)
-#pragma unroll
-                for (int j = 0; j < 2; ++j)
-#pragma unroll
-                    for (int r = 0; r < 16; ++r) {
-                        acc64[i][j][r] = fma((double)acc[i][j][r], sc, acc64[i][j][r]);
-                        acc[i][j][r] = 0.f;
-                    }
-            since_flush = 0;
-            if (kt_in_seg == tps) {
-                kt_in_seg = 0;
-                ++seg;
-            }
-        }
-        if (t + 1 < ntiles) lstore(lds + ((t + 1) & 1) * STAGE);
-        __syncthreads();
-    }
+    slice::tile_loop<T, FLUSH, 2, 2>(a.A, a.B, a.Mp, a.Np, a.Kp, a.nseg, ArgSegs{a}, m0, n0, lds, acc64);
 
     // Epilogue: C = beta C + 2^(eA+eB+base) acc64.
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
-            const int64_t col = n0 + wn + 32 * j + (lane & 31);
+            const int64_t col = n0 + slice::acc_col(wn, j, lane);
             if (col >= a.N) continue;
             const int eb = a.eB[col];
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
-                const int64_t row = m0 + wm + 32 * i + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+                const int64_t row = m0 + slice::acc_row(wm, i, r, lane);
                 if (row >= a.M) continue;
                 const int ea = a.eA[row];
                 double v = 0.0;
@@ -403,6 +294,42 @@ void launch_split_s(int S, const double *X, int64_t ld, bool lines, int64_t nx, 
 int split_radix(int carrier) { return carrier == SPLIT_FP16 ? 10 : 9; }
 int split_flush(int carrier) { return carrier == SPLIT_FP16 ? 64 : 256; }
 
+std::vector<SplitGroup> split_groups(int S, int carrier, int64_t Kp) {
+    // Group the triangular pairs (s + t <= S + 1) by level, from the highest
+    // level (smallest weight) down, keeping each group's FP64 sum exact:
+    // sum_L 2^(R (Lhi - L)) npairs(L) Kp 2^(2R-2) < 2^52.
+    std::vector<SplitGroup> out;
+    const int R = split_radix(carrier);
+    const double lim = std::ldexp(1.0, 52);
+    int L = S - 1;
+    while (L >= 0) {
+        SplitGroup g{};
+        g.Lhi = L;
+        g.Llo = L;
+        double bound = (double)(L + 1) * (double)Kp * std::ldexp(1.0, 2 * R - 2);
+        while (g.Llo > 0) {
+            const int Ln = g.Llo - 1;
+            const double nb = bound + (double)(Ln + 1) * (double)Kp * std::ldexp(1.0, 2 * R - 2 + R * (L - Ln));
+            if (nb >= lim) break;
+            bound = nb;
+            g.Llo = Ln;
+        }
+        g.base_exp = -(2 * R - 2) - R * g.Lhi;
+        for (int Lv = g.Lhi; Lv >= g.Llo; --Lv)
+            for (int s = 0; s <= Lv; ++s) {
+                const int t = Lv - s;
+                if (s >= S || t >= S) continue;
+                g.seg_a[g.nseg] = s;
+                g.seg_b[g.nseg] = t;
+                g.seg_shift[g.nseg] = R * (g.Lhi - Lv);
+                ++g.nseg;
+            }
+        out.push_back(g);
+        L = g.Llo - 1;
+    }
+    return out;
+}
+
 SplitPlan plan_split(int64_t M, int64_t N, int64_t K, int slices, int carrier, int check_rows) {
     SplitPlan pl;
     pl.S = slices;
@@ -419,27 +346,7 @@ SplitPlan plan_split(int64_t M, int64_t N, int64_t K, int slices, int carrier, i
     pl.off_residual = pl.off_err + 256;
     pl.bytes = pl.off_residual + split_residual_bytes(N, K, (int)std::min<int64_t>(check_rows, M));
 
-    // Group the triangular pairs (s + t <= S + 1) by level, from the highest
-    // level (smallest weight) down, keeping each group's FP64 sum exact:
-    // sum_L 2^(R (Lhi - L)) npairs(L) Kp 2^(2R-2) < 2^52.
-    const int R = split_radix(carrier);
-    const double lim = std::ldexp(1.0, 52);
-    int L = pl.S - 1;
-    while (L >= 0) {
-        SplitGroup g;
-        g.Lhi = L;
-        g.Llo = L;
-        double bound = (double)(L + 1) * (double)pl.Kp * std::ldexp(1.0, 2 * R - 2);
-        while (g.Llo > 0) {
-            const int Ln = g.Llo - 1;
-            const double nb = bound + (double)(Ln + 1) * (double)pl.Kp * std::ldexp(1.0, 2 * R - 2 + R * (L - Ln));
-            if (nb >= lim) break;
-            bound = nb;
-            g.Llo = Ln;
-        }
-        pl.groups.push_back(g);
-        L = g.Llo - 1;
-    }
+    pl.groups = split_groups(pl.S, carrier, pl.Kp);
     return pl;
 }
 
@@ -468,7 +375,6 @@ hipError_t launch_split_gemm(const SplitPlan &pl, int opA, int opB, int64_t M, i
 
     // slice products, smallest weights first
     if (ev_begin) hipEventRecord(ev_begin, stream);
-    const int R = split_radix(pl.carrier);
     bool first = true;
     for (const SplitGroup &g : pl.groups) {
         SliceGemmArgs a{};
@@ -484,17 +390,13 @@ hipError_t launch_split_gemm(const SplitPlan &pl, int opA, int opB, int64_t M, i
         a.C = C;
         a.ldc = ldc;
         a.beta = first ? 0 : 1;
-        a.base_exp = -(2 * R - 2) - R * g.Lhi;
-        a.nseg = 0;
-        for (int L = g.Lhi; L >= g.Llo; --L)
-            for (int s = 0; s <= L; ++s) {
-                const int t = L - s;
-                if (s >= S || t >= S) continue;
-                a.seg_a[a.nseg] = s;
-                a.seg_b[a.nseg] = t;
-                a.seg_shift[a.nseg] = R * (g.Lhi - L);
-                ++a.nseg;
-            }
+        a.base_exp = g.base_exp;
+        a.nseg = g.nseg;
+        for (int q = 0; q < g.nseg; ++q) {
+            a.seg_a[q] = g.seg_a[q];
+            a.seg_b[q] = g.seg_b[q];
+            a.seg_shift[q] = g.seg_shift[q];
+        }
         const unsigned blocks = (unsigned)((pl.Mp / SBM) * (pl.Np / SBN));
         if (fp16)
             hipLaunchKernelGGL((slice_gemm_kernel<_Float16, 64>), dim3(blocks), dim3(256), 0, stream, a);

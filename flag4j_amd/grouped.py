@@ -18,7 +18,10 @@ class GroupedPlan:
     """A plan over device-resident operands (device pointers as ints)."""
 
     def __init__(self, shapes, a_ptrs, b_ptrs, c_ptrs, lda, ldb, ldc, opA="N", opB="N", complex_=False,
-                 precision="fp64"):
+                 precision="fp64", slices=0, use3m=True):
+        """precision "fp64", or "split_bf16" / "split_fp16" for the batched split
+        form (`slices` digits, 0 = default; complex products as three real
+        products when `use3m`, else four)."""
         n = len(shapes)
         M = np.array([s[0] for s in shapes], dtype=np.int64)
         N = np.array([s[1] for s in shapes], dtype=np.int64)
@@ -30,10 +33,17 @@ class GroupedPlan:
         self._keep += [A, B, C, la, lb, lc]
         p = lambda a: ctypes.c_void_p(a.ctypes.data) if a.size else None
         h = ctypes.c_void_p()
-        _native.check(_native.load().f4j_grouped_plan_create(
-            int(complex_), _native.OP[opA], _native.OP[opB], n, p(M), p(N), p(K), p(A), p(la), p(B), p(lb), p(C),
-            p(lc), _native.PREC[precision], ctypes.byref(h)))
+        self.split = precision in ("split_bf16", "split_fp16")
+        if self.split:
+            _native.check(_native.load().f4j_grouped_plan_create_split(
+                int(complex_), _native.OP[opA], _native.OP[opB], n, p(M), p(N), p(K), p(A), p(la), p(B), p(lb),
+                p(C), p(lc), _native.PREC[precision], int(slices), int(bool(use3m)), ctypes.byref(h)))
+        else:
+            _native.check(_native.load().f4j_grouped_plan_create(
+                int(complex_), _native.OP[opA], _native.OP[opB], n, p(M), p(N), p(K), p(A), p(la), p(B), p(lb),
+                p(C), p(lc), _native.PREC[precision], ctypes.byref(h)))
         self.handle = h
+        self.count = n
         self.flops_real = float(np.sum(2.0 * M * N * K)) * (4 if complex_ else 1)
 
     def run(self, stream=0):
@@ -49,6 +59,15 @@ class GroupedPlan:
         _native.check(_native.load().f4j_grouped_plan_info(self.handle, ctypes.byref(fl), cfg, tiles, 4))
         return {"flops": fl.value, "tile_cfg": list(cfg), "tiles": list(tiles)}
 
+    def split_report(self):
+        """After a run of a split plan: (per-problem max |C - AB| / (|A||B|) over
+        the checked rows, -1 where unchecked; real slice products per run)."""
+        err = np.zeros(self.count, dtype=np.float64)
+        prods = ctypes.c_int()
+        _native.check(_native.load().f4j_grouped_plan_split_report(
+            self.handle, ctypes.c_void_p(err.ctypes.data) if self.count else None, self.count, ctypes.byref(prods)))
+        return err, prods.value
+
     def close(self):
         if self.handle:
             _native.check(_native.load().f4j_grouped_plan_destroy(self.handle))
@@ -61,9 +80,10 @@ class GroupedPlan:
             pass
 
 
-def mult_all(As, Bs, opA="N", opB="N", precision="fp64"):
+def mult_all(As, Bs, opA="N", opB="N", precision="fp64", slices=0, use3m=True, report=False):
     """[op(A_i) op(B_i) for i] for numpy arrays (float64 or complex128), in one
-    grouped plan: operands are packed into one device buffer each way."""
+    grouped plan: operands are packed into one device buffer each way.  With a
+    split precision and report=True, also returns the plan's split report."""
     cplx = any(np.iscomplexobj(a) for a in As) or any(np.iscomplexobj(b) for b in Bs)
     dt = np.complex128 if cplx else np.float64
     As = [np.ascontiguousarray(a, dtype=dt) for a in As]
@@ -89,9 +109,11 @@ def mult_all(As, Bs, opA="N", opB="N", precision="fp64"):
                            [dB.ptr + int(o) * es for o in offs_b[:-1]],
                            [dC.ptr + int(o) * es for o in offs_c[:-1]],
                            [max(a.shape[1], 1) for a in As], [max(b.shape[1], 1) for b in Bs],
-                           [max(n, 1) for _, n, _ in shapes], opA, opB, cplx, precision)
+                           [max(n, 1) for _, n, _ in shapes], opA, opB, cplx, precision, slices, use3m)
         plan.run()
         _native.check(_native.load().f4j_stream_sync(None))
         out = dC.download(dt, int(offs_c[-1])) if offs_c[-1] else np.zeros(0, dt)
+        rep = plan.split_report() if (report and plan.split) else None
         plan.close()
-    return [out[offs_c[i]:offs_c[i + 1]].reshape(shapes[i][0], shapes[i][1]) for i in range(len(shapes))]
+    res = [out[offs_c[i]:offs_c[i + 1]].reshape(shapes[i][0], shapes[i][1]) for i in range(len(shapes))]
+    return (res, rep) if report else res

@@ -139,6 +139,21 @@ int f4j_grouped_plan_create(int is_complex, int opA, int opB, int64_t count,
                             const double *const *B, const int64_t *ldb,
                             double *const *C, const int64_t *ldc, int precision,
                             f4j_grouped_plan **plan);
+/* Batched split-precision products (no reference counterpart: Flag4j is FP64-only; the batch
+ * replaces a loop of Matrix.mult / A.H().mult(B) calls).  precision F4J_PREC_SPLIT_BF16 or
+ * F4J_PREC_SPLIT_FP16; slices 0 = 6 (BF16) / 5 (FP16); complex products run as three real
+ * products (3M, Gauss) when use3m != 0, else four.  splitCheckRows rows of every product are
+ * verified after each run.  f4j_grouped_plan_create with a SPLIT_* precision is this call with
+ * slices 0 and use3m 1.  Plans run, tune (a no-op: one tile shape) and destroy like FP64 plans. */
+int f4j_grouped_plan_create_split(int is_complex, int opA, int opB, int64_t count,
+                                  const int64_t *M, const int64_t *N, const int64_t *K,
+                                  const double *const *A, const int64_t *lda,
+                                  const double *const *B, const int64_t *ldb,
+                                  double *const *C, const int64_t *ldc, int precision, int slices,
+                                  int use3m, f4j_grouped_plan **plan);
+/* After a run of a split plan: err[p] = max |C - AB| / (|A||B|) over the checked rows of problem p
+ * (-1 if none), *products = real slice products per run.  Waits for the last run. */
+int f4j_grouped_plan_split_report(f4j_grouped_plan *plan, double *err, int64_t count, int *products);
 int f4j_grouped_plan_run(f4j_grouped_plan *plan, void *stream);
 int f4j_grouped_plan_destroy(f4j_grouped_plan *plan);
 /* Work of the plan (flops, classical count) and, per size bucket (nb <= 4 entries), the tile
