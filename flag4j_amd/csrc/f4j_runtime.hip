@@ -24,6 +24,7 @@
 #include "f4j_internal.h"
 #include "gemm_tile.h"
 #include "crt_gemm.h"
+#include "split_batch.h"
 #include "split_gemm.h"
 
 namespace f4j {
@@ -277,6 +278,72 @@ int run_split(int opA, int opB, int64_t M, int64_t N, int64_t K, const double *A
     return F4J_OK;
 }
 
+// FP64 MFMA GEMM of real or complex (real-expanded) operands.
+hipError_t launch_fp64(bool cplx, int opA, int opB, int64_t M, int64_t N, int64_t K, const double *A, int64_t lda,
+                       const double *B, int64_t ldb, double *C, int64_t ldc, hipStream_t stream) {
+    const int w = cplx ? 2 : 1;
+    GemmProblem p{M, N * w, K * w, A, lda * w, B, ldb * w, C, ldc * w, 0};
+    return launch_gemm_f64(kind_of(opA, cplx), kind_of(opB, cplx), p, stream, pick_tile(p.M, p.N));
+}
+
+// Complex split-precision GEMM: the batched split engine on a batch of one
+// (three real products in the 3M form when use3m, else four), with the same
+// residual check, fallback and report as the real form.
+int run_split_cplx(int opA, int opB, int64_t M, int64_t N, int64_t K, const double *A, int64_t lda, const double *B,
+                   int64_t ldb, double *C, int64_t ldc, int prec, int slices, int use3m, hipStream_t stream) {
+    const int carrier = prec == F4J_PREC_SPLIT_FP16 ? SPLIT_FP16 : SPLIT_BF16;
+    const int S_ = slices > 0 ? slices : default_slices_for(prec);
+    if (S_ < 1 || S_ > SPLIT_MAX_SLICES)
+        return fail(F4J_ERR_INVALID_ARG, "slices must be in 1.." + std::to_string(SPLIT_MAX_SLICES));
+    Config c;
+    {
+        std::lock_guard<std::mutex> g(S().cfg_mu);
+        c = S().cfg;
+    }
+    const std::vector<BatchProblem> pr{BatchProblem{M, N, K, A, lda, B, ldb, C, ldc}};
+    std::string err;
+    int status = F4J_OK;
+    SplitBatch *b = split_batch_create(true, opA, opB, carrier, S_, use3m != 0, c.split_check_rows, pr, &err, &status);
+    if (!b) return fail(status, err);
+    State &s = S();
+    std::lock_guard<std::mutex> g(s.ws_mu);
+    hipEvent_t *ev = c.split_timing ? s.tev : nullptr;
+    hipError_t e = split_batch_enqueue(b, stream, ev ? ev[0] : nullptr, ev ? ev[1] : nullptr);
+    f4j_split_report rep{};
+    rep.precision = prec;
+    rep.slices = S_;
+    rep.products = split_batch_real_products(b);
+    rep.launches = 1;
+    rep.checked_rows = (int)std::min<int64_t>(c.split_check_rows, M);
+    rep.max_rel_err = -1.0;
+    rep.product_ms = -1.0;
+    double perr = -1.0;
+    if (e == hipSuccess) e = split_batch_errors(b, &perr, 1, stream);      // synchronises
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    split_batch_destroy(b);
+    if (e != hipSuccess) return hip_fail(e, "complex split gemm");
+    if (ev) {
+        float ms = 0.f;
+        F4J_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]), "hipEventElapsedTime");
+        rep.product_ms = ms;
+    }
+    if (rep.checked_rows > 0) {
+        rep.max_rel_err = perr;
+        if (!(perr <= c.split_tolerance)) {
+            if (!c.split_fallback) {
+                t_split = rep;
+                return fail(F4J_ERR_ACCURACY, "complex split product error " + std::to_string(perr) +
+                                                  " above tolerance " + std::to_string(c.split_tolerance));
+            }
+            F4J_HIP(launch_fp64(true, opA, opB, M, N, K, A, lda, B, ldb, C, ldc, stream), "fp64 fallback");
+            F4J_HIP(hipStreamSynchronize(stream), "stream synchronize");
+            rep.precision = F4J_PREC_FP64;
+        }
+    }
+    t_split = rep;
+    return F4J_OK;
+}
+
 // Modular (CRT) split-precision real GEMM on the INT8 matrix cores;
 // `slices` = number of moduli (0: enough for FP64 accuracy at this K).
 int run_crt(int opA, int opB, int64_t M, int64_t N, int64_t K, const double *A, int64_t lda, const double *B,
@@ -343,8 +410,6 @@ int run_crt(int opA, int opB, int64_t M, int64_t N, int64_t K, const double *A, 
 int gemm_dev(bool cplx, int opA, int opB, int64_t M, int64_t N, int64_t K, const double *A, int64_t lda,
              const double *B, int64_t ldb, double *C, int64_t ldc, int precision, int slices, int use3m,
              hipStream_t stream) {
-    (void)slices;
-    (void)use3m;
     if (M == 0 || N == 0) return F4J_OK;
     const int w = cplx ? 2 : 1;
     if (K == 0) {
@@ -358,6 +423,8 @@ int gemm_dev(bool cplx, int opA, int opB, int64_t M, int64_t N, int64_t K, const
     }
     if ((prec == F4J_PREC_SPLIT_BF16 || prec == F4J_PREC_SPLIT_FP16) && !cplx)
         return run_split(opA, opB, M, N, K, A, lda, B, ldb, C, ldc, prec, slices, stream);
+    if ((prec == F4J_PREC_SPLIT_BF16 || prec == F4J_PREC_SPLIT_FP16) && cplx)
+        return run_split_cplx(opA, opB, M, N, K, A, lda, B, ldb, C, ldc, prec, slices, use3m, stream);
     if (prec == F4J_PREC_SPLIT_INT8 && !cplx)
         return run_crt(opA, opB, M, N, K, A, lda, B, ldb, C, ldc, slices, stream);
     if (prec != F4J_PREC_FP64)

@@ -30,7 +30,9 @@ struct SplitBatch;
 SplitBatch *split_batch_create(bool cplx, int opA, int opB, int carrier, int slices, bool use3m, int check_rows,
                                const std::vector<BatchProblem> &probs, std::string *err, int *status);
 // Enqueue the whole batch on `stream` (no allocation, no memset: capturable).
-hipError_t split_batch_enqueue(const SplitBatch *b, hipStream_t stream);
+// ev_begin / ev_end (optional) are recorded around the slice-product launch.
+hipError_t split_batch_enqueue(const SplitBatch *b, hipStream_t stream, hipEvent_t ev_begin = nullptr,
+                               hipEvent_t ev_end = nullptr);
 // Per-problem max |C - AB| / (|A||B|) over the checked rows of the last run
 // (-1 where nothing was checked); synchronises `stream`.
 hipError_t split_batch_errors(const SplitBatch *b, double *out, int64_t count, hipStream_t stream);

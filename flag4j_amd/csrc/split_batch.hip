@@ -546,7 +546,7 @@ SplitBatch *split_batch_create(bool cplx, int opA, int opB, int carrier, int sli
     return b;
 }
 
-hipError_t split_batch_enqueue(const SplitBatch *b, hipStream_t st) {
+hipError_t split_batch_enqueue(const SplitBatch *b, hipStream_t st, hipEvent_t ev_begin, hipEvent_t ev_end) {
     for (const BatchProblem &q : b->zero_k) {
         const int w = b->cplx ? 2 : 1;
         const hipError_t e = launch_zero_fill(q.C, q.M, q.N * w, q.ldc * w, st);
@@ -561,6 +561,7 @@ hipError_t split_batch_enqueue(const SplitBatch *b, hipStream_t st) {
     else
         launch_bsplit<9, __bf16>(b->S, dim3((unsigned)b->n_split), st, b->d_sides, b->d_split_work, b->d_exps,
                                  (__bf16 *)b->d_digits);
+    if (ev_begin) hipEventRecord(ev_begin, st);
     if (fp16)
         hipLaunchKernelGGL((bgemm_kernel<_Float16, 64>), dim3((unsigned)b->workers), dim3(256), 0, st, b->d_items,
                            b->d_begin, b->d_probs, b->d_jobs, b->d_sides, b->d_groups, (const _Float16 *)b->d_digits,
@@ -569,6 +570,7 @@ hipError_t split_batch_enqueue(const SplitBatch *b, hipStream_t st) {
         hipLaunchKernelGGL((bgemm_kernel<__bf16, 256>), dim3((unsigned)b->workers), dim3(256), 0, st, b->d_items,
                            b->d_begin, b->d_probs, b->d_jobs, b->d_sides, b->d_groups, (const __bf16 *)b->d_digits,
                            b->d_exps, (int)b->cplx);
+    if (ev_end) hipEventRecord(ev_end, st);
     if (b->n_res > 0) {
         const int64_t np = (int64_t)b->prob_index.size();
         (void)np;

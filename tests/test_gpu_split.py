@@ -252,3 +252,61 @@ def test_product_timing_report(prec, S):
         L.config_set("splitTiming", 0)
     _, rep = split_mult(A, B, prec, S)
     assert rep.product_ms == -1.0
+
+
+# ---------------------------------------------------------------- complex products (batched engine, batch of one)
+
+def zsplit_mult(A, B, prec, slices, use3m, opA="N", opB="N"):
+    M = A.shape[0] if opA == "N" else A.shape[1]
+    K = A.shape[1] if opA == "N" else A.shape[0]
+    N = B.shape[1] if opB == "N" else B.shape[0]
+    A, B = np.ascontiguousarray(A, np.complex128), np.ascontiguousarray(B, np.complex128)
+    C = np.empty((M, N), np.complex128)
+    L.check(L.load().f4j_zgemm(L.OP[opA], L.OP[opB], M, N, K, A.ctypes.data, A.shape[1], B.ctypes.data,
+                               B.shape[1], C.ctypes.data, N, L.PREC[prec], slices, int(use3m)))
+    return C, L.last_split_report()
+
+
+def zsampled_err(C, A, B, opA, opB, n=300, seed=0):
+    r = np.random.default_rng(seed)
+    I, J = r.integers(0, C.shape[0], n), r.integers(0, C.shape[1], n)
+    rh, rl, ih, il, ab = o.cplx_dot2(A, B, I, J, opA, opB)
+    return float(np.max(np.maximum(np.abs((C[I, J].real - rh) - rl), np.abs((C[I, J].imag - ih) - il)) / ab))
+
+
+@pytest.mark.parametrize("opA,opB", [("N", "N"), ("C", "N"), ("N", "T"), ("T", "C")])
+@pytest.mark.parametrize("use3m", [True, False])
+def test_complex_split_single(opA, opB, use3m):
+    M, N, K = 200, 150, 300
+    sa = (M, K) if opA == "N" else (K, M)
+    sb = (K, N) if opB == "N" else (N, K)
+    A = rng.uniform(41, *sa) + 1j * rng.uniform(42, *sa)
+    B = rng.uniform(43, *sb) + 1j * rng.uniform(44, *sb)
+    C, rep = zsplit_mult(A, B, "split_fp16", 6, use3m, opA, opB)
+    assert rep.precision == L.PREC["split_fp16"]
+    assert rep.products == (3 if use3m else 4) * 21
+    assert 0 <= rep.max_rel_err <= 1e-15
+    assert zsampled_err(C, A, B, opA, opB) <= 1e-15
+
+
+def test_complex_split_config2_shape_bf16():
+    """CMatrix.mult at 2048 x 1536 x 2048 (uniform[-1,1) parts), split BF16, 3M."""
+    A = rng.uniform(45, 2048, 1536) + 1j * rng.uniform(46, 2048, 1536)
+    B = rng.uniform(47, 1536, 2048) + 1j * rng.uniform(48, 1536, 2048)
+    C, rep = zsplit_mult(A, B, "split_bf16", 6, True)
+    assert rep.precision == L.PREC["split_bf16"] and 0 <= rep.max_rel_err <= 1e-15
+    assert zsampled_err(C, A, B, "N", "N", n=200) <= 1e-15
+
+
+def test_complex_split_fallback():
+    A = rng.uniform(49, 64, 64) + 1j * rng.uniform(50, 64, 64)
+    B = rng.uniform(51, 64, 64) + 1j * rng.uniform(52, 64, 64)
+    L.config_set("splitTolerance", 1e-30)
+    C, rep = zsplit_mult(A, B, "split_fp16", 2, True)
+    assert rep.precision == L.PREC["fp64"]
+    assert zsampled_err(C, A, B, "N", "N") <= 1e-15
+    L.config_set("splitFallback", 0)
+    C = np.empty((64, 64), np.complex128)
+    st = L.load().f4j_zgemm(0, 0, 64, 64, 64, A.ctypes.data, 64, B.ctypes.data, 64, C.ctypes.data, 64,
+                            L.PREC["split_fp16"], 2, 1)
+    assert st == L.ERR_ACCURACY
