@@ -13,8 +13,11 @@ slices; FP16 digits, 5 slices); and the modular form (INT8 residues modulo
 covers exponents, splitting, slice products, recombination and the FP64
 residual check.  Each form's error is measured against exact sums on sampled
 entries.  `value` is the fastest form whose sampled error is <= 1e-15 of
-(|A||B|)_ij.  The batched workload (10,000 products with M, N, K log-uniform
-on [16, 512], N(0,1) entries) is reported beside it.
+(|A||B|)_ij.  Reported beside it: the batched workload (10,000 products with
+M, N, K log-uniform on [16, 512], N(0,1) entries; FP64 and split-BF16 grouped
+plans) and the complex batched workload (2,000 A^H B with M, N, K log-uniform
+on [32, 1024], parts spread over 10^+-4; 3M on split-FP16 and the FP64 grouped
+plan).
 
 Prints one JSON line.  `roofline` is for the dominant kernel of the form that
 gives `value` (the FP64 GEMM, or the slice-product launch of a split form),
@@ -105,7 +108,8 @@ def exact_entries(np, A, B, I, J):
 def bench_batched(np, torch, f4j, rng, dev, count=10000, reps=10):
     """10,000 FP64 products, M, N, K log-uniform on [16, 512] (java.util.Random
     seed 4), N(0,1) entries drawn on the device; one grouped plan (tuned), its
-    hipGraph replayed `reps` times."""
+    hipGraph replayed `reps` times.  The same batch again as a split-BF16 plan
+    (7 slices: K goes down to 16), with its per-product residual-check error."""
     from flag4j_amd import grouped
     dims = rng.log_uniform_ints(4, 3 * count, 16, 512).reshape(count, 3)
     M, N, K = dims[:, 0], dims[:, 1], dims[:, 2]
@@ -117,30 +121,104 @@ def bench_batched(np, torch, f4j, rng, dev, count=10000, reps=10):
     dA = torch.randn(int(oa[-1]), dtype=torch.float64, device=dev, generator=g)
     dB = torch.randn(int(ob[-1]), dtype=torch.float64, device=dev, generator=g)
     dC = torch.empty(int(oc[-1]), dtype=torch.float64, device=dev)
-    plan = grouped.GroupedPlan(list(zip(M.tolist(), N.tolist(), K.tolist())),
-                               [dA.data_ptr() + 8 * int(x) for x in oa[:-1]],
-                               [dB.data_ptr() + 8 * int(x) for x in ob[:-1]],
-                               [dC.data_ptr() + 8 * int(x) for x in oc[:-1]], K.tolist(), N.tolist(), N.tolist())
-    plan.tune(2)
+    args = (list(zip(M.tolist(), N.tolist(), K.tolist())), [dA.data_ptr() + 8 * int(x) for x in oa[:-1]],
+            [dB.data_ptr() + 8 * int(x) for x in ob[:-1]], [dC.data_ptr() + 8 * int(x) for x in oc[:-1]],
+            K.tolist(), N.tolist(), N.tolist())
     st = torch.cuda.Stream(device=dev)
-    plan.run(st.cuda_stream)
-    torch.cuda.synchronize()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record(st)
-    for _ in range(reps):
+
+    def timed(plan):
         plan.run(st.cuda_stream)
-    e1.record(st)
-    torch.cuda.synchronize()
-    ms = e0.elapsed_time(e1) / reps
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        for _ in range(reps):
+            plan.run(st.cuda_stream)
+        e1.record(st)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    plan = grouped.GroupedPlan(*args)
+    plan.tune(2)
+    ms = timed(plan)
     flops = float(np.sum(2.0 * M * N * K))
     roof = float(np.sum(np.maximum(2.0 * M * N * K / (PEAK_FP64_TFLOPS * 1e12), 8.0 * (M * K + K * N + M * N)
                                    / (PEAK_HBM_GBS * 1e9))))
     info = plan.info()
     plan.close()
-    return {"workload": "10000 FP64 products, M,N,K log-uniform on [16,512], N(0,1)", "ms": round(ms, 4),
-            "tflops": round(flops / ms / 1e9, 2), "per_problem_roofline_ms": round(roof * 1e3, 4),
-            "roofline_frac": round(roof * 1e3 / ms, 4), "tile_cfg_per_bucket": info["tile_cfg"],
-            "tiles_per_bucket": info["tiles"]}
+    out = {"workload": "10000 FP64 products, M,N,K log-uniform on [16,512], N(0,1)", "ms": round(ms, 4),
+           "tflops": round(flops / ms / 1e9, 2), "per_problem_roofline_ms": round(roof * 1e3, 4),
+           "roofline_frac": round(roof * 1e3 / ms, 4), "tile_cfg_per_bucket": info["tile_cfg"],
+           "tiles_per_bucket": info["tiles"]}
+    f4j._native.config_set("splitCheckRows", 1)
+    plan = grouped.GroupedPlan(*args, precision="split_bf16", slices=7)
+    ms_s = timed(plan)
+    errs, prods = plan.split_report()
+    plan.close()
+    f4j._native.config_set("splitCheckRows", 4)
+    out["split_bf16"] = {"slices": 7, "ms": round(ms_s, 4), "tflops_fp64_accurate": round(flops / ms_s / 1e9, 2),
+                         "real_slice_products": prods, "max_err_checked_rows": float(np.max(errs)),
+                         "products_over_1e-15": int(np.sum(errs > 1e-15)), "checked_rows_per_product": 1}
+    return out
+
+
+def bench_complex_batched(np, torch, f4j, rng, dev, count=2000, reps=3, slices=8):
+    """2,000 complex products A^H B (A stored K x M, B K x N), M, N, K log-uniform
+    on [32, 1024] (java.util.Random seed 5); real and imaginary parts
+    (2r-1) 10^(4 (2r'-1)) drawn on the device.  The 3M form on FP16 digits
+    (`slices`), one grouped split plan in a hipGraph, error from the residual
+    check (1 row of every product); the FP64 grouped plan on the same data."""
+    from flag4j_amd import grouped
+    dims = rng.log_uniform_ints(5, 3 * count, 32, 1024).reshape(count, 3)
+    M, N, K = dims[:, 0], dims[:, 1], dims[:, 2]
+    oa = np.cumsum(np.r_[0, K * M])
+    ob = np.cumsum(np.r_[0, K * N])
+    oc = np.cumsum(np.r_[0, M * N])
+    g = torch.Generator(device=dev)
+    g.manual_seed(2)
+
+    def wide(n):
+        r = torch.rand(n, dtype=torch.float64, device=dev, generator=g)
+        rp = torch.rand(n, dtype=torch.float64, device=dev, generator=g)
+        return (2 * r - 1) * torch.pow(10.0, 4 * (2 * rp - 1))
+
+    dA = torch.complex(wide(int(oa[-1])), wide(int(oa[-1])))
+    dB = torch.complex(wide(int(ob[-1])), wide(int(ob[-1])))
+    dC = torch.empty(int(oc[-1]), dtype=torch.complex128, device=dev)
+    args = (list(zip(M.tolist(), N.tolist(), K.tolist())), [dA.data_ptr() + 16 * int(x) for x in oa[:-1]],
+            [dB.data_ptr() + 16 * int(x) for x in ob[:-1]], [dC.data_ptr() + 16 * int(x) for x in oc[:-1]],
+            M.tolist(), N.tolist(), N.tolist(), "C", "N", True)
+    st = torch.cuda.Stream(device=dev)
+
+    def timed(plan):
+        plan.run(st.cuda_stream)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        for _ in range(reps):
+            plan.run(st.cuda_stream)
+        e1.record(st)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    flops = float(np.sum(8.0 * M * N * K))
+    f4j._native.config_set("splitCheckRows", 1)
+    plan = grouped.GroupedPlan(*args, precision="split_fp16", slices=slices, use3m=True)
+    ms_s = timed(plan)
+    errs, prods = plan.split_report()
+    plan.close()
+    f4j._native.config_set("splitCheckRows", 4)
+    plan = grouped.GroupedPlan(*args)
+    plan.tune(1)
+    ms_d = timed(plan)
+    plan.close()
+    del dA, dB, dC
+    torch.cuda.empty_cache()
+    return {"workload": "2000 complex128 A^H B, M,N,K log-uniform on [32,1024], parts (2r-1) 10^(4(2r'-1))",
+            "form": "3M on split-FP16", "slices": slices, "ms": round(ms_s, 3),
+            "tflops_classical": round(flops / ms_s / 1e9, 2), "real_slice_products": prods,
+            "max_err_checked_rows": float(np.max(errs)), "products_over_1e-15": int(np.sum(errs > 1e-15)),
+            "checked_rows_per_product": 1, "fp64_grouped_ms": round(ms_d, 3),
+            "fp64_grouped_tflops": round(flops / ms_d / 1e9, 2)}
 
 
 def load_traffic(name):
@@ -271,10 +349,12 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
-    batched = None
+    batched = complex_batched = None
     if not args.no_batched:
         batched = bench_batched(np, torch, f4j, rng, dev)
         log(f"[bench] batched: {batched}")
+        complex_batched = bench_complex_batched(np, torch, f4j, rng, dev)
+        log(f"[bench] complex batched: {complex_batched}")
 
     # End-to-end through the host-array ABI (PCIe included), one call.
     Ce = np.empty((n, n))
@@ -314,6 +394,7 @@ def main():
         "accuracy": {"max_err_over_absAabsB_sampled": forms[best[0]]["max_err"], "samples": 512,
                      "reference": "exact sums (TwoProduct + fsum) on sampled entries"},
         "batched": batched,
+        "complex_batched": complex_batched,
         "end_to_end_gflops_host_arrays": round(flops / e2e_s / 1e9, 2),
     }
     print(json.dumps(out), flush=True)
