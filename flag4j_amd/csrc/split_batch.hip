@@ -90,7 +90,8 @@ __device__ inline int exp_of(double v) {
 // Work item (side, first line): contiguous sides take 4 lines per block (a
 // wave per line, lanes along k); strided sides 256 lines (a thread per line,
 // lanes along the lines).  Lines past nx (padding) get INT_MIN.
-__global__ __launch_bounds__(256) void bexp_kernel(const Side *sides, const int2 *work, int *epool) {
+__global__ __launch_bounds__(256) void bexp_kernel(const Side *__restrict__ sides, const int2 *__restrict__ work,
+                                                   int *__restrict__ epool) {
     const int2 w = work[blockIdx.x];
     const Side s = sides[w.x];
     if (s.contig) {
@@ -115,7 +116,8 @@ __global__ __launch_bounds__(256) void bexp_kernel(const Side *sides, const int2
 // along k (contiguous sides) or along the lines (strided sides); thread t then
 // splits line t / 8, 8 consecutive k, and writes 16 bytes per digit slice.
 template <int R, int S, class T>
-__global__ __launch_bounds__(256) void bsplit_kernel(const Side *sides, const int4 *work, const int *epool, T *dpool) {
+__global__ __launch_bounds__(256) void bsplit_kernel(const Side *__restrict__ sides, const int4 *__restrict__ work,
+                                                     const int *__restrict__ epool, T *__restrict__ dpool) {
     __shared__ double tile[32][65];
     const int4 w = work[blockIdx.x];
     const Side s = sides[w.x];
@@ -152,7 +154,7 @@ __global__ __launch_bounds__(256) void bsplit_kernel(const Side *sides, const in
 }
 
 struct GroupSegs {
-    const SplitGroup *g;
+    const SplitGroup *__restrict__ g;
     __device__ int a(int q) const { return g->seg_a[q]; }
     __device__ int b(int q) const { return g->seg_b[q]; }
     __device__ int shift(int q) const { return g->seg_shift[q]; }
@@ -160,9 +162,15 @@ struct GroupSegs {
 
 // Persistent slice-product kernel: workgroup w runs items begin[w] .. begin[w+1].
 template <class T, int FLUSH>
-__global__ __launch_bounds__(256, 2) void bgemm_kernel(const Item *items, const int *begin, const Prob *probs,
-                                                       const Job *jobs, const Side *sides, const SplitGroup *groups,
-                                                       const T *dpool, const int *epool, int cplx) {
+// Tables are read-only for the kernel's lifetime (__restrict__): their loads
+// are uniform and go through the scalar cache into SGPRs, which leaves the
+// kernel at 165 VGPRs: 3 waves per SIMD (LDS 3 x 36 KB per CU).
+__global__ __launch_bounds__(256, 3) void bgemm_kernel(const Item *__restrict__ items, const int *__restrict__ begin,
+                                                       const Prob *__restrict__ probs, const Job *__restrict__ jobs,
+                                                       const Side *__restrict__ sides,
+                                                       const SplitGroup *__restrict__ groups,
+                                                       const T *__restrict__ dpool, const int *__restrict__ epool,
+                                                       int cplx) {
     __shared__ __attribute__((aligned(16))) T lds[2 * BTile::STAGE];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
@@ -233,8 +241,8 @@ __global__ __launch_bounds__(256) void fill_u64_kernel(unsigned long long *p, in
 // problem): compensated dot products (Dot2; Ogita, Rump and Oishi 2005) of
 // row i = y M / w against every column; max |C - AB| / (|A||B|) per problem,
 // |z| = |re| + |im| for complex entries.
-__global__ __launch_bounds__(256) void bres_kernel(const Prob *probs, const int4 *work, int cplx, int opA, int opB,
-                                                   unsigned long long *err) {
+__global__ __launch_bounds__(256) void bres_kernel(const Prob *__restrict__ probs, const int4 *__restrict__ work,
+                                                   int cplx, int opA, int opB, unsigned long long *__restrict__ err) {
     const int4 w = work[blockIdx.x];
     const Prob &p = probs[w.x];
     const int64_t i = (int64_t)w.y * p.M / w.w;
