@@ -211,31 +211,45 @@ __global__ __launch_bounds__(256, 2) void slice_gemm_kernel(SliceGemmArgs a) {
 
 constexpr int RES_KCHUNK = 256;
 
+constexpr int RES_ROWS = 4;                       // sampled rows per thread (one read of B serves all)
+
 // Pass 1: block (x, y, z) forms compensated partial dot products of sampled
-// row y over k-chunk z for 256 columns.
+// rows RES_ROWS y .. RES_ROWS y + 3 over k-chunk z for 256 columns.
 __global__ __launch_bounds__(256) void residual_partial_kernel(const double *A, int64_t lda, int opA, const double *B,
                                                                int64_t ldb, int opB, int64_t M, int64_t N, int64_t K,
-                                                               int64_t row_step, double *part) {
+                                                               int64_t row_step, int nrows, double *part) {
     const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    const int64_t i = min(M - 1, (int64_t)blockIdx.y * row_step);
     if (j >= N) return;
+    const int r0 = blockIdx.y * RES_ROWS;
+    int64_t ir[RES_ROWS];
+#pragma unroll
+    for (int u = 0; u < RES_ROWS; ++u) ir[u] = min(M - 1, (int64_t)min(r0 + u, nrows - 1) * row_step);
     const int64_t k0 = (int64_t)blockIdx.z * RES_KCHUNK, k1 = min(K, k0 + RES_KCHUNK);
-    double p = 0.0, q = 0.0, sa = 0.0;
+    double p[RES_ROWS], q[RES_ROWS], sa[RES_ROWS];
+#pragma unroll
+    for (int u = 0; u < RES_ROWS; ++u) p[u] = q[u] = sa[u] = 0.0;
     for (int64_t k = k0; k < k1; ++k) {
-        const double x = opA ? A[k * lda + i] : A[i * lda + k];
         const double y = opB ? B[j * ldb + k] : B[k * ldb + j];
-        const double h = x * y;
-        const double r = fma(x, y, -h);
-        const double s = p + h;
-        const double z = s - p;
-        q += ((p - (s - z)) + (h - z)) + r;
-        p = s;
-        sa = fma(fabs(x), fabs(y), sa);
+#pragma unroll
+        for (int u = 0; u < RES_ROWS; ++u) {
+            const double x = opA ? A[k * lda + ir[u]] : A[ir[u] * lda + k];
+            const double h = x * y;
+            const double r = fma(x, y, -h);
+            const double s = p[u] + h;
+            const double z = s - p[u];
+            q[u] += ((p[u] - (s - z)) + (h - z)) + r;
+            p[u] = s;
+            sa[u] = fma(fabs(x), fabs(y), sa[u]);
+        }
     }
-    double *o = part + (((int64_t)blockIdx.y * gridDim.z + blockIdx.z) * N + j) * 3;
-    o[0] = p;
-    o[1] = q;
-    o[2] = sa;
+#pragma unroll
+    for (int u = 0; u < RES_ROWS; ++u) {
+        if (r0 + u >= nrows) break;
+        double *o = part + (((int64_t)(r0 + u) * gridDim.z + blockIdx.z) * N + j) * 3;
+        o[0] = p[u];
+        o[1] = q[u];
+        o[2] = sa[u];
+    }
 }
 
 // Pass 2: combine the chunk partials of (row y, column j) with TwoSum and
@@ -446,8 +460,9 @@ hipError_t launch_residual_check(int opA, int opB, int64_t M, int64_t N, int64_t
     if (r <= 0) return hipGetLastError();
     const int64_t step = r > 1 ? (M - 1) / (r - 1) : 0;
     const int nch = (int)((K + RES_KCHUNK - 1) / RES_KCHUNK);
-    hipLaunchKernelGGL(residual_partial_kernel, dim3((unsigned)((N + 255) / 256), (unsigned)r, (unsigned)nch), dim3(256),
-                       0, stream, A, lda, opA, B, ldb, opB, M, N, K, step, part);
+    hipLaunchKernelGGL(residual_partial_kernel,
+                       dim3((unsigned)((N + 255) / 256), (unsigned)((r + RES_ROWS - 1) / RES_ROWS), (unsigned)nch),
+                       dim3(256), 0, stream, A, lda, opA, B, ldb, opB, M, N, K, step, r, part);
     hipLaunchKernelGGL(residual_combine_kernel, dim3((unsigned)((N + 255) / 256), (unsigned)r), dim3(256), 0, stream, C,
                        ldc, M, N, step, nch, part, err);
     return hipGetLastError();
