@@ -40,11 +40,31 @@ int crt_default_moduli(int64_t K);
 // outside [2, CRT_MAX_MODULI] or gives fewer than 1 bit for this K.
 bool plan_crt(int64_t M, int64_t N, int64_t K, int n, int check_rows, CrtPlan *plan);
 
+// A real operand of the products as the kernels read it: line x (row of op(A),
+// column of op(B)), element k is ca re + cb im of X at x sx + k sk doubles
+// (real data: the value itself).  `lines`: consecutive k are adjacent.
+struct CrtOperand {
+    const double *X;
+    int64_t sx, sk;
+    double ca, cb;
+    int cplx, lines;
+};
+// Operand view of X (leading dimension ld in elements; complex X interleaved).
+CrtOperand crt_operand(const double *X, int64_t ld, bool lines, bool cplx, double ca = 1.0, double cb = 0.0);
+
+// Where the recombined product goes: real C (C = AB), or the parts of an
+// interleaved complex C (Re C (+)= cre AB, Im C (+)= cim AB; first: assign).
+struct CrtOutput {
+    bool cplx = false;
+    double cre = 1.0, cim = 0.0;
+    bool first = true;
+};
+
 // Enqueue exponents, residue slices, the n residue products and the FP64
-// reconstruction into C.  `ws` holds plan.bytes bytes.  op codes: 0 = N, 1 = T.
+// reconstruction into C.  `ws` holds plan.bytes bytes.
 // ev_begin / ev_end (optional) are recorded on `stream` around the product launch.
-hipError_t launch_crt_gemm(const CrtPlan &pl, int opA, int opB, int64_t M, int64_t N, int64_t K, const double *A,
-                           int64_t lda, const double *B, int64_t ldb, double *C, int64_t ldc, void *ws,
-                           hipStream_t stream, hipEvent_t ev_begin = nullptr, hipEvent_t ev_end = nullptr);
+hipError_t launch_crt_gemm(const CrtPlan &pl, const CrtOperand &oa, const CrtOperand &ob, int64_t M, int64_t N,
+                           int64_t K, double *C, int64_t ldc, const CrtOutput &out, void *ws, hipStream_t stream,
+                           hipEvent_t ev_begin = nullptr, hipEvent_t ev_end = nullptr);
 
 }  // namespace f4j

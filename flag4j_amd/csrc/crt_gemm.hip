@@ -168,14 +168,54 @@ __device__ inline unsigned pack4(int a, int b, int c, int d) {
     return (unsigned)(a & 255) | ((unsigned)(b & 255) << 8) | ((unsigned)(c & 255) << 16) | ((unsigned)d << 24);
 }
 
+__device__ inline double opnd_value(const CrtOperand &o, int64_t x, int64_t k) {
+    const double *p = o.X + x * o.sx + k * o.sk;
+    return o.cplx ? fma(o.ca, p[0], o.cb * p[1]) : p[0];
+}
+
+__device__ inline int exp_of(double v) {
+    if (v == 0.0) return INT_MIN;
+    int e;
+    (void)frexp(v, &e);
+    return e;
+}
+
+__global__ __launch_bounds__(256) void crt_fill_kernel(int *p, int64_t n, int v) {
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) p[i] = v;
+}
+
+// Line exponents (smallest E with |value| < 2^E over the line; INT_MIN for an
+// all-zero line).  Contiguous lines: one block per line, lanes along k.
+__global__ __launch_bounds__(256) void crt_exp_lines_kernel(CrtOperand o, int64_t K, int *e) {
+    const int64_t x = blockIdx.x;
+    int m = INT_MIN;
+    for (int64_t k = threadIdx.x; k < K; k += 256) m = max(m, exp_of(opnd_value(o, x, k)));
+    for (int off = 32; off > 0; off >>= 1) m = max(m, __shfl_xor(m, off));
+    __shared__ int red[4];
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) e[x] = max(max(red[0], red[1]), max(red[2], red[3]));
+}
+
+// Strided lines: thread per line (lanes along the lines), one k-chunk per
+// block row, merged with atomicMax into e (filled with INT_MIN first).
+__global__ __launch_bounds__(256) void crt_exp_cols_kernel(CrtOperand o, int64_t nx, int64_t K, int64_t kchunk,
+                                                           int *e) {
+    const int64_t x = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (x >= nx) return;
+    const int64_t k0 = (int64_t)blockIdx.y * kchunk, k1 = min(K, k0 + kchunk);
+    int m = INT_MIN;
+    for (int64_t k = k0; k < k1; ++k) m = max(m, exp_of(opnd_value(o, x, k)));
+    if (m != INT_MIN) atomicMax(e + x, m);
+}
+
 // One block: 32 lines x 128 k.  The FP64 block is staged through LDS with
 // coalesced reads (along k when lines are contiguous, along x when they are
 // strided); thread t then takes line t % 32, 16 k at 16 ((t / 32) % 2) + 32 (t / 64)
 // and writes one 16-byte fragment piece per modulus (a wave writes 1 KB).
 template <bool WIDE, bool LINES>
-__global__ __launch_bounds__(256) void crt_split_kernel(const double *X, int64_t ld, int64_t nx, int64_t K,
-                                                        const int *e, signed char *out, int64_t Xp, int64_t Kp,
-                                                        CrtTables tb) {
+__global__ __launch_bounds__(256) void crt_split_kernel(CrtOperand o, int64_t nx, int64_t K, const int *e,
+                                                        signed char *out, int64_t Xp, int64_t Kp, CrtTables tb) {
     __shared__ double tile[32][129];
     const int64_t k0 = (int64_t)blockIdx.x * 128, x0 = (int64_t)blockIdx.y * 32;
     const int t = threadIdx.x;
@@ -183,13 +223,13 @@ __global__ __launch_bounds__(256) void crt_split_kernel(const double *X, int64_t
         const int c = t & 127;
         for (int r = t >> 7; r < 32; r += 2) {
             const int64_t x = x0 + r, k = k0 + c;
-            tile[r][c] = (x < nx && k < K) ? X[x * ld + k] : 0.0;
+            tile[r][c] = (x < nx && k < K) ? opnd_value(o, x, k) : 0.0;
         }
     } else {
         const int r = t & 31;
         for (int c = t >> 5; c < 128; c += 8) {
             const int64_t x = x0 + r, k = k0 + c;
-            tile[r][c] = (x < nx && k < K) ? X[k * ld + x] : 0.0;
+            tile[r][c] = (x < nx && k < K) ? opnd_value(o, x, k) : 0.0;
         }
     }
     __syncthreads();
@@ -369,9 +409,12 @@ constexpr int RV = 8;                             // product entries per thread
 
 // Thread -> RV consecutive entries of the flattened M x N product: one 8-byte
 // load per modulus (planes are 256-byte aligned) instead of RV byte loads.
+// Output: real C (cplx = 0): C_ij = value; complex C (interleaved):
+// Re C_ij (+)= cre value, Im C_ij (+)= cim value (first: assign).
 __global__ __launch_bounds__(256) void crt_reconstruct_kernel(const unsigned char *R, int64_t plane, int64_t M,
                                                               int64_t N, const int *eA, const int *eB, double *C,
-                                                              int64_t ldc, CrtTables tb) {
+                                                              int64_t ldc, CrtTables tb, int cplx, double cre,
+                                                              double cim, int first) {
     const int64_t g0 = ((int64_t)blockIdx.x * 256 + threadIdx.x) * RV;
     const int64_t total = M * N;
     if (g0 >= total) return;
@@ -423,7 +466,13 @@ __global__ __launch_bounds__(256) void crt_reconstruct_kernel(const unsigned cha
                 const double acc = sm + (((top - (sm - z)) + (hi - z)) + lo);
                 out = ldexp(acc, ea + eb - 2 * tb.t);
             }
-            C[i * ldc + j] = out;
+            if (cplx) {
+                double *c = C + 2 * (i * ldc + j);
+                c[0] = first ? cre * out : c[0] + cre * out;
+                c[1] = first ? cim * out : c[1] + cim * out;
+            } else {
+                C[i * ldc + j] = out;
+            }
         }
         if (++j == N) {
             j = 0;
@@ -495,36 +544,60 @@ bool plan_crt(int64_t M, int64_t N, int64_t K, int n, int check_rows, CrtPlan *p
     return true;
 }
 
-hipError_t launch_crt_gemm(const CrtPlan &pl, int opA, int opB, int64_t M, int64_t N, int64_t K, const double *A,
-                           int64_t lda, const double *B, int64_t ldb, double *C, int64_t ldc, void *ws,
-                           hipStream_t stream, hipEvent_t ev_begin, hipEvent_t ev_end) {
+CrtOperand crt_operand(const double *X, int64_t ld, bool lines, bool cplx, double ca, double cb) {
+    CrtOperand o{};
+    const int w = cplx ? 2 : 1;
+    o.X = X;
+    o.sx = (lines ? ld : 1) * w;
+    o.sk = (lines ? 1 : ld) * w;
+    o.ca = ca;
+    o.cb = cb;
+    o.cplx = cplx;
+    o.lines = lines;
+    return o;
+}
+
+hipError_t launch_crt_gemm(const CrtPlan &pl, const CrtOperand &oa, const CrtOperand &ob, int64_t M, int64_t N,
+                           int64_t K, double *C, int64_t ldc, const CrtOutput &out, void *ws, hipStream_t stream,
+                           hipEvent_t ev_begin, hipEvent_t ev_end) {
     char *w = (char *)ws;
     signed char *ra = (signed char *)(w + pl.off_ra), *rb = (signed char *)(w + pl.off_rb);
     unsigned char *res = (unsigned char *)(w + pl.off_res);
     int *ea = (int *)(w + pl.off_ea), *eb = (int *)(w + pl.off_eb);
-    hipError_t e = launch_exponents(opA, opB, M, N, K, A, lda, B, ldb, ea, eb, stream);
-    if (e != hipSuccess) return e;
-    const bool wide = pl.tb.t > 58;
-    auto split = [&](const double *X, int64_t ld, bool lines, int64_t nx, int64_t Xp, const int *ex, signed char *out) {
-        const dim3 grid((unsigned)(pl.Kp / 128), (unsigned)(Xp / 32));
-        if (lines) {
-            if (wide)
-                hipLaunchKernelGGL((crt_split_kernel<true, true>), grid, dim3(256), 0, stream, X, ld, nx, K, ex, out,
-                                   Xp, pl.Kp, pl.tb);
-            else
-                hipLaunchKernelGGL((crt_split_kernel<false, true>), grid, dim3(256), 0, stream, X, ld, nx, K, ex, out,
-                                   Xp, pl.Kp, pl.tb);
+    auto exps = [&](const CrtOperand &o, int64_t nx, int *e) {
+        if (nx <= 0) return;
+        if (o.lines) {
+            hipLaunchKernelGGL(crt_exp_lines_kernel, dim3((unsigned)nx), dim3(256), 0, stream, o, K, e);
         } else {
-            if (wide)
-                hipLaunchKernelGGL((crt_split_kernel<true, false>), grid, dim3(256), 0, stream, X, ld, nx, K, ex, out,
-                                   Xp, pl.Kp, pl.tb);
-            else
-                hipLaunchKernelGGL((crt_split_kernel<false, false>), grid, dim3(256), 0, stream, X, ld, nx, K, ex,
-                                   out, Xp, pl.Kp, pl.tb);
+            const int64_t kchunk = 256;
+            hipLaunchKernelGGL(crt_fill_kernel, dim3(64), dim3(256), 0, stream, e, nx, INT_MIN);
+            hipLaunchKernelGGL(crt_exp_cols_kernel, dim3((unsigned)((nx + 255) / 256), (unsigned)((K + kchunk - 1) / kchunk)),
+                               dim3(256), 0, stream, o, nx, K, kchunk, e);
         }
     };
-    split(A, lda, opA == 0, M, pl.Mp, ea, ra);
-    split(B, ldb, opB == 1, N, pl.Np, eb, rb);
+    exps(oa, M, ea);
+    exps(ob, N, eb);
+    const bool wide = pl.tb.t > 58;
+    auto split = [&](const CrtOperand &o, int64_t nx, int64_t Xp, const int *ex, signed char *dst) {
+        const dim3 grid((unsigned)(pl.Kp / 128), (unsigned)(Xp / 32));
+        if (o.lines) {
+            if (wide)
+                hipLaunchKernelGGL((crt_split_kernel<true, true>), grid, dim3(256), 0, stream, o, nx, K, ex, dst, Xp,
+                                   pl.Kp, pl.tb);
+            else
+                hipLaunchKernelGGL((crt_split_kernel<false, true>), grid, dim3(256), 0, stream, o, nx, K, ex, dst, Xp,
+                                   pl.Kp, pl.tb);
+        } else {
+            if (wide)
+                hipLaunchKernelGGL((crt_split_kernel<true, false>), grid, dim3(256), 0, stream, o, nx, K, ex, dst, Xp,
+                                   pl.Kp, pl.tb);
+            else
+                hipLaunchKernelGGL((crt_split_kernel<false, false>), grid, dim3(256), 0, stream, o, nx, K, ex, dst,
+                                   Xp, pl.Kp, pl.tb);
+        }
+    };
+    split(oa, M, pl.Mp, ea, ra);
+    split(ob, N, pl.Np, eb, rb);
     CrtGemmArgs a{ra, rb, pl.Mp, pl.Np, pl.Kp, M, N, res, pl.plane};
     if (ev_begin) hipEventRecord(ev_begin, stream);
     hipLaunchKernelGGL(crt_gemm_kernel, dim3((unsigned)((pl.Mp / CBM) * (pl.Np / CBN)), (unsigned)pl.tb.n), dim3(256),
@@ -532,7 +605,7 @@ hipError_t launch_crt_gemm(const CrtPlan &pl, int opA, int opB, int64_t M, int64
     if (ev_end) hipEventRecord(ev_end, stream);
     const int64_t threads = (M * N + RV - 1) / RV;
     hipLaunchKernelGGL(crt_reconstruct_kernel, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, stream, res,
-                       pl.plane, M, N, ea, eb, C, ldc, pl.tb);
+                       pl.plane, M, N, ea, eb, C, ldc, pl.tb, (int)out.cplx, out.cre, out.cim, (int)out.first);
     return hipGetLastError();
 }
 

@@ -61,7 +61,7 @@ struct State {
     DevBuf bufA, bufB, bufC;
     std::mutex ws_mu;                 // guards ws and tev (split-precision workspace, product timing)
     DevBuf ws;
-    hipEvent_t tev[2] = {nullptr, nullptr};
+    hipEvent_t tev[8] = {};                 // product timing: a begin / end pair per real product (up to 4)
 };
 
 State &S() {
@@ -346,8 +346,8 @@ int run_split_cplx(int opA, int opB, int64_t M, int64_t N, int64_t K, const doub
 
 // Modular (CRT) split-precision real GEMM on the INT8 matrix cores;
 // `slices` = number of moduli (0: enough for FP64 accuracy at this K).
-int run_crt(int opA, int opB, int64_t M, int64_t N, int64_t K, const double *A, int64_t lda, const double *B,
-            int64_t ldb, double *C, int64_t ldc, int slices, hipStream_t stream) {
+int run_crt(bool cplx, int opA, int opB, int64_t M, int64_t N, int64_t K, const double *A, int64_t lda,
+            const double *B, int64_t ldb, double *C, int64_t ldc, int slices, int use3m, hipStream_t stream) {
     Config c;
     {
         std::lock_guard<std::mutex> g(S().cfg_mu);
@@ -360,27 +360,55 @@ int run_crt(int opA, int opB, int64_t M, int64_t N, int64_t K, const double *A, 
         return fail(F4J_ERR_INVALID_ARG, "split_int8: " + std::to_string(n) +
                                              " moduli cannot represent products with K = " + std::to_string(K) +
                                              " (need 2..20 moduli and K < 131072)");
+    // Real products: one job.  Complex products op(A) = a + ib, op(B) = c + id
+    // (b, d carry the conjugation sign): 3M (Gauss) T1 = ac, T2 = bd,
+    // T3 = (a + b)(c + d), Re = T1 - T2, Im = T3 - T1 - T2; or 4M.
+    struct Job {
+        double ca_a, cb_a, ca_b, cb_b, cre, cim;
+    };
+    const double sa = opA == F4J_OP_C ? -1.0 : 1.0, sb = opB == F4J_OP_C ? -1.0 : 1.0;
+    std::vector<Job> jobs;
+    if (!cplx) jobs = {{1, 0, 1, 0, 1, 0}};
+    else if (use3m) jobs = {{1, 0, 1, 0, 1, -1}, {0, sa, 0, sb, -1, -1}, {1, sa, 1, sb, 0, 1}};
+    else jobs = {{1, 0, 1, 0, 1, 0}, {0, sa, 0, sb, -1, 0}, {1, 0, 0, sb, 0, 1}, {0, sa, 1, 0, 0, 1}};
     State &s = S();
     std::lock_guard<std::mutex> g(s.ws_mu);
     int rc = grow(s.ws, pl.bytes);
     if (rc != F4J_OK) return rc;
-    hipEvent_t *ev = c.split_timing ? s.tev : nullptr;
-    F4J_HIP(launch_crt_gemm(pl, opA, opB, M, N, K, A, lda, B, ldb, C, ldc, s.ws.p, stream, ev ? ev[0] : nullptr,
-                            ev ? ev[1] : nullptr),
-            "crt gemm launch");
+    const bool timing = c.split_timing != 0;
+    for (size_t jb = 0; jb < jobs.size(); ++jb) {
+        const Job &j = jobs[jb];
+        const CrtOperand oa = crt_operand(A, lda, opA == F4J_OP_N, cplx, j.ca_a, j.cb_a);
+        const CrtOperand ob = crt_operand(B, ldb, opB != F4J_OP_N, cplx, j.ca_b, j.cb_b);
+        CrtOutput out;
+        out.cplx = cplx;
+        out.cre = j.cre;
+        out.cim = j.cim;
+        out.first = jb == 0;
+        F4J_HIP(launch_crt_gemm(pl, oa, ob, M, N, K, C, ldc, out, s.ws.p, stream, timing ? s.tev[2 * jb] : nullptr,
+                                timing ? s.tev[2 * jb + 1] : nullptr),
+                "crt gemm launch");
+    }
     f4j_split_report rep{};
     rep.product_ms = -1.0;
     rep.precision = F4J_PREC_SPLIT_INT8;
     rep.slices = n;
-    rep.products = n;
-    rep.launches = 1;
+    rep.products = n * (int)jobs.size();
+    rep.launches = (int)jobs.size();
     rep.checked_rows = (int)std::min<int64_t>(c.split_check_rows, M);
     rep.max_rel_err = -1.0;
     if (rep.checked_rows > 0) {
         char *w = (char *)s.ws.p;
-        F4J_HIP(launch_residual_check(opA, opB, M, N, K, A, lda, B, ldb, C, ldc, rep.checked_rows,
-                                      (double *)(w + pl.off_part), (unsigned long long *)(w + pl.off_err), stream),
-                "crt residual launch");
+        double *part = (double *)(w + pl.off_part);
+        unsigned long long *eb = (unsigned long long *)(w + pl.off_err);
+        if (cplx)
+            F4J_HIP(launch_residual_check_cplx(opA, opB, M, N, K, A, lda, B, ldb, C, ldc, rep.checked_rows, part, eb,
+                                               stream),
+                    "crt residual launch");
+        else
+            F4J_HIP(launch_residual_check(opA, opB, M, N, K, A, lda, B, ldb, C, ldc, rep.checked_rows, part, eb,
+                                          stream),
+                    "crt residual launch");
         double err = 0.0;
         F4J_HIP(hipMemcpyAsync(&err, w + pl.off_err, sizeof(double), hipMemcpyDeviceToHost, stream), "copy residual");
         F4J_HIP(hipStreamSynchronize(stream), "stream synchronize");
@@ -391,16 +419,18 @@ int run_crt(int opA, int opB, int64_t M, int64_t N, int64_t K, const double *A, 
                 return fail(F4J_ERR_ACCURACY, "split_int8 product error " + std::to_string(err) +
                                                   " above tolerance " + std::to_string(c.split_tolerance));
             }
-            GemmProblem p{M, N, K, A, lda, B, ldb, C, ldc, 0};
-            F4J_HIP(launch_gemm_f64(opA ? R_T : R_N, opB ? R_T : R_N, p, stream, pick_tile(M, N)), "fp64 fallback");
+            F4J_HIP(launch_fp64(cplx, opA, opB, M, N, K, A, lda, B, ldb, C, ldc, stream), "fp64 fallback");
             rep.precision = F4J_PREC_FP64;
         }
     }
     F4J_HIP(hipStreamSynchronize(stream), "stream synchronize");
-    if (ev) {
-        float ms = 0.f;
-        F4J_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]), "hipEventElapsedTime");
-        rep.product_ms = ms;
+    if (timing) {
+        rep.product_ms = 0.0;
+        for (size_t jb = 0; jb < jobs.size(); ++jb) {
+            float ms = 0.f;
+            F4J_HIP(hipEventElapsedTime(&ms, s.tev[2 * jb], s.tev[2 * jb + 1]), "hipEventElapsedTime");
+            rep.product_ms += ms;
+        }
     }
     t_split = rep;
     return F4J_OK;
@@ -425,8 +455,8 @@ int gemm_dev(bool cplx, int opA, int opB, int64_t M, int64_t N, int64_t K, const
         return run_split(opA, opB, M, N, K, A, lda, B, ldb, C, ldc, prec, slices, stream);
     if ((prec == F4J_PREC_SPLIT_BF16 || prec == F4J_PREC_SPLIT_FP16) && cplx)
         return run_split_cplx(opA, opB, M, N, K, A, lda, B, ldb, C, ldc, prec, slices, use3m, stream);
-    if (prec == F4J_PREC_SPLIT_INT8 && !cplx)
-        return run_crt(opA, opB, M, N, K, A, lda, B, ldb, C, ldc, slices, stream);
+    if (prec == F4J_PREC_SPLIT_INT8)
+        return run_crt(cplx, opA, opB, M, N, K, A, lda, B, ldb, C, ldc, slices, use3m, stream);
     if (prec != F4J_PREC_FP64)
         return fail(F4J_ERR_UNSUPPORTED, "precision " + std::to_string(prec) + " is not available for " +
                                              (cplx ? "complex" : "real") + " products in this build");

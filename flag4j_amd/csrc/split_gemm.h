@@ -73,6 +73,12 @@ hipError_t launch_split_residual(const SplitPlan &pl, int opA, int opB, int64_t 
 hipError_t launch_exponents(int opA, int opB, int64_t M, int64_t N, int64_t K, const double *A, int64_t lda,
                             const double *B, int64_t ldb, int *eA, int *eB, hipStream_t stream);
 
+// Complex form (interleaved data, op codes 0 = N, 1 = T, 2 = C): |z| = |re| + |im|,
+// an entry's error is the larger of its two parts.
+hipError_t launch_residual_check_cplx(int opA, int opB, int64_t M, int64_t N, int64_t K, const double *A,
+                                      int64_t lda, const double *B, int64_t ldb, const double *C, int64_t ldc,
+                                      int rows, double *part, unsigned long long *err, hipStream_t stream);
+
 // FP64 residual check of `rows` evenly spaced rows of C against compensated
 // dot products; max |C - AB| / (|A||B|) left as double bits in *err.
 // `part` holds split_residual_bytes(N, K, rows) bytes.

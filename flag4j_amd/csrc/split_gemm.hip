@@ -275,6 +275,72 @@ __global__ __launch_bounds__(256) void residual_combine_kernel(const double *C, 
     atomicMax(max_err_bits, __builtin_bit_cast(unsigned long long, err));
 }
 
+// Complex form of the check (interleaved data, op codes N = 0, T = 1, C = 2):
+// |z| = |re| + |im|, the error of an entry is the larger of its two parts.
+__device__ inline void cload(const double *X, int op, int64_t ld, int64_t r, int64_t c, double &re, double &im) {
+    // element (r, c) of op(X)
+    const double *p = op ? X + 2 * (c * ld + r) : X + 2 * (r * ld + c);
+    re = p[0];
+    im = op == 2 ? -p[1] : p[1];
+}
+
+__device__ inline void dot2_step(double &p, double &q, double x, double y) {
+    const double h = x * y;
+    const double r = fma(x, y, -h);
+    const double s = p + h;
+    const double z = s - p;
+    q += ((p - (s - z)) + (h - z)) + r;
+    p = s;
+}
+
+__global__ __launch_bounds__(256) void residual_partial_cplx_kernel(const double *A, int64_t lda, int opA,
+                                                                    const double *B, int64_t ldb, int opB, int64_t M,
+                                                                    int64_t N, int64_t K, int64_t row_step,
+                                                                    double *part) {
+    const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t i = min(M - 1, (int64_t)blockIdx.y * row_step);
+    if (j >= N) return;
+    const int64_t k0 = (int64_t)blockIdx.z * RES_KCHUNK, k1 = min(K, k0 + RES_KCHUNK);
+    double rp = 0.0, rq = 0.0, ip = 0.0, iq = 0.0, sa = 0.0;
+    for (int64_t k = k0; k < k1; ++k) {
+        double ar, ai, br, bi;
+        cload(A, opA, lda, i, k, ar, ai);
+        cload(B, opB, ldb, k, j, br, bi);
+        dot2_step(rp, rq, ar, br);
+        dot2_step(rp, rq, -ai, bi);
+        dot2_step(ip, iq, ar, bi);
+        dot2_step(ip, iq, ai, br);
+        sa = fma(fabs(ar) + fabs(ai), fabs(br) + fabs(bi), sa);
+    }
+    double *o = part + (((int64_t)blockIdx.y * gridDim.z + blockIdx.z) * N + j) * 5;
+    o[0] = rp;
+    o[1] = rq;
+    o[2] = ip;
+    o[3] = iq;
+    o[4] = sa;
+}
+
+__global__ __launch_bounds__(256) void residual_combine_cplx_kernel(const double *C, int64_t ldc, int64_t M, int64_t N,
+                                                                    int64_t row_step, int nchunks, const double *part,
+                                                                    unsigned long long *max_err_bits) {
+    const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t i = min(M - 1, (int64_t)blockIdx.y * row_step);
+    if (j >= N) return;
+    double rp = 0.0, rq = 0.0, ip = 0.0, iq = 0.0, sa = 0.0;
+    for (int c = 0; c < nchunks; ++c) {
+        const double *o = part + (((int64_t)blockIdx.y * nchunks + c) * N + j) * 5;
+        dot2_step(rp, rq, o[0], 1.0);
+        rq += o[1];
+        dot2_step(ip, iq, o[2], 1.0);
+        iq += o[3];
+        sa += o[4];
+    }
+    const double *c = C + 2 * (i * ldc + j);
+    const double e = fmax(fabs((c[0] - rp) - rq), fabs((c[1] - ip) - iq));
+    const double err = sa > 0.0 ? e / sa : (e == 0.0 ? 0.0 : INFINITY);
+    atomicMax(max_err_bits, __builtin_bit_cast(unsigned long long, err));
+}
+
 template <int R, int S, class T>
 void launch_split_one(const double *X, int64_t ld, bool lines, int64_t nx, int64_t K, const int *e, T *out, int64_t Xp,
                       int64_t Kp, hipStream_t stream) {
@@ -468,9 +534,24 @@ hipError_t launch_residual_check(int opA, int opB, int64_t M, int64_t N, int64_t
     return hipGetLastError();
 }
 
+hipError_t launch_residual_check_cplx(int opA, int opB, int64_t M, int64_t N, int64_t K, const double *A,
+                                      int64_t lda, const double *B, int64_t ldb, const double *C, int64_t ldc,
+                                      int rows, double *part, unsigned long long *err, hipStream_t stream) {
+    hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(256), 0, stream, (int *)err, 2, 0);
+    const int r = (int)std::min<int64_t>(rows, M);
+    if (r <= 0) return hipGetLastError();
+    const int64_t step = r > 1 ? (M - 1) / (r - 1) : 0;
+    const int nch = (int)((K + RES_KCHUNK - 1) / RES_KCHUNK);
+    hipLaunchKernelGGL(residual_partial_cplx_kernel, dim3((unsigned)((N + 255) / 256), (unsigned)r, (unsigned)nch),
+                       dim3(256), 0, stream, A, lda, opA, B, ldb, opB, M, N, K, step, part);
+    hipLaunchKernelGGL(residual_combine_cplx_kernel, dim3((unsigned)((N + 255) / 256), (unsigned)r), dim3(256), 0,
+                       stream, C, ldc, M, N, step, nch, part, err);
+    return hipGetLastError();
+}
+
 size_t split_residual_bytes(int64_t N, int64_t K, int rows) {
     const int64_t nch = (K + RES_KCHUNK - 1) / RES_KCHUNK;
-    return (size_t)rows * (size_t)nch * (size_t)N * 3 * sizeof(double);
+    return (size_t)rows * (size_t)nch * (size_t)N * 5 * sizeof(double);     // room for the complex check
 }
 
 }  // namespace f4j

@@ -114,8 +114,8 @@ int f4j_dgemm_dev(int opA, int opB, int64_t M, int64_t N, int64_t K,
  *           DenseSemiringMatMultTranspose.* M:linalg/ops/dense/semiring_ops/DenseSemiringMatMultTranspose.java:54-178
  *   (C, N): A.H().mult(B)                 M:arrays/backend/field_arrays/AbstractDenseFieldMatrix.java:98-102
  *           with the conjugate transpose fused into the operand load.
- * In SPLIT_BF16 / SPLIT_FP16 a complex product runs as three real split products (3M, Gauss)
- * when `use_3m` != 0, else four; SPLIT_INT8 is available for real products only. */
+ * In the SPLIT precisions a complex product runs as three real split products (3M, Gauss)
+ * when `use_3m` != 0, else four. */
 int f4j_zgemm(int opA, int opB, int64_t M, int64_t N, int64_t K,
               const double *A, int64_t lda, const double *B, int64_t ldb,
               double *C, int64_t ldc, int precision, int slices, int use_3m);

@@ -310,3 +310,39 @@ def test_complex_split_fallback():
     st = L.load().f4j_zgemm(0, 0, 64, 64, 64, A.ctypes.data, 64, B.ctypes.data, 64, C.ctypes.data, 64,
                             L.PREC["split_fp16"], 2, 1)
     assert st == L.ERR_ACCURACY
+
+
+@pytest.mark.parametrize("opA,opB", [("N", "N"), ("C", "N"), ("N", "T"), ("T", "C")])
+@pytest.mark.parametrize("use3m", [True, False])
+def test_complex_crt(opA, opB, use3m):
+    """Complex products in the modular form: 3 (3M) or 4 real INT8 products."""
+    M, N, K = 300, 200, 500
+    sa = (M, K) if opA == "N" else (K, M)
+    sb = (K, N) if opB == "N" else (N, K)
+    A = rng.uniform(61, *sa) + 1j * rng.uniform(62, *sa)
+    B = rng.uniform(63, *sb) + 1j * rng.uniform(64, *sb)
+    L.config_set("splitTiming", 1)
+    try:
+        C, rep = zsplit_mult(A, B, "split_int8", 0, use3m, opA, opB)
+    finally:
+        L.config_set("splitTiming", 0)
+    assert rep.precision == L.PREC["split_int8"]
+    assert rep.launches == (3 if use3m else 4) and rep.products == rep.launches * rep.slices
+    assert 0 <= rep.max_rel_err <= 1e-15 and rep.product_ms > 0
+    assert zsampled_err(C, A, B, opA, opB) <= 1e-15
+
+
+def test_complex_crt_gaussian_integers_exact():
+    """Gaussian integers below 2^19 in each part: T1, T2, T3 are exact integer
+    products and Re, Im are exact sums of them."""
+    r = np.random.default_rng(7)
+    A = (r.integers(-(1 << 19), 1 << 19, (70, 90)) + 1j * r.integers(-(1 << 19), 1 << 19, (70, 90)))
+    B = (r.integers(-(1 << 19), 1 << 19, (90, 50)) + 1j * r.integers(-(1 << 19), 1 << 19, (90, 50)))
+    C, rep = zsplit_mult(A, B, "split_int8", 0, True)
+    Ar, Ai = A.real.astype(object), A.imag.astype(object)
+    Br, Bi = B.real.astype(object), B.imag.astype(object)
+    re = Ar.dot(Br) - Ai.dot(Bi)
+    im = Ar.dot(Bi) + Ai.dot(Br)
+    assert np.array_equal(C.real, np.array(re, dtype=np.float64))
+    assert np.array_equal(C.imag, np.array(im, dtype=np.float64))
+    assert rep.precision == L.PREC["split_int8"]
