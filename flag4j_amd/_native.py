@@ -34,7 +34,7 @@ class DeviceInfo(ctypes.Structure):
                 ("peak_hbm_gbs", ctypes.c_double), ("name", ctypes.c_char * 64), ("arch", ctypes.c_char * 32)]
 
 
-ABI_VERSION = 2          # F4J_ABI_VERSION the structures below match
+ABI_VERSION = 3          # F4J_ABI_VERSION the structures below match
 
 
 class SplitReport(ctypes.Structure):
@@ -58,6 +58,10 @@ SIGNATURES = {
     "f4j_dgemm_dev": ([_i, _i, _i64, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _i, _i, _vp], _i),
     "f4j_zgemm": ([_i, _i, _i64, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _i, _i, _i], _i),
     "f4j_zgemm_dev": ([_i, _i, _i64, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _i, _i, _i, _vp], _i),
+    "f4j_dgemv": ([_i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _i], _i),
+    "f4j_dgemv_dev": ([_i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _i, _vp], _i),
+    "f4j_zgemv": ([_i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _i], _i),
+    "f4j_zgemv_dev": ([_i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _i, _vp], _i),
     "f4j_grouped_plan_create": ([_i, _i, _i, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i,
                                  ctypes.POINTER(_vp)], _i),
     "f4j_grouped_plan_create_split": ([_i, _i, _i, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i,

@@ -136,10 +136,13 @@ class Matrix(_DenseMatrix):
         return Matrix(Shape(n, n), np.eye(n))
 
     def mult(self, b, precision="auto", slices=0):
-        """C = this * b: Matrix.mult (M:arrays/dense/Matrix.java:638-640) via the GPU dispatcher.
+        """C = this * b: Matrix.mult (M:arrays/dense/Matrix.java:638-640) via the GPU dispatcher;
+        for a Vector b, Matrix.mult(Vector) (Matrix.java:1406-1408).
 
         Raises LinearAlgebraException if this.numCols != b.numRows
         (ReDeMatMultDispatcher.java:163-164; ValidateParameters.java:62-67)."""
+        if isinstance(b, Vector):
+            return self._mult_vector(b, precision)
         if self.numCols != b.numRows:
             raise LinearAlgebraException(_mat_mult_shape_err(self.shape, b.shape))
         M, K, N = self.numRows, self.numCols, b.numCols
@@ -147,6 +150,18 @@ class Matrix(_DenseMatrix):
         _native.check(_native.load().f4j_dgemm(0, 0, M, N, K, _ptr(self.data), max(K, 1), _ptr(b.data),
                                                max(N, 1), _ptr(out), max(N, 1), _native.PREC[precision], slices))
         return Matrix(Shape(M, N), out)
+
+    def _mult_vector(self, b, precision):
+        """Matrix.mult(Vector) -> ReDeMatVecMultDispatcher.dispatch
+        (M:linalg/ops/dispatch/ReDeMatVecMultDispatcher.java:141-181): LinearAlgebraException on a
+        size mismatch (:147-148, ValidateParameters.ensureMatMultShapes)."""
+        if self.numCols != b.size:
+            raise LinearAlgebraException(_mat_mult_shape_err(self.shape, b.shape))
+        M, K = self.numRows, self.numCols
+        out = np.empty(M, dtype=np.float64)
+        _native.check(_native.load().f4j_dgemv(M, K, _ptr(self.data), max(K, 1), _ptr(b.data), 1, _ptr(out), 1,
+                                               _native.PREC[precision]))
+        return Vector(out)
 
     def multTranspose(self, b, precision="auto", slices=0):
         """this * b^T without forming b^T (M:arrays/dense/Matrix.java:655-663).
@@ -238,7 +253,10 @@ class CMatrix(_DenseMatrix):
         Raises LinearAlgebraException on an inner-dimension mismatch
         (Cm128DeMatMultDispatcher.java:162-163).  Unlike the reference (whose
         kernels size dest as rows(a)*rows(b), Cm128DeMatMultKernels.java:47-77),
-        any B shape is handled."""
+        any B shape is handled.  For a CVector b: CMatrix.mult(CVector)
+        (M:arrays/backend/semiring_arrays/AbstractDenseSemiringMatrix.java:295-298)."""
+        if isinstance(b, CVector):
+            return self._mult_vector(b, precision)
         if self.numCols != b.numRows:
             raise LinearAlgebraException(_mat_mult_shape_err(self.shape, b.shape))
         M, K, N = self.numRows, self.numCols, b.numCols
@@ -249,6 +267,18 @@ class CMatrix(_DenseMatrix):
                                                _ptr(out), max(N, 1), _native.PREC[precision], slices,
                                                int(use_3m)))
         return CMatrix(Shape(M, N), out)
+
+    def _mult_vector(self, b, precision):
+        """DenseSemiringMatMultDispatcher.dispatchVector (M:linalg/ops/dense/semiring_ops/
+        DenseSemiringMatMultDispatcher.java:120-142): LinearAlgebraException on a size
+        mismatch (ensureMatMultShapes, :124)."""
+        if self.numCols != b.size:
+            raise LinearAlgebraException(_mat_mult_shape_err(self.shape, Shape(b.size, 1)))
+        M, K = self.numRows, self.numCols
+        out = np.empty(M, dtype=np.complex128)
+        _native.check(_native.load().f4j_zgemv(M, K, _ptr(self.data), max(K, 1), _ptr(b.data), 1, _ptr(out), 1,
+                                               _native.PREC[precision]))
+        return CVector(out)
 
     def multTranspose(self, b, precision="auto", slices=0):
         """this * b^T, transpose without conjugation
@@ -287,6 +317,61 @@ class CMatrix(_DenseMatrix):
         """CMatrix.equals (M:arrays/dense/CMatrix.java:1096-1103): Complex128.equals is
         re == re && im == im (M:numbers/Complex128.java:211-219)."""
         if not isinstance(other, CMatrix) or self.shape != other.shape:
+            return False
+        a, b = self.data, other.data
+        return bool(np.all((a.real == b.real) & (a.imag == b.imag)))
+
+    __eq__ = equals
+    __hash__ = None
+
+
+class _DenseVector:
+    _dtype = np.float64
+
+    def __init__(self, entries):
+        arr = np.ascontiguousarray(np.array(entries, dtype=self._dtype).reshape(-1))
+        self.shape = Shape(arr.size)
+        self._data = arr
+
+    @property
+    def data(self):
+        return self._data
+
+    @property
+    def size(self):
+        return self.shape.get(0)
+
+    def numpy(self):
+        return self._data
+
+    def __repr__(self):
+        return f"{type(self).__name__}{self.shape}"
+
+
+class Vector(_DenseVector):
+    """Real dense vector (M:arrays/dense/Vector.java)."""
+
+    _dtype = np.float64
+
+    def equals(self, other):
+        """Vector.equals: same size and == elementwise, NaN equal to NaN (RealDenseEquals)."""
+        if not isinstance(other, Vector) or self.shape != other.shape:
+            return False
+        a, b = self.data, other.data
+        return bool(np.all((a == b) | (np.isnan(a) & np.isnan(b))))
+
+    __eq__ = equals
+    __hash__ = None
+
+
+class CVector(_DenseVector):
+    """Complex128 dense vector (M:arrays/dense/CVector.java)."""
+
+    _dtype = np.complex128
+
+    def equals(self, other):
+        """CVector.equals: Complex128.equals elementwise (M:numbers/Complex128.java:211-219)."""
+        if not isinstance(other, CVector) or self.shape != other.shape:
             return False
         a, b = self.data, other.data
         return bool(np.all((a.real == b.real) & (a.imag == b.imag)))

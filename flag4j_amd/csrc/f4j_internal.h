@@ -52,6 +52,17 @@ hipError_t launch_gemm_strict(bool cplx, int opA, int opB, int64_t M, int64_t N,
                               int64_t lda, const double *B, int64_t ldb, double *C, int64_t ldc,
                               hipStream_t stream);
 
+// Matrix-vector products in the reference's summation orders (gemv.hip):
+// GEMV_BLOCKED  real, 64-wide k blocks summed from +0.0 and added in block order;
+// GEMV_CHAIN    real, one k-ascending sum per row;
+// GEMV_CPLX     complex, one k-ascending sum of complex products per row.
+// y[i * incy] = sum_k A[i * lda + k] x[k * incx]; lda, incx, incy in element
+// units (complex elements for GEMV_CPLX), conjx conjugates x (complex only).
+enum GemvKind : int { GEMV_BLOCKED = 0, GEMV_CHAIN = 1, GEMV_CPLX = 2 };
+bool gemv_supported(int kind, int64_t K, int64_t lda, int64_t incx);
+hipError_t launch_gemv(int kind, int64_t M, int64_t K, const double *A, int64_t lda, const double *x, int64_t incx,
+                       double *y, int64_t incy, bool conjx, hipStream_t stream);
+
 // Fill rows x cols (leading dimension ld, in doubles) with zeros.
 hipError_t launch_zero_fill(double *C, int64_t rows, int64_t cols, int64_t ld, hipStream_t stream);
 

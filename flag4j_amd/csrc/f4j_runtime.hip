@@ -436,6 +436,29 @@ int run_crt(bool cplx, int opA, int opB, int64_t M, int64_t N, int64_t K, const 
     return F4J_OK;
 }
 
+// Matrix-vector kind the reference's dispatch gives y = A x (A M x K): small
+// products (M K + K < smallThreshold, ReDeMatVecMultDispatcher.java:144-151,
+// ReDeMatMultDispatcher.java:158-168) and Matrix.multTranspose with one row
+// (RealDenseMatMultTranspose.java:53-82) are one k-ascending sum per row; larger
+// real products take the 64-block form (ReDeMatVecMultDispatcher.java:167-181,
+// ReDeMatMultDispatcher.java:183-198).  Complex products are always one chain.
+int gemv_kind(bool cplx, bool b_transposed, int64_t M, int64_t K) {
+    if (cplx) return GEMV_CPLX;
+    if (b_transposed) return GEMV_CHAIN;
+    std::lock_guard<std::mutex> g(S().cfg_mu);
+    return M * K + K < S().cfg.small_threshold ? GEMV_CHAIN : GEMV_BLOCKED;
+}
+
+// Launch y = A x in the reference's order; false when the shape is outside the
+// matrix-vector kernels' limits (the caller then takes another path).
+bool try_gemv(int kind, int64_t M, int64_t K, const double *A, int64_t lda, const double *x, int64_t incx, double *y,
+              int64_t incy, bool conjx, hipStream_t stream, hipError_t *err) {
+    if (!gemv_supported(kind, K, lda, incx)) return false;
+    if (kind == GEMV_CPLX && ((uintptr_t)A & 15) != 0) return false;
+    *err = launch_gemv(kind, M, K, A, lda, x, incx, y, incy, conjx, stream);
+    return true;
+}
+
 // Device-level GEMM, all pointers on the device, async on `stream`.
 int gemm_dev(bool cplx, int opA, int opB, int64_t M, int64_t N, int64_t K, const double *A, int64_t lda,
              const double *B, int64_t ldb, double *C, int64_t ldc, int precision, int slices, int use3m,
@@ -447,6 +470,16 @@ int gemm_dev(bool cplx, int opA, int opB, int64_t M, int64_t N, int64_t K, const
         return F4J_OK;
     }
     const int prec = resolve_precision(precision, M, N, K, opA, opB);
+    // N = 1 with A stored M x K: the reference dispatches a matrix-vector kernel
+    // (ReDeMatMultDispatcher.java:183-198); so do FP64 and STRICT here, in its order.
+    if (N == 1 && opA == F4J_OP_N && (prec == F4J_PREC_STRICT || prec == F4J_PREC_FP64)) {
+        hipError_t e = hipSuccess;
+        const int kind = gemv_kind(cplx, opB != F4J_OP_N, M, K);
+        if (try_gemv(kind, M, K, A, lda, B, opB == F4J_OP_N ? ldb : 1, C, ldc, opB == F4J_OP_C, stream, &e)) {
+            F4J_HIP(e, "matrix-vector launch");
+            return F4J_OK;
+        }
+    }
     if (prec == F4J_PREC_STRICT) {
         F4J_HIP(launch_gemm_strict(cplx, opA, opB, M, N, K, A, lda, B, ldb, C, ldc, stream), "strict gemm launch");
         return F4J_OK;
@@ -567,6 +600,77 @@ int config_set_locked(Config &c, const std::string &key, const std::string &valu
 }  // namespace
 
 void set_error(const std::string &msg) { t_err = msg; }
+
+// y = A x on the device (sizes and strides in elements).
+int gemv_dev(bool cplx, int64_t M, int64_t K, const double *A, int64_t lda, const double *x, int64_t incx, double *y,
+             int64_t incy, int precision, hipStream_t stream) {
+    if (M == 0) return F4J_OK;
+    const int w = cplx ? 2 : 1;
+    if (K == 0) {
+        F4J_HIP(launch_zero_fill(y, M, w, incy * w, stream), "zero fill");
+        return F4J_OK;
+    }
+    const int kind = gemv_kind(cplx, false, M, K);
+    hipError_t e = hipSuccess;
+    if (try_gemv(kind, M, K, A, lda, x, incx, y, incy, false, stream, &e)) {
+        F4J_HIP(e, "matrix-vector launch");
+        return F4J_OK;
+    }
+    if (kind != GEMV_BLOCKED) {
+        // a single k-ascending chain per row is also what the strict kernel computes
+        F4J_HIP(launch_gemm_strict(cplx, F4J_OP_N, F4J_OP_N, M, 1, K, A, lda, x, incx, y, incy, stream),
+                "strict matrix-vector launch");
+        return F4J_OK;
+    }
+    return fail(F4J_ERR_UNSUPPORTED, "matrix-vector product beyond the kernel limits (slab of 8 rows or x over 2 GiB)");
+}
+
+int validate_gemv(bool cplx, int64_t M, int64_t K, const void *A, int64_t lda, const void *x, int64_t incx,
+                  const void *y, int64_t incy, int precision) {
+    (void)cplx;
+    if (M < 0 || K < 0) return fail(F4J_ERR_INVALID_ARG, "negative dimension");
+    if (precision < F4J_PREC_AUTO || precision > F4J_PREC_SPLIT_INT8)
+        return fail(F4J_ERR_INVALID_ARG, "invalid precision code " + std::to_string(precision));
+    if (precision != F4J_PREC_AUTO && precision != F4J_PREC_FP64 && precision != F4J_PREC_STRICT)
+        return fail(F4J_ERR_UNSUPPORTED, "matrix-vector products run in FP64 (AUTO, FP64 or STRICT): "
+                                         "they are bound by reading A once");
+    if (incx < 1 || incy < 1) return fail(F4J_ERR_INVALID_ARG, "incx and incy must be >= 1");
+    if (M > 0 && !y) return fail(F4J_ERR_INVALID_ARG, "y is null");
+    if (M > 0 && K > 0) {
+        if (!A || !x) return fail(F4J_ERR_INVALID_ARG, "A or x is null");
+        if (lda < K) return fail(F4J_ERR_INVALID_ARG, "lda < K");
+    }
+    return F4J_OK;
+}
+
+// Host-array y = A x: stage A, x and y through the pooled device buffers.
+int gemv_host(bool cplx, int64_t M, int64_t K, const double *A, int64_t lda, const double *x, int64_t incx, double *y,
+              int64_t incy, int precision) {
+    int rc = validate_gemv(cplx, M, K, A, lda, x, incx, y, incy, precision);
+    if (rc != F4J_OK || M == 0) return rc;
+    if ((rc = ensure_ready()) != F4J_OK) return rc;
+    State &s = S();
+    std::lock_guard<std::mutex> g(s.mu);
+    const size_t es = cplx ? 16 : 8;
+    if (K > 0) {
+        if ((rc = grow(s.bufA, (size_t)(M * K) * es)) != F4J_OK) return rc;
+        if ((rc = grow(s.bufB, (size_t)K * es)) != F4J_OK) return rc;
+    }
+    if ((rc = grow(s.bufC, (size_t)M * es)) != F4J_OK) return rc;
+    if (K > 0) {
+        F4J_HIP(hipMemcpy2DAsync(s.bufA.p, K * es, A, lda * es, K * es, M, hipMemcpyHostToDevice, s.stream), "copy A");
+        F4J_HIP(hipMemcpy2DAsync(s.bufB.p, es, x, incx * es, es, K, hipMemcpyHostToDevice, s.stream), "copy x");
+    }
+    rc = gemv_dev(cplx, M, K, (const double *)s.bufA.p, K, (const double *)s.bufB.p, 1, (double *)s.bufC.p, 1,
+                  precision, s.stream);
+    if (rc != F4J_OK) {
+        (void)hipStreamSynchronize(s.stream);
+        return rc;
+    }
+    F4J_HIP(hipMemcpy2DAsync(y, incy * es, s.bufC.p, es, es, M, hipMemcpyDeviceToHost, s.stream), "copy y");
+    F4J_HIP(hipStreamSynchronize(s.stream), "stream synchronize");
+    return F4J_OK;
+}
 
 int runtime_ready() { return ensure_ready(); }
 
@@ -705,6 +809,32 @@ int f4j_zgemm_dev(int opA, int opB, int64_t M, int64_t N, int64_t K, const doubl
     return gemm_dev(true, opA, opB, M, N, K, A, lda, B, ldb, C, ldc, precision, slices, use_3m, pick_stream(stream));
 }
 
+int f4j_dgemv(int64_t M, int64_t K, const double *A, int64_t lda, const double *x, int64_t incx, double *y,
+              int64_t incy, int precision) {
+    return gemv_host(false, M, K, A, lda, x, incx, y, incy, precision);
+}
+
+int f4j_dgemv_dev(int64_t M, int64_t K, const double *A, int64_t lda, const double *x, int64_t incx, double *y,
+                  int64_t incy, int precision, void *stream) {
+    int rc = validate_gemv(false, M, K, A, lda, x, incx, y, incy, precision);
+    if (rc != F4J_OK) return rc;
+    if ((rc = ensure_ready()) != F4J_OK) return rc;
+    return gemv_dev(false, M, K, A, lda, x, incx, y, incy, precision, pick_stream(stream));
+}
+
+int f4j_zgemv(int64_t M, int64_t K, const double *A, int64_t lda, const double *x, int64_t incx, double *y,
+              int64_t incy, int precision) {
+    return gemv_host(true, M, K, A, lda, x, incx, y, incy, precision);
+}
+
+int f4j_zgemv_dev(int64_t M, int64_t K, const double *A, int64_t lda, const double *x, int64_t incx, double *y,
+                  int64_t incy, int precision, void *stream) {
+    int rc = validate_gemv(true, M, K, A, lda, x, incx, y, incy, precision);
+    if (rc != F4J_OK) return rc;
+    if ((rc = ensure_ready()) != F4J_OK) return rc;
+    return gemv_dev(true, M, K, A, lda, x, incx, y, incy, precision, pick_stream(stream));
+}
+
 int f4j_last_split_report(f4j_split_report *out) {
     if (!out) return fail(F4J_ERR_INVALID_ARG, "out is null");
     *out = t_split;
@@ -780,7 +910,10 @@ int f4j_dispatch_name(int is_complex, int opA, int opB, int64_t M, int64_t N, in
     const int prec = resolve_precision(precision, M, N, K, opA, opB);
     if (M == 0 || N == 0) name = "EMPTY";
     else if (K == 0) name = "ZERO_FILL";
-    else if (prec == F4J_PREC_STRICT) name = "STRICT";
+    else if (N == 1 && opA == F4J_OP_N && (prec == F4J_PREC_STRICT || prec == F4J_PREC_FP64)) {
+        const int kind = gemv_kind(cplx, opB != F4J_OP_N, M, K);
+        name = kind == GEMV_BLOCKED ? "GEMV_BLOCKED" : (kind == GEMV_CHAIN ? "GEMV_CHAIN" : "GEMV_COMPLEX");
+    } else if (prec == F4J_PREC_STRICT) name = "STRICT";
     else if (prec == F4J_PREC_FP64) name = tile_name(pick_tile(M, N * (cplx ? 2 : 1)));
     else name = std::string("SPLIT_") + prec_names[prec];
     std::snprintf(buf, buflen, "%s", name.c_str());

@@ -1,0 +1,510 @@
+// Matrix-vector products in the reference's own summation orders, streamed
+// from HBM once.
+//
+// Real, y = A x (A M x K row-major):
+//   Matrix.mult(Vector) (M:arrays/dense/Matrix.java:1406-1408) ->
+//   ReDeMatVecMultDispatcher (M:linalg/ops/dispatch/ReDeMatVecMultDispatcher.java:141-181), and
+//   the N = 1 case of Matrix.mult(Matrix) (ReDeMatMultDispatcher.java:183-198) select
+//   RealDenseMatMult.blockedVector / concurrentBlockedVector
+//   (M:linalg/ops/dense/real/RealDenseMatMult.java:458-491, :537-569): y_i starts at +0.0 and,
+//   for each 64-wide k block in order, y_i += (the block's k-ascending sum from +0.0).
+//   Small problems take standardVector / standard (:426-447, :50-79), and
+//   Matrix.multTranspose with a one-row B takes RealDenseMatMultTranspose (:53-82): both
+//   are one k-ascending sum per row ("plain chain").
+// Complex, y = A x:
+//   CMatrix.mult(CVector) (M:arrays/backend/semiring_arrays/AbstractDenseSemiringMatrix.java:295-298
+//   -> DenseSemiringMatMultDispatcher.dispatchVector :120-142 -> DenseSemiringMatMult.*Vector
+//   :435-574) and the N = 1 case of CMatrix.mult(CMatrix): every variant reloads the running
+//   sum from dest, so y_i is one k-ascending sum of Complex128.mult products
+//   (M:numbers/Complex128.java:276-278, :336-341).
+//
+// Every result is a chain of dependent additions.  A lane runs one chain:
+// a (row, 64-block) chain for the blocked form (8 rows x 8 blocks per wave, the
+// block sums then added in block order across lanes), a row for the plain real
+// form (64 rows per wave) and a (row, component) pair for complex (32 rows per
+// wave).  The wave streams its slab of A in stages of 16 elements per chain
+// (8 KB): coalesced 8-byte buffer loads into registers D stages ahead, then a
+// write into LDS rotated so that the 64 lanes read their own chains' elements
+// without bank conflicts.  Loads past K or past the last row are given an
+// out-of-range offset and return zeros, which leave every chain unchanged (a
+// sum that starts at +0.0 is never -0.0, so adding +0.0 is exact).  Products and
+// sums are separately rounded (no contraction), so the results equal the
+// reference's bit for bit.  The kernels are bound by reading A once.
+#include <type_traits>
+
+#include "f4j_internal.h"
+
+#pragma clang fp contract(off)
+
+namespace f4j {
+namespace {
+
+constexpr uint32_t OOR = 0x80000000u;      // offset past every slab (slabs are < 2^31 bytes)
+
+struct GemvArgs {
+    int64_t M, K;
+    const double *A;
+    int64_t lda;              // doubles between rows (complex: 2 x complex lda)
+    const double *x;
+    int64_t incx;             // doubles between elements (complex: 2 x complex stride)
+    double *y;
+    int64_t incy;             // doubles between results (complex: 2 x complex stride)
+    int64_t nstage;           // a multiple of the prefetch depth
+    int conjx;
+};
+
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+
+__device__ inline __amdgpu_buffer_rsrc_t rsrc(const void *p, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc((void *)p, (short)0, (int)bytes, 0x00020000);
+}
+
+// Rows of A per wave, A loads (VEC doubles each) and x loads (one double each) per lane per stage.
+template <int KIND>
+constexpr int rows_per_wave() { return KIND == GEMV_BLOCKED ? 8 : (KIND == GEMV_CHAIN ? 64 : 32); }
+template <int KIND>
+constexpr int x_loads() { return KIND == GEMV_BLOCKED ? 2 : 1; }
+
+// Stage s covers k = kstage(s) + [0, 64 J) with J = 8 blocks (blocked form) or
+// 1 (otherwise); within it, the k a lane loads depends only on the lane and
+// the load index.
+template <int KIND>
+__device__ inline int kstage(int64_t s) {
+    if constexpr (KIND == GEMV_BLOCKED) return (int)(512 * (s >> 2) + 16 * (s & 3));
+    else return (int)(16 * s);
+}
+
+// Row (within the wave's slab) and k offset (within the stage) of A load i of
+// lane l.  A stage holds 16 elements of each of the 64 chains.  Load i of lane l
+// lands in LDS at double 64 i + l (VEC 1) or 128 i + 2 l (VEC 2); the element
+// each lane fetches is rotated so that the reading lanes spread over the banks.
+template <int KIND, int VEC>
+__device__ inline void a_slot(int i, int l, int &r, int &kl) {
+    if constexpr (KIND == GEMV_CPLX) {
+        r = 4 * i + (l >> 4);                          // rows 4i..4i+3, one complex element per lane
+        kl = ((l & 15) - r) & 15;                      // reader of row r finds element e at (e + r) & 15
+    } else {
+        int ch, el;
+        if constexpr (VEC == 2) {
+            const int b = l >> 3;
+            ch = 8 * i + b;                            // reader of chain c finds pair p at (p + c/8 + c%8) & 7
+            el = 2 * (((l & 7) - i - b) & 7);
+        } else {
+            const int b = l >> 4;
+            ch = 4 * i + b;                            // reader of chain c finds element e at (e + c/4 + c%4) & 15
+            el = ((l & 15) - i - b) & 15;
+        }
+        if constexpr (KIND == GEMV_BLOCKED) {
+            r = ch >> 3;                               // chain = (row ch / 8, block ch % 8)
+            kl = 64 * (ch & 7) + el;
+        } else {
+            r = ch;                                    // chain = row
+            kl = el;
+        }
+    }
+}
+
+// k offset (within the stage), component and sign of x load h of lane l.
+template <int KIND>
+__device__ inline void x_slot(int h, int l, int conjx, int &kl, int &part, bool &neg) {
+    part = 0;
+    neg = false;
+    if constexpr (KIND == GEMV_BLOCKED) {
+        const int j = 4 * h + (l >> 4);                // block j: element e at lane position (e + j) & 15
+        kl = 64 * j + (((l & 15) - j) & 15);
+    } else if constexpr (KIND == GEMV_CHAIN) {
+        kl = l & 15;
+    } else {
+        // Two copies of the stage's 16 elements b' (b' = conj(b) when conjx):
+        // lanes 0-31 hold (b're, -b'im), lanes 32-63 hold (b'im, b're).
+        const int copy = l >> 5, d = l & 31, p = d & 1;
+        kl = d >> 1;
+        part = copy == 0 ? p : 1 - p;
+        neg = part == 1 && ((copy == 0) != (conjx != 0));
+    }
+}
+
+template <int KIND, int VEC, int D>
+__global__ __launch_bounds__(64) void gemv_kernel(GemvArgs g) {
+    constexpr int R = rows_per_wave<KIND>(), XN = x_loads<KIND>(), NA = 16 / VEC;
+    static_assert(D * (NA + XN) <= 63, "loads in flight must fit the vmcnt counter");
+    __shared__ __attribute__((aligned(16))) double lds[1024 + 64 * XN];
+    const int l = threadIdx.x;
+    const int64_t row0 = (int64_t)blockIdx.x * R;
+    const int rows = (int)((g.M - row0) < R ? (g.M - row0) : R);
+    const int ew = KIND == GEMV_CPLX ? 2 : 1;
+    const __amdgpu_buffer_rsrc_t ra = rsrc(g.A + row0 * g.lda, (uint32_t)((int64_t)rows * g.lda * 8));
+    const __amdgpu_buffer_rsrc_t rx = rsrc(g.x, (uint32_t)(((g.K - 1) * g.incx + ew) * 8));
+
+    typedef typename std::conditional<VEC == 2, f64x2, double>::type AV;
+    // Per-lane load geometry: k offset within a stage (a row past the slab
+    // counts as past K) and byte offset at stage 0.  K < 2^28 (slabs < 2^31 bytes).
+    const int K = (int)g.K, incx = (int)g.incx;
+    int akl[NA], xkl[XN];
+    uint32_t aoff[NA], xoff[XN];
+    bool xneg[XN];
+#pragma unroll
+    for (int i = 0; i < NA; ++i) {
+        int r, kl;
+        a_slot<KIND, VEC>(i, l, r, kl);
+        akl[i] = r < rows ? kl : K;
+        aoff[i] = (uint32_t)(((int64_t)r * g.lda + (int64_t)kl * ew) * 8);
+    }
+#pragma unroll
+    for (int h = 0; h < XN; ++h) {
+        int part;
+        x_slot<KIND>(h, l, g.conjx, xkl[h], part, xneg[h]);
+        xoff[h] = (uint32_t)(((int64_t)xkl[h] * incx + part) * 8);
+    }
+    AV va[D][NA];
+    double vx[D][XN];
+    auto issue = [&](AV (&a)[NA], double (&xv)[XN], int64_t s) {
+        const int ks = kstage<KIND>(s);
+        const uint32_t ka = (uint32_t)ks * (uint32_t)(8 * ew), kx = (uint32_t)ks * (uint32_t)incx * 8u;
+#pragma unroll
+        for (int i = 0; i < NA; ++i) {
+            const uint32_t off = ks + akl[i] < K ? aoff[i] + ka : OOR;
+            if constexpr (VEC == 2)
+                a[i] = __builtin_bit_cast(f64x2, __builtin_amdgcn_raw_buffer_load_b128(ra, off, 0, 0));
+            else
+                a[i] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(ra, off, 0, 0));
+        }
+#pragma unroll
+        for (int h = 0; h < XN; ++h) {
+            const uint32_t off = ks + xkl[h] < K ? xoff[h] + kx : OOR;
+            const double v = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rx, off, 0, 0));
+            xv[h] = xneg[h] ? -v : v;
+        }
+    };
+
+    // Where this lane's chain finds its elements in LDS.
+    int a_base, a_rot, x_base, x_rot = 0;
+    if constexpr (KIND == GEMV_CPLX) {
+        const int r = l & 31;
+        a_base = 128 * (r >> 2) + 32 * (r & 3);
+        a_rot = r;
+        x_base = 1024 + 32 * (l >> 5);
+    } else {
+        if constexpr (VEC == 2) {
+            a_base = 128 * (l >> 3) + 16 * (l & 7);
+            a_rot = (l >> 3) + (l & 7);
+        } else {
+            a_base = 64 * (l >> 2) + 16 * (l & 3);
+            a_rot = (l >> 2) + (l & 3);
+        }
+        if constexpr (KIND == GEMV_BLOCKED) {
+            const int j = l & 7;
+            x_base = 1024 + 64 * (j >> 2) + 16 * (j & 3);
+            x_rot = j;
+        } else {
+            x_base = 1024;
+        }
+    }
+
+    double s_acc = 0.0;    // this lane's chain
+    double dest = 0.0;     // blocked form: the row's running sum (lanes l % 8 == 0)
+
+#pragma unroll
+    for (int d = 0; d < D; ++d) issue(va[d], vx[d], d);
+
+    for (int64_t s0 = 0; s0 < g.nstage; s0 += D) {
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            const int64_t s = s0 + d;
+#pragma unroll
+            for (int i = 0; i < NA; ++i) {
+                if constexpr (VEC == 2) *(f64x2 *)&lds[128 * i + 2 * l] = va[d][i];
+                else lds[64 * i + l] = va[d][i];
+            }
+#pragma unroll
+            for (int h = 0; h < XN; ++h) lds[1024 + 64 * h + l] = vx[d][h];
+            issue(va[d], vx[d], s + D);       // past the last stage the offsets are out of range: zeros
+            if constexpr (KIND == GEMV_CPLX) {
+#pragma unroll
+                for (int e = 0; e < 16; ++e) {
+                    const f64x2 a = *(const f64x2 *)&lds[a_base + 2 * ((e + a_rot) & 15)];
+                    const f64x2 w = *(const f64x2 *)&lds[x_base + 2 * e];
+                    const double p = __dadd_rn(__dmul_rn(a.x, w.x), __dmul_rn(a.y, w.y));
+                    s_acc = __dadd_rn(s_acc, p);
+                }
+            } else if constexpr (VEC == 2) {
+#pragma unroll
+                for (int p = 0; p < 8; ++p) {
+                    const f64x2 a = *(const f64x2 *)&lds[a_base + 2 * ((p + a_rot) & 7)];
+                    const double x0 = lds[x_base + ((2 * p + x_rot) & 15)];
+                    const double x1 = lds[x_base + ((2 * p + 1 + x_rot) & 15)];
+                    s_acc = __dadd_rn(s_acc, __dmul_rn(a.x, x0));
+                    s_acc = __dadd_rn(s_acc, __dmul_rn(a.y, x1));
+                }
+            } else {
+#pragma unroll
+                for (int e = 0; e < 16; ++e) {
+                    const double a = lds[a_base + ((e + a_rot) & 15)];
+                    const double xv = lds[x_base + ((e + x_rot) & 15)];
+                    s_acc = __dadd_rn(s_acc, __dmul_rn(a, xv));
+                }
+            }
+            if constexpr (KIND == GEMV_BLOCKED) {
+                if ((s & 3) == 3) {
+                    // the 8 block sums of row l / 8 sit in lanes 8 (l / 8) + j: add them in block order
+                    double t = dest;
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) t = __dadd_rn(t, __shfl(s_acc, (l & ~7) + j));
+                    dest = t;
+                    s_acc = 0.0;
+                }
+            }
+        }
+    }
+
+    if constexpr (KIND == GEMV_BLOCKED) {
+        const int r = l >> 3;
+        if ((l & 7) == 0 && r < rows) g.y[(row0 + r) * g.incy] = dest;
+    } else if constexpr (KIND == GEMV_CHAIN) {
+        if (l < rows) g.y[(row0 + l) * g.incy] = s_acc;
+    } else {
+        const int r = l & 31;
+        if (r < rows) g.y[(row0 + r) * g.incy + (l >> 5)] = s_acc;
+    }
+}
+
+// ---- direct-to-LDS form (16-byte aligned A with even K and lda; complex always) ----
+//
+// The same chains and LDS layout as above, but every load writes LDS directly
+// (buffer_load ... lds), into a ring of D stage slots, D - 1 stages ahead.
+// Loads produce no registers, so the waits are explicit: each step issues the
+// same number of loads (offsets clamped into the slab), and at the top of step
+// s vmcnt((D - 2) L) retires stage s.  Elements past K are skipped by the
+// reader (a uniform test picks the masked path for the last stages only).
+typedef __attribute__((address_space(3))) void lds_void;
+#define F4J_GEMV_WAIT_VMCNT(n) __builtin_amdgcn_s_waitcnt(((n) & 15) | (7 << 4) | (15 << 8) | (((n) >> 4) << 14))
+
+constexpr int SLOT = 9216;                 // bytes per stage: 8 KB of A, 1 KB of x
+
+// x loads per stage: XV 2 = 16-byte pairs (x aligned, unit stride, K even), XV 1 = dwords.
+template <int KIND, int XV>
+constexpr int x_dma_loads() { return KIND == GEMV_CPLX ? 2 : (KIND == GEMV_BLOCKED && XV == 1 ? 4 : 1); }
+
+template <int KIND, int XV, int D>
+__global__ __launch_bounds__(64) void gemv_dma_kernel(GemvArgs g) {
+    constexpr int R = rows_per_wave<KIND>(), XL = x_dma_loads<KIND, XV>(), L = 8 + XL;
+    constexpr int ew = KIND == GEMV_CPLX ? 2 : 1;
+    static_assert((D - 1) * L <= 63, "loads in flight must fit the vmcnt counter");
+    __shared__ __attribute__((aligned(1024))) char lds[D * SLOT];
+    const int l = threadIdx.x;
+    const int64_t row0 = (int64_t)blockIdx.x * R;
+    const int rows = (int)((g.M - row0) < R ? (g.M - row0) : R);
+    const int K = (int)g.K, incx = (int)g.incx;
+    const __amdgpu_buffer_rsrc_t ra = rsrc(g.A + row0 * g.lda, (uint32_t)((int64_t)rows * g.lda * 8));
+    const __amdgpu_buffer_rsrc_t rx = rsrc(g.x, (uint32_t)(((g.K - 1) * g.incx + ew) * 8));
+    // last valid load start: a pair (real) or an element (complex) / x element
+    const int ka_max = KIND == GEMV_CPLX ? K - 1 : K - 2;
+    const int kx_max = XV == 2 ? K - 2 : K - 1;
+
+    // per-lane load geometry
+    int akl[8];
+    uint32_t arow[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        int r, kl;
+        a_slot<KIND, 2>(i, l, r, kl);
+        akl[i] = kl;
+        arow[i] = (uint32_t)((int64_t)(r < rows ? r : rows - 1) * g.lda * 8);
+    }
+    int xkl[XL], xadd[XL];
+#pragma unroll
+    for (int h = 0; h < XL; ++h) {
+        if constexpr (KIND == GEMV_CPLX) {
+            const int m = l >> 1, part = m & 1;          // copy h, double m = (element m / 2, part)
+            xkl[h] = m >> 1;
+            xadd[h] = 8 * (h == 0 ? part : 1 - part) + 4 * (l & 1);
+        } else if constexpr (XV == 2) {
+            if constexpr (KIND == GEMV_BLOCKED) {
+                const int j = l >> 3;                    // block j: pair p at lane position (p + j) & 7
+                xkl[h] = 64 * j + 2 * (((l & 7) - j) & 7);
+            } else {
+                xkl[h] = 2 * (l & 7);
+            }
+            xadd[h] = 0;
+        } else {
+            const int m = 32 * h + (l >> 1);             // double m of the stage's x area
+            if constexpr (KIND == GEMV_BLOCKED) {
+                const int j = m >> 4;                    // block j: element e at (e + j) & 15
+                xkl[h] = 64 * j + (((m & 15) - j) & 15);
+            } else {
+                xkl[h] = m & 15;
+            }
+            xadd[h] = 4 * (l & 1);
+        }
+    }
+
+    auto issue = [&](int64_t s) {
+        const int ks = kstage<KIND>(s);
+        char *slot = lds + (int)(s % D) * SLOT;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int k = min(ks + akl[i], ka_max);
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(ra, (lds_void *)(slot + 1024 * i), 16,
+                                                     arow[i] + (uint32_t)k * (8u * ew), 0, 0, 0);
+        }
+#pragma unroll
+        for (int h = 0; h < XL; ++h) {
+            const int k = min(ks + xkl[h], kx_max);
+            const uint32_t off = (uint32_t)k * (uint32_t)incx * 8u + xadd[h];   // incx in doubles
+            if constexpr (XV == 2 && KIND != GEMV_CPLX)
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(rx, (lds_void *)(slot + 8192), 16, off, 0, 0, 0);
+            else
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(rx, (lds_void *)(slot + 8192 + 256 * h), 4, off, 0, 0, 0);
+        }
+    };
+
+    // reader geometry (bytes within a slot)
+    int a_base, a_rot, x_base = 8192, x_rot = 0, kb = 0;
+    uint32_t sg1 = 0, sg2 = 0;                        // complex: sign masks of the two products
+    if constexpr (KIND == GEMV_CPLX) {
+        const int r = l & 31;
+        a_base = 1024 * (r >> 2) + 256 * (r & 3);
+        a_rot = r;
+        const bool im = l >= 32, cj = g.conjx != 0;
+        x_base = 8192 + (im ? 256 : 0);
+        // re = t1 - s t2, im = s t1 + t2 with t1 = ar w0, t2 = ai w1, s = -1 for conj(x)
+        sg1 = (im && cj) ? 0x80000000u : 0u;
+        sg2 = (!im && !cj) ? 0x80000000u : 0u;
+    } else {
+        a_base = 1024 * (l >> 3) + 128 * (l & 7);
+        a_rot = (l >> 3) + (l & 7);
+        if constexpr (KIND == GEMV_BLOCKED) {
+            const int j = l & 7;
+            kb = 64 * j;
+            x_base = 8192 + (XV == 2 ? 128 : 128) * j;
+            x_rot = j;
+        }
+    }
+
+    double s_acc = 0.0, dest = 0.0;
+    auto compute = [&](int64_t s, bool masked) {
+        const char *slot = lds + (int)(s % D) * SLOT;
+        const int ks = kstage<KIND>(s);
+        if constexpr (KIND == GEMV_CPLX) {
+#pragma unroll
+            for (int e = 0; e < 16; ++e) {
+                const f64x2 a = *(const f64x2 *)(slot + a_base + 16 * ((e + a_rot) & 15));
+                const f64x2 w = *(const f64x2 *)(slot + x_base + 16 * e);
+                double t1 = __dmul_rn(a.x, w.x), t2 = __dmul_rn(a.y, w.y);
+                t1 = __builtin_bit_cast(double, __builtin_bit_cast(uint64_t, t1) ^ ((uint64_t)sg1 << 32));
+                t2 = __builtin_bit_cast(double, __builtin_bit_cast(uint64_t, t2) ^ ((uint64_t)sg2 << 32));
+                const double v = __dadd_rn(s_acc, __dadd_rn(t1, t2));
+                s_acc = (!masked || ks + e < K) ? v : s_acc;
+            }
+        } else {
+#pragma unroll
+            for (int p = 0; p < 8; ++p) {
+                const f64x2 a = *(const f64x2 *)(slot + a_base + 16 * ((p + a_rot) & 7));
+                double x0, x1;
+                if constexpr (XV == 2 && KIND == GEMV_BLOCKED) {
+                    const f64x2 w = *(const f64x2 *)(slot + x_base + 16 * ((p + x_rot) & 7));
+                    x0 = w.x;
+                    x1 = w.y;
+                } else if constexpr (KIND == GEMV_BLOCKED) {
+                    x0 = *(const double *)(slot + x_base + 8 * ((2 * p + x_rot) & 15));
+                    x1 = *(const double *)(slot + x_base + 8 * ((2 * p + 1 + x_rot) & 15));
+                } else {
+                    const f64x2 w = *(const f64x2 *)(slot + x_base + 16 * p);
+                    x0 = w.x;
+                    x1 = w.y;
+                }
+                const double v = __dadd_rn(__dadd_rn(s_acc, __dmul_rn(a.x, x0)), __dmul_rn(a.y, x1));
+                s_acc = (!masked || ks + kb + 2 * p < K) ? v : s_acc;
+            }
+        }
+    };
+
+#pragma unroll
+    for (int d = 0; d < D - 1; ++d) issue(d);
+    // a stage is complete when every chain's 16 elements lie below K
+    const int span = KIND == GEMV_BLOCKED ? 64 * 7 + 16 : 16;
+    for (int64_t s = 0; s < g.nstage; ++s) {
+        __builtin_amdgcn_sched_barrier(0);
+        F4J_GEMV_WAIT_VMCNT((D - 2) * L);
+        __builtin_amdgcn_sched_barrier(0);
+        issue(s + D - 1);                 // past the last stage: clamped reloads, never read
+        __builtin_amdgcn_sched_barrier(0);
+        if (kstage<KIND>(s) + span <= K) compute(s, false);
+        else compute(s, true);
+        if constexpr (KIND == GEMV_BLOCKED) {
+            if ((s & 3) == 3) {
+                double t = dest;
+#pragma unroll
+                for (int j = 0; j < 8; ++j) t = __dadd_rn(t, __shfl(s_acc, (l & ~7) + j));
+                dest = t;
+                s_acc = 0.0;
+            }
+        }
+    }
+    F4J_GEMV_WAIT_VMCNT(0);               // no LDS write in flight when the wave ends
+
+    if constexpr (KIND == GEMV_BLOCKED) {
+        const int r = l >> 3;
+        if ((l & 7) == 0 && r < rows) g.y[(row0 + r) * g.incy] = dest;
+    } else if constexpr (KIND == GEMV_CHAIN) {
+        if (l < rows) g.y[(row0 + l) * g.incy] = s_acc;
+    } else {
+        const int r = l & 31;
+        if (r < rows) g.y[(row0 + r) * g.incy + (l >> 5)] = s_acc;
+    }
+}
+
+template <int KIND, int XV, int D>
+void launch_dma(GemvArgs g, int64_t stages, hipStream_t stream) {
+    constexpr int R = rows_per_wave<KIND>();
+    g.nstage = stages;
+    hipLaunchKernelGGL((gemv_dma_kernel<KIND, XV, D>), dim3((unsigned)((g.M + R - 1) / R)), dim3(64), 0, stream, g);
+}
+
+template <int KIND, int VEC, int D>
+void launch(GemvArgs g, int64_t stages, hipStream_t stream) {
+    constexpr int R = rows_per_wave<KIND>();
+    g.nstage = (stages + D - 1) / D * D;
+    hipLaunchKernelGGL((gemv_kernel<KIND, VEC, D>), dim3((unsigned)((g.M + R - 1) / R)), dim3(64), 0, stream, g);
+}
+
+}  // namespace
+
+bool gemv_supported(int kind, int64_t K, int64_t lda, int64_t incx) {
+    // 32-bit offsets: the wave's slab of A and all of x stay below 2^31 bytes,
+    // also for the stage offsets computed past the end (K + 1024 elements).
+    const int64_t ew = kind == GEMV_CPLX ? 2 : 1;
+    const int64_t R = kind == GEMV_BLOCKED ? 8 : (kind == GEMV_CHAIN ? 64 : 32);
+    const int64_t lim = 0x7FFFFFF0;
+    return K >= 1 && lda >= K && incx >= 1 && R * lda * ew * 8 <= lim &&
+           (K + 4096) * incx * ew * 8 <= lim;
+}
+
+hipError_t launch_gemv(int kind, int64_t M, int64_t K, const double *A, int64_t lda, const double *x, int64_t incx,
+                       double *y, int64_t incy, bool conjx, hipStream_t stream) {
+    if (M == 0) return hipSuccess;
+    if (!gemv_supported(kind, K, lda, incx)) return hipErrorInvalidValue;
+    const int64_t ew = kind == GEMV_CPLX ? 2 : 1;
+    GemvArgs g{M, K, A, lda * ew, x, incx * ew, y, incy * ew, 0, conjx ? 1 : 0};
+    // Direct-to-LDS loads move 16 bytes: 16-byte aligned pairs (complex: elements)
+    // that never straddle a row end.  Otherwise the register-staged form.
+    const bool a16 = ((uintptr_t)A & 15) == 0 && (kind == GEMV_CPLX || (lda % 2 == 0 && K % 2 == 0));
+    const bool x16 = ((uintptr_t)x & 15) == 0 && incx == 1 && K % 2 == 0;
+    if (kind == GEMV_CPLX && !a16) return hipErrorInvalidValue;
+    if (kind == GEMV_BLOCKED) {
+        const int64_t stages = 4 * ((K + 511) / 512);
+        if (a16 && x16) launch_dma<GEMV_BLOCKED, 2, 4>(g, stages, stream);
+        else if (a16) launch_dma<GEMV_BLOCKED, 1, 4>(g, stages, stream);
+        else launch<GEMV_BLOCKED, 1, 3>(g, stages, stream);
+    } else if (kind == GEMV_CHAIN) {
+        const int64_t stages = (K + 15) / 16;
+        if (a16 && x16) launch_dma<GEMV_CHAIN, 2, 7>(g, stages, stream);
+        else if (a16) launch_dma<GEMV_CHAIN, 1, 7>(g, stages, stream);
+        else launch<GEMV_CHAIN, 1, 3>(g, stages, stream);
+    } else {
+        launch_dma<GEMV_CPLX, 1, 7>(g, (K + 15) / 16, stream);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace f4j

@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define F4J_ABI_VERSION 2
+#define F4J_ABI_VERSION 3
 
 typedef enum f4j_status {
     F4J_OK = 0,
@@ -122,6 +122,33 @@ int f4j_zgemm(int opA, int opB, int64_t M, int64_t N, int64_t K,
 int f4j_zgemm_dev(int opA, int opB, int64_t M, int64_t N, int64_t K,
                   const double *A, int64_t lda, const double *B, int64_t ldb,
                   double *C, int64_t ldc, int precision, int slices, int use_3m, void *stream);
+
+/* ---- matrix-vector products ----
+ * Replace:
+ *   real:    Matrix.mult(Vector)      M:arrays/dense/Matrix.java:1406-1408 ->
+ *            ReDeMatVecMultDispatcher  M:linalg/ops/dispatch/ReDeMatVecMultDispatcher.java:141-181 ->
+ *            RealDenseMatMult.standardVector / blockedVector / concurrentBlockedVector
+ *                                      M:linalg/ops/dense/real/RealDenseMatMult.java:426-569
+ *   complex: CMatrix.mult(CVector)    M:arrays/backend/semiring_arrays/AbstractDenseSemiringMatrix.java:295-298 ->
+ *            DenseSemiringMatMultDispatcher.dispatchVector
+ *                                      M:linalg/ops/dense/semiring_ops/DenseSemiringMatMultDispatcher.java:120-142 ->
+ *            DenseSemiringMatMult.*Vector M:linalg/ops/dense/semiring_ops/DenseSemiringMatMult.java:435-574
+ * y (M) = A (M x K, row-major, lda) x (K); incx / incy are the element strides of x and y (>= 1).
+ * The results equal the reference's bit for bit: real products with M*K + K < smallThreshold are
+ * k-ascending sums (standardVector); larger ones sum each 64-wide k block from +0.0 and add the
+ * block sums in order (blockedVector); complex products are k-ascending sums of Complex128
+ * products.  precision: F4J_PREC_AUTO, F4J_PREC_FP64 or F4J_PREC_STRICT, which all run these
+ * kernels (the product is bound by reading A once); a SPLIT precision returns
+ * F4J_ERR_UNSUPPORTED.  f4j_dgemm / f4j_zgemm with N = 1 and opA = N run the same kernels in
+ * FP64 and STRICT precision (the reference's N = 1 dispatch, ReDeMatMultDispatcher.java:183-198). */
+int f4j_dgemv(int64_t M, int64_t K, const double *A, int64_t lda, const double *x, int64_t incx,
+              double *y, int64_t incy, int precision);
+int f4j_dgemv_dev(int64_t M, int64_t K, const double *A, int64_t lda, const double *x, int64_t incx,
+                  double *y, int64_t incy, int precision, void *stream);
+int f4j_zgemv(int64_t M, int64_t K, const double *A, int64_t lda, const double *x, int64_t incx,
+              double *y, int64_t incy, int precision);
+int f4j_zgemv_dev(int64_t M, int64_t K, const double *A, int64_t lda, const double *x, int64_t incx,
+                  double *y, int64_t incy, int precision, void *stream);
 
 /* ---- grouped (batched) products ----
  * No reference counterpart: Flag4j multiplies one pair per call; a batch is a Java loop of

@@ -30,7 +30,7 @@ def test_exports_every_declared_symbol():
 
 
 def test_abi_version():
-    assert _native.load().f4j_abi_version() == 2
+    assert _native.load().f4j_abi_version() == 3
 
 
 def _dgemm(*args):
@@ -103,6 +103,37 @@ def test_dispatcher_small_path_threshold():
     assert _name(0, 0, 0, 4, 2, 3, "fp64").startswith("MFMA_F64")
 
 
+def test_dispatcher_matrix_vector_routes():
+    """N = 1 with A stored M x K is a matrix-vector product: the reference picks
+    standard (small) or the 64-block vector kernels (ReDeMatMultDispatcher.java:158-198),
+    multTranspose a k-ascending sum (RealDenseMatMultTranspose.java:53-82), complex
+    products a k-ascending sum (DenseSemiringMatMult.java:435-574)."""
+    assert _name(0, 0, 0, 40, 1, 40) == "GEMV_CHAIN"             # 1600 + 40 < 2450: small path
+    assert _name(0, 0, 0, 100, 1, 100) == "GEMV_BLOCKED"
+    assert _name(0, 0, 0, 100, 1, 100, "strict") == "GEMV_BLOCKED"
+    assert _name(0, 0, 0, 100, 1, 100, "fp64") == "GEMV_BLOCKED"
+    assert _name(0, 0, 1, 100, 1, 100) == "GEMV_CHAIN"           # multTranspose
+    assert _name(1, 0, 0, 100, 1, 100) == "GEMV_COMPLEX"
+    assert _name(1, 0, 2, 100, 1, 100, "strict") == "GEMV_COMPLEX"
+    assert _name(0, 1, 0, 100, 1, 100).startswith("MFMA")        # op(A) = T: not the vector path
+    assert _name(0, 0, 0, 100, 1, 100, "split_int8") == "SPLIT_split_int8"
+
+
+def test_gemv_arguments_checked_before_device_use():
+    L = _native.load()
+    a = np.zeros(8)
+    p = ctypes.c_void_p(a.ctypes.data)
+    assert L.f4j_dgemv(-1, 2, p, 2, p, 1, p, 1, 0) == _native.ERR_INVALID_ARG
+    assert L.f4j_dgemv(2, 3, p, 2, p, 1, p, 1, 0) == _native.ERR_INVALID_ARG     # lda < K
+    assert L.f4j_dgemv(2, 2, p, 2, p, 0, p, 1, 0) == _native.ERR_INVALID_ARG     # incx < 1
+    assert L.f4j_dgemv(2, 2, None, 2, p, 1, p, 1, 0) == _native.ERR_INVALID_ARG
+    assert L.f4j_zgemv(2, 2, p, 2, p, 1, None, 1, 0) == _native.ERR_INVALID_ARG
+    assert L.f4j_dgemv(2, 2, p, 2, p, 1, p, 1, 9) == _native.ERR_INVALID_ARG
+    assert L.f4j_dgemv(2, 2, p, 2, p, 1, p, 1, _native.PREC["split_bf16"]) == _native.ERR_UNSUPPORTED
+    assert b"FP64" in L.f4j_last_error()
+    assert L.f4j_dgemv(0, 5, None, 5, None, 1, None, 1, 0) == 0                 # empty: no device
+
+
 def test_config_keys_validated():
     L = _native.load()
     buf = ctypes.create_string_buffer(32)
@@ -145,6 +176,8 @@ def test_strict_kernel_has_no_fused_multiply_add():
     asm = open(os.path.join(ROOT, "_build", "obj", "gemm_strict.s")).read()
     assert "v_mul_f64" in asm and "v_add_f64" in asm
     assert "v_fma_f64" not in asm
+    gemv = open(os.path.join(ROOT, "_build", "obj", "gemv.s")).read()
+    assert "gemv_dma_kernel" in gemv and "v_fma_f64" not in gemv
 
 
 def test_tuner_cache_file_roundtrip(tmp_path):

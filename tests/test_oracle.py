@@ -121,6 +121,38 @@ def test_complex_column_vector_golden(golden):
     assert np.array_equal(o.cplx_gemm(A, B).reshape(-1), exp)
 
 
+def test_real_matrix_vector_golden(golden):
+    """T:arrays/dense/matrix/MatrixVectorTests.java:27-45 (Matrix.mult(Vector), small path:
+    standardVector; with K <= 64 the blocked form gives the same bits)."""
+    g = golden["real_api_matvec"]
+    A, x, exp = real_mat(g["a"]), real_mat(g["b"]), real_mat(g["c"])
+    assert np.array_equal(bits(o.real_standard(A, x.reshape(-1, 1)).reshape(-1)), bits(exp))
+    assert np.array_equal(bits(o.real_blocked_vector(A, x, 64)), bits(exp))
+
+
+def test_complex_matrix_vector_golden(golden):
+    """T:arrays/dense/complex_matrix/CMatrixMatVecMultTests.java:62-84 (CMatrix.mult(CVector))."""
+    g = golden["cplx_api_matvec"]
+    A, B, exp = cplx_mat(g["a"]), cplx_mat(g["b"]), cplx_mat(g["c"])
+    act = o.cplx_gemm(A, B).reshape(-1)
+    assert np.array_equal(act.view(np.float64), exp.view(np.float64))
+    # a swapped product term (re*im + re*im) is a plausible mistake the vector must reject
+    wrong = o.cplx_gemm(A.conj(), B).reshape(-1)
+    assert not np.array_equal(wrong, exp)
+
+
+def test_matrix_vector_shape_errors(golden):
+    """Size mismatches raise LinearAlgebraException before any device work
+    (MatrixVectorTests.java:47-58, CMatrixMatVecMultTests.java:86-97)."""
+    import flag4j_amd as f4j
+    g = golden["real_api_matvec"]
+    with pytest.raises(f4j.LinearAlgebraException):
+        f4j.Matrix(real_mat(g["a"])).mult(f4j.Vector(real_mat(g["b_bad"])))
+    g = golden["cplx_api_matvec"]
+    with pytest.raises(f4j.LinearAlgebraException):
+        f4j.CMatrix(cplx_mat(g["a"])).mult(f4j.CVector(cplx_mat(g["b_bad"]).reshape(-1)))
+
+
 def test_complex_pow_golden(golden):
     """T:arrays/dense/complex_matrix/CMatrixMultTests.java:198-219."""
     g = golden["cplx_api_pow"]
