@@ -58,6 +58,10 @@ SIGNATURES = {
     "f4j_dgemm_dev": ([_i, _i, _i64, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _i, _i, _vp], _i),
     "f4j_zgemm": ([_i, _i, _i64, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _i, _i, _i], _i),
     "f4j_zgemm_dev": ([_i, _i, _i64, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _i, _i, _i, _vp], _i),
+    "f4j_dzgemm": ([_i64, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _i, _i], _i),
+    "f4j_dzgemm_dev": ([_i64, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _i, _i, _vp], _i),
+    "f4j_zdgemm": ([_i64, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _i, _i], _i),
+    "f4j_zdgemm_dev": ([_i64, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _i, _i, _vp], _i),
     "f4j_dgemv": ([_i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _i], _i),
     "f4j_dgemv_dev": ([_i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _i, _vp], _i),
     "f4j_zgemv": ([_i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _i], _i),

@@ -143,6 +143,8 @@ class Matrix(_DenseMatrix):
         (ReDeMatMultDispatcher.java:163-164; ValidateParameters.java:62-67)."""
         if isinstance(b, Vector):
             return self._mult_vector(b, precision)
+        if isinstance(b, (CMatrix, CVector)):
+            return _mixed_mult(self, b, precision, slices)
         if self.numCols != b.numRows:
             raise LinearAlgebraException(_mat_mult_shape_err(self.shape, b.shape))
         M, K, N = self.numRows, self.numCols, b.numCols
@@ -257,6 +259,8 @@ class CMatrix(_DenseMatrix):
         (M:arrays/backend/semiring_arrays/AbstractDenseSemiringMatrix.java:295-298)."""
         if isinstance(b, CVector):
             return self._mult_vector(b, precision)
+        if isinstance(b, (Matrix, Vector)):
+            return _mixed_mult(self, b, precision, slices)
         if self.numCols != b.numRows:
             raise LinearAlgebraException(_mat_mult_shape_err(self.shape, b.shape))
         M, K, N = self.numRows, self.numCols, b.numCols
@@ -323,6 +327,23 @@ class CMatrix(_DenseMatrix):
 
     __eq__ = equals
     __hash__ = None
+
+
+def _mixed_mult(a, b, precision, slices):
+    """Real x complex products: Matrix.mult(CMatrix / CVector) (M:arrays/dense/Matrix.java:1793-1797,
+    :1851-1853) and CMatrix.mult(Matrix / Vector) (M:arrays/dense/CMatrix.java:878-881, :894-896)
+    -> MatrixMultiplyDispatcher (M:linalg/ops/MatrixMultiplyDispatcher.java:71-260), which raises
+    LinearAlgebraException on an inner-dimension mismatch (ValidateParameters.ensureMatMultShapes)."""
+    vec = isinstance(b, _DenseVector)
+    K2, N = (b.size, 1) if vec else (b.numRows, b.numCols)
+    if a.numCols != K2:
+        raise LinearAlgebraException(_mat_mult_shape_err(a.shape, b.shape))
+    M, K = a.numRows, a.numCols
+    out = np.empty(M * N, dtype=np.complex128)
+    f = _native.load().f4j_zdgemm if isinstance(a, CMatrix) else _native.load().f4j_dzgemm
+    _native.check(f(M, N, K, _ptr(a.data), max(K, 1), _ptr(b.data), max(N, 1), _ptr(out), max(N, 1),
+                    _native.PREC[precision], slices))
+    return CVector(out) if vec else CMatrix(Shape(M, N), out)
 
 
 class _DenseVector:

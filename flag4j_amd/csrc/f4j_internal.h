@@ -52,16 +52,32 @@ hipError_t launch_gemm_strict(bool cplx, int opA, int opB, int64_t M, int64_t N,
                               int64_t lda, const double *B, int64_t ldb, double *C, int64_t ldc,
                               hipStream_t stream);
 
+// Flag4j-order (bitwise) mixed GEMM, no transposition: mode 0 real A x complex B,
+// mode 1 complex A x real B; C complex.  Sizes and leading dimensions in elements.
+hipError_t launch_gemm_strict_mixed(int mode, int64_t M, int64_t N, int64_t K, const double *A, int64_t lda,
+                                    const double *B, int64_t ldb, double *C, int64_t ldc, hipStream_t stream);
+
 // Matrix-vector products in the reference's summation orders (gemv.hip):
 // GEMV_BLOCKED  real, 64-wide k blocks summed from +0.0 and added in block order;
 // GEMV_CHAIN    real, one k-ascending sum per row;
-// GEMV_CPLX     complex, one k-ascending sum of complex products per row.
-// y[i * incy] = sum_k A[i * lda + k] x[k * incx]; lda, incx, incy in element
-// units (complex elements for GEMV_CPLX), conjx conjugates x (complex only).
-enum GemvKind : int { GEMV_BLOCKED = 0, GEMV_CHAIN = 1, GEMV_CPLX = 2 };
+// GEMV_CPLX     complex, one k-ascending sum of complex products per row;
+// GEMV_RX       real A, complex x: per component one k-ascending sum of x_k a_ik;
+// GEMV_CX       complex A, real x: per component one k-ascending sum of a_ik x_k.
+// y[i * incy] = sum_k A[i * lda + k] x[k * incx]; lda, incx, incy in units of
+// each operand's elements (complex or real), conjx conjugates x (GEMV_CPLX only).
+// Returns hipErrorInvalidValue where the kernels do not apply (see gemv_supported;
+// complex A, and real A with a complex x, must also be 16-byte aligned).
+enum GemvKind : int { GEMV_BLOCKED = 0, GEMV_CHAIN = 1, GEMV_CPLX = 2, GEMV_RX = 3, GEMV_CX = 4 };
 bool gemv_supported(int kind, int64_t K, int64_t lda, int64_t incx);
+bool gemv_applicable(int kind, int64_t K, const double *A, int64_t lda, int64_t incx);   // supported and aligned
 hipError_t launch_gemv(int kind, int64_t M, int64_t K, const double *A, int64_t lda, const double *x, int64_t incx,
                        double *y, int64_t incy, bool conjx, hipStream_t stream);
+
+// Complex x real products as one real product (mixed.hip): split A (complex
+// M x K, lda in complex elements) into W = [Re A; Im A] (2 x M x K doubles), and
+// merge V (2 x M x N doubles) into the interleaved complex C (ldc in complex elements).
+hipError_t launch_split_planes(const double *A, int64_t lda, int64_t M, int64_t K, double *W, hipStream_t stream);
+hipError_t launch_merge_planes(const double *V, int64_t M, int64_t N, double *C, int64_t ldc, hipStream_t stream);
 
 // Fill rows x cols (leading dimension ld, in doubles) with zeros.
 hipError_t launch_zero_fill(double *C, int64_t rows, int64_t cols, int64_t ld, hipStream_t stream);

@@ -61,6 +61,8 @@ struct State {
     DevBuf bufA, bufB, bufC;
     std::mutex ws_mu;                 // guards ws and tev (split-precision workspace, product timing)
     DevBuf ws;
+    std::mutex mix_mu;                // guards mixws (planes of complex x real products)
+    DevBuf mixws;
     hipEvent_t tev[8] = {};                 // product timing: a begin / end pair per real product (up to 4)
 };
 
@@ -453,8 +455,7 @@ int gemv_kind(bool cplx, bool b_transposed, int64_t M, int64_t K) {
 // matrix-vector kernels' limits (the caller then takes another path).
 bool try_gemv(int kind, int64_t M, int64_t K, const double *A, int64_t lda, const double *x, int64_t incx, double *y,
               int64_t incy, bool conjx, hipStream_t stream, hipError_t *err) {
-    if (!gemv_supported(kind, K, lda, incx)) return false;
-    if (kind == GEMV_CPLX && ((uintptr_t)A & 15) != 0) return false;
+    if (!gemv_applicable(kind, K, A, lda, incx)) return false;
     *err = launch_gemv(kind, M, K, A, lda, x, incx, y, incy, conjx, stream);
     return true;
 }
@@ -600,6 +601,83 @@ int config_set_locked(Config &c, const std::string &key, const std::string &valu
 }  // namespace
 
 void set_error(const std::string &msg) { t_err = msg; }
+
+// Mixed product on the device: mode 0 C = A B with A real, B complex; mode 1
+// A complex, B real (RealFieldDenseMatMult, M:linalg/ops/dense/real_field_ops/
+// RealFieldDenseMatMult.java:58-1090).  Sizes and leading dimensions in elements.
+int mixed_dev(int mode, int64_t M, int64_t N, int64_t K, const double *A, int64_t lda, const double *B, int64_t ldb,
+              double *C, int64_t ldc, int precision, int slices, hipStream_t stream) {
+    if (M == 0 || N == 0) return F4J_OK;
+    if (K == 0) {
+        F4J_HIP(launch_zero_fill(C, M, 2 * N, 2 * ldc, stream), "zero fill");
+        return F4J_OK;
+    }
+    const int prec = resolve_precision(precision, M, N, K, F4J_OP_N, F4J_OP_N);
+    if (prec == F4J_PREC_STRICT) {
+        F4J_HIP(launch_gemm_strict_mixed(mode, M, N, K, A, lda, B, ldb, C, ldc, stream), "strict mixed launch");
+        return F4J_OK;
+    }
+    if (N == 1 && prec == F4J_PREC_FP64) {
+        // matrix-vector forms: one k-ascending sum per component, as in the reference
+        hipError_t e = hipSuccess;
+        if (!try_gemv(mode == 0 ? GEMV_RX : GEMV_CX, M, K, A, lda, B, ldb, C, ldc, false, stream, &e))
+            e = launch_gemm_strict_mixed(mode, M, N, K, A, lda, B, ldb, C, ldc, stream);
+        F4J_HIP(e, "mixed matrix-vector launch");
+        return F4J_OK;
+    }
+    if (mode == 0) {
+        // Each component of C takes the same products and sums as a real product with
+        // B's interleaved storage viewed as a K x 2N real matrix.
+        return gemm_dev(false, F4J_OP_N, F4J_OP_N, M, 2 * N, K, A, lda, B, 2 * ldb, C, 2 * ldc, prec, slices, 0,
+                        stream);
+    }
+    // mode 1: [Re A; Im A] (2M x K) B (K x N) = [Re C; Im C], then interleave.
+    State &s = S();
+    std::lock_guard<std::mutex> g(s.mix_mu);
+    const size_t wbytes = (size_t)(2 * M * K) * 8, vbytes = (size_t)(2 * M * N) * 8;
+    int rc = grow(s.mixws, wbytes + vbytes);
+    if (rc != F4J_OK) return rc;
+    double *W = (double *)s.mixws.p, *V = (double *)((char *)s.mixws.p + wbytes);
+    F4J_HIP(launch_split_planes(A, lda, M, K, W, stream), "split planes");
+    rc = gemm_dev(false, F4J_OP_N, F4J_OP_N, 2 * M, N, K, W, K, B, ldb, V, N, prec, slices, 0, stream);
+    if (rc == F4J_OK) {
+        F4J_HIP(launch_merge_planes(V, M, N, C, ldc, stream), "merge planes");
+    }
+    // the planes buffer is shared: it is free again only once the stream has used it
+    const hipError_t e = hipStreamSynchronize(stream);
+    if (rc != F4J_OK) return rc;
+    F4J_HIP(e, "stream synchronize");
+    return F4J_OK;
+}
+
+// Host-array mixed product: stage A, B and C through the pooled device buffers.
+int mixed_host(int mode, int64_t M, int64_t N, int64_t K, const double *A, int64_t lda, const double *B, int64_t ldb,
+               double *C, int64_t ldc, int precision, int slices) {
+    int rc = validate(false, F4J_OP_N, F4J_OP_N, M, N, K, A, lda, B, ldb, C, ldc, precision);
+    if (rc != F4J_OK || M == 0 || N == 0) return rc;
+    if ((rc = ensure_ready()) != F4J_OK) return rc;
+    State &s = S();
+    std::lock_guard<std::mutex> g(s.mu);
+    const size_t ea = mode == 1 ? 16 : 8, eb = mode == 0 ? 16 : 8;
+    if (K > 0) {
+        if ((rc = grow(s.bufA, (size_t)(M * K) * ea)) != F4J_OK) return rc;
+        if ((rc = grow(s.bufB, (size_t)(K * N) * eb)) != F4J_OK) return rc;
+    }
+    if ((rc = grow(s.bufC, (size_t)(M * N) * 16)) != F4J_OK) return rc;
+    if (K > 0) {
+        F4J_HIP(hipMemcpy2DAsync(s.bufA.p, K * ea, A, lda * ea, K * ea, M, hipMemcpyHostToDevice, s.stream), "copy A");
+        F4J_HIP(hipMemcpy2DAsync(s.bufB.p, N * eb, B, ldb * eb, N * eb, K, hipMemcpyHostToDevice, s.stream), "copy B");
+    }
+    rc = mixed_dev(mode, M, N, K, (const double *)s.bufA.p, K, (const double *)s.bufB.p, N, (double *)s.bufC.p, N,
+                   precision, slices, s.stream);
+    if (rc != F4J_OK) {
+        (void)hipStreamSynchronize(s.stream);
+        return rc;
+    }
+    F4J_HIP(hipMemcpy2DAsync(C, ldc * 16, s.bufC.p, N * 16, N * 16, M, hipMemcpyDeviceToHost, s.stream), "copy C");
+    F4J_HIP(hipStreamSynchronize(s.stream), "stream synchronize");
+    return F4J_OK;
+}
 
 // y = A x on the device (sizes and strides in elements).
 int gemv_dev(bool cplx, int64_t M, int64_t K, const double *A, int64_t lda, const double *x, int64_t incx, double *y,
@@ -807,6 +885,32 @@ int f4j_zgemm_dev(int opA, int opB, int64_t M, int64_t N, int64_t K, const doubl
     if (rc != F4J_OK) return rc;
     if ((rc = ensure_ready()) != F4J_OK) return rc;
     return gemm_dev(true, opA, opB, M, N, K, A, lda, B, ldb, C, ldc, precision, slices, use_3m, pick_stream(stream));
+}
+
+int f4j_dzgemm(int64_t M, int64_t N, int64_t K, const double *A, int64_t lda, const double *B, int64_t ldb, double *C,
+               int64_t ldc, int precision, int slices) {
+    return mixed_host(0, M, N, K, A, lda, B, ldb, C, ldc, precision, slices);
+}
+
+int f4j_dzgemm_dev(int64_t M, int64_t N, int64_t K, const double *A, int64_t lda, const double *B, int64_t ldb,
+                   double *C, int64_t ldc, int precision, int slices, void *stream) {
+    int rc = validate(false, F4J_OP_N, F4J_OP_N, M, N, K, A, lda, B, ldb, C, ldc, precision);
+    if (rc != F4J_OK) return rc;
+    if ((rc = ensure_ready()) != F4J_OK) return rc;
+    return mixed_dev(0, M, N, K, A, lda, B, ldb, C, ldc, precision, slices, pick_stream(stream));
+}
+
+int f4j_zdgemm(int64_t M, int64_t N, int64_t K, const double *A, int64_t lda, const double *B, int64_t ldb, double *C,
+               int64_t ldc, int precision, int slices) {
+    return mixed_host(1, M, N, K, A, lda, B, ldb, C, ldc, precision, slices);
+}
+
+int f4j_zdgemm_dev(int64_t M, int64_t N, int64_t K, const double *A, int64_t lda, const double *B, int64_t ldb,
+                   double *C, int64_t ldc, int precision, int slices, void *stream) {
+    int rc = validate(false, F4J_OP_N, F4J_OP_N, M, N, K, A, lda, B, ldb, C, ldc, precision);
+    if (rc != F4J_OK) return rc;
+    if ((rc = ensure_ready()) != F4J_OK) return rc;
+    return mixed_dev(1, M, N, K, A, lda, B, ldb, C, ldc, precision, slices, pick_stream(stream));
 }
 
 int f4j_dgemv(int64_t M, int64_t K, const double *A, int64_t lda, const double *x, int64_t incx, double *y,

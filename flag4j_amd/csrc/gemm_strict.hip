@@ -69,7 +69,45 @@ __global__ __launch_bounds__(256) void strict_cplx_kernel(StrictArgs a) {
     a.C[2 * (i * a.ldc + j) + 1] = si;
 }
 
+// Mixed products (RealFieldDenseMatMult, M:linalg/ops/dense/real_field_ops/
+// RealFieldDenseMatMult.java:58-1090): each component of C_ij is a k-ascending
+// sum from +0.0 of (component of the complex factor) * (real factor), with
+// Complex128.mult(double) = (re b, im b) (M:numbers/Complex128.java:402-404).
+// mode 0: A real (M x K), B complex (K x N); mode 1: A complex, B real.  Both
+// stored row-major, leading dimensions in elements, no transposition.
+__global__ __launch_bounds__(256) void strict_mixed_kernel(StrictArgs a, int mode) {
+    const int64_t j = (int64_t)blockIdx.x * 16 + (threadIdx.x & 15);
+    const int64_t i = (int64_t)blockIdx.y * 16 + (threadIdx.x >> 4);
+    if (i >= a.M || j >= a.N) return;
+    double sr = 0.0, si = 0.0;
+    for (int64_t k = 0; k < a.K; ++k) {
+        double re, im, r;
+        if (mode == 0) {
+            r = a.A[i * a.lda + k];
+            re = a.B[2 * (k * a.ldb + j)];
+            im = a.B[2 * (k * a.ldb + j) + 1];
+        } else {
+            re = a.A[2 * (i * a.lda + k)];
+            im = a.A[2 * (i * a.lda + k) + 1];
+            r = a.B[k * a.ldb + j];
+        }
+        sr = __dadd_rn(sr, __dmul_rn(re, r));
+        si = __dadd_rn(si, __dmul_rn(im, r));
+    }
+    a.C[2 * (i * a.ldc + j)] = sr;
+    a.C[2 * (i * a.ldc + j) + 1] = si;
+}
+
 }  // namespace
+
+hipError_t launch_gemm_strict_mixed(int mode, int64_t M, int64_t N, int64_t K, const double *A, int64_t lda,
+                                    const double *B, int64_t ldb, double *C, int64_t ldc, hipStream_t stream) {
+    if (M == 0 || N == 0) return hipSuccess;
+    StrictArgs a{M, N, K, A, lda, B, ldb, C, ldc, 0, 0};
+    dim3 grid((unsigned)((N + 15) / 16), (unsigned)((M + 15) / 16));
+    hipLaunchKernelGGL(strict_mixed_kernel, grid, dim3(256), 0, stream, a, mode);
+    return hipGetLastError();
+}
 
 hipError_t launch_gemm_strict(bool cplx, int opA, int opB, int64_t M, int64_t N, int64_t K, const double *A,
                               int64_t lda, const double *B, int64_t ldb, double *C, int64_t ldc,

@@ -59,9 +59,15 @@ __device__ inline __amdgpu_buffer_rsrc_t rsrc(const void *p, uint32_t bytes) {
     return __builtin_amdgcn_make_buffer_rsrc((void *)p, (short)0, (int)bytes, 0x00020000);
 }
 
+// Complex A (elements of two doubles) / complex x.
+template <int KIND>
+constexpr bool a_cplx() { return KIND == GEMV_CPLX || KIND == GEMV_CX; }
+template <int KIND>
+constexpr bool x_cplx() { return KIND == GEMV_CPLX || KIND == GEMV_RX; }
+
 // Rows of A per wave, A loads (VEC doubles each) and x loads (one double each) per lane per stage.
 template <int KIND>
-constexpr int rows_per_wave() { return KIND == GEMV_BLOCKED ? 8 : (KIND == GEMV_CHAIN ? 64 : 32); }
+constexpr int rows_per_wave() { return KIND == GEMV_BLOCKED ? 8 : (a_cplx<KIND>() ? 32 : 64); }
 template <int KIND>
 constexpr int x_loads() { return KIND == GEMV_BLOCKED ? 2 : 1; }
 
@@ -80,7 +86,7 @@ __device__ inline int kstage(int64_t s) {
 // each lane fetches is rotated so that the reading lanes spread over the banks.
 template <int KIND, int VEC>
 __device__ inline void a_slot(int i, int l, int &r, int &kl) {
-    if constexpr (KIND == GEMV_CPLX) {
+    if constexpr (a_cplx<KIND>()) {
         r = 4 * i + (l >> 4);                          // rows 4i..4i+3, one complex element per lane
         kl = ((l & 15) - r) & 15;                      // reader of row r finds element e at (e + r) & 15
     } else {
@@ -288,7 +294,7 @@ constexpr int x_dma_loads() { return KIND == GEMV_CPLX ? 2 : (KIND == GEMV_BLOCK
 template <int KIND, int XV, int D>
 __global__ __launch_bounds__(64) void gemv_dma_kernel(GemvArgs g) {
     constexpr int R = rows_per_wave<KIND>(), XL = x_dma_loads<KIND, XV>(), L = 8 + XL;
-    constexpr int ew = KIND == GEMV_CPLX ? 2 : 1;
+    constexpr int ew = a_cplx<KIND>() ? 2 : 1, xw = x_cplx<KIND>() ? 2 : 1;
     static_assert((D - 1) * L <= 63, "loads in flight must fit the vmcnt counter");
     __shared__ __attribute__((aligned(1024))) char lds[D * SLOT];
     const int l = threadIdx.x;
@@ -296,9 +302,9 @@ __global__ __launch_bounds__(64) void gemv_dma_kernel(GemvArgs g) {
     const int rows = (int)((g.M - row0) < R ? (g.M - row0) : R);
     const int K = (int)g.K, incx = (int)g.incx;
     const __amdgpu_buffer_rsrc_t ra = rsrc(g.A + row0 * g.lda, (uint32_t)((int64_t)rows * g.lda * 8));
-    const __amdgpu_buffer_rsrc_t rx = rsrc(g.x, (uint32_t)(((g.K - 1) * g.incx + ew) * 8));
+    const __amdgpu_buffer_rsrc_t rx = rsrc(g.x, (uint32_t)(((g.K - 1) * g.incx + xw) * 8));
     // last valid load start: a pair (real) or an element (complex) / x element
-    const int ka_max = KIND == GEMV_CPLX ? K - 1 : K - 2;
+    const int ka_max = a_cplx<KIND>() ? K - 1 : K - 2;
     const int kx_max = XV == 2 ? K - 2 : K - 1;
 
     // per-lane load geometry
@@ -318,6 +324,12 @@ __global__ __launch_bounds__(64) void gemv_dma_kernel(GemvArgs g) {
             const int m = l >> 1, part = m & 1;          // copy h, double m = (element m / 2, part)
             xkl[h] = m >> 1;
             xadd[h] = 8 * (h == 0 ? part : 1 - part) + 4 * (l & 1);
+        } else if constexpr (KIND == GEMV_RX) {
+            xkl[h] = l >> 2;                             // complex element l / 4, dword l % 4
+            xadd[h] = 4 * (l & 3);
+        } else if constexpr (KIND == GEMV_CX) {
+            xkl[h] = (l >> 1) & 15;                      // double l / 2 (lanes 32-63 repeat 0-31)
+            xadd[h] = 4 * (l & 1);
         } else if constexpr (XV == 2) {
             if constexpr (KIND == GEMV_BLOCKED) {
                 const int j = l >> 3;                    // block j: pair p at lane position (p + j) & 7
@@ -361,7 +373,11 @@ __global__ __launch_bounds__(64) void gemv_dma_kernel(GemvArgs g) {
     // reader geometry (bytes within a slot)
     int a_base, a_rot, x_base = 8192, x_rot = 0, kb = 0;
     uint32_t sg1 = 0, sg2 = 0;                        // complex: sign masks of the two products
-    if constexpr (KIND == GEMV_CPLX) {
+    if constexpr (KIND == GEMV_CX) {
+        const int r = l & 31;
+        a_base = 1024 * (r >> 2) + 256 * (r & 3);
+        a_rot = r;
+    } else if constexpr (KIND == GEMV_CPLX) {
         const int r = l & 31;
         a_base = 1024 * (r >> 2) + 256 * (r & 3);
         a_rot = r;
@@ -381,11 +397,34 @@ __global__ __launch_bounds__(64) void gemv_dma_kernel(GemvArgs g) {
         }
     }
 
-    double s_acc = 0.0, dest = 0.0;
+    double s_acc = 0.0, dest = 0.0, s_im = 0.0;   // s_im: second chain of the real-A, complex-x form
     auto compute = [&](int64_t s, bool masked) {
         const char *slot = lds + (int)(s % D) * SLOT;
         const int ks = kstage<KIND>(s);
-        if constexpr (KIND == GEMV_CPLX) {
+        if constexpr (KIND == GEMV_CX) {
+            // y_c = sum_k a_c x_k: Complex128.mult(double) (M:numbers/Complex128.java:402-404)
+            const bool im = l >= 32;
+#pragma unroll
+            for (int e = 0; e < 16; ++e) {
+                const f64x2 a = *(const f64x2 *)(slot + a_base + 16 * ((e + a_rot) & 15));
+                const double w = *(const double *)(slot + 8192 + 8 * e);
+                const double v = __dadd_rn(s_acc, __dmul_rn(im ? a.y : a.x, w));
+                s_acc = (!masked || ks + e < K) ? v : s_acc;
+            }
+        } else if constexpr (KIND == GEMV_RX) {
+            // y = sum_k x_k a_k, both components: x.mult(a) = (x.re a, x.im a)
+#pragma unroll
+            for (int p = 0; p < 8; ++p) {
+                const f64x2 a = *(const f64x2 *)(slot + a_base + 16 * ((p + a_rot) & 7));
+                const f64x2 w0 = *(const f64x2 *)(slot + 8192 + 32 * p);
+                const f64x2 w1 = *(const f64x2 *)(slot + 8192 + 32 * p + 16);
+                const double vr = __dadd_rn(__dadd_rn(s_acc, __dmul_rn(w0.x, a.x)), __dmul_rn(w1.x, a.y));
+                const double vi = __dadd_rn(__dadd_rn(s_im, __dmul_rn(w0.y, a.x)), __dmul_rn(w1.y, a.y));
+                const bool ok = !masked || ks + 2 * p < K;
+                s_acc = ok ? vr : s_acc;
+                s_im = ok ? vi : s_im;
+            }
+        } else if constexpr (KIND == GEMV_CPLX) {
 #pragma unroll
             for (int e = 0; e < 16; ++e) {
                 const f64x2 a = *(const f64x2 *)(slot + a_base + 16 * ((e + a_rot) & 15));
@@ -448,6 +487,11 @@ __global__ __launch_bounds__(64) void gemv_dma_kernel(GemvArgs g) {
         if ((l & 7) == 0 && r < rows) g.y[(row0 + r) * g.incy] = dest;
     } else if constexpr (KIND == GEMV_CHAIN) {
         if (l < rows) g.y[(row0 + l) * g.incy] = s_acc;
+    } else if constexpr (KIND == GEMV_RX) {
+        if (l < rows) {
+            g.y[(row0 + l) * g.incy] = s_acc;
+            g.y[(row0 + l) * g.incy + 1] = s_im;
+        }
     } else {
         const int r = l & 31;
         if (r < rows) g.y[(row0 + r) * g.incy + (l >> 5)] = s_acc;
@@ -470,27 +514,37 @@ void launch(GemvArgs g, int64_t stages, hipStream_t stream) {
 
 }  // namespace
 
+namespace {
+int a_width(int kind) { return kind == GEMV_CPLX || kind == GEMV_CX ? 2 : 1; }
+int x_width(int kind) { return kind == GEMV_CPLX || kind == GEMV_RX ? 2 : 1; }
+int y_width(int kind) { return kind == GEMV_BLOCKED || kind == GEMV_CHAIN ? 1 : 2; }
+int rows_of(int kind) { return kind == GEMV_BLOCKED ? 8 : (a_width(kind) == 2 ? 32 : 64); }
+}  // namespace
+
 bool gemv_supported(int kind, int64_t K, int64_t lda, int64_t incx) {
     // 32-bit offsets: the wave's slab of A and all of x stay below 2^31 bytes,
-    // also for the stage offsets computed past the end (K + 1024 elements).
-    const int64_t ew = kind == GEMV_CPLX ? 2 : 1;
-    const int64_t R = kind == GEMV_BLOCKED ? 8 : (kind == GEMV_CHAIN ? 64 : 32);
+    // also for the stage offsets computed past the end (K + 4096 elements).
     const int64_t lim = 0x7FFFFFF0;
-    return K >= 1 && lda >= K && incx >= 1 && R * lda * ew * 8 <= lim &&
-           (K + 4096) * incx * ew * 8 <= lim;
+    return K >= 1 && lda >= K && incx >= 1 && rows_of(kind) * lda * a_width(kind) * 8 <= lim &&
+           (K + 4096) * incx * x_width(kind) * 8 <= lim;
+}
+
+bool gemv_applicable(int kind, int64_t K, const double *A, int64_t lda, int64_t incx) {
+    if (!gemv_supported(kind, K, lda, incx)) return false;
+    // complex A, and real A with a complex x, stream 16-byte pairs / elements only
+    const bool a16 = ((uintptr_t)A & 15) == 0 && (a_width(kind) == 2 || (lda % 2 == 0 && K % 2 == 0));
+    return a16 || kind == GEMV_BLOCKED || kind == GEMV_CHAIN;
 }
 
 hipError_t launch_gemv(int kind, int64_t M, int64_t K, const double *A, int64_t lda, const double *x, int64_t incx,
                        double *y, int64_t incy, bool conjx, hipStream_t stream) {
     if (M == 0) return hipSuccess;
-    if (!gemv_supported(kind, K, lda, incx)) return hipErrorInvalidValue;
-    const int64_t ew = kind == GEMV_CPLX ? 2 : 1;
-    GemvArgs g{M, K, A, lda * ew, x, incx * ew, y, incy * ew, 0, conjx ? 1 : 0};
+    if (!gemv_applicable(kind, K, A, lda, incx)) return hipErrorInvalidValue;
+    GemvArgs g{M, K, A, lda * a_width(kind), x, incx * x_width(kind), y, incy * y_width(kind), 0, conjx ? 1 : 0};
     // Direct-to-LDS loads move 16 bytes: 16-byte aligned pairs (complex: elements)
     // that never straddle a row end.  Otherwise the register-staged form.
-    const bool a16 = ((uintptr_t)A & 15) == 0 && (kind == GEMV_CPLX || (lda % 2 == 0 && K % 2 == 0));
+    const bool a16 = ((uintptr_t)A & 15) == 0 && (a_width(kind) == 2 || (lda % 2 == 0 && K % 2 == 0));
     const bool x16 = ((uintptr_t)x & 15) == 0 && incx == 1 && K % 2 == 0;
-    if (kind == GEMV_CPLX && !a16) return hipErrorInvalidValue;
     if (kind == GEMV_BLOCKED) {
         const int64_t stages = 4 * ((K + 511) / 512);
         if (a16 && x16) launch_dma<GEMV_BLOCKED, 2, 4>(g, stages, stream);
@@ -501,6 +555,10 @@ hipError_t launch_gemv(int kind, int64_t M, int64_t K, const double *A, int64_t 
         if (a16 && x16) launch_dma<GEMV_CHAIN, 2, 7>(g, stages, stream);
         else if (a16) launch_dma<GEMV_CHAIN, 1, 7>(g, stages, stream);
         else launch<GEMV_CHAIN, 1, 3>(g, stages, stream);
+    } else if (kind == GEMV_RX) {
+        launch_dma<GEMV_RX, 1, 7>(g, (K + 15) / 16, stream);
+    } else if (kind == GEMV_CX) {
+        launch_dma<GEMV_CX, 1, 7>(g, (K + 15) / 16, stream);
     } else {
         launch_dma<GEMV_CPLX, 1, 7>(g, (K + 15) / 16, stream);
     }

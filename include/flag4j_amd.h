@@ -123,6 +123,30 @@ int f4j_zgemm_dev(int opA, int opB, int64_t M, int64_t N, int64_t K,
                   const double *A, int64_t lda, const double *B, int64_t ldb,
                   double *C, int64_t ldc, int precision, int slices, int use_3m, void *stream);
 
+/* ---- mixed real / complex products ----
+ * Replace:
+ *   Matrix.mult(CMatrix)  M:arrays/dense/Matrix.java:1793-1797, Matrix.mult(CVector) Matrix.java:1851-1853,
+ *   CMatrix.mult(Matrix)  M:arrays/dense/CMatrix.java:878-881,  CMatrix.mult(Vector) CMatrix.java:894-896
+ *   -> MatrixMultiplyDispatcher M:linalg/ops/MatrixMultiplyDispatcher.java:71-260
+ *   -> RealFieldDenseMatMult.*  M:linalg/ops/dense/real_field_ops/RealFieldDenseMatMult.java:58-1090
+ * f4j_dzgemm: C = A B with A real (M x K) and B complex (K x N); f4j_zdgemm: A complex (M x K) and
+ * B real (K x N).  C is complex (M x N); no transposition; leading dimensions in each operand's
+ * elements.  In the reference each component of C_ij is a k-ascending sum of
+ * (component) x (real entry) products.  STRICT reproduces that bit for bit, as does AUTO for small
+ * products (M*K + K*N < smallThreshold) and FP64 / AUTO for N = 1 (the matrix-vector forms).  FP64
+ * otherwise runs the MFMA GEMM: f4j_dzgemm on B's interleaved storage viewed as a K x 2N real
+ * matrix, f4j_zdgemm on [Re A; Im A] stacked into one 2M x K real matrix (the planes go through a
+ * library-owned buffer, so f4j_zdgemm_dev synchronises `stream` before it returns).  The SPLIT
+ * precisions run the split forms on the same real views. */
+int f4j_dzgemm(int64_t M, int64_t N, int64_t K, const double *A, int64_t lda, const double *B, int64_t ldb,
+               double *C, int64_t ldc, int precision, int slices);
+int f4j_dzgemm_dev(int64_t M, int64_t N, int64_t K, const double *A, int64_t lda, const double *B, int64_t ldb,
+                   double *C, int64_t ldc, int precision, int slices, void *stream);
+int f4j_zdgemm(int64_t M, int64_t N, int64_t K, const double *A, int64_t lda, const double *B, int64_t ldb,
+               double *C, int64_t ldc, int precision, int slices);
+int f4j_zdgemm_dev(int64_t M, int64_t N, int64_t K, const double *A, int64_t lda, const double *B, int64_t ldb,
+                   double *C, int64_t ldc, int precision, int slices, void *stream);
+
 /* ---- matrix-vector products ----
  * Replace:
  *   real:    Matrix.mult(Vector)      M:arrays/dense/Matrix.java:1406-1408 ->

@@ -270,6 +270,36 @@ void f4o_cplx_gemm(int opA, int opB, int M, int N, int K, const double *A, const
         }
 }
 
+/* Mixed real / complex products: RealFieldDenseMatMult
+ * (M:linalg/ops/dense/real_field_ops/RealFieldDenseMatMult.java:58-1090).
+ * Every variant (standard, reordered, blocked, concurrent, vector) sums each
+ * C_ij from Complex128.ZERO in k order, sum = sum.add(z.mult(r)) with
+ * Complex128.mult(double) = (re r, im r) (M:numbers/Complex128.java:402-404):
+ * each component is a k-ascending sum of (component) * (real entry).
+ * mode 0: A real M x K, B complex K x N; mode 1: A complex, B real.  C complex. */
+void f4o_mixed_gemm(int mode, int M, int N, int K, const double *A, const double *B, double *C) {
+    for (int i = 0; i < M; i++)
+        for (int j = 0; j < N; j++) {
+            double sr = 0.0, si = 0.0;
+            for (int k = 0; k < K; k++) {
+                double zr, zi, r;
+                if (mode == 0) {
+                    r = A[(int64_t)i * K + k];
+                    zr = B[2 * ((int64_t)k * N + j)];
+                    zi = B[2 * ((int64_t)k * N + j) + 1];
+                } else {
+                    zr = A[2 * ((int64_t)i * K + k)];
+                    zi = A[2 * ((int64_t)i * K + k) + 1];
+                    r = B[(int64_t)k * N + j];
+                }
+                sr = sr + zr * r;
+                si = si + zi * r;
+            }
+            C[2 * ((int64_t)i * N + j)] = sr;
+            C[2 * ((int64_t)i * N + j) + 1] = si;
+        }
+}
+
 /* ------------------------------------------------------------------------ */
 /* Dispatcher decision trees                                                */
 /* ------------------------------------------------------------------------ */

@@ -13,6 +13,11 @@
  *                         invariants (RealDenseMatMultTransposeTests).
  *   - f4o_cplx_*        : bitwise against CMatrixMultiplyTests, CMatrixMultTests
  *                         and the ComplexDenseMatMultTransposeTests invariants.
+ *   - f4o_mixed_gemm    : bitwise against MatrixMultTests.matMultComplexTestCase,
+ *                         CMatrixMultTests.matMultTestCase and the mixed
+ *                         matrix-vector tests (MatrixVectorTests,
+ *                         CMatrixMatVecMultTests); agrees with f4o_cplx_gemm on a
+ *                         real factor promoted to complex (inputs without zeros).
  *   - f4o_real_blocked_vector : weakly pinned (K = 3) by the column-vector
  *                         golden vectors; its 64-k partial-sum rule is checked
  *                         against a brute-force restatement at K > 64.
@@ -60,6 +65,8 @@ void f4o_real_mt_blocked_reordered(int M, int N, int K, const double *A, const d
 /* op codes: 0 = N, 1 = T, 2 = C (conjugate transpose).  op(A) is MxK, op(B) is KxN.
  * Storage: opA==N -> A is MxK; opA in {T,C} -> A is KxM.  Same for B (KxN / NxK). */
 void f4o_cplx_gemm(int opA, int opB, int M, int N, int K, const double *A, const double *B, double *C);
+/* Mixed products (RealFieldDenseMatMult.java:58-1090): mode 0 real A x complex B, mode 1 complex A x real B. */
+void f4o_mixed_gemm(int mode, int M, int N, int K, const double *A, const double *B, double *C);
 void f4o_cplx_mt_reordered(int M, int N, int K, const double *A, const double *B, double *C,
                            int row0, int row1, int threads);
 

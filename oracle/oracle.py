@@ -57,6 +57,7 @@ def lib():
         L.f4o_real_mt_reordered.argtypes = [_i, _i, _i, _dp, _dp, _dp, _i, _i, _i]
         L.f4o_real_mt_blocked_reordered.argtypes = [_i, _i, _i, _dp, _dp, _dp, _i, _i, _i, _i]
         L.f4o_cplx_gemm.argtypes = [_i, _i, _i, _i, _i, _dp, _dp, _dp]
+        L.f4o_mixed_gemm.argtypes = [_i, _i, _i, _i, _dp, _dp, _dp]
         L.f4o_cplx_mt_reordered.argtypes = [_i, _i, _i, _dp, _dp, _dp, _i, _i, _i]
         L.f4o_select_real.argtypes = [_i, _i, _i, ctypes.POINTER(Thresholds)]
         L.f4o_select_cplx.argtypes = [_i, _i, _i, ctypes.POINTER(Thresholds)]
@@ -173,6 +174,18 @@ def real_mt_blocked_reordered(A, B, C=None, rows=None, threads=1, bs=64):
 def _op_shape(X, op):
     r, c = X.shape
     return (r, c) if op == "N" else (c, r)
+
+
+def mixed_gemm(A, B):
+    """C = A @ B with one real and one complex factor, in RealFieldDenseMatMult's order."""
+    mode = 1 if np.iscomplexobj(A) else 0
+    A = _c128(A) if mode == 1 else _f64(A)
+    B = _f64(B) if mode == 1 else _c128(B)
+    M, K = A.shape
+    N = B.shape[1]
+    C = np.empty((M, N), dtype=np.complex128)
+    lib().f4o_mixed_gemm(mode, M, N, K, _p(A.view(np.float64)), _p(B.view(np.float64)), _p(C.view(np.float64)))
+    return C
 
 
 def cplx_gemm(A, B, opA="N", opB="N"):

@@ -18,6 +18,8 @@ Test files read (T: = src/test/java/org/flag4j/):
   T:linalg/ops/dense/complex/CMatrixMultiplyTests.java:17-169
   T:arrays/dense/complex_matrix/CMatrixMultTests.java:64-102, :198-256
   T:linalg/ops/dense/complex/ComplexDenseMatMultTransposeTests.java:40-107
+  T:arrays/dense/matrix/MatrixMultTests.java:66-102 (Matrix.mult(CMatrix))
+  T:arrays/dense/complex_matrix/CMatrixMultTests.java:20-62 (CMatrix.mult(Matrix))
   T:arrays/dense/matrix/MatrixVectorTests.java:27-98 (Matrix.mult(Vector), Matrix.mult(CVector))
   T:arrays/dense/complex_matrix/CMatrixMatVecMultTests.java:22-100 (CMatrix.mult(Vector / CVector))
 """
@@ -64,6 +66,10 @@ SPECS = [
      {"a": ("entriesA", 0), "b": ("entriesB", 0)}),
     ("cplx_T_rect", "linalg/ops/dense/complex/ComplexDenseMatMultTransposeTests.java", "rectangleTestCase",
      {"a": ("entriesA", 0), "b": ("entriesB", 0)}),
+    ("real_cmat_api_mult", "arrays/dense/matrix/MatrixMultTests.java", "matMultComplexTestCase",
+     {"a": ("aEntries", 0), "b": ("bEntries", 0), "c": ("expCEntries", 0), "b_bad": ("bEntries", 1)}),
+    ("cplx_rmat_api_mult", "arrays/dense/complex_matrix/CMatrixMultTests.java", "matMultTestCase",
+     {"a": ("aEntries", 0), "b": ("bEntries", 0), "c": ("expEntries", 0), "b_bad": ("bEntries", 1)}),
     ("real_api_matvec", "arrays/dense/matrix/MatrixVectorTests.java", "matVecMultTestCase",
      {"a": ("aEntries", 0), "b": ("bEntries", 0), "c": ("expEntries", 0), "b_bad": ("bEntries", 1)}),
     ("real_cvec_api_matvec", "arrays/dense/matrix/MatrixVectorTests.java", "matVecMultComplexTestCase",

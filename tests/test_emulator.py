@@ -12,14 +12,23 @@ EMU_SRC = os.path.join(ROOT, "tests", "emu", "gemm_emu.cpp")
 EMU_BIN = os.path.join(ROOT, "_build", "gemm_emu")
 
 
+def build_once(src, binary, deps):
+    """Compile when a dependency is newer; concurrent test processes each compile
+    into their own file and rename it into place, so none runs a half-written binary."""
+    os.makedirs(os.path.dirname(binary), exist_ok=True)
+    if os.path.exists(binary) and all(os.path.getmtime(p) <= os.path.getmtime(binary) for p in deps + [src]):
+        return binary
+    tmp = f"{binary}.{os.getpid()}"
+    subprocess.run(["g++", "-O2", "-std=c++17", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include", "-o", tmp, src],
+                   check=True)
+    os.replace(tmp, binary)
+    return binary
+
+
 @pytest.fixture(scope="module")
 def emu():
-    os.makedirs(os.path.dirname(EMU_BIN), exist_ok=True)
     hdrs = [os.path.join(ROOT, "flag4j_amd", "csrc", h) for h in ("gemm_tile.h", "f4j_internal.h")]
-    if not os.path.exists(EMU_BIN) or any(os.path.getmtime(p) > os.path.getmtime(EMU_BIN) for p in hdrs + [EMU_SRC]):
-        subprocess.run(["g++", "-O2", "-std=c++17", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include", "-o", EMU_BIN,
-                        EMU_SRC], check=True)
-    return EMU_BIN
+    return build_once(EMU_SRC, EMU_BIN, hdrs)
 
 
 KINDS = {"R_N": 0, "R_T": 1, "Z_N": 2, "Z_T": 3, "Z_C": 4}
@@ -48,12 +57,8 @@ CRT_BIN = os.path.join(ROOT, "_build", "crt_emu")
 def crt_emu():
     """Host emulation of the modular GEMM (fragment-major residue layout, LDS
     stages, epilogue) with the kernels' own offset functions."""
-    os.makedirs(os.path.dirname(CRT_BIN), exist_ok=True)
-    deps = [os.path.join(ROOT, "flag4j_amd", "csrc", h) for h in ("gemm_tile.h", "crt_layout.h")] + [CRT_SRC]
-    if not os.path.exists(CRT_BIN) or any(os.path.getmtime(p) > os.path.getmtime(CRT_BIN) for p in deps):
-        subprocess.run(["g++", "-O2", "-std=c++17", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include", "-o", CRT_BIN,
-                        CRT_SRC], check=True)
-    return CRT_BIN
+    deps = [os.path.join(ROOT, "flag4j_amd", "csrc", h) for h in ("gemm_tile.h", "crt_layout.h")]
+    return build_once(CRT_SRC, CRT_BIN, deps)
 
 
 @pytest.mark.parametrize("shape", [(1, 1, 1), (7, 300, 13), (130, 70, 257), (300, 200, 500), (257, 129, 384),

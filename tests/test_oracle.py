@@ -141,6 +141,43 @@ def test_complex_matrix_vector_golden(golden):
     assert not np.array_equal(wrong, exp)
 
 
+MIXED = ["real_cmat_api_mult", "cplx_rmat_api_mult", "real_cvec_api_matvec", "cplx_vec_api_matvec"]
+
+
+def _mixed_operands(g, case):
+    a = cplx_mat(g["a"]) if case.startswith("cplx") else real_mat(g["a"])
+    b = real_mat(g["b"]) if case.startswith("cplx") else cplx_mat(g["b"])
+    return a, (b.reshape(-1, 1) if b.ndim == 1 else b), cplx_mat(g["c"])
+
+
+@pytest.mark.parametrize("case", MIXED)
+def test_mixed_golden(golden, case):
+    """T:arrays/dense/matrix/MatrixMultTests.java:66-87, complex_matrix/CMatrixMultTests.java:20-41,
+    matrix/MatrixVectorTests.java:62-83, complex_matrix/CMatrixMatVecMultTests.java:22-45
+    (CVector / CMatrix equality: == per component, M:numbers/Complex128.java:211-219)."""
+    A, B, exp = _mixed_operands(golden[case], case)
+    act = o.mixed_gemm(A, B).reshape(exp.shape)
+    assert np.all((act.real == exp.real) & (act.imag == exp.imag))
+    # plausible mistakes: a conjugated complex factor, a transposed real factor
+    if np.iscomplexobj(B):
+        bad = o.mixed_gemm(A, B.conj()).reshape(exp.shape)
+    else:
+        bad = o.mixed_gemm(A.conj(), B).reshape(exp.shape)
+    assert not np.all((bad.real == exp.real) & (bad.imag == exp.imag))
+
+
+def test_mixed_equals_complex_on_promoted_factor():
+    """With no zero entries (so no signed-zero differences) the mixed order equals the
+    complex kernel's on the real factor promoted to complex: ar*br - 0*bi = ar*br exactly."""
+    rng = np.random.default_rng(4)
+    A = rng.standard_normal((9, 70)) * 10.0 ** rng.integers(-5, 6, (9, 70))
+    B = (rng.standard_normal((70, 5)) + 1j * rng.standard_normal((70, 5))) * 10.0 ** rng.integers(-5, 6, (70, 5))
+    assert np.array_equal(o.mixed_gemm(A, B).view(np.float64), o.cplx_gemm(A.astype(complex), B).view(np.float64))
+    Bt = B.T.copy()
+    At = A.T.copy()
+    assert np.array_equal(o.mixed_gemm(Bt, At).view(np.float64), o.cplx_gemm(Bt, At.astype(complex)).view(np.float64))
+
+
 def test_matrix_vector_shape_errors(golden):
     """Size mismatches raise LinearAlgebraException before any device work
     (MatrixVectorTests.java:47-58, CMatrixMatVecMultTests.java:86-97)."""
@@ -151,6 +188,18 @@ def test_matrix_vector_shape_errors(golden):
     g = golden["cplx_api_matvec"]
     with pytest.raises(f4j.LinearAlgebraException):
         f4j.CMatrix(cplx_mat(g["a"])).mult(f4j.CVector(cplx_mat(g["b_bad"]).reshape(-1)))
+    g = golden["real_cmat_api_mult"]                # MatrixMultTests.java:89-102
+    with pytest.raises(f4j.LinearAlgebraException):
+        f4j.Matrix(real_mat(g["a"])).mult(f4j.CMatrix(cplx_mat(g["b_bad"])))
+    g = golden["cplx_rmat_api_mult"]                # CMatrixMultTests.java:43-62
+    with pytest.raises(f4j.LinearAlgebraException):
+        f4j.CMatrix(cplx_mat(g["a"])).mult(f4j.Matrix(real_mat(g["b_bad"])))
+    g = golden["real_cvec_api_matvec"]              # MatrixVectorTests.java:86-98
+    with pytest.raises(f4j.LinearAlgebraException):
+        f4j.Matrix(real_mat(g["a"])).mult(f4j.CVector(cplx_mat(g["b_bad"])))
+    g = golden["cplx_vec_api_matvec"]               # CMatrixMatVecMultTests.java:47-59
+    with pytest.raises(f4j.LinearAlgebraException):
+        f4j.CMatrix(cplx_mat(g["a"])).mult(f4j.Vector(real_mat(g["b_bad"]).reshape(-1)))
 
 
 def test_complex_pow_golden(golden):
