@@ -13,7 +13,9 @@ slices; FP16 digits, 5 slices); and the modular form (INT8 residues modulo
 covers exponents, splitting, slice products, recombination and the FP64
 residual check.  Each form's error is measured against exact sums on sampled
 entries.  `value` is the fastest form whose sampled error is <= 1e-15 of
-(|A||B|)_ij.  Reported beside it: the batched workload (10,000 products with
+(|A||B|)_ij.  Reported beside it: the matrix-vector product on the same
+operand (Matrix.mult(Vector), and CMatrix.mult(CVector) at the same size, both
+in the reference's summation order, against the HBM roofline); the batched workload (10,000 products with
 M, N, K log-uniform on [16, 512], N(0,1) entries; FP64 and split-BF16 grouped
 plans) and the complex batched workload (2,000 A^H B with M, N, K log-uniform
 on [32, 1024], parts spread over 10^+-4; 3M on split-FP16 and the FP64 grouped
@@ -54,6 +56,7 @@ def parse():
     ap.add_argument("--no-split", action="store_true", help="plain FP64 only")
     ap.add_argument("--forms", default=",".join(FORMS), help="comma-separated subset of " + ",".join(FORMS))
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-gemv", action="store_true")
     ap.add_argument("--profile-copy", action="store_true",
                     help="after the timed region, copy 512 MiB on the device once (calibrates HBM byte counters)")
     return ap.parse_args()
@@ -221,6 +224,45 @@ def bench_complex_batched(np, torch, f4j, rng, dev, count=2000, reps=3, slices=8
             "fp64_grouped_tflops": round(flops / ms_d / 1e9, 2)}
 
 
+def bench_gemv(np, torch, f4j, dA, x, stream, reps=50):
+    """Matrix.mult(Vector) on the square operand (Flag4j's blockedVector order, bit for bit) and
+    CMatrix.mult(CVector) on an n x n complex matrix: R launches back to back between two events
+    on the launch stream.  Algorithmic bytes: the matrix once, plus x and y."""
+    n = dA.shape[0]
+    L = f4j._native.load()
+    sh = stream.cuda_stream
+
+    def timed(fn):
+        for _ in range(3):
+            fn()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) * 1e3 / reps
+
+    dy = torch.empty(n, dtype=torch.float64, device=dA.device)
+    us = timed(lambda: f4j._native.check(L.f4j_dgemv_dev(n, n, dA.data_ptr(), n, x.data_ptr(), 1, dy.data_ptr(), 1,
+                                                          0, sh)))
+    gbs = (8.0 * n * n + 16.0 * n) / (us * 1e-6) / 1e9
+    Ac = torch.empty((n, 2 * n), dtype=torch.float64, device=dA.device).uniform_(-1, 1)
+    xc = torch.empty(2 * n, dtype=torch.float64, device=dA.device).uniform_(-1, 1)
+    yc = torch.empty(2 * n, dtype=torch.float64, device=dA.device)
+    usc = timed(lambda: f4j._native.check(L.f4j_zgemv_dev(n, n, Ac.data_ptr(), n, xc.data_ptr(), 1, yc.data_ptr(), 1,
+                                                           0, sh)))
+    gbc = (16.0 * n * n + 32.0 * n) / (usc * 1e-6) / 1e9
+    del Ac, xc, yc
+    return {"workload": f"Matrix.mult(Vector) {n}x{n} FP64, blockedVector order (bit for bit), device-resident",
+            "us": round(us, 2), "gb_s": round(gbs, 1),
+            "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                         "frac": round(gbs / PEAK_HBM_GBS, 4), "traffic": None},
+            "complex": {"workload": f"CMatrix.mult(CVector) {n}x{n} complex128, k-ascending order (bit for bit)",
+                        "us": round(usc, 2), "gb_s": round(gbc, 1), "frac": round(gbc / PEAK_HBM_GBS, 4)}}
+
+
 def load_traffic(name):
     """HBM bytes per launch from the committed rocprofv3 --pmc summary, if present."""
     path = os.path.join(ROOT, "profiles", name)
@@ -349,6 +391,11 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
+    gemv = None
+    if not args.no_gemv:
+        gemv = bench_gemv(np, torch, f4j, dA, dB[0].contiguous(), stream)
+        log(f"[bench] gemv: {gemv}")
+
     batched = complex_batched = None
     if not args.no_batched:
         batched = bench_batched(np, torch, f4j, rng, dev)
@@ -395,6 +442,7 @@ def main():
                      "reference": "exact sums (TwoProduct + fsum) on sampled entries"},
         "batched": batched,
         "complex_batched": complex_batched,
+        "gemv": gemv,
         "end_to_end_gflops_host_arrays": round(flops / e2e_s / 1e9, 2),
     }
     print(json.dumps(out), flush=True)
