@@ -19,6 +19,7 @@ struct CrtTables {
     int P;                               // 40-bit chunks
     int m[CRT_MAX_MODULI];
     double minv[CRT_MAX_MODULI];         // 1 / m_l rounded
+    double c26[CRT_MAX_MODULI];          // 2^26 mod m_l
     double w[CRT_MAX_MODULI][CRT_MAX_CHUNKS];
     double mu[CRT_MAX_CHUNKS];           // chunks of M
     double scale[CRT_MAX_CHUNKS];        // 2^(exponent of chunk p)

@@ -34,6 +34,7 @@
 
 #include "crt_gemm.h"
 #include "crt_layout.h"
+#include "crt_residue.h"
 #include "f4j_internal.h"
 #include "gemm_tile.h"
 #include "split_gemm.h"
@@ -142,26 +143,6 @@ int t_for(int n, int64_t K) {
 // Residue slices: A'(x, k) mod m_l as signed bytes, fragment-major (below).
 // ------------------------------------------------------------------------
 
-// v is an integer.  Signed residue of v modulo m, in [-128, 127] for m = 256
-// and [-(m-1)/2, (m-1)/2] for odd m.
-//   |v| < 2^58: q = rint(v / m) from one fma against 1.5 2^52 (the product
-//   v (1/m) is exact inside the fma; 1/m rounded moves it by < 1/4), so
-//   v - q m is in (-m, m) and formed exactly; one correction each side.
-//   2^58 <= |v| < 2^100 (t > 58; 20 moduli give at most t = 75): a first
-//   round with rint leaves v - q m below m + 2^-52 |v| < 2^49 in magnitude
-//   (an integer, formed exactly), then the step above.
-template <bool WIDE>
-__device__ inline int residue(double v, int m, double minv) {
-    constexpr double MAGIC = 6755399441055744.0;              // 1.5 2^52
-    if (WIDE) v = fma(-rint(v * minv), (double)m, v);
-    const double q = fma(v, minv, MAGIC) - MAGIC;
-    int r = (int)fma(-q, (double)m, v);
-    const int hi = (m - 1) >> 1;                              // 127 for m = 256
-    r = r > hi ? r - m : r;
-    r = r < -(m >> 1) ? r + m : r;
-    return r;
-}
-
 __device__ inline double scaled_int(double v, int e, int t) { return e == INT_MIN ? 0.0 : rint(ldexp(v, t - e)); }
 
 __device__ inline unsigned pack4(int a, int b, int c, int d) {
@@ -236,24 +217,32 @@ __global__ __launch_bounds__(256) void crt_split_kernel(CrtOperand o, int64_t nx
     const int r = crt::split_line(t), kb = crt::split_k(t);
     const int64_t x = x0 + r;
     const int ex = x < nx ? e[x] : INT_MIN;
-    double v[16];
+    // scaled integers, split once (v = vh 2^26 + vl) for the per-modulus residues
+    double vh[16], vl[16];
 #pragma unroll
-    for (int u = 0; u < 16; ++u) v[u] = scaled_int(tile[r][kb + u], ex, tb.t);
+    for (int u = 0; u < 16; ++u) {
+        const double v = scaled_int(tile[r][kb + u], ex, tb.t);
+        if (WIDE) {
+            vh[u] = v;               // kept whole: the wide step reduces it per modulus
+            vl[u] = 0.0;
+        } else {
+            crt::residue_split(v, vh[u], vl[u]);
+        }
+    }
     const int64_t KC = Kp >> 5;
     int4 *dst = (int4 *)(out + crt::frag_offset(x, k0 + kb, KC));
     const int64_t lstride = Xp * Kp / 16;
     for (int l = 0; l < tb.n; ++l) {
-        const int m = tb.m[l];
-        const double mi = tb.minv[l];
+        const double m = (double)tb.m[l], mi = tb.minv[l], c26 = tb.c26[l];
+        int rs[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u)
+            rs[u] = WIDE ? crt::residue_wide(vh[u], m, mi, c26) : crt::residue_of(vh[u], vl[u], m, mi, c26);
         int4 w;
-        w.x = (int)pack4(residue<WIDE>(v[0], m, mi), residue<WIDE>(v[1], m, mi), residue<WIDE>(v[2], m, mi),
-                         residue<WIDE>(v[3], m, mi));
-        w.y = (int)pack4(residue<WIDE>(v[4], m, mi), residue<WIDE>(v[5], m, mi), residue<WIDE>(v[6], m, mi),
-                         residue<WIDE>(v[7], m, mi));
-        w.z = (int)pack4(residue<WIDE>(v[8], m, mi), residue<WIDE>(v[9], m, mi), residue<WIDE>(v[10], m, mi),
-                         residue<WIDE>(v[11], m, mi));
-        w.w = (int)pack4(residue<WIDE>(v[12], m, mi), residue<WIDE>(v[13], m, mi), residue<WIDE>(v[14], m, mi),
-                         residue<WIDE>(v[15], m, mi));
+        w.x = (int)pack4(rs[0], rs[1], rs[2], rs[3]);
+        w.y = (int)pack4(rs[4], rs[5], rs[6], rs[7]);
+        w.z = (int)pack4(rs[8], rs[9], rs[10], rs[11]);
+        w.w = (int)pack4(rs[12], rs[13], rs[14], rs[15]);
         dst[l * lstride] = w;
     }
 }
@@ -377,9 +366,8 @@ __global__ __launch_bounds__(256, 2) void crt_gemm_kernel(CrtGemmArgs a, CrtTabl
     if (t < nt) step(t, ra0, ra1);
     F4J_WAIT_VMCNT(0);                            // no LDS write in flight when the block ends
 
-    // acc mod m in [0, m): q = floor(acc / m) from FP64 (|acc| < 2^31, error far
-    // below 1/m), then one correction.
-    const int m = tb.m[l];
+    // acc mod m (crt::product_residue: in [0, m], m kept for exact multiples)
+    const double m = (double)tb.m[l];
     const double minv = tb.minv[l];
     unsigned char *R = a.R + (int64_t)l * a.plane;
 #pragma unroll
@@ -392,11 +380,7 @@ __global__ __launch_bounds__(256, 2) void crt_gemm_kernel(CrtGemmArgs a, CrtTabl
             for (int r = 0; r < 16; ++r) {
                 const int64_t row = m0 + crt::acc_row(i, r, lane);
                 if (row >= a.M) continue;
-                const int v = acc[i][j][r];
-                int rr = v - (int)floor((double)v * minv) * m;
-                if (rr < 0) rr += m;
-                if (rr >= m) rr -= m;
-                R[row * a.N + col] = (unsigned char)rr;
+                R[row * a.N + col] = (unsigned char)crt::product_residue(acc[i][j][r], m, minv);
             }
         }
 }
@@ -424,26 +408,56 @@ __global__ __launch_bounds__(256) void crt_reconstruct_kernel(const unsigned cha
     for (int p = 0; p < CRT_MAX_CHUNKS; ++p)
 #pragma unroll
         for (int e = 0; e < RV; ++e) S[p][e] = 0.0;
-    for (int l = 0; l < tb.n; ++l) {
-        const unsigned char *src = R + (int64_t)l * plane + g0;
-        unsigned char r[RV];
-        if (cnt == RV) {
-            const uint2 v = *(const uint2 *)src;
-            __builtin_memcpy(r, &v, RV);
-        } else {
+    // All residue words are loaded before any is used, so every load of the
+    // thread is in flight at once (the loop over moduli would otherwise wait
+    // for each 8-byte load in turn).
+    // exponents of every entry's row and column, loaded up front as well
+    int ea_e[RV], eb_e[RV];
+    {
+        int64_t i = g0 / N, j = g0 - i * N;
 #pragma unroll
-            for (int e = 0; e < RV; ++e) r[e] = e < cnt ? src[e] : 0;
+        for (int e = 0; e < RV; ++e) {
+            const int64_t ii = e < cnt ? i : 0, jj = e < cnt ? j : 0;
+            ea_e[e] = eA[ii];
+            eb_e[e] = eB[jj];
+            if (++j == N) {
+                j = 0;
+                ++i;
+            }
         }
+    }
+    uint2 word[CRT_MAX_MODULI];
+    if (cnt == RV) {
+        // unconditional loads (moduli past n re-read plane 0, a cache hit, unused):
+        // a load issued on only some paths would make the compiler wait for all of them
 #pragma unroll
-        for (int p = 0; p < CRT_MAX_CHUNKS; ++p)
+        for (int l = 0; l < CRT_MAX_MODULI; ++l)
+            word[l] = *(const uint2 *)(R + (int64_t)(l < tb.n ? l : 0) * plane + g0);
+    } else {
 #pragma unroll
-            for (int e = 0; e < RV; ++e) S[p][e] = fma((double)r[e], tb.w[l][p], S[p][e]);   // exact integers
+        for (int l = 0; l < CRT_MAX_MODULI; ++l) {
+            unsigned char r[RV];
+#pragma unroll
+            for (int e = 0; e < RV; ++e) r[e] = (l < tb.n && e < cnt) ? R[(int64_t)l * plane + g0 + e] : 0;
+            __builtin_memcpy(&word[l], r, RV);
+        }
+    }
+#pragma unroll
+    for (int l = 0; l < CRT_MAX_MODULI; ++l) {
+        if (l < tb.n) {          // uniform; no early exit, so word[] keeps static indices (registers)
+            unsigned char r[RV];
+            __builtin_memcpy(r, &word[l], RV);
+#pragma unroll
+            for (int p = 0; p < CRT_MAX_CHUNKS; ++p)
+#pragma unroll
+                for (int e = 0; e < RV; ++e) S[p][e] = fma((double)r[e], tb.w[l][p], S[p][e]);   // exact integers
+        }
     }
     int64_t i = g0 / N, j = g0 - i * N;
 #pragma unroll
     for (int e = 0; e < RV; ++e) {
         if (e < cnt) {
-            const int ea = eA[i], eb = eB[j];
+            const int ea = ea_e[e], eb = eb_e[e];
             double out = 0.0;
             if (ea != INT_MIN && eb != INT_MIN) {
                 double approx = 0.0;
@@ -520,6 +534,7 @@ bool plan_crt(int64_t M, int64_t N, int64_t K, int n, int check_rows, CrtPlan *p
         for (int p = 0; p < CRT_MAX_CHUNKS; ++p) tb.w[l][p] = 0.0;
         tb.m[l] = MODULI[l];
         tb.minv[l] = 1.0 / MODULI[l];
+        tb.c26[l] = (double)((1 << 26) % MODULI[l]);
         if (l >= n) continue;
         uint32_t rem = 0;
         const Big Ml = big_div_small(Mb, (uint32_t)MODULI[l], nullptr);

@@ -1,0 +1,62 @@
+// Residue arithmetic of the modular (CRT) GEMM, shared by its kernels and by
+// the host test that checks it against exact integer residues
+// (tests/emu/residue_emu.cpp).
+//
+// Operand residues.  v is an integer-valued double (a scaled operand entry,
+// |v| <= 2^t).  For |v| < 2^58 it is split once, exactly, into
+//   v = vh 2^26 + vl,  vh = floor(v 2^-26),  vl in [0, 2^26),
+// and for each modulus m (c26 = 2^26 mod m, minv = 1/m rounded)
+//   u = vh c26 + vl           exact: |u| < 2^41, u = v (mod m)
+//   q = floor((u + 128) / m)  from one fma: the computed quotient is within
+//                             2^-19 of (u + 128) / m, whose fractional part is a
+//                             multiple of 1/m >= 2^-8, so q is exact unless
+//                             (u + 128) / m is an integer, where q may be one less;
+//   r = u - q m               exact, r = v (mod m), r in [-128, m - 129], or
+//                             r = m - 128 in that integer case.
+// So r always lies in [-128, 127] (m <= 255), or [-128, 128] for m = 256, whose
+// 128 is the byte of -128: every residue fits a signed byte with no correction.
+// Wider v (2^58 <= |v| <= 2^75) is first reduced to |v| < 2^24 by one exact step
+// v - rint(v / m) m (the computed quotient is off by at most 2^16).
+//
+// Product residues.  For an int32 accumulator a, r = a - floor(a / m) m with the
+// quotient from one multiply (error below 2^-28, fractional parts multiples of
+// 1/m): r in [0, m), or r = m when a is a multiple of m.  The value m is kept:
+// the reconstruction only uses r modulo m (m W_l = 0 modulo M).
+#pragma once
+#include <cmath>
+#include <cstdint>
+
+#ifndef F4J_HD
+#define F4J_HD __host__ __device__ inline
+#endif
+
+namespace f4j {
+namespace crt {
+
+constexpr double TWO26 = 67108864.0;
+
+F4J_HD void residue_split(double v, double &vh, double &vl) {
+    vh = floor(ldexp(v, -26));
+    vl = fma(-vh, TWO26, v);
+}
+
+F4J_HD int residue_of(double vh, double vl, double m, double minv, double c26) {
+    const double u = fma(vh, c26, vl);
+    const double q = floor(fma(u, minv, 128.0 * minv));
+    return (int)fma(-q, m, u);
+}
+
+// 2^58 <= |v| <= 2^75: one exact reduction, then the small-value step.
+F4J_HD int residue_wide(double v, double m, double minv, double c26) {
+    const double v1 = fma(-rint(v * minv), m, v);
+    return residue_of(0.0, v1, m, minv, c26);
+}
+
+// Product residue in [0, m].
+F4J_HD int product_residue(int a, double m, double minv) {
+    const double d = (double)a;
+    return (int)fma(-floor(d * minv), m, d);
+}
+
+}  // namespace crt
+}  // namespace f4j

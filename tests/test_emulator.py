@@ -66,3 +66,16 @@ def crt_emu():
 def test_crt_kernel_index_math(crt_emu, shape):
     r = subprocess.run([crt_emu, *map(str, shape)], capture_output=True, text=True, timeout=300)
     assert r.returncode == 0 and r.stdout.strip().startswith("OK"), r.stdout
+
+
+RES_SRC = os.path.join(ROOT, "tests", "emu", "residue_emu.cpp")
+RES_BIN = os.path.join(ROOT, "_build", "residue_emu")
+
+
+def test_crt_residue_arithmetic():
+    """The modular GEMM's residue arithmetic (crt_residue.h: operand residues for
+    t = 1 .. 75, product residues of int32 accumulators) against exact integer
+    residues, including the exact-quotient and extreme cases."""
+    binary = build_once(RES_SRC, RES_BIN, [os.path.join(ROOT, "flag4j_amd", "csrc", "crt_residue.h")])
+    r = subprocess.run([binary], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0 and r.stdout.strip().startswith("OK"), r.stdout
