@@ -190,25 +190,19 @@ __global__ __launch_bounds__(256) void crt_exp_cols_kernel(CrtOperand o, int64_t
     if (m != INT_MIN) atomicMax(e + x, m);
 }
 
-// One block: 32 lines x 128 k.  The FP64 block is staged through LDS with
-// coalesced reads (along k when lines are contiguous, along x when they are
-// strided); thread t then takes line t % 32, 16 k at 16 ((t / 32) % 2) + 32 (t / 64)
-// and writes one 16-byte fragment piece per modulus (a wave writes 1 KB).
-template <bool WIDE, bool LINES>
+// Contiguous lines (a line is a row of the stored matrix): one block is 32
+// lines x 128 k, staged through LDS with coalesced reads along k; thread t then
+// takes line t % 32, 16 k at 16 ((t / 32) % 2) + 32 (t / 64) and writes one
+// 16-byte fragment piece per modulus (a wave writes 1 KB).
+template <bool WIDE>
 __global__ __launch_bounds__(256) void crt_split_kernel(CrtOperand o, int64_t nx, int64_t K, const int *e,
                                                         signed char *out, int64_t Xp, int64_t Kp, CrtTables tb) {
     __shared__ double tile[32][129];
     const int64_t k0 = (int64_t)blockIdx.x * 128, x0 = (int64_t)blockIdx.y * 32;
     const int t = threadIdx.x;
-    if (LINES) {
+    {
         const int c = t & 127;
         for (int r = t >> 7; r < 32; r += 2) {
-            const int64_t x = x0 + r, k = k0 + c;
-            tile[r][c] = (x < nx && k < K) ? opnd_value(o, x, k) : 0.0;
-        }
-    } else {
-        const int r = t & 31;
-        for (int c = t >> 5; c < 128; c += 8) {
             const int64_t x = x0 + r, k = k0 + c;
             tile[r][c] = (x < nx && k < K) ? opnd_value(o, x, k) : 0.0;
         }
@@ -224,6 +218,55 @@ __global__ __launch_bounds__(256) void crt_split_kernel(CrtOperand o, int64_t nx
         const double v = scaled_int(tile[r][kb + u], ex, tb.t);
         if (WIDE) {
             vh[u] = v;               // kept whole: the wide step reduces it per modulus
+            vl[u] = 0.0;
+        } else {
+            crt::residue_split(v, vh[u], vl[u]);
+        }
+    }
+    const int64_t KC = Kp >> 5;
+    int4 *dst = (int4 *)(out + crt::frag_offset(x, k0 + kb, KC));
+    const int64_t lstride = Xp * Kp / 16;
+    for (int l = 0; l < tb.n; ++l) {
+        const double m = (double)tb.m[l], mi = tb.minv[l], c26 = tb.c26[l];
+        int rs[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u)
+            rs[u] = WIDE ? crt::residue_wide(vh[u], m, mi, c26) : crt::residue_of(vh[u], vl[u], m, mi, c26);
+        int4 w;
+        w.x = (int)pack4(rs[0], rs[1], rs[2], rs[3]);
+        w.y = (int)pack4(rs[4], rs[5], rs[6], rs[7]);
+        w.z = (int)pack4(rs[8], rs[9], rs[10], rs[11]);
+        w.w = (int)pack4(rs[12], rs[13], rs[14], rs[15]);
+        dst[l * lstride] = w;
+    }
+}
+
+// Strided lines (a line is a column of the stored matrix): one block is 128
+// lines x 32 k, so every k row of the block is read as 1 KB of contiguous
+// memory; thread t then takes line t % 128 and 16 k at 16 (t / 128).
+template <bool WIDE>
+__global__ __launch_bounds__(256) void crt_split_cols_kernel(CrtOperand o, int64_t nx, int64_t K, const int *e,
+                                                             signed char *out, int64_t Xp, int64_t Kp, CrtTables tb) {
+    __shared__ double tile[128][33];
+    const int64_t k0 = (int64_t)blockIdx.x * 32, x0 = (int64_t)blockIdx.y * 128;
+    const int t = threadIdx.x;
+    {
+        const int c = t & 127;
+        for (int r = t >> 7; r < 32; r += 2) {
+            const int64_t x = x0 + c, k = k0 + r;
+            tile[c][r] = (x < nx && k < K) ? opnd_value(o, x, k) : 0.0;
+        }
+    }
+    __syncthreads();
+    const int xl = crt::split_cols_line(t), kb = crt::split_cols_k(t);
+    const int64_t x = x0 + xl;
+    const int ex = x < nx ? e[x] : INT_MIN;
+    double vh[16], vl[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+        const double v = scaled_int(tile[xl][kb + u], ex, tb.t);
+        if (WIDE) {
+            vh[u] = v;
             vl[u] = 0.0;
         } else {
             crt::residue_split(v, vh[u], vl[u]);
@@ -597,18 +640,19 @@ hipError_t launch_crt_gemm(const CrtPlan &pl, const CrtOperand &oa, const CrtOpe
         const dim3 grid((unsigned)(pl.Kp / 128), (unsigned)(Xp / 32));
         if (o.lines) {
             if (wide)
-                hipLaunchKernelGGL((crt_split_kernel<true, true>), grid, dim3(256), 0, stream, o, nx, K, ex, dst, Xp,
-                                   pl.Kp, pl.tb);
+                hipLaunchKernelGGL((crt_split_kernel<true>), grid, dim3(256), 0, stream, o, nx, K, ex, dst, Xp, pl.Kp,
+                                   pl.tb);
             else
-                hipLaunchKernelGGL((crt_split_kernel<false, true>), grid, dim3(256), 0, stream, o, nx, K, ex, dst, Xp,
+                hipLaunchKernelGGL((crt_split_kernel<false>), grid, dim3(256), 0, stream, o, nx, K, ex, dst, Xp,
                                    pl.Kp, pl.tb);
         } else {
+            const dim3 gc((unsigned)(pl.Kp / 32), (unsigned)(Xp / 128));
             if (wide)
-                hipLaunchKernelGGL((crt_split_kernel<true, false>), grid, dim3(256), 0, stream, o, nx, K, ex, dst, Xp,
+                hipLaunchKernelGGL((crt_split_cols_kernel<true>), gc, dim3(256), 0, stream, o, nx, K, ex, dst, Xp,
                                    pl.Kp, pl.tb);
             else
-                hipLaunchKernelGGL((crt_split_kernel<false, false>), grid, dim3(256), 0, stream, o, nx, K, ex, dst,
-                                   Xp, pl.Kp, pl.tb);
+                hipLaunchKernelGGL((crt_split_cols_kernel<false>), gc, dim3(256), 0, stream, o, nx, K, ex, dst, Xp,
+                                   pl.Kp, pl.tb);
         }
     };
     split(oa, M, pl.Mp, ea, ra);

@@ -34,6 +34,12 @@ F4J_HD int64_t frag_offset(int64_t x, int64_t k, int64_t KC) {
 F4J_HD int split_line(int t) { return t & 31; }
 F4J_HD int split_k(int t) { return 16 * ((t >> 5) & 1) + 32 * (t >> 6); }
 
+// Split kernel for strided lines (columns of a row-major operand; block = 128
+// lines x 32 k, so each k row of the block is 1 KB of contiguous memory):
+// thread t writes the 16 bytes of line x0 + split_cols_line(t) at k0 + split_cols_k(t).
+F4J_HD int split_cols_line(int t) { return t & 127; }
+F4J_HD int split_cols_k(int t) { return 16 * (t >> 7); }
+
 // Product kernel.  Wave w of a block owns rows m0w = tile_m BM + 64 w .. + 63
 // (register sets i = 0, 1 of 32 rows); the block's columns are n0 .. n0 + 127.
 // A load for k slice t, set i, step ks: uniform base + soffset + per-lane voffset.

@@ -33,16 +33,19 @@ static int g_fail = 0;
         }                                          \
     } while (0)
 
-// Residue slice as the split kernel writes it (32 lines x 128 k per block).
+// Residue slice as the split kernels write it: contiguous lines in blocks of
+// 32 lines x 128 k, strided lines (cols) in blocks of 128 lines x 32 k.
 static std::vector<signed char> split_write(const std::vector<int> &X, int64_t nx, int64_t K, int64_t Xp,
-                                            int64_t Kp) {
+                                            int64_t Kp, bool cols) {
     const int64_t KC = Kp / 32;
+    const int bx = cols ? 128 : 32, bk = cols ? 32 : 128;
     std::vector<signed char> out(Xp * Kp, 0);
     std::vector<int> hits(Xp * Kp, 0);
-    for (int64_t xb = 0; xb < Xp / 32; ++xb)
-        for (int64_t kb = 0; kb < Kp / 128; ++kb)
+    for (int64_t xb = 0; xb < Xp / bx; ++xb)
+        for (int64_t kb = 0; kb < Kp / bk; ++kb)
             for (int t = 0; t < 256; ++t) {
-                const int64_t x = xb * 32 + crt::split_line(t), k = kb * 128 + crt::split_k(t);
+                const int64_t x = xb * bx + (cols ? crt::split_cols_line(t) : crt::split_line(t));
+                const int64_t k = kb * bk + (cols ? crt::split_cols_k(t) : crt::split_k(t));
                 const int64_t off = crt::frag_offset(x, k, KC);
                 for (int u = 0; u < 16; ++u) {
                     CHECK(off + u >= 0 && off + u < Xp * Kp, "split write out of bounds x=%ld k=%ld", (long)x,
@@ -74,7 +77,8 @@ int main(int argc, char **argv) {
     std::vector<int> A(M * K), B(N * K);            // A: M lines of K; B: N lines (columns of op(B)) of K
     for (auto &v : A) v = d(g);
     for (auto &v : B) v = d(g);
-    const std::vector<signed char> Ar = split_write(A, M, K, Mp, Kp), Br = split_write(B, N, K, Np, Kp);
+    // A through the contiguous-line split, B through the strided one (as for op(B) = N)
+    const std::vector<signed char> Ar = split_write(A, M, K, Mp, Kp, false), Br = split_write(B, N, K, Np, Kp, true);
 
     std::vector<int> R(M * N, -1), Rhits(M * N, 0);
     const int64_t tiles_m = Mp / crt::BM, tiles_n = Np / crt::BN;
