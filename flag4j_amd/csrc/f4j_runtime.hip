@@ -64,6 +64,8 @@ struct State {
     std::mutex mix_mu;                // guards mixws (planes of complex x real products)
     DevBuf mixws;
     hipEvent_t tev[8] = {};                 // product timing: a begin / end pair per real product (up to 4)
+    hipStream_t side = nullptr;             // residual sums beside the modular products (guarded by ws_mu)
+    hipEvent_t fork_ev = nullptr, join_ev = nullptr;
 };
 
 State &S() {
@@ -101,6 +103,9 @@ int init_locked(State &s, int device) {
     F4J_HIP(hipGetDeviceProperties(&s.prop, device), "hipGetDeviceProperties");
     F4J_HIP(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking), "hipStreamCreate");
     for (hipEvent_t &e : s.tev) F4J_HIP(hipEventCreate(&e), "hipEventCreate");
+    F4J_HIP(hipStreamCreateWithFlags(&s.side, hipStreamNonBlocking), "hipStreamCreate");
+    F4J_HIP(hipEventCreateWithFlags(&s.fork_ev, hipEventDisableTiming), "hipEventCreate");
+    F4J_HIP(hipEventCreateWithFlags(&s.join_ev, hipEventDisableTiming), "hipEventCreate");
     s.device = device;
     s.inited = true;
     return F4J_OK;
@@ -378,6 +383,27 @@ int run_crt(bool cplx, int opA, int opB, int64_t M, int64_t N, int64_t K, const 
     int rc = grow(s.ws, pl.bytes);
     if (rc != F4J_OK) return rc;
     const bool timing = c.split_timing != 0;
+    const int check_rows = (int)std::min<int64_t>(c.split_check_rows, M);
+    char *w = (char *)s.ws.p;
+    double *part = (double *)(w + pl.off_part);
+    // Whatever path leaves this function, the side stream is idle first: its
+    // work writes the shared workspace, which the next call may grow.
+    struct SideIdle {
+        hipStream_t side = nullptr;
+        ~SideIdle() {
+            if (side) (void)hipStreamSynchronize(side);
+        }
+    } side_idle;
+    if (check_rows > 0) {
+        // The check's compensated sums need only A and B: they run on the side
+        // stream while the residue products run on this one.
+        side_idle.side = s.side;
+        F4J_HIP(hipEventRecord(s.fork_ev, stream), "hipEventRecord");
+        F4J_HIP(hipStreamWaitEvent(s.side, s.fork_ev, 0), "hipStreamWaitEvent");
+        F4J_HIP(launch_residual_partial(cplx, opA, opB, M, N, K, A, lda, B, ldb, check_rows, part, s.side),
+                "crt residual launch");
+        F4J_HIP(hipEventRecord(s.join_ev, s.side), "hipEventRecord");
+    }
     for (size_t jb = 0; jb < jobs.size(); ++jb) {
         const Job &j = jobs[jb];
         const CrtOperand oa = crt_operand(A, lda, opA == F4J_OP_N, cplx, j.ca_a, j.cb_a);
@@ -397,20 +423,13 @@ int run_crt(bool cplx, int opA, int opB, int64_t M, int64_t N, int64_t K, const 
     rep.slices = n;
     rep.products = n * (int)jobs.size();
     rep.launches = (int)jobs.size();
-    rep.checked_rows = (int)std::min<int64_t>(c.split_check_rows, M);
+    rep.checked_rows = check_rows;
     rep.max_rel_err = -1.0;
     if (rep.checked_rows > 0) {
-        char *w = (char *)s.ws.p;
-        double *part = (double *)(w + pl.off_part);
         unsigned long long *eb = (unsigned long long *)(w + pl.off_err);
-        if (cplx)
-            F4J_HIP(launch_residual_check_cplx(opA, opB, M, N, K, A, lda, B, ldb, C, ldc, rep.checked_rows, part, eb,
-                                               stream),
-                    "crt residual launch");
-        else
-            F4J_HIP(launch_residual_check(opA, opB, M, N, K, A, lda, B, ldb, C, ldc, rep.checked_rows, part, eb,
-                                          stream),
-                    "crt residual launch");
+        F4J_HIP(hipStreamWaitEvent(stream, s.join_ev, 0), "hipStreamWaitEvent");
+        F4J_HIP(launch_residual_combine(cplx, M, N, K, C, ldc, rep.checked_rows, part, eb, stream),
+                "crt residual launch");
         double err = 0.0;
         F4J_HIP(hipMemcpyAsync(&err, w + pl.off_err, sizeof(double), hipMemcpyDeviceToHost, stream), "copy residual");
         F4J_HIP(hipStreamSynchronize(stream), "stream synchronize");
@@ -774,7 +793,8 @@ int f4j_shutdown(void) {
     if (!s.inited) return F4J_OK;
     F4J_HIP(hipSetDevice(s.device), "hipSetDevice");
     F4J_HIP(hipStreamSynchronize(s.stream), "stream synchronize");
-    for (DevBuf *b : {&s.bufA, &s.bufB, &s.bufC, &s.ws})
+    F4J_HIP(hipStreamSynchronize(s.side), "stream synchronize");
+    for (DevBuf *b : {&s.bufA, &s.bufB, &s.bufC, &s.ws, &s.mixws})
         if (b->p) {
             F4J_HIP(hipFree(b->p), "hipFree");
             b->p = nullptr;
@@ -785,6 +805,13 @@ int f4j_shutdown(void) {
             F4J_HIP(hipEventDestroy(e), "hipEventDestroy");
             e = nullptr;
         }
+    for (hipEvent_t *e : {&s.fork_ev, &s.join_ev})
+        if (*e) {
+            F4J_HIP(hipEventDestroy(*e), "hipEventDestroy");
+            *e = nullptr;
+        }
+    F4J_HIP(hipStreamDestroy(s.side), "hipStreamDestroy");
+    s.side = nullptr;
     F4J_HIP(hipStreamDestroy(s.stream), "hipStreamDestroy");
     s.stream = nullptr;
     s.inited = false;

@@ -85,5 +85,12 @@ hipError_t launch_residual_check_cplx(int opA, int opB, int64_t M, int64_t N, in
 hipError_t launch_residual_check(int opA, int opB, int64_t M, int64_t N, int64_t K, const double *A, int64_t lda,
                                  const double *B, int64_t ldb, const double *C, int64_t ldc, int rows, double *part,
                                  unsigned long long *err, hipStream_t stream);
+// The check in two phases: the compensated sums of the sampled rows need only
+// A and B (they may run beside the product); the comparison needs C.
+hipError_t launch_residual_partial(bool cplx, int opA, int opB, int64_t M, int64_t N, int64_t K, const double *A,
+                                   int64_t lda, const double *B, int64_t ldb, int rows, double *part,
+                                   hipStream_t stream);
+hipError_t launch_residual_combine(bool cplx, int64_t M, int64_t N, int64_t K, const double *C, int64_t ldc, int rows,
+                                   const double *part, unsigned long long *err, hipStream_t stream);
 
 }  // namespace f4j

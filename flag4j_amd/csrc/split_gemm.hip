@@ -518,35 +518,53 @@ hipError_t launch_split_residual(const SplitPlan &pl, int opA, int opB, int64_t 
                                  (unsigned long long *)((char *)ws + pl.off_err), stream);
 }
 
-hipError_t launch_residual_check(int opA, int opB, int64_t M, int64_t N, int64_t K, const double *A, int64_t lda,
-                                 const double *B, int64_t ldb, const double *C, int64_t ldc, int rows, double *part,
-                                 unsigned long long *err, hipStream_t stream) {
+hipError_t launch_residual_partial(bool cplx, int opA, int opB, int64_t M, int64_t N, int64_t K, const double *A,
+                                   int64_t lda, const double *B, int64_t ldb, int rows, double *part,
+                                   hipStream_t stream) {
+    const int r = (int)std::min<int64_t>(rows, M);
+    if (r <= 0) return hipSuccess;
+    const int64_t step = r > 1 ? (M - 1) / (r - 1) : 0;
+    const int nch = (int)((K + RES_KCHUNK - 1) / RES_KCHUNK);
+    if (cplx)
+        hipLaunchKernelGGL(residual_partial_cplx_kernel, dim3((unsigned)((N + 255) / 256), (unsigned)r, (unsigned)nch),
+                           dim3(256), 0, stream, A, lda, opA, B, ldb, opB, M, N, K, step, part);
+    else
+        hipLaunchKernelGGL(residual_partial_kernel,
+                           dim3((unsigned)((N + 255) / 256), (unsigned)((r + RES_ROWS - 1) / RES_ROWS), (unsigned)nch),
+                           dim3(256), 0, stream, A, lda, opA, B, ldb, opB, M, N, K, step, r, part);
+    return hipGetLastError();
+}
+
+hipError_t launch_residual_combine(bool cplx, int64_t M, int64_t N, int64_t K, const double *C, int64_t ldc, int rows,
+                                   const double *part, unsigned long long *err, hipStream_t stream) {
     hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(256), 0, stream, (int *)err, 2, 0);
     const int r = (int)std::min<int64_t>(rows, M);
     if (r <= 0) return hipGetLastError();
     const int64_t step = r > 1 ? (M - 1) / (r - 1) : 0;
     const int nch = (int)((K + RES_KCHUNK - 1) / RES_KCHUNK);
-    hipLaunchKernelGGL(residual_partial_kernel,
-                       dim3((unsigned)((N + 255) / 256), (unsigned)((r + RES_ROWS - 1) / RES_ROWS), (unsigned)nch),
-                       dim3(256), 0, stream, A, lda, opA, B, ldb, opB, M, N, K, step, r, part);
-    hipLaunchKernelGGL(residual_combine_kernel, dim3((unsigned)((N + 255) / 256), (unsigned)r), dim3(256), 0, stream, C,
-                       ldc, M, N, step, nch, part, err);
+    if (cplx)
+        hipLaunchKernelGGL(residual_combine_cplx_kernel, dim3((unsigned)((N + 255) / 256), (unsigned)r), dim3(256), 0,
+                           stream, C, ldc, M, N, step, nch, part, err);
+    else
+        hipLaunchKernelGGL(residual_combine_kernel, dim3((unsigned)((N + 255) / 256), (unsigned)r), dim3(256), 0,
+                           stream, C, ldc, M, N, step, nch, part, err);
     return hipGetLastError();
+}
+
+hipError_t launch_residual_check(int opA, int opB, int64_t M, int64_t N, int64_t K, const double *A, int64_t lda,
+                                 const double *B, int64_t ldb, const double *C, int64_t ldc, int rows, double *part,
+                                 unsigned long long *err, hipStream_t stream) {
+    const hipError_t e = launch_residual_partial(false, opA, opB, M, N, K, A, lda, B, ldb, rows, part, stream);
+    if (e != hipSuccess) return e;
+    return launch_residual_combine(false, M, N, K, C, ldc, rows, part, err, stream);
 }
 
 hipError_t launch_residual_check_cplx(int opA, int opB, int64_t M, int64_t N, int64_t K, const double *A,
                                       int64_t lda, const double *B, int64_t ldb, const double *C, int64_t ldc,
                                       int rows, double *part, unsigned long long *err, hipStream_t stream) {
-    hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(256), 0, stream, (int *)err, 2, 0);
-    const int r = (int)std::min<int64_t>(rows, M);
-    if (r <= 0) return hipGetLastError();
-    const int64_t step = r > 1 ? (M - 1) / (r - 1) : 0;
-    const int nch = (int)((K + RES_KCHUNK - 1) / RES_KCHUNK);
-    hipLaunchKernelGGL(residual_partial_cplx_kernel, dim3((unsigned)((N + 255) / 256), (unsigned)r, (unsigned)nch),
-                       dim3(256), 0, stream, A, lda, opA, B, ldb, opB, M, N, K, step, part);
-    hipLaunchKernelGGL(residual_combine_cplx_kernel, dim3((unsigned)((N + 255) / 256), (unsigned)r), dim3(256), 0,
-                       stream, C, ldc, M, N, step, nch, part, err);
-    return hipGetLastError();
+    const hipError_t e = launch_residual_partial(true, opA, opB, M, N, K, A, lda, B, ldb, rows, part, stream);
+    if (e != hipSuccess) return e;
+    return launch_residual_combine(true, M, N, K, C, ldc, rows, part, err, stream);
 }
 
 size_t split_residual_bytes(int64_t N, int64_t K, int rows) {
