@@ -211,6 +211,28 @@ def test_crt_ragged(M, N, K):
     assert sampled_err(C, A, B, n=200) <= 1e-15
 
 
+@pytest.mark.parametrize("N,opB", [(1000, "N"), (2056, "N"), (2056, "T"), (1001, "N")])
+def test_crt_column_block_matches_whole(N, opB):
+    """Column exponents depend only on the column and t only on K, so the first 512
+    columns of a wide product must equal, bit for bit, the product of the first 512
+    columns alone; every entry meets the bar (real, and complex 3M on the same shapes)."""
+    M, K = 200, 384
+    A = rng.uniform(41, M, K)
+    B = rng.uniform(42, *((K, N) if opB == "N" else (N, K)))
+    C, rep = split_mult(A, B, "split_int8", 0, "N", opB)
+    assert rep.precision == L.PREC["split_int8"]
+    Bs = np.ascontiguousarray(B[:, :512] if opB == "N" else B[:512, :])
+    C512, _ = split_mult(A, Bs, "split_int8", 0, "N", opB)
+    assert np.array_equal(C[:, :512].view(np.int64), C512.view(np.int64))
+    Bop = np.ascontiguousarray(B if opB == "N" else B.T)
+    assert sampled_err(C, A, Bop, n=400) <= 1e-15
+    Ac = A + 1j * rng.uniform(43, M, K)
+    Bc = Bop + 1j * rng.uniform(44, K, N)
+    Cc, rep = zsplit_mult(Ac, Bc, "split_int8", 0, True)
+    assert rep.precision == L.PREC["split_int8"]
+    assert zsampled_err(Cc, Ac, Bc, "N", "N") <= 1e-15
+
+
 @pytest.mark.parametrize("phi,n", [(4, 18), (8, 20)])
 def test_crt_zero_lines_and_wide_range(phi, n):
     """Entries spread over 10^(+-phi): the default moduli (t ~ 53) leave the small
