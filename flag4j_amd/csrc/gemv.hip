@@ -22,7 +22,8 @@
 // a (row, 64-block) chain for the blocked form (8 rows x 8 blocks per wave, the
 // block sums then added in block order across lanes), a row for the plain real
 // form (64 rows per wave) and a (row, component) pair for complex (32 rows per
-// wave).  The wave streams its slab of A in stages of 16 elements per chain
+// wave; below about 12K-24K rows these take 8 or 16 rows per wave, with the
+// other lanes idle, to spread the rows over more waves).  The wave streams its slab of A in stages of 16 elements per chain
 // (8 KB): coalesced 8-byte buffer loads into registers D stages ahead, then a
 // write into LDS rotated so that the 64 lanes read their own chains' elements
 // without bank conflicts.  Loads past K or past the last row are given an
@@ -282,18 +283,27 @@ __global__ __launch_bounds__(64) void gemv_kernel(GemvArgs g) {
 // same number of loads (offsets clamped into the slab), and at the top of step
 // s vmcnt((D - 2) L) retires stage s.  Elements past K are skipped by the
 // reader (a uniform test picks the masked path for the last stages only).
+//
+// NL (the one-chain-per-row kinds): A loads per stage, 8 for a full wave of
+// chains.  With NL < 8 a wave takes NL / 8 of the rows (a stage holds NL KB
+// of A) and only that share of its lanes carries a chain; the other lanes
+// repeat those chains and store nothing.  Fewer rows per wave put more waves
+// (and more loads in flight) on each CU when M is a few thousand rows.
 typedef __attribute__((address_space(3))) void lds_void;
 #define F4J_GEMV_WAIT_VMCNT(n) __builtin_amdgcn_s_waitcnt(((n) & 15) | (7 << 4) | (15 << 8) | (((n) >> 4) << 14))
 
-constexpr int SLOT = 9216;                 // bytes per stage: 8 KB of A, 1 KB of x
+// Bytes per stage slot: NL KB of A, then 1 KB of x.
+constexpr int slot_bytes(int nl) { return 1024 * nl + 1024; }
 
 // x loads per stage: XV 2 = 16-byte pairs (x aligned, unit stride, K even), XV 1 = dwords.
 template <int KIND, int XV>
 constexpr int x_dma_loads() { return KIND == GEMV_CPLX ? 2 : (KIND == GEMV_BLOCKED && XV == 1 ? 4 : 1); }
 
-template <int KIND, int XV, int D>
+template <int KIND, int XV, int D, int NL>
 __global__ __launch_bounds__(64) void gemv_dma_kernel(GemvArgs g) {
-    constexpr int R = rows_per_wave<KIND>(), XL = x_dma_loads<KIND, XV>(), L = 8 + XL;
+    static_assert(NL == 8 || (KIND != GEMV_BLOCKED && (NL == 2 || NL == 1)), "rows per wave");
+    constexpr int R = rows_per_wave<KIND>() * NL / 8, XL = x_dma_loads<KIND, XV>(), L = NL + XL;
+    constexpr int SLOT = slot_bytes(NL), XOFF = 1024 * NL;
     constexpr int ew = a_cplx<KIND>() ? 2 : 1, xw = x_cplx<KIND>() ? 2 : 1;
     static_assert((D - 1) * L <= 63, "loads in flight must fit the vmcnt counter");
     __shared__ __attribute__((aligned(1024))) char lds[D * SLOT];
@@ -308,10 +318,10 @@ __global__ __launch_bounds__(64) void gemv_dma_kernel(GemvArgs g) {
     const int kx_max = XV == 2 ? K - 2 : K - 1;
 
     // per-lane load geometry
-    int akl[8];
-    uint32_t arow[8];
+    int akl[8];           // (sized 8, not NL: an array of template-dependent size captured by the
+    uint32_t arow[8];     //  lambdas below makes hipcc drop the kernel's host stub without a diagnostic)
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
+    for (int i = 0; i < NL; ++i) {
         int r, kl;
         a_slot<KIND, 2>(i, l, r, kl);
         akl[i] = kl;
@@ -354,7 +364,7 @@ __global__ __launch_bounds__(64) void gemv_dma_kernel(GemvArgs g) {
         const int ks = kstage<KIND>(s);
         char *slot = lds + (int)(s % D) * SLOT;
 #pragma unroll
-        for (int i = 0; i < 8; ++i) {
+        for (int i = 0; i < NL; ++i) {
             const int k = min(ks + akl[i], ka_max);
             __builtin_amdgcn_raw_ptr_buffer_load_lds(ra, (lds_void *)(slot + 1024 * i), 16,
                                                      arow[i] + (uint32_t)k * (8u * ew), 0, 0, 0);
@@ -364,35 +374,37 @@ __global__ __launch_bounds__(64) void gemv_dma_kernel(GemvArgs g) {
             const int k = min(ks + xkl[h], kx_max);
             const uint32_t off = (uint32_t)k * (uint32_t)incx * 8u + xadd[h];   // incx in doubles
             if constexpr (XV == 2 && KIND != GEMV_CPLX)
-                __builtin_amdgcn_raw_ptr_buffer_load_lds(rx, (lds_void *)(slot + 8192), 16, off, 0, 0, 0);
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(rx, (lds_void *)(slot + XOFF), 16, off, 0, 0, 0);
             else
-                __builtin_amdgcn_raw_ptr_buffer_load_lds(rx, (lds_void *)(slot + 8192 + 256 * h), 4, off, 0, 0, 0);
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(rx, (lds_void *)(slot + XOFF + 256 * h), 4, off, 0, 0, 0);
         }
     };
 
     // reader geometry (bytes within a slot)
-    int a_base, a_rot, x_base = 8192, x_rot = 0, kb = 0;
+    // (lanes past the wave's R chains read the chain of row lane % R)
+    int a_base, a_rot, x_base = XOFF, x_rot = 0, kb = 0;
     uint32_t sg1 = 0, sg2 = 0;                        // complex: sign masks of the two products
     if constexpr (KIND == GEMV_CX) {
-        const int r = l & 31;
+        const int r = (l & 31) % R;
         a_base = 1024 * (r >> 2) + 256 * (r & 3);
         a_rot = r;
     } else if constexpr (KIND == GEMV_CPLX) {
-        const int r = l & 31;
+        const int r = (l & 31) % R;
         a_base = 1024 * (r >> 2) + 256 * (r & 3);
         a_rot = r;
         const bool im = l >= 32, cj = g.conjx != 0;
-        x_base = 8192 + (im ? 256 : 0);
+        x_base = XOFF + (im ? 256 : 0);
         // re = t1 - s t2, im = s t1 + t2 with t1 = ar w0, t2 = ai w1, s = -1 for conj(x)
         sg1 = (im && cj) ? 0x80000000u : 0u;
         sg2 = (!im && !cj) ? 0x80000000u : 0u;
     } else {
-        a_base = 1024 * (l >> 3) + 128 * (l & 7);
-        a_rot = (l >> 3) + (l & 7);
+        const int c = KIND == GEMV_BLOCKED ? l : l % R;   // blocked: chain (row l / 8, block l % 8)
+        a_base = 1024 * (c >> 3) + 128 * (c & 7);
+        a_rot = (c >> 3) + (c & 7);
         if constexpr (KIND == GEMV_BLOCKED) {
             const int j = l & 7;
             kb = 64 * j;
-            x_base = 8192 + (XV == 2 ? 128 : 128) * j;
+            x_base = XOFF + 128 * j;
             x_rot = j;
         }
     }
@@ -407,7 +419,7 @@ __global__ __launch_bounds__(64) void gemv_dma_kernel(GemvArgs g) {
 #pragma unroll
             for (int e = 0; e < 16; ++e) {
                 const f64x2 a = *(const f64x2 *)(slot + a_base + 16 * ((e + a_rot) & 15));
-                const double w = *(const double *)(slot + 8192 + 8 * e);
+                const double w = *(const double *)(slot + XOFF + 8 * e);
                 const double v = __dadd_rn(s_acc, __dmul_rn(im ? a.y : a.x, w));
                 s_acc = (!masked || ks + e < K) ? v : s_acc;
             }
@@ -416,8 +428,8 @@ __global__ __launch_bounds__(64) void gemv_dma_kernel(GemvArgs g) {
 #pragma unroll
             for (int p = 0; p < 8; ++p) {
                 const f64x2 a = *(const f64x2 *)(slot + a_base + 16 * ((p + a_rot) & 7));
-                const f64x2 w0 = *(const f64x2 *)(slot + 8192 + 32 * p);
-                const f64x2 w1 = *(const f64x2 *)(slot + 8192 + 32 * p + 16);
+                const f64x2 w0 = *(const f64x2 *)(slot + XOFF + 32 * p);
+                const f64x2 w1 = *(const f64x2 *)(slot + XOFF + 32 * p + 16);
                 const double vr = __dadd_rn(__dadd_rn(s_acc, __dmul_rn(w0.x, a.x)), __dmul_rn(w1.x, a.y));
                 const double vi = __dadd_rn(__dadd_rn(s_im, __dmul_rn(w0.y, a.x)), __dmul_rn(w1.y, a.y));
                 const bool ok = !masked || ks + 2 * p < K;
@@ -498,11 +510,29 @@ __global__ __launch_bounds__(64) void gemv_dma_kernel(GemvArgs g) {
     }
 }
 
-template <int KIND, int XV, int D>
+template <int KIND, int XV, int D, int NL = 8>
 void launch_dma(GemvArgs g, int64_t stages, hipStream_t stream) {
-    constexpr int R = rows_per_wave<KIND>();
+    constexpr int R = rows_per_wave<KIND>() * NL / 8;
     g.nstage = stages;
-    hipLaunchKernelGGL((gemv_dma_kernel<KIND, XV, D>), dim3((unsigned)((g.M + R - 1) / R)), dim3(64), 0, stream, g);
+    hipLaunchKernelGGL((gemv_dma_kernel<KIND, XV, D, NL>), dim3((unsigned)((g.M + R - 1) / R)), dim3(64), 0, stream,
+                       g);
+}
+
+// One-chain-per-row kinds: rows per wave by M, as measured on MI355X
+// (2048 <= M <= 32768, A of 512 MB / 1 GB): full waves need M of about 12K
+// (complex) or 24K (real) rows to keep enough chains and loads in flight on
+// every CU; below that, 8 rows per wave (16 for real A from 12K rows), with the
+// ring depth grown so a wave still has 16-24 KB of A in flight.
+template <int KIND, int XV>
+void launch_rows(GemvArgs g, int64_t stages, hipStream_t stream) {
+    if constexpr (a_cplx<KIND>()) {
+        if (g.M >= 12288) launch_dma<KIND, XV, 7, 8>(g, stages, stream);
+        else launch_dma<KIND, XV, 13, 2>(g, stages, stream);
+    } else {
+        if (g.M >= 24576) launch_dma<KIND, XV, 7, 8>(g, stages, stream);
+        else if (g.M >= 12288) launch_dma<KIND, XV, 13, 2>(g, stages, stream);
+        else launch_dma<KIND, XV, 17, 1>(g, stages, stream);
+    }
 }
 
 template <int KIND, int VEC, int D>
@@ -552,15 +582,15 @@ hipError_t launch_gemv(int kind, int64_t M, int64_t K, const double *A, int64_t 
         else launch<GEMV_BLOCKED, 1, 3>(g, stages, stream);
     } else if (kind == GEMV_CHAIN) {
         const int64_t stages = (K + 15) / 16;
-        if (a16 && x16) launch_dma<GEMV_CHAIN, 2, 7>(g, stages, stream);
-        else if (a16) launch_dma<GEMV_CHAIN, 1, 7>(g, stages, stream);
+        if (a16 && x16) launch_rows<GEMV_CHAIN, 2>(g, stages, stream);
+        else if (a16) launch_rows<GEMV_CHAIN, 1>(g, stages, stream);
         else launch<GEMV_CHAIN, 1, 3>(g, stages, stream);
     } else if (kind == GEMV_RX) {
-        launch_dma<GEMV_RX, 1, 7>(g, (K + 15) / 16, stream);
+        launch_rows<GEMV_RX, 1>(g, (K + 15) / 16, stream);
     } else if (kind == GEMV_CX) {
-        launch_dma<GEMV_CX, 1, 7>(g, (K + 15) / 16, stream);
+        launch_rows<GEMV_CX, 1>(g, (K + 15) / 16, stream);
     } else {
-        launch_dma<GEMV_CPLX, 1, 7>(g, (K + 15) / 16, stream);
+        launch_rows<GEMV_CPLX, 1>(g, (K + 15) / 16, stream);
     }
     return hipGetLastError();
 }

@@ -200,3 +200,29 @@ def test_full_size_blocked_bitwise():
     A = rng.random((M, K)) * 2 - 1
     x = rng.random(K) * 2 - 1
     assert np.array_equal(bits(dgemv(A, x)), bits(o.real_blocked_vector(A, x, 64)))
+
+
+@pytest.mark.parametrize("M", [24577, 12289, 12287, 100])
+def test_rows_per_wave_thresholds(M):
+    """The one-chain-per-row kernels take 64 / 16 / 8 rows per wave (real A) or 32 / 8
+    (complex A) by M; each size class, with a ragged last wave, against the oracle bit
+    for bit: Matrix.multTranspose with a one-row B (plain chain, K even: direct
+    loads), CMatrix.mult(CVector), Matrix.mult(CVector) and CMatrix.mult(Vector)."""
+    rng = np.random.default_rng(M)
+    L = _native.load()
+    K = 38
+    A, x = wide(rng, M, K), wide(rng, K)
+    C = np.full(M, np.nan)
+    _native.check(L.f4j_dgemm(0, 1, M, 1, K, p(A), K, p(x), K, p(C), 1, _native.PREC["auto"], 0))
+    assert np.array_equal(bits(C), bits(o.real_mult_transpose(A, x.reshape(1, -1)).reshape(-1)))
+    xc = wide_c(rng, K)
+    yc = np.full(M, np.nan + 0j)
+    _native.check(L.f4j_dzgemm(M, 1, K, p(A), K, p(xc), 1, p(yc), 1, _native.PREC["auto"], 0))
+    assert np.array_equal(yc.view(np.float64), o.mixed_gemm(A, xc.reshape(-1, 1)).reshape(-1).view(np.float64))
+    Kc = 37
+    Ac, xcc, xr = wide_c(rng, M, Kc), wide_c(rng, Kc), wide(rng, Kc)
+    y = zgemv(Ac, xcc)
+    assert np.array_equal(y.view(np.float64), o.cplx_gemm(Ac, xcc.reshape(-1, 1)).reshape(-1).view(np.float64))
+    yc = np.full(M, np.nan + 0j)
+    _native.check(L.f4j_zdgemm(M, 1, Kc, p(Ac), Kc, p(xr), 1, p(yc), 1, _native.PREC["auto"], 0))
+    assert np.array_equal(yc.view(np.float64), o.mixed_gemm(Ac, xr.reshape(-1, 1)).reshape(-1).view(np.float64))
