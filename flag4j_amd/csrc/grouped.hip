@@ -8,9 +8,11 @@
 // This is the batched entry point that replaces a Java loop of Matrix.mult /
 // A.H().mult(B) calls.
 //
-// Plan.  Problems are split into size buckets (by the larger of M and N~).
-// Each bucket runs one persistent launch of the FP64 MFMA tile kernel with
-// the bucket's tile configuration; its tiles are assigned to workgroups by
+// Plan.  Problems are split into tile-shape buckets: each problem goes to the
+// shape (64 x 64, 128 x 64, 64 x 128 or 128 x 128) with the least padded work
+// per unit of measured efficiency, so a 400 x 40 product takes 128 x 64 tiles
+// rather than 128 x 128.  Each bucket runs one persistent launch of the FP64
+// MFMA tile kernel with the bucket's tile configuration; its tiles are assigned to workgroups by
 // longest-processing-time-first list scheduling on an estimated cost, so the
 // workgroups finish together.  The launches are captured into one hipGraph at
 // plan creation; running the plan replays the graph.  The autotuner times each
@@ -45,12 +47,17 @@ namespace f4j {
 namespace {
 
 constexpr int NUM_BUCKETS = 4;
-// Upper bound of max(M, N~) per bucket (the last is unbounded).
-constexpr int64_t BUCKET_LIMIT[NUM_BUCKETS - 1] = {64, 160, 320};
+// Tile shape of each bucket and the fraction of the FP64 MFMA rate its default
+// configuration reaches on padded work in batches of 16..512-sized products
+// (measured on MI355X; the 64-wide tiles carry more load and barrier overhead
+// per MFMA).
+constexpr int BUCKET_BM[NUM_BUCKETS] = {64, 128, 64, 128};
+constexpr int BUCKET_BN[NUM_BUCKETS] = {64, 64, 128, 128};
+constexpr double BUCKET_EFF[NUM_BUCKETS] = {0.50, 0.60, 0.60, 0.68};
 // Tile configurations the autotuner tries (ids of F4J_FP64_TILES).
 constexpr int CANDIDATES[] = {1, 7, 8, 0, 4};
-// Untuned defaults per bucket.
-constexpr int DEFAULT_CFG[NUM_BUCKETS] = {1, 1, 1, 4};
+// Untuned defaults per bucket: the bucket's own tile shape.
+constexpr int DEFAULT_CFG[NUM_BUCKETS] = {1, 8, 7, 4};
 
 struct TileGeom {
     int BM, BN, BK, threads;
@@ -66,11 +73,21 @@ TileGeom geom_of(int cfg) {
     }
 }
 
+// The bucket whose tile shape gives the least padded M x N~ area per unit of
+// efficiency (K is padded alike by every shape).  Ties go to the smaller tile.
 int bucket_of(int64_t M, int64_t Nx) {
-    const int64_t x = std::max(M, Nx);
-    for (int b = 0; b < NUM_BUCKETS - 1; ++b)
-        if (x <= BUCKET_LIMIT[b]) return b;
-    return NUM_BUCKETS - 1;
+    int best = 0;
+    double best_cost = 0.0;
+    for (int b = 0; b < NUM_BUCKETS; ++b) {
+        const double pm = (double)((M + BUCKET_BM[b] - 1) / BUCKET_BM[b] * BUCKET_BM[b]);
+        const double pn = (double)((Nx + BUCKET_BN[b] - 1) / BUCKET_BN[b] * BUCKET_BN[b]);
+        const double cost = pm * pn / BUCKET_EFF[b];
+        if (b == 0 || cost < best_cost) {
+            best = b;
+            best_cost = cost;
+        }
+    }
+    return best;
 }
 
 // Tuner cache: key "arch kinds bucket" -> cfg.

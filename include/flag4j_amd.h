@@ -180,8 +180,9 @@ int f4j_zgemv_dev(int64_t M, int64_t K, const double *A, int64_t lda, const doub
  * dispatcher lookup and a pool fork/join (M:concurrency/ThreadManager.java:155-183).
  * A plan fixes `count` independent products C_i (M_i x N_i) = op(A_i) op(B_i), all with the
  * same op codes, on DEVICE pointers (complex: interleaved, sizes in complex elements).  It
- * schedules their tiles over persistent launches (one per size bucket, tile shape per bucket
- * from the autotuner cache) and captures them into a hipGraph; f4j_grouped_plan_run replays the
+ * schedules their tiles over persistent launches (one per tile-shape bucket: each product goes
+ * to the shape that pads it least per unit of efficiency; tile configuration per bucket from the
+ * autotuner cache) and captures them into a hipGraph; f4j_grouped_plan_run replays the
  * graph on `stream` (asynchronous).  The operand pointers are baked into the plan: rerun it after
  * refilling the same buffers.  Precision: F4J_PREC_FP64 (or AUTO, which means FP64 here). */
 typedef struct f4j_grouped_plan f4j_grouped_plan;
@@ -208,7 +209,7 @@ int f4j_grouped_plan_create_split(int is_complex, int opA, int opB, int64_t coun
 int f4j_grouped_plan_split_report(f4j_grouped_plan *plan, double *err, int64_t count, int *products);
 int f4j_grouped_plan_run(f4j_grouped_plan *plan, void *stream);
 int f4j_grouped_plan_destroy(f4j_grouped_plan *plan);
-/* Work of the plan (flops, classical count) and, per size bucket (nb <= 4 entries), the tile
+/* Work of the plan (flops, classical count) and, per shape bucket (nb <= 4 entries), the tile
  * configuration id and the number of tiles. */
 int f4j_grouped_plan_info(const f4j_grouped_plan *plan, double *flops, int *tile_cfg, int64_t *tiles, int nb);
 /* Autotuner: time every candidate tile configuration per bucket on this plan's own problems
