@@ -61,11 +61,21 @@ struct CrtOutput {
     bool first = true;
 };
 
+// A second stream for B's exponents and residues: `fork` is recorded on the
+// main stream and waited for by `stream`, `join` recorded after B's split and
+// waited for by the main stream before the products.
+struct CrtSide {
+    hipStream_t stream;
+    hipEvent_t fork, join;
+};
+
 // Enqueue exponents, residue slices, the n residue products and the FP64
 // reconstruction into C.  `ws` holds plan.bytes bytes.
 // ev_begin / ev_end (optional) are recorded on `stream` around the product launch.
+// With `side`, B's exponents and residues run on side->stream beside A's.
 hipError_t launch_crt_gemm(const CrtPlan &pl, const CrtOperand &oa, const CrtOperand &ob, int64_t M, int64_t N,
                            int64_t K, double *C, int64_t ldc, const CrtOutput &out, void *ws, hipStream_t stream,
-                           hipEvent_t ev_begin = nullptr, hipEvent_t ev_end = nullptr);
+                           hipEvent_t ev_begin = nullptr, hipEvent_t ev_end = nullptr,
+                           const CrtSide *side = nullptr);
 
 }  // namespace f4j

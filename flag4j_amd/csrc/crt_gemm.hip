@@ -621,12 +621,12 @@ CrtOperand crt_operand(const double *X, int64_t ld, bool lines, bool cplx, doubl
 
 hipError_t launch_crt_gemm(const CrtPlan &pl, const CrtOperand &oa, const CrtOperand &ob, int64_t M, int64_t N,
                            int64_t K, double *C, int64_t ldc, const CrtOutput &out, void *ws, hipStream_t stream,
-                           hipEvent_t ev_begin, hipEvent_t ev_end) {
+                           hipEvent_t ev_begin, hipEvent_t ev_end, const CrtSide *side) {
     char *w = (char *)ws;
     signed char *ra = (signed char *)(w + pl.off_ra), *rb = (signed char *)(w + pl.off_rb);
     unsigned char *res = (unsigned char *)(w + pl.off_res);
     int *ea = (int *)(w + pl.off_ea), *eb = (int *)(w + pl.off_eb);
-    auto exps = [&](const CrtOperand &o, int64_t nx, int *e) {
+    auto exps = [&](const CrtOperand &o, int64_t nx, int *e, hipStream_t stream) {
         if (nx <= 0) return;
         if (o.lines) {
             hipLaunchKernelGGL(crt_exp_lines_kernel, dim3((unsigned)nx), dim3(256), 0, stream, o, K, e);
@@ -637,10 +637,9 @@ hipError_t launch_crt_gemm(const CrtPlan &pl, const CrtOperand &oa, const CrtOpe
                                dim3(256), 0, stream, o, nx, K, kchunk, e);
         }
     };
-    exps(oa, M, ea);
-    exps(ob, N, eb);
     const bool wide = pl.tb.t > 58;
-    auto split = [&](const CrtOperand &o, int64_t nx, int64_t Xp, const int *ex, signed char *dst) {
+    auto split = [&](const CrtOperand &o, int64_t nx, int64_t Xp, const int *ex, signed char *dst,
+                     hipStream_t stream) {
         const dim3 grid((unsigned)(pl.Kp / 128), (unsigned)(Xp / 32));
         if (o.lines) {
             if (wide)
@@ -659,8 +658,22 @@ hipError_t launch_crt_gemm(const CrtPlan &pl, const CrtOperand &oa, const CrtOpe
                                    pl.Kp, pl.tb);
         }
     };
-    split(oa, M, pl.Mp, ea, ra);
-    split(ob, N, pl.Np, eb, rb);
+    hipStream_t sb = stream;
+    if (side) {
+        sb = side->stream;
+        if (hipError_t e = hipEventRecord(side->fork, stream); e != hipSuccess) return e;
+        if (hipError_t e = hipStreamWaitEvent(sb, side->fork, 0); e != hipSuccess) return e;
+    }
+    exps(ob, N, eb, sb);
+    split(ob, N, pl.Np, eb, rb, sb);
+    if (side) {
+        if (hipError_t e = hipEventRecord(side->join, sb); e != hipSuccess) return e;
+    }
+    exps(oa, M, ea, stream);
+    split(oa, M, pl.Mp, ea, ra, stream);
+    if (side) {
+        if (hipError_t e = hipStreamWaitEvent(stream, side->join, 0); e != hipSuccess) return e;
+    }
     CrtGemmArgs a{ra, rb, pl.Mp, pl.Np, pl.Kp, M, N, res, pl.plane};
     if (ev_begin) (void)hipEventRecord(ev_begin, stream);
     hipLaunchKernelGGL(crt_gemm_kernel, dim3((unsigned)((pl.Mp / CBM) * (pl.Np / CBN)), (unsigned)pl.tb.n), dim3(256),

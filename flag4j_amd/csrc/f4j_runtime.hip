@@ -64,8 +64,9 @@ struct State {
     std::mutex mix_mu;                // guards mixws (planes of complex x real products)
     DevBuf mixws;
     hipEvent_t tev[8] = {};                 // product timing: a begin / end pair per real product (up to 4)
-    hipStream_t side = nullptr;             // residual sums beside the modular products (guarded by ws_mu)
-    hipEvent_t fork_ev = nullptr, join_ev = nullptr;
+    hipStream_t side = nullptr;             // B's split and the residual sums beside A's split and the
+                                            // modular products (guarded by ws_mu)
+    hipEvent_t fork_ev = nullptr, join_ev = nullptr, split_fork_ev = nullptr, split_join_ev = nullptr;
 };
 
 State &S() {
@@ -106,6 +107,8 @@ int init_locked(State &s, int device) {
     F4J_HIP(hipStreamCreateWithFlags(&s.side, hipStreamNonBlocking), "hipStreamCreate");
     F4J_HIP(hipEventCreateWithFlags(&s.fork_ev, hipEventDisableTiming), "hipEventCreate");
     F4J_HIP(hipEventCreateWithFlags(&s.join_ev, hipEventDisableTiming), "hipEventCreate");
+    F4J_HIP(hipEventCreateWithFlags(&s.split_fork_ev, hipEventDisableTiming), "hipEventCreate");
+    F4J_HIP(hipEventCreateWithFlags(&s.split_join_ev, hipEventDisableTiming), "hipEventCreate");
     s.device = device;
     s.inited = true;
     return F4J_OK;
@@ -394,16 +397,7 @@ int run_crt(bool cplx, int opA, int opB, int64_t M, int64_t N, int64_t K, const 
             if (side) (void)hipStreamSynchronize(side);
         }
     } side_idle;
-    if (check_rows > 0) {
-        // The check's compensated sums need only A and B: they run on the side
-        // stream while the residue products run on this one.
-        side_idle.side = s.side;
-        F4J_HIP(hipEventRecord(s.fork_ev, stream), "hipEventRecord");
-        F4J_HIP(hipStreamWaitEvent(s.side, s.fork_ev, 0), "hipStreamWaitEvent");
-        F4J_HIP(launch_residual_partial(cplx, opA, opB, M, N, K, A, lda, B, ldb, check_rows, part, s.side),
-                "crt residual launch");
-        F4J_HIP(hipEventRecord(s.join_ev, s.side), "hipEventRecord");
-    }
+    side_idle.side = s.side;
     for (size_t jb = 0; jb < jobs.size(); ++jb) {
         const Job &j = jobs[jb];
         const CrtOperand oa = crt_operand(A, lda, opA == F4J_OP_N, cplx, j.ca_a, j.cb_a);
@@ -413,9 +407,19 @@ int run_crt(bool cplx, int opA, int opB, int64_t M, int64_t N, int64_t K, const 
         out.cre = j.cre;
         out.cim = j.cim;
         out.first = jb == 0;
+        // B's exponents and residues on the side stream, beside A's on this one
+        const CrtSide side{s.side, s.split_fork_ev, s.split_join_ev};
         F4J_HIP(launch_crt_gemm(pl, oa, ob, M, N, K, C, ldc, out, s.ws.p, stream, timing ? s.tev[2 * jb] : nullptr,
-                                timing ? s.tev[2 * jb + 1] : nullptr),
+                                timing ? s.tev[2 * jb + 1] : nullptr, &side),
                 "crt gemm launch");
+        if (jb == 0 && check_rows > 0) {
+            // The check's compensated sums need only A and B: they follow B's
+            // split on the side stream (which has waited for the inputs) and run
+            // beside the residue products.
+            F4J_HIP(launch_residual_partial(cplx, opA, opB, M, N, K, A, lda, B, ldb, check_rows, part, s.side),
+                    "crt residual launch");
+            F4J_HIP(hipEventRecord(s.join_ev, s.side), "hipEventRecord");
+        }
     }
     f4j_split_report rep{};
     rep.product_ms = -1.0;
@@ -805,7 +809,7 @@ int f4j_shutdown(void) {
             F4J_HIP(hipEventDestroy(e), "hipEventDestroy");
             e = nullptr;
         }
-    for (hipEvent_t *e : {&s.fork_ev, &s.join_ev})
+    for (hipEvent_t *e : {&s.fork_ev, &s.join_ev, &s.split_fork_ev, &s.split_join_ev})
         if (*e) {
             F4J_HIP(hipEventDestroy(*e), "hipEventDestroy");
             *e = nullptr;

@@ -569,7 +569,7 @@ hipError_t split_batch_enqueue(const SplitBatch *b, hipStream_t st, hipEvent_t e
     else
         launch_bsplit<9, __bf16>(b->S, dim3((unsigned)b->n_split), st, b->d_sides, b->d_split_work, b->d_exps,
                                  (__bf16 *)b->d_digits);
-    if (ev_begin) hipEventRecord(ev_begin, st);
+    if (ev_begin) (void)hipEventRecord(ev_begin, st);
     if (fp16)
         hipLaunchKernelGGL((bgemm_kernel<_Float16, 64>), dim3((unsigned)b->workers), dim3(256), 0, st, b->d_items,
                            b->d_begin, b->d_probs, b->d_jobs, b->d_sides, b->d_groups, (const _Float16 *)b->d_digits,
@@ -578,7 +578,7 @@ hipError_t split_batch_enqueue(const SplitBatch *b, hipStream_t st, hipEvent_t e
         hipLaunchKernelGGL((bgemm_kernel<__bf16, 256>), dim3((unsigned)b->workers), dim3(256), 0, st, b->d_items,
                            b->d_begin, b->d_probs, b->d_jobs, b->d_sides, b->d_groups, (const __bf16 *)b->d_digits,
                            b->d_exps, (int)b->cplx);
-    if (ev_end) hipEventRecord(ev_end, st);
+    if (ev_end) (void)hipEventRecord(ev_end, st);
     if (b->n_res > 0) {
         const int64_t np = (int64_t)b->prob_index.size();
         (void)np;

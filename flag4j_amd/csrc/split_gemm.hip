@@ -439,7 +439,8 @@ hipError_t launch_split_gemm(const SplitPlan &pl, int opA, int opB, int64_t M, i
     const int S = pl.S;
     const bool fp16 = pl.carrier == SPLIT_FP16;
 
-    launch_exponents(opA, opB, M, N, K, A, lda, B, ldb, ea, eb, stream);
+    if (hipError_t e = launch_exponents(opA, opB, M, N, K, A, lda, B, ldb, ea, eb, stream); e != hipSuccess)
+        return e;
 
     // digit slices
     auto split = [&](const double *X, int64_t ld, bool lines, int64_t nx, int64_t Xp, const int *e, void *out) {
@@ -454,7 +455,7 @@ hipError_t launch_split_gemm(const SplitPlan &pl, int opA, int opB, int64_t M, i
     if (err != hipSuccess) return err;
 
     // slice products, smallest weights first
-    if (ev_begin) hipEventRecord(ev_begin, stream);
+    if (ev_begin) (void)hipEventRecord(ev_begin, stream);
     bool first = true;
     for (const SplitGroup &g : pl.groups) {
         SliceGemmArgs a{};
@@ -484,7 +485,7 @@ hipError_t launch_split_gemm(const SplitPlan &pl, int opA, int opB, int64_t M, i
             hipLaunchKernelGGL((slice_gemm_kernel<__bf16, 256>), dim3(blocks), dim3(256), 0, stream, a);
         first = false;
     }
-    if (ev_end) hipEventRecord(ev_end, stream);
+    if (ev_end) (void)hipEventRecord(ev_end, stream);
     return hipGetLastError();
 }
 
