@@ -313,21 +313,24 @@ typedef __attribute__((address_space(3))) void lds_void;
 // vmcnt(n) only (gfx9 encoding: expcnt and lgkmcnt at their maxima).
 #define F4J_WAIT_VMCNT(n) __builtin_amdgcn_s_waitcnt(((n) & 15) | (7 << 4) | (15 << 8) | (((n) >> 4) << 14))
 
-// Block tile 256 x 128, four waves of 64 x 128 (2 x 4 MFMA blocks, 128 int32
-// accumulators per lane).  Each wave reads its own 64 rows of A straight into
-// registers (no other wave needs them); B is shared through a three-stage LDS
-// ring filled by buffer loads that write LDS directly.  B runs two 128-byte k
-// slices ahead of the MFMAs and A one, with one barrier per slice.  With the
-// fragment-major layout every load (and every LDS fragment read) moves 1 KB
-// of contiguous memory.
-__global__ __launch_bounds__(256, 2) void crt_gemm_kernel(CrtGemmArgs a, CrtTables tb) {
+// Block tile 256 x 256, eight waves of 64 x 128 (2 x 4 MFMA blocks, 128 int32
+// accumulators per lane): wave w takes row group w % 4 and column half w / 4.
+// Both operands reach LDS by buffer loads that write LDS directly, each
+// fragment once per block (waves 0-3 load A's, 4-7 B's), through a ring of
+// STAGES steps of SK k bytes, AHEAD steps ahead of the MFMAs, one barrier per
+// step.  Loading A once per block rather than once per wave into registers
+// takes a third of the bytes off the CU's vector-memory return path, which
+// was the busiest unit; gfx950's LDS carries the extra fragment reads.  With
+// the fragment-major layout every load and every fragment read moves 1 KB of
+// contiguous memory.
+__global__ __launch_bounds__(512, 2) void crt_gemm_kernel(CrtGemmArgs a, CrtTables tb) {
     __shared__ __attribute__((aligned(1024))) signed char lds[crt::STAGES * crt::STAGE];
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);   // uniform: keeps descriptors scalar
     const int l = blockIdx.y;
     int64_t tm, tn;
     tile_of(blockIdx.x, a.Mp / CBM, a.Np / CBN, tm, tn);
-    const int64_t m0 = (int64_t)__builtin_amdgcn_readfirstlane((int)tm) * CBM + wave * 64;
+    const int64_t m0 = (int64_t)__builtin_amdgcn_readfirstlane((int)tm) * CBM;
     const int64_t n0 = (int64_t)__builtin_amdgcn_readfirstlane((int)tn) * CBN;
 
     i32x16 acc[2][4];
@@ -338,90 +341,74 @@ __global__ __launch_bounds__(256, 2) void crt_gemm_kernel(CrtGemmArgs a, CrtTabl
 #pragma unroll
             for (int r = 0; r < 16; ++r) acc[i][j][r] = 0;
 
-    // Offsets from crt_layout.h: slice t is k blocks 4 t .. 4 t + 3.
     const int KC = (int)(a.Kp >> 5);
-    const signed char *pa = a.A + (int64_t)l * a.Mp * a.Kp;
-    const signed char *pb = a.B + (int64_t)l * a.Np * a.Kp;
-    const int nt = (int)(a.Kp / CBK);
+    // this wave's operand: A's slice (waves 0-3) or B's (waves 4-7) of modulus l
+    const bool is_a = wave < 4;
+    const signed char *src = is_a ? a.A + (int64_t)l * a.Mp * a.Kp : a.B + (int64_t)l * a.Np * a.Kp;
+    const int64_t x0 = is_a ? m0 : n0;
+    const int nt = (int)(a.Kp / crt::SK);
 
-    i32x4 ra0[2][4], ra1[2][4];
-    auto issue_a = [&](int t, i32x4 (&dst)[2][4]) {
-        const __amdgpu_buffer_rsrc_t rsa = __builtin_amdgcn_make_buffer_rsrc(
-            (void *)(pa + crt::a_base(m0, t, KC)), (short)0, 0x7FFFFFF0, 0x00020000);
+    auto issue = [&](int slot, int t) {          // step t into stage slot % STAGES: LOADS fragments per wave
+        const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+            (void *)(src + crt::load_base(x0, wave, t, KC)), (short)0, 0x7FFFFFF0, 0x00020000);
 #pragma unroll
-        for (int i = 0; i < 2; ++i)
-#pragma unroll
-            for (int ks = 0; ks < 4; ++ks)
-                dst[i][ks] = __builtin_bit_cast(
-                    i32x4, __builtin_amdgcn_raw_buffer_load_b128(rsa, crt::a_voff(ks, lane), crt::a_soff(i, KC), 0));
+        for (int p = 0; p < crt::LOADS; ++p)
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, (lds_void *)(lds + crt::load_lds(slot, wave, p)), 16,
+                                                     crt::load_voff(lane), crt::load_soff(p, KC), 0, 0);
     };
-    auto issue_b = [&](int slot, int t) {        // slice t into stage slot % 3; wave w loads column block w
-        const __amdgpu_buffer_rsrc_t rsb = __builtin_amdgcn_make_buffer_rsrc(
-            (void *)(pb + crt::b_base(n0, t, KC)), (short)0, 0x7FFFFFF0, 0x00020000);
+    // Every step issues the same LOADS loads (past the last step they repeat
+    // step nt - 1 into the free stage), so at the top of step t the
+    // LOADS (AHEAD - 1) most recent loads are steps t + 1 .. t + AHEAD - 1 and
+    // that vmcnt retires step t.
 #pragma unroll
-        for (int p = 0; p < 4; ++p)
-            __builtin_amdgcn_raw_ptr_buffer_load_lds(rsb, (lds_void *)(lds + crt::b_lds(slot, wave, p)), 16,
-                                                     crt::b_voff(lane), crt::b_soff(wave, p, KC), 0, 0);
-    };
-    // Issue order: B(0), B(1), A(0), then per step t: B(t + 2) (4 loads) and
-    // A(t + 1) (8 loads).  Past the last slice the loads repeat slice nt - 1
-    // into the free stage and the unused register set, so every step issues
-    // the same 12 loads: the compiler's wait counts then stay exact (a load
-    // issued only on some paths makes it wait for the newest loads), and at
-    // the top of step t the 12 most recent loads are B(t + 1) and A(t), so
-    // vmcnt(12) retires B(t) and everything before it.
-    auto step = [&](int t, i32x4 (&cur)[2][4], i32x4 (&nxt)[2][4]) {
+    for (int t = 0; t < crt::AHEAD; ++t) issue(t, min(t, nt - 1));
+    for (int t = 0; t < nt; ++t) {
         __builtin_amdgcn_sched_barrier(0);
-        F4J_WAIT_VMCNT(12);
+        F4J_WAIT_VMCNT(crt::LOADS * (crt::AHEAD - 1));
         __builtin_amdgcn_s_waitcnt(0xC07F);      // lgkmcnt(0)
-        __builtin_amdgcn_s_barrier();
+        __builtin_amdgcn_s_barrier();            // every wave's step-t loads are in LDS; stage t - 1 is free
         __builtin_amdgcn_sched_barrier(0);
-        issue_b(t + 2, min(t + 2, nt - 1));
-        issue_a(min(t + 1, nt - 1), nxt);
+        issue(t + crt::AHEAD, min(t + crt::AHEAD, nt - 1));
         __builtin_amdgcn_sched_barrier(0);
-        i32x4 fb[2][4];
+        i32x4 fa[2][2], fb[2][4];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) fb[0][j] = *(const i32x4 *)(lds + crt::b_frag(t, j, 0, lane));
+        for (int i = 0; i < 2; ++i) fa[0][i] = *(const i32x4 *)(lds + crt::a_frag(t, wave, i, 0, lane));
 #pragma unroll
-        for (int ks = 0; ks < CBK / 32; ++ks) {
-            if (ks + 1 < CBK / 32) {
+        for (int j = 0; j < 4; ++j) fb[0][j] = *(const i32x4 *)(lds + crt::b_frag(t, wave, j, 0, lane));
+#pragma unroll
+        for (int ks = 0; ks < crt::KB; ++ks) {
+            if (ks + 1 < crt::KB) {
+#pragma unroll
+                for (int i = 0; i < 2; ++i)
+                    fa[(ks + 1) & 1][i] = *(const i32x4 *)(lds + crt::a_frag(t, wave, i, ks + 1, lane));
 #pragma unroll
                 for (int j = 0; j < 4; ++j)
-                    fb[(ks + 1) & 1][j] = *(const i32x4 *)(lds + crt::b_frag(t, j, ks + 1, lane));
+                    fb[(ks + 1) & 1][j] = *(const i32x4 *)(lds + crt::b_frag(t, wave, j, ks + 1, lane));
             }
-            __builtin_amdgcn_sched_barrier(0);   // next step's fragment reads ahead of this step's MFMAs
+            __builtin_amdgcn_sched_barrier(0);   // next k step's fragment reads ahead of these MFMAs
 #pragma unroll
             for (int i = 0; i < 2; ++i)
 #pragma unroll
                 for (int j = 0; j < 4; ++j)
-                    acc[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(cur[i][ks], fb[ks & 1][j], acc[i][j], 0, 0, 0);
+                    acc[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[ks & 1][i], fb[ks & 1][j], acc[i][j], 0, 0, 0);
         }
-    };
-
-    issue_b(0, 0);
-    issue_b(1, min(1, nt - 1));
-    issue_a(0, ra0);
-    int t = 0;
-    for (; t + 1 < nt; t += 2) {
-        step(t, ra0, ra1);
-        step(t + 1, ra1, ra0);
     }
-    if (t < nt) step(t, ra0, ra1);
     F4J_WAIT_VMCNT(0);                            // no LDS write in flight when the block ends
 
     // acc mod m (crt::product_residue: in [0, m], m kept for exact multiples)
     const double m = (double)tb.m[l];
     const double minv = tb.minv[l];
     unsigned char *R = a.R + (int64_t)l * a.plane;
+    const int64_t rw = m0 + crt::wave_row(wave), cw = n0 + crt::wave_col(wave);
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            const int64_t col = n0 + crt::acc_col(j, lane);
+            const int64_t col = cw + crt::acc_col(j, lane);
             if (col >= a.N) continue;
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
-                const int64_t row = m0 + crt::acc_row(i, r, lane);
+                const int64_t row = rw + crt::acc_row(i, r, lane);
                 if (row >= a.M) continue;
                 R[row * a.N + col] = (unsigned char)crt::product_residue(acc[i][j][r], m, minv);
             }
@@ -676,8 +663,8 @@ hipError_t launch_crt_gemm(const CrtPlan &pl, const CrtOperand &oa, const CrtOpe
     }
     CrtGemmArgs a{ra, rb, pl.Mp, pl.Np, pl.Kp, M, N, res, pl.plane};
     if (ev_begin) (void)hipEventRecord(ev_begin, stream);
-    hipLaunchKernelGGL(crt_gemm_kernel, dim3((unsigned)((pl.Mp / CBM) * (pl.Np / CBN)), (unsigned)pl.tb.n), dim3(256),
-                       0, stream, a, pl.tb);
+    hipLaunchKernelGGL(crt_gemm_kernel, dim3((unsigned)((pl.Mp / CBM) * (pl.Np / CBN)), (unsigned)pl.tb.n),
+                       dim3(64 * crt::WAVES), 0, stream, a, pl.tb);
     if (ev_end) (void)hipEventRecord(ev_end, stream);
     const int64_t threads = (M * N + RV - 1) / RV;
     hipLaunchKernelGGL(crt_reconstruct_kernel, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, stream, res,

@@ -19,11 +19,16 @@
 namespace f4j {
 namespace crt {
 
-constexpr int BM = 256, BN = 128, BK = 128;   // block tile (rows, columns); k slice in bytes
+constexpr int BM = 256, BN = 256;             // product block tile (rows, columns)
+constexpr int BK = 128;                       // k padding unit of the residue slices, bytes
 constexpr int FRAG = 1024;                    // bytes of one 32 x 32 fragment
-constexpr int SLICE = (BK / 32) * FRAG;       // bytes of one line block in one k slice
-constexpr int STAGES = 3;                     // B slices in flight through LDS
-constexpr int STAGE = BN * BK;                // LDS bytes of one B stage (4 column blocks x 4 k steps)
+constexpr int WAVES = 8;                      // product block: 4 row groups of 64 x 2 column halves of 128
+constexpr int SK = 128;                       // k bytes per product step
+constexpr int KB = SK / 32;                   // 32-byte MFMA k steps (k blocks) per product step
+constexpr int STAGES = 2;                     // steps in the LDS ring
+constexpr int AHEAD = STAGES - 1;             // steps in flight ahead of the MFMAs
+constexpr int LOADS = 2 * KB;                 // fragment loads per wave per step
+constexpr int STAGE = (BM + BN) * SK;         // LDS bytes of one step: its A fragments, then its B fragments
 
 F4J_HD int64_t frag_offset(int64_t x, int64_t k, int64_t KC) {
     return (((x >> 5) * KC + (k >> 5)) << 10) + (((x & 31) + ((k >> 4) & 1) * 32) << 4) + (k & 15);
@@ -40,22 +45,32 @@ F4J_HD int split_k(int t) { return 16 * ((t >> 5) & 1) + 32 * (t >> 6); }
 F4J_HD int split_cols_line(int t) { return t & 127; }
 F4J_HD int split_cols_k(int t) { return 16 * (t >> 7); }
 
-// Product kernel.  Wave w of a block owns rows m0w = tile_m BM + 64 w .. + 63
-// (register sets i = 0, 1 of 32 rows); the block's columns are n0 .. n0 + 127.
-// A load for k slice t, set i, step ks: uniform base + soffset + per-lane voffset.
-F4J_HD int64_t a_base(int64_t m0w, int t, int KC) { return (((m0w >> 5) * KC) << 10) + (int64_t)t * SLICE; }
-F4J_HD int a_soff(int i, int KC) { return (i * KC) << 10; }
-F4J_HD int a_voff(int ks, int lane) { return FRAG * ks + 16 * lane; }
-// B: wave w loads column block w of slice t (4 fragments, p = k step) into
-// LDS stage slot % STAGES; the hardware writes lane L's 16 bytes at +16 L.
-F4J_HD int64_t b_base(int64_t n0, int t, int KC) { return (((n0 >> 5) * KC) << 10) + (int64_t)t * SLICE; }
-F4J_HD int b_soff(int w, int p, int KC) { return ((w * KC) << 10) + FRAG * p; }
-F4J_HD int b_voff(int lane) { return 16 * lane; }
-F4J_HD int b_lds(int slot, int w, int p) { return (slot % STAGES) * STAGE + 4 * FRAG * w + FRAG * p; }
-// Fragment read of column block j, k step ks, from stage slot.
-F4J_HD int b_frag(int slot, int j, int ks, int lane) {
-    return (slot % STAGES) * STAGE + 4 * FRAG * j + FRAG * ks + 16 * lane;
+// Product kernel.  A block computes a 256 x 256 tile.  Every step t (k bytes
+// SK t .. SK t + SK - 1) both operands' fragments go straight from the residue
+// slices into LDS stage t % STAGES, each fragment loaded once per block:
+// stage fragment g < 8 KB is A's line block g / KB, k block g % KB (of the
+// step); g >= 8 KB is B's line block (g - 8 KB) / KB, k block g % KB.  Wave w
+// loads fragments LOADS w .. LOADS w + LOADS - 1 (waves 0-3 from A, 4-7 from B):
+// line blocks 2 (w % 4) + p / KB and k blocks p % KB of its operand, by buffer
+// loads that write lane L's 16 bytes at +16 L.
+// Base (bytes into the operand's slice, 64-bit) of wave w's loads at step t for a block
+// whose operand lines start at x0 (m0 for A, n0 for B), and the per-load offset.
+F4J_HD int64_t load_base(int64_t x0, int w, int t, int KC) {
+    return ((((x0 >> 5) + 2 * (w & 3)) * KC + KB * (int64_t)t) << 10);
 }
+F4J_HD int load_soff(int p, int KC) { return (((p / KB) * KC) + (p % KB)) << 10; }
+F4J_HD int load_voff(int lane) { return 16 * lane; }
+F4J_HD int load_lds(int slot, int w, int p) { return (slot % STAGES) * STAGE + (LOADS * w + p) * FRAG; }
+// Wave w computes rows 64 (w % 4) .. + 63 (line blocks i = 0, 1) and columns
+// 128 (w / 4) .. + 127 (line blocks j = 0..3) of the tile.  Fragment reads at k step ks (0 .. KB - 1):
+F4J_HD int a_frag(int slot, int w, int i, int ks, int lane) {
+    return (slot % STAGES) * STAGE + ((2 * (w & 3) + i) * KB + ks) * FRAG + 16 * lane;
+}
+F4J_HD int b_frag(int slot, int w, int j, int ks, int lane) {
+    return (slot % STAGES) * STAGE + (8 * KB + (4 * (w >> 2) + j) * KB + ks) * FRAG + 16 * lane;
+}
+F4J_HD int wave_row(int w) { return 64 * (w & 3); }
+F4J_HD int wave_col(int w) { return 128 * (w >> 2); }
 // MFMA result item r of `lane` in block (i, j): row and column within the wave tile.
 F4J_HD int acc_row(int i, int r, int lane) { return 32 * i + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3); }
 F4J_HD int acc_col(int j, int lane) { return 32 * j + (lane & 31); }

@@ -70,7 +70,7 @@ int main(int argc, char **argv) {
     const int m = 251;
     const int64_t Mp = (M + crt::BM - 1) / crt::BM * crt::BM, Np = (N + crt::BN - 1) / crt::BN * crt::BN;
     const int64_t Kp = (K + crt::BK - 1) / crt::BK * crt::BK;
-    const int KC = (int)(Kp / 32), nt = (int)(Kp / crt::BK);
+    const int KC = (int)(Kp / 32);
 
     std::mt19937 g(seed);
     std::uniform_int_distribution<int> d(-128, 127);
@@ -82,6 +82,7 @@ int main(int argc, char **argv) {
 
     std::vector<int> R(M * N, -1), Rhits(M * N, 0);
     const int64_t tiles_m = Mp / crt::BM, tiles_n = Np / crt::BN;
+    const int nt = (int)(Kp / crt::SK);
     std::vector<int> tile_hits(tiles_m * tiles_n, 0);
     std::vector<signed char> lds(crt::STAGES * crt::STAGE);
     std::vector<int> tag(crt::STAGES * crt::STAGE, -1);
@@ -91,49 +92,47 @@ int main(int argc, char **argv) {
         tile_of(bid, tiles_m, tiles_n, tm, tn);
         CHECK(tm >= 0 && tm < tiles_m && tn >= 0 && tn < tiles_n, "tile out of range");
         ++tile_hits[tm * tiles_n + tn];
-        const int64_t n0 = tn * crt::BN;
+        const int64_t m0 = tm * crt::BM, n0 = tn * crt::BN;
         std::fill(tag.begin(), tag.end(), -1);
-        auto issue_b = [&](int slot, int t) {
-            for (int w = 0; w < 4; ++w)
-                for (int p = 0; p < 4; ++p)
+        // every wave's 4 loads of step t into stage slot (waves 0-3 from A, 4-7 from B)
+        auto issue = [&](int slot, int t) {
+            for (int w = 0; w < crt::WAVES; ++w) {
+                const bool is_a = w < 4;
+                const std::vector<signed char> &src = is_a ? Ar : Br;
+                const int64_t x0 = is_a ? m0 : n0, size = is_a ? Mp * Kp : Np * Kp;
+                for (int p = 0; p < crt::LOADS; ++p)
                     for (int lane = 0; lane < 64; ++lane)
                         for (int v = 0; v < 16; ++v) {
-                            const int64_t src = crt::b_base(n0, t, KC) + crt::b_soff(w, p, KC) + crt::b_voff(lane) + v;
-                            const int dst = crt::b_lds(slot, w, p) + 16 * lane + v;
-                            CHECK(src >= 0 && src < Np * Kp, "B load out of bounds");
+                            const int64_t off = crt::load_base(x0, w, t, KC) + crt::load_soff(p, KC) +
+                                                crt::load_voff(lane) + v;
+                            const int dst = crt::load_lds(slot, w, p) + 16 * lane + v;
+                            CHECK(off >= 0 && off < size, "%c load out of bounds", is_a ? 'A' : 'B');
                             CHECK(dst >= 0 && dst < crt::STAGES * crt::STAGE, "LDS write out of bounds");
-                            if (src < 0 || src >= Np * Kp || dst < 0 || dst >= crt::STAGES * crt::STAGE) continue;
-                            lds[dst] = Br[src];
+                            if (off < 0 || off >= size || dst < 0 || dst >= crt::STAGES * crt::STAGE) continue;
+                            lds[dst] = src[off];
                             tag[dst] = t;
                         }
+            }
         };
-        // acc[w][i][j][lane][r]
-        std::vector<long long> acc(4 * 2 * 4 * 64 * 16, 0);
-        auto A_at = [&](int w, int i, int t, int ks, int lane, int v) -> int {
-            const int64_t m0w = tm * crt::BM + 64 * w;
-            const int64_t off = crt::a_base(m0w, t, KC) + crt::a_soff(i, KC) + crt::a_voff(ks, lane) + v;
-            CHECK(off >= 0 && off < Mp * Kp, "A load out of bounds");
-            return (off >= 0 && off < Mp * Kp) ? Ar[off] : 0;
-        };
-        auto B_at = [&](int t, int j, int ks, int lane, int v) -> int {
-            const int off = crt::b_frag(t, j, ks, lane) + v;
+        auto read = [&](int off, int t) -> int {
             CHECK(off >= 0 && off < crt::STAGES * crt::STAGE, "LDS read out of bounds");
             if (off < 0 || off >= crt::STAGES * crt::STAGE) return 0;
-            CHECK(tag[off] == t, "LDS read of slice %d finds slice %d", t, tag[off]);
+            CHECK(tag[off] == t, "LDS read of step %d finds step %d", t, tag[off]);
             return lds[off];
         };
-        issue_b(0, 0);
-        issue_b(1, std::min(1, nt - 1));
+        // acc[w][i][j][lane][r]
+        std::vector<long long> acc(crt::WAVES * 2 * 4 * 64 * 16, 0);
+        for (int t = 0; t < crt::AHEAD; ++t) issue(t, std::min(t, nt - 1));
         for (int t = 0; t < nt; ++t) {
-            issue_b(t + 2, std::min(t + 2, nt - 1));    // into the stage no wave reads in step t
-            for (int w = 0; w < 4; ++w)
-                for (int ks = 0; ks < 4; ++ks) {
+            issue(t + crt::AHEAD, std::min(t + crt::AHEAD, nt - 1));   // into the stage no wave reads in step t
+            for (int w = 0; w < crt::WAVES; ++w)
+                for (int ks = 0; ks < crt::KB; ++ks) {
                     // operands as the lanes hold them
                     int a[2][64][16], b[4][64][16];
                     for (int lane = 0; lane < 64; ++lane)
                         for (int v = 0; v < 16; ++v) {
-                            for (int i = 0; i < 2; ++i) a[i][lane][v] = A_at(w, i, t, ks, lane, v);
-                            for (int j = 0; j < 4; ++j) b[j][lane][v] = B_at(t, j, ks, lane, v);
+                            for (int i = 0; i < 2; ++i) a[i][lane][v] = read(crt::a_frag(t, w, i, ks, lane) + v, t);
+                            for (int j = 0; j < 4; ++j) b[j][lane][v] = read(crt::b_frag(t, w, j, ks, lane) + v, t);
                         }
                     // documented MFMA semantics
                     for (int i = 0; i < 2; ++i)
@@ -141,23 +140,23 @@ int main(int argc, char **argv) {
                             for (int lane = 0; lane < 64; ++lane)
                                 for (int r = 0; r < 16; ++r) {
                                     const int row = 8 * (r / 4) + 4 * (lane / 32) + r % 4, col = lane % 32;
-                                    long long s = 0;
+                                    long long sum = 0;
                                     for (int kk = 0; kk < 32; ++kk) {
                                         const int la = row + 32 * (kk / 16), lb = col + 32 * (kk / 16);
-                                        s += (long long)a[i][la][kk % 16] * b[j][lb][kk % 16];
+                                        sum += (long long)a[i][la][kk % 16] * b[j][lb][kk % 16];
                                     }
-                                    acc[(((w * 2 + i) * 4 + j) * 64 + lane) * 16 + r] += s;
+                                    acc[(((w * 2 + i) * 4 + j) * 64 + lane) * 16 + r] += sum;
                                 }
                 }
         }
         // epilogue
-        for (int w = 0; w < 4; ++w)
+        for (int w = 0; w < crt::WAVES; ++w)
             for (int i = 0; i < 2; ++i)
                 for (int j = 0; j < 4; ++j)
                     for (int lane = 0; lane < 64; ++lane)
                         for (int r = 0; r < 16; ++r) {
-                            const int64_t row = tm * crt::BM + 64 * w + crt::acc_row(i, r, lane);
-                            const int64_t col = n0 + crt::acc_col(j, lane);
+                            const int64_t row = m0 + crt::wave_row(w) + crt::acc_row(i, r, lane);
+                            const int64_t col = n0 + crt::wave_col(w) + crt::acc_col(j, lane);
                             if (row >= M || col >= N) continue;
                             long long v = acc[(((w * 2 + i) * 4 + j) * 64 + lane) * 16 + r] % m;
                             if (v < 0) v += m;
