@@ -343,7 +343,7 @@ __global__ __launch_bounds__(512, 2) void crt_gemm_kernel(CrtGemmArgs a, CrtTabl
 
     const int KC = (int)(a.Kp >> 5);
     // this wave's operand: A's slice (waves 0-3) or B's (waves 4-7) of modulus l
-    const bool is_a = wave < 4;
+    const bool is_a = wave < crt::WAVES / 2;
     const signed char *src = is_a ? a.A + (int64_t)l * a.Mp * a.Kp : a.B + (int64_t)l * a.Np * a.Kp;
     const int64_t x0 = is_a ? m0 : n0;
     const int nt = (int)(a.Kp / crt::SK);

@@ -97,7 +97,7 @@ int main(int argc, char **argv) {
         // every wave's 4 loads of step t into stage slot (waves 0-3 from A, 4-7 from B)
         auto issue = [&](int slot, int t) {
             for (int w = 0; w < crt::WAVES; ++w) {
-                const bool is_a = w < 4;
+                const bool is_a = w < crt::WAVES / 2;
                 const std::vector<signed char> &src = is_a ? Ar : Br;
                 const int64_t x0 = is_a ? m0 : n0, size = is_a ? Mp * Kp : Np * Kp;
                 for (int p = 0; p < crt::LOADS; ++p)
