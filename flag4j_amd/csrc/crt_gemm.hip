@@ -368,13 +368,14 @@ __global__ __launch_bounds__(512, 2) void crt_gemm_kernel(CrtGemmArgs a, CrtTabl
         __builtin_amdgcn_s_waitcnt(0xC07F);      // lgkmcnt(0)
         __builtin_amdgcn_s_barrier();            // every wave's step-t loads are in LDS; stage t - 1 is free
         __builtin_amdgcn_sched_barrier(0);
-        issue(t + crt::AHEAD, min(t + crt::AHEAD, nt - 1));
-        __builtin_amdgcn_sched_barrier(0);
         i32x4 fa[2][2], fb[2][4];
 #pragma unroll
         for (int i = 0; i < 2; ++i) fa[0][i] = *(const i32x4 *)(lds + crt::a_frag(t, wave, i, 0, lane));
 #pragma unroll
         for (int j = 0; j < 4; ++j) fb[0][j] = *(const i32x4 *)(lds + crt::b_frag(t, wave, j, 0, lane));
+        __builtin_amdgcn_sched_barrier(0);       // the first fragment reads go out before the next step's loads
+        issue(t + crt::AHEAD, min(t + crt::AHEAD, nt - 1));
+        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int ks = 0; ks < crt::KB; ++ks) {
             if (ks + 1 < crt::KB) {
