@@ -356,43 +356,60 @@ __global__ __launch_bounds__(512, 2) void crt_gemm_kernel(CrtGemmArgs a, CrtTabl
             __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, (lds_void *)(lds + crt::load_lds(slot, wave, p)), 16,
                                                      crt::load_voff(lane), crt::load_soff(p, KC), 0, 0);
     };
+    // Two stages, pipelined across steps: the MFMAs of a step's last k step
+    // run while the waves meet at the barrier, issue the loads two steps ahead
+    // and read the next step's first fragments, so the next step's first MFMAs
+    // find their operands ready.  At the barrier of step t every wave has read all of stage
+    // t % 2 and has its own loads of step t + 1 (stage (t + 1) % 2) retired, so
+    // after it step t + 1 may be read and step t + 2 may overwrite stage t % 2.
     // Every step issues the same LOADS loads (past the last step they repeat
-    // step nt - 1 into the free stage), so at the top of step t the
-    // LOADS (AHEAD - 1) most recent loads are steps t + 1 .. t + AHEAD - 1 and
-    // that vmcnt retires step t.
+    // step nt - 1 into the free stage), so vmcnt(0) before the barrier retires
+    // exactly the next step's loads.
+    static_assert(crt::STAGES == 2 && crt::KB % 2 == 0, "two-stage ring, even k steps per stage");
+    i32x4 fa[2][2], fb[2][4];
+    issue(0, 0);
+    F4J_WAIT_VMCNT(0);
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-    for (int t = 0; t < crt::AHEAD; ++t) issue(t, min(t, nt - 1));
+    for (int i = 0; i < 2; ++i) fa[0][i] = *(const i32x4 *)(lds + crt::a_frag(0, wave, i, 0, lane));
+#pragma unroll
+    for (int j = 0; j < 4; ++j) fb[0][j] = *(const i32x4 *)(lds + crt::b_frag(0, wave, j, 0, lane));
+    __builtin_amdgcn_sched_barrier(0);
+    issue(1, min(1, nt - 1));
+    auto mfmas = [&](int buf) {
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                acc[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[buf][i], fb[buf][j], acc[i][j], 0, 0, 0);
+    };
     for (int t = 0; t < nt; ++t) {
-        __builtin_amdgcn_sched_barrier(0);
-        F4J_WAIT_VMCNT(crt::LOADS * (crt::AHEAD - 1));
-        __builtin_amdgcn_s_waitcnt(0xC07F);      // lgkmcnt(0)
-        __builtin_amdgcn_s_barrier();            // every wave's step-t loads are in LDS; stage t - 1 is free
-        __builtin_amdgcn_sched_barrier(0);
-        i32x4 fa[2][2], fb[2][4];
 #pragma unroll
-        for (int i = 0; i < 2; ++i) fa[0][i] = *(const i32x4 *)(lds + crt::a_frag(t, wave, i, 0, lane));
-#pragma unroll
-        for (int j = 0; j < 4; ++j) fb[0][j] = *(const i32x4 *)(lds + crt::b_frag(t, wave, j, 0, lane));
-        __builtin_amdgcn_sched_barrier(0);       // the first fragment reads go out before the next step's loads
-        issue(t + crt::AHEAD, min(t + crt::AHEAD, nt - 1));
-        __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-        for (int ks = 0; ks < crt::KB; ++ks) {
-            if (ks + 1 < crt::KB) {
-#pragma unroll
-                for (int i = 0; i < 2; ++i)
-                    fa[(ks + 1) & 1][i] = *(const i32x4 *)(lds + crt::a_frag(t, wave, i, ks + 1, lane));
-#pragma unroll
-                for (int j = 0; j < 4; ++j)
-                    fb[(ks + 1) & 1][j] = *(const i32x4 *)(lds + crt::b_frag(t, wave, j, ks + 1, lane));
-            }
-            __builtin_amdgcn_sched_barrier(0);   // next k step's fragment reads ahead of these MFMAs
+        for (int ks = 0; ks + 1 < crt::KB; ++ks) {
+            __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
             for (int i = 0; i < 2; ++i)
+                fa[(ks + 1) & 1][i] = *(const i32x4 *)(lds + crt::a_frag(t, wave, i, ks + 1, lane));
 #pragma unroll
-                for (int j = 0; j < 4; ++j)
-                    acc[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[ks & 1][i], fb[ks & 1][j], acc[i][j], 0, 0, 0);
+            for (int j = 0; j < 4; ++j)
+                fb[(ks + 1) & 1][j] = *(const i32x4 *)(lds + crt::b_frag(t, wave, j, ks + 1, lane));
+            mfmas(ks & 1);                        // (the compiler interleaves them with those reads)
         }
+        __builtin_amdgcn_sched_barrier(0);
+        F4J_WAIT_VMCNT(0);                        // this wave's loads of step t + 1 are in LDS
+        __builtin_amdgcn_s_waitcnt(0xC07F);      // lgkmcnt(0): its reads of stage t % 2 are done
+        __builtin_amdgcn_s_barrier();
+        __builtin_amdgcn_sched_barrier(0);
+        issue(t + 2, min(t + 2, nt - 1));        // into stage t % 2
+        __builtin_amdgcn_sched_barrier(0);
+        // step t + 1's first fragments (past the last step: the repeated stage, never used)
+#pragma unroll
+        for (int i = 0; i < 2; ++i) fa[0][i] = *(const i32x4 *)(lds + crt::a_frag(t + 1, wave, i, 0, lane));
+#pragma unroll
+        for (int j = 0; j < 4; ++j) fb[0][j] = *(const i32x4 *)(lds + crt::b_frag(t + 1, wave, j, 0, lane));
+        __builtin_amdgcn_sched_barrier(0);
+        mfmas((crt::KB - 1) & 1);                 // step t's last k step, beside the barrier and reads
     }
     F4J_WAIT_VMCNT(0);                            // no LDS write in flight when the block ends
 
