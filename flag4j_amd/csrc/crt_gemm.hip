@@ -317,8 +317,8 @@ typedef __attribute__((address_space(3))) void lds_void;
 // accumulators per lane): wave w takes row group w % 4 and column half w / 4.
 // Both operands reach LDS by buffer loads that write LDS directly, each
 // fragment once per block (waves 0-3 load A's, 4-7 B's), through a ring of
-// STAGES steps of SK k bytes, AHEAD steps ahead of the MFMAs, one barrier per
-// step.  Loading A once per block rather than once per wave into registers
+// STAGES steps of SK k bytes, loaded up to STAGES steps ahead of the MFMAs,
+// one barrier per step.  Loading A once per block rather than once per wave into registers
 // takes a third of the bytes off the CU's vector-memory return path, which
 // was the busiest unit; gfx950's LDS carries the extra fragment reads.  With
 // the fragment-major layout every load and every fragment read moves 1 KB of
@@ -356,19 +356,21 @@ __global__ __launch_bounds__(512, 2) void crt_gemm_kernel(CrtGemmArgs a, CrtTabl
             __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, (lds_void *)(lds + crt::load_lds(slot, wave, p)), 16,
                                                      crt::load_voff(lane), crt::load_soff(p, KC), 0, 0);
     };
-    // Two stages, pipelined across steps: the MFMAs of a step's last k step
-    // run while the waves meet at the barrier, issue the loads two steps ahead
-    // and read the next step's first fragments, so the next step's first MFMAs
-    // find their operands ready.  At the barrier of step t every wave has read all of stage
-    // t % 2 and has its own loads of step t + 1 (stage (t + 1) % 2) retired, so
-    // after it step t + 1 may be read and step t + 2 may overwrite stage t % 2.
+    // Pipelined across steps: the MFMAs of a step's last k step run while the
+    // waves meet at the barrier, issue the loads STAGES steps ahead and read the
+    // next step's first fragments, so the next step's first MFMAs find their
+    // operands ready.  At the barrier of step t every wave has read all of stage
+    // t % STAGES and has its own loads of step t + 1 retired, so after it step
+    // t + 1 may be read and step t + STAGES may overwrite stage t % STAGES.
     // Every step issues the same LOADS loads (past the last step they repeat
-    // step nt - 1 into the free stage), so vmcnt(0) before the barrier retires
-    // exactly the next step's loads.
-    static_assert(crt::STAGES == 2 && crt::KB % 2 == 0, "two-stage ring, even k steps per stage");
+    // step nt - 1 into the free stage), so at that point the LOADS (STAGES - 2)
+    // most recent loads are steps t + 2 .. t + STAGES - 1 and that vmcnt
+    // retires step t + 1.
+    static_assert(crt::KB % 2 == 0, "even k steps per stage");
     i32x4 fa[2][2], fb[2][4];
-    issue(0, 0);
-    F4J_WAIT_VMCNT(0);
+#pragma unroll
+    for (int t = 0; t < crt::STAGES - 1; ++t) issue(t, min(t, nt - 1));
+    F4J_WAIT_VMCNT(crt::LOADS * (crt::STAGES - 2));
     __builtin_amdgcn_s_barrier();
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
@@ -376,7 +378,7 @@ __global__ __launch_bounds__(512, 2) void crt_gemm_kernel(CrtGemmArgs a, CrtTabl
 #pragma unroll
     for (int j = 0; j < 4; ++j) fb[0][j] = *(const i32x4 *)(lds + crt::b_frag(0, wave, j, 0, lane));
     __builtin_amdgcn_sched_barrier(0);
-    issue(1, min(1, nt - 1));
+    issue(crt::STAGES - 1, min(crt::STAGES - 1, nt - 1));
     auto mfmas = [&](int buf) {
 #pragma unroll
         for (int i = 0; i < 2; ++i)
@@ -397,11 +399,11 @@ __global__ __launch_bounds__(512, 2) void crt_gemm_kernel(CrtGemmArgs a, CrtTabl
             mfmas(ks & 1);                        // (the compiler interleaves them with those reads)
         }
         __builtin_amdgcn_sched_barrier(0);
-        F4J_WAIT_VMCNT(0);                        // this wave's loads of step t + 1 are in LDS
-        __builtin_amdgcn_s_waitcnt(0xC07F);      // lgkmcnt(0): its reads of stage t % 2 are done
+        F4J_WAIT_VMCNT(crt::LOADS * (crt::STAGES - 2));   // this wave's loads of step t + 1 are in LDS
+        __builtin_amdgcn_s_waitcnt(0xC07F);      // lgkmcnt(0): its reads of stage t % STAGES are done
         __builtin_amdgcn_s_barrier();
         __builtin_amdgcn_sched_barrier(0);
-        issue(t + 2, min(t + 2, nt - 1));        // into stage t % 2
+        issue(t + crt::STAGES, min(t + crt::STAGES, nt - 1));   // into stage t % STAGES
         __builtin_amdgcn_sched_barrier(0);
         // step t + 1's first fragments (past the last step: the repeated stage, never used)
 #pragma unroll
