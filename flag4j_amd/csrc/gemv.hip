@@ -22,9 +22,9 @@
 // a (row, 64-block) chain for the blocked form (8 rows x 8 blocks per wave, the
 // block sums then added in block order across lanes), a row for the plain real
 // form (64 rows per wave) and a (row, component) pair for complex (32 rows per
-// wave; below about 12K-24K rows these take 8 or 16 rows per wave, with the
-// other lanes idle, to spread the rows over more waves).  The wave streams its slab of A in stages of 16 elements per chain
-// (8 KB): coalesced 8-byte buffer loads into registers D stages ahead, then a
+// wave; real A below about 32K rows takes 8 or 16 rows per wave, with the
+// other lanes idle, to spread the rows over more waves).  The wave streams
+// its slab of A in stages of 16 elements per chain (8 KB): coalesced 8-byte buffer loads into registers D stages ahead, then a
 // write into LDS rotated so that the 64 lanes read their own chains' elements
 // without bank conflicts.  Loads past K or past the last row are given an
 // out-of-range offset and return zeros, which leave every chain unchanged (a
@@ -518,20 +518,19 @@ void launch_dma(GemvArgs g, int64_t stages, hipStream_t stream) {
                        g);
 }
 
-// One-chain-per-row kinds: rows per wave by M, as measured on MI355X
-// (2048 <= M <= 32768, A of 512 MB / 1 GB): full waves need M of about 12K
-// (complex) or 24K (real) rows to keep enough chains and loads in flight on
-// every CU; below that, 8 rows per wave (16 for real A from 12K rows), with the
-// ring depth grown so a wave still has 16-24 KB of A in flight.
+// One-chain-per-row kinds: rows per wave and ring depth, from a sweep on
+// MI355X over M = 8K..32K rows (A of 512 MB real / 1 GB complex) of 8 / 16 /
+// 32-64 rows per wave against ring depths 4..17.  Complex A: full waves with
+// a 4-stage ring (best or within 8% of best at every M).  Real A: 8 rows per
+// wave, 9 stages, below 16K rows; 16 rows, 5 stages, above — deep rings on
+// many small waves leave too few of them resident per CU.
 template <int KIND, int XV>
 void launch_rows(GemvArgs g, int64_t stages, hipStream_t stream) {
     if constexpr (a_cplx<KIND>()) {
-        if (g.M >= 12288) launch_dma<KIND, XV, 7, 8>(g, stages, stream);
-        else launch_dma<KIND, XV, 13, 2>(g, stages, stream);
+        launch_dma<KIND, XV, 4, 8>(g, stages, stream);
     } else {
-        if (g.M >= 24576) launch_dma<KIND, XV, 7, 8>(g, stages, stream);
-        else if (g.M >= 12288) launch_dma<KIND, XV, 13, 2>(g, stages, stream);
-        else launch_dma<KIND, XV, 17, 1>(g, stages, stream);
+        if (g.M >= 16384) launch_dma<KIND, XV, 5, 2>(g, stages, stream);
+        else launch_dma<KIND, XV, 9, 1>(g, stages, stream);
     }
 }
 

@@ -202,11 +202,10 @@ def test_full_size_blocked_bitwise():
     assert np.array_equal(bits(dgemv(A, x)), bits(o.real_blocked_vector(A, x, 64)))
 
 
-@pytest.mark.parametrize("M", [24577, 12289, 12287, 100])
+@pytest.mark.parametrize("M", [16385, 16383, 100])
 def test_rows_per_wave_thresholds(M):
-    """The one-chain-per-row kernels take 64 / 16 / 8 rows per wave (real A) or 32 / 8
-    (complex A) by M; each size class, with a ragged last wave, against the oracle bit
-    for bit: Matrix.multTranspose with a one-row B (plain chain, K even: direct
+    """The one-chain-per-row kernels take 16 / 8 rows per wave (real A, by M) or 32 (complex
+    A); each size class, with a ragged last wave, against the oracle bit for bit: Matrix.multTranspose with a one-row B (plain chain, K even: direct
     loads), CMatrix.mult(CVector), Matrix.mult(CVector) and CMatrix.mult(Vector)."""
     rng = np.random.default_rng(M)
     L = _native.load()
