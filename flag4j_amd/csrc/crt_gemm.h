@@ -31,6 +31,7 @@ struct CrtPlan {
     int64_t Mp = 0, Np = 0, Kp = 0;      // padded sizes of the residue slices
     int64_t plane = 0;                   // bytes between the product residues of two moduli (M N, 256-aligned)
     size_t off_ra = 0, off_rb = 0, off_res = 0, off_ea = 0, off_eb = 0, off_err = 0, off_part = 0, bytes = 0;
+    size_t off_flag = 0;                 // int: set by the exponent kernels when an operand holds NaN or Inf
 };
 
 // Smallest number of moduli whose product gives t >= 53 bits for inner size K

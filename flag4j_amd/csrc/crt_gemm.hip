@@ -166,11 +166,18 @@ __global__ __launch_bounds__(256) void crt_fill_kernel(int *p, int64_t n, int v)
 }
 
 // Line exponents (smallest E with |value| < 2^E over the line; INT_MIN for an
-// all-zero line).  Contiguous lines: one block per line, lanes along k.
-__global__ __launch_bounds__(256) void crt_exp_lines_kernel(CrtOperand o, int64_t K, int *e) {
+// all-zero line).  Contiguous lines: one block per line, lanes along k.  A NaN
+// or infinite entry sets *nonfinite (the residues cannot represent it).
+__global__ __launch_bounds__(256) void crt_exp_lines_kernel(CrtOperand o, int64_t K, int *e, int *nonfinite) {
     const int64_t x = blockIdx.x;
     int m = INT_MIN;
-    for (int64_t k = threadIdx.x; k < K; k += 256) m = max(m, exp_of(opnd_value(o, x, k)));
+    bool bad = false;
+    for (int64_t k = threadIdx.x; k < K; k += 256) {
+        const double v = opnd_value(o, x, k);
+        bad |= !__builtin_isfinite(v);
+        m = max(m, exp_of(v));
+    }
+    if (bad) *nonfinite = 1;
     for (int off = 32; off > 0; off >>= 1) m = max(m, __shfl_xor(m, off));
     __shared__ int red[4];
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
@@ -181,12 +188,18 @@ __global__ __launch_bounds__(256) void crt_exp_lines_kernel(CrtOperand o, int64_
 // Strided lines: thread per line (lanes along the lines), one k-chunk per
 // block row, merged with atomicMax into e (filled with INT_MIN first).
 __global__ __launch_bounds__(256) void crt_exp_cols_kernel(CrtOperand o, int64_t nx, int64_t K, int64_t kchunk,
-                                                           int *e) {
+                                                           int *e, int *nonfinite) {
     const int64_t x = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (x >= nx) return;
     const int64_t k0 = (int64_t)blockIdx.y * kchunk, k1 = min(K, k0 + kchunk);
     int m = INT_MIN;
-    for (int64_t k = k0; k < k1; ++k) m = max(m, exp_of(opnd_value(o, x, k)));
+    bool bad = false;
+    for (int64_t k = k0; k < k1; ++k) {
+        const double v = opnd_value(o, x, k);
+        bad |= !__builtin_isfinite(v);
+        m = max(m, exp_of(v));
+    }
+    if (bad) *nonfinite = 1;
     if (m != INT_MIN) atomicMax(e + x, m);
 }
 
@@ -607,6 +620,7 @@ bool plan_crt(int64_t M, int64_t N, int64_t K, int n, int check_rows, CrtPlan *p
     pl.off_ea = pl.off_res + (size_t)n * pl.plane;
     pl.off_eb = pl.off_ea + al((size_t)M * 4);
     pl.off_err = pl.off_eb + al((size_t)N * 4);
+    pl.off_flag = pl.off_err + 128;
     pl.off_part = pl.off_err + 256;
     pl.bytes = pl.off_part + split_residual_bytes(N, K, (int)std::min<int64_t>(check_rows, M));
     *plan = pl;
@@ -633,17 +647,19 @@ hipError_t launch_crt_gemm(const CrtPlan &pl, const CrtOperand &oa, const CrtOpe
     signed char *ra = (signed char *)(w + pl.off_ra), *rb = (signed char *)(w + pl.off_rb);
     unsigned char *res = (unsigned char *)(w + pl.off_res);
     int *ea = (int *)(w + pl.off_ea), *eb = (int *)(w + pl.off_eb);
+    int *nonfinite = (int *)(w + pl.off_flag);
     auto exps = [&](const CrtOperand &o, int64_t nx, int *e, hipStream_t stream) {
         if (nx <= 0) return;
         if (o.lines) {
-            hipLaunchKernelGGL(crt_exp_lines_kernel, dim3((unsigned)nx), dim3(256), 0, stream, o, K, e);
+            hipLaunchKernelGGL(crt_exp_lines_kernel, dim3((unsigned)nx), dim3(256), 0, stream, o, K, e, nonfinite);
         } else {
             const int64_t kchunk = 256;
             hipLaunchKernelGGL(crt_fill_kernel, dim3(64), dim3(256), 0, stream, e, nx, INT_MIN);
             hipLaunchKernelGGL(crt_exp_cols_kernel, dim3((unsigned)((nx + 255) / 256), (unsigned)((K + kchunk - 1) / kchunk)),
-                               dim3(256), 0, stream, o, nx, K, kchunk, e);
+                               dim3(256), 0, stream, o, nx, K, kchunk, e, nonfinite);
         }
     };
+    if (out.first) hipLaunchKernelGGL(crt_fill_kernel, dim3(1), dim3(256), 0, stream, nonfinite, 1, 0);
     const bool wide = pl.tb.t > 58;
     auto split = [&](const CrtOperand &o, int64_t nx, int64_t Xp, const int *ex, signed char *dst,
                      hipStream_t stream) {

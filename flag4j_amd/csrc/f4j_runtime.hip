@@ -259,13 +259,25 @@ int run_split(int opA, int opB, int64_t M, int64_t N, int64_t K, const double *A
     rep.launches = (int)pl.groups.size();
     rep.checked_rows = (int)std::min<int64_t>(c.split_check_rows, M);
     rep.max_rel_err = -1.0;
+    // NaN or Inf among the inputs (flagged by the exponent kernels): the digit
+    // slices cannot carry them; FP64 arithmetic propagates them as the reference's does.
+    int nonfinite = 0;
+    F4J_HIP(hipMemcpyAsync(&nonfinite, (char *)s.ws.p + pl.off_flag, sizeof(int), hipMemcpyDeviceToHost, stream),
+            "copy flag");
+    double err = 0.0;
     if (rep.checked_rows > 0) {
         F4J_HIP(launch_split_residual(pl, opA, opB, M, N, K, A, lda, B, ldb, C, ldc, rep.checked_rows, s.ws.p, stream),
                 "split residual launch");
-        double err = 0.0;
         F4J_HIP(hipMemcpyAsync(&err, (char *)s.ws.p + pl.off_err, sizeof(double), hipMemcpyDeviceToHost, stream),
                 "copy residual");
-        F4J_HIP(hipStreamSynchronize(stream), "stream synchronize");
+    }
+    F4J_HIP(hipStreamSynchronize(stream), "stream synchronize");
+    if (nonfinite) {
+        GemmProblem p{M, N, K, A, lda, B, ldb, C, ldc, 0};
+        F4J_HIP(launch_gemm_f64(opA ? R_T : R_N, opB ? R_T : R_N, p, stream, pick_tile(M, N)),
+                "fp64 for non-finite inputs");
+        rep.precision = F4J_PREC_FP64;
+    } else if (rep.checked_rows > 0) {
         rep.max_rel_err = err;
         if (!(err <= c.split_tolerance)) {
             if (!c.split_fallback) {
@@ -337,7 +349,13 @@ int run_split_cplx(int opA, int opB, int64_t M, int64_t N, int64_t K, const doub
         F4J_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]), "hipEventElapsedTime");
         rep.product_ms = ms;
     }
-    if (rep.checked_rows > 0) {
+    if (std::isnan(perr)) {
+        // NaN or Inf among the inputs: the digit slices cannot carry them; FP64
+        // arithmetic propagates them as the reference's does.
+        F4J_HIP(launch_fp64(true, opA, opB, M, N, K, A, lda, B, ldb, C, ldc, stream), "fp64 for non-finite inputs");
+        F4J_HIP(hipStreamSynchronize(stream), "stream synchronize");
+        rep.precision = F4J_PREC_FP64;
+    } else if (rep.checked_rows > 0) {
         rep.max_rel_err = perr;
         if (!(perr <= c.split_tolerance)) {
             if (!c.split_fallback) {
@@ -429,14 +447,24 @@ int run_crt(bool cplx, int opA, int opB, int64_t M, int64_t N, int64_t K, const 
     rep.launches = (int)jobs.size();
     rep.checked_rows = check_rows;
     rep.max_rel_err = -1.0;
+    // NaN or Inf among the inputs (flagged by the exponent kernels): the
+    // residues cannot carry them, so the product is recomputed in FP64, whose
+    // arithmetic propagates them as the reference's does.
+    int nonfinite = 0;
+    F4J_HIP(hipMemcpyAsync(&nonfinite, w + pl.off_flag, sizeof(int), hipMemcpyDeviceToHost, stream), "copy flag");
+    double err = 0.0;
     if (rep.checked_rows > 0) {
         unsigned long long *eb = (unsigned long long *)(w + pl.off_err);
         F4J_HIP(hipStreamWaitEvent(stream, s.join_ev, 0), "hipStreamWaitEvent");
         F4J_HIP(launch_residual_combine(cplx, M, N, K, C, ldc, rep.checked_rows, part, eb, stream),
                 "crt residual launch");
-        double err = 0.0;
         F4J_HIP(hipMemcpyAsync(&err, w + pl.off_err, sizeof(double), hipMemcpyDeviceToHost, stream), "copy residual");
-        F4J_HIP(hipStreamSynchronize(stream), "stream synchronize");
+    }
+    F4J_HIP(hipStreamSynchronize(stream), "stream synchronize");
+    if (nonfinite) {
+        F4J_HIP(launch_fp64(cplx, opA, opB, M, N, K, A, lda, B, ldb, C, ldc, stream), "fp64 for non-finite inputs");
+        rep.precision = F4J_PREC_FP64;
+    } else if (rep.checked_rows > 0) {
         rep.max_rel_err = err;
         if (!(err <= c.split_tolerance)) {
             if (!c.split_fallback) {

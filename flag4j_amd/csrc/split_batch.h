@@ -34,7 +34,8 @@ SplitBatch *split_batch_create(bool cplx, int opA, int opB, int carrier, int sli
 hipError_t split_batch_enqueue(const SplitBatch *b, hipStream_t stream, hipEvent_t ev_begin = nullptr,
                                hipEvent_t ev_end = nullptr);
 // Per-problem max |C - AB| / (|A||B|) over the checked rows of the last run
-// (-1 where nothing was checked); synchronises `stream`.
+// (-1 where nothing was checked; NaN where the problem's operands hold NaN or
+// Inf, which the digit slices cannot carry); synchronises `stream`.
 hipError_t split_batch_errors(const SplitBatch *b, double *out, int64_t count, hipStream_t stream);
 int split_batch_real_products(const SplitBatch *b);     // real slice products per run (jobs x pairs)
 int split_batch_checked_rows(const SplitBatch *b);

@@ -28,6 +28,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <limits>
 #include <climits>
 #include <cmath>
 #include <map>
@@ -55,7 +56,13 @@ struct Side {                 // one operand of one real job
     int64_t nx, K, Xp, Kp;    // lines, inner size, padded to the tile / the k step
     int64_t dig;              // offset (carrier elements) of the digit slices [S][Xp][Kp]
     int64_t eoff;             // offset of the line exponents [Xp]
+    int prob;                 // the product's slot in the per-product error array
 };
+
+// Error-array value that marks a product whose operands hold NaN or Inf: the
+// bits of a quiet NaN, above every residual error's bits (finite or +Inf), so
+// the residual check's atomicMax keeps it.
+constexpr unsigned long long NONFINITE_BITS = 0x7FF8000000000000ull;
 
 struct Job {
     int sa, sb;               // sides of op(A) and op(B)
@@ -90,16 +97,22 @@ __device__ inline int exp_of(double v) {
 // Work item (side, first line): contiguous sides take 4 lines per block (a
 // wave per line, lanes along k); strided sides 256 lines (a thread per line,
 // lanes along the lines).  Lines past nx (padding) get INT_MIN.
+// A NaN or infinite entry marks the side's product in err (NONFINITE_BITS).
 __global__ __launch_bounds__(256) void bexp_kernel(const Side *__restrict__ sides, const int2 *__restrict__ work,
-                                                   int *__restrict__ epool) {
+                                                   int *__restrict__ epool, unsigned long long *__restrict__ err) {
     const int2 w = work[blockIdx.x];
     const Side s = sides[w.x];
+    bool bad = false;
     if (s.contig) {
         const int64_t x = w.y + (threadIdx.x >> 6);
         const int lane = threadIdx.x & 63;
         int m = INT_MIN;
         if (x < s.nx)
-            for (int64_t k = lane; k < s.K; k += 64) m = max(m, exp_of(side_value(s, x, k)));
+            for (int64_t k = lane; k < s.K; k += 64) {
+                const double v = side_value(s, x, k);
+                bad |= !__builtin_isfinite(v);
+                m = max(m, exp_of(v));
+            }
         for (int off = 32; off > 0; off >>= 1) m = max(m, __shfl_xor(m, off));
         if (lane == 0 && x < s.Xp) epool[s.eoff + x] = m;
     } else {
@@ -107,9 +120,14 @@ __global__ __launch_bounds__(256) void bexp_kernel(const Side *__restrict__ side
         if (x >= s.Xp) return;
         int m = INT_MIN;
         if (x < s.nx)
-            for (int64_t k = 0; k < s.K; ++k) m = max(m, exp_of(side_value(s, x, k)));
+            for (int64_t k = 0; k < s.K; ++k) {
+                const double v = side_value(s, x, k);
+                bad |= !__builtin_isfinite(v);
+                m = max(m, exp_of(v));
+            }
         epool[s.eoff + x] = m;
     }
+    if (bad) atomicMax(err + s.prob, NONFINITE_BITS);
 }
 
 // Work item (side, x0, k0): 32 lines x 64 k staged through LDS with reads
@@ -424,6 +442,7 @@ SplitBatch *split_batch_create(bool cplx, int opA, int opB, int carrier, int sli
             c.sx = (c.contig ? q.ldb : 1) * w;
             c.sk = (c.contig ? 1 : q.ldb) * w;
             for (Side *s : {&a, &c}) {
+                s->prob = (int)dprobs.size();
                 s->dig = dig;
                 dig += (int64_t)slices * s->Xp * s->Kp;
                 s->eoff = eoff;
@@ -562,7 +581,10 @@ hipError_t split_batch_enqueue(const SplitBatch *b, hipStream_t st, hipEvent_t e
     }
     if (b->n_items == 0) return hipGetLastError();
     const bool fp16 = b->carrier == SPLIT_FP16;
-    hipLaunchKernelGGL(bexp_kernel, dim3((unsigned)b->n_exp), dim3(256), 0, st, b->d_sides, b->d_exp_work, b->d_exps);
+    const int64_t nprob = (int64_t)std::count_if(b->prob_index.begin(), b->prob_index.end(), [](int v) { return v >= 0; });
+    hipLaunchKernelGGL(fill_u64_kernel, dim3(64), dim3(256), 0, st, b->d_err, nprob);
+    hipLaunchKernelGGL(bexp_kernel, dim3((unsigned)b->n_exp), dim3(256), 0, st, b->d_sides, b->d_exp_work, b->d_exps,
+                       b->d_err);
     if (fp16)
         launch_bsplit<10, _Float16>(b->S, dim3((unsigned)b->n_split), st, b->d_sides, b->d_split_work, b->d_exps,
                                     (_Float16 *)b->d_digits);
@@ -580,10 +602,6 @@ hipError_t split_batch_enqueue(const SplitBatch *b, hipStream_t st, hipEvent_t e
                            b->d_exps, (int)b->cplx);
     if (ev_end) (void)hipEventRecord(ev_end, st);
     if (b->n_res > 0) {
-        const int64_t np = (int64_t)b->prob_index.size();
-        (void)np;
-        hipLaunchKernelGGL(fill_u64_kernel, dim3(64), dim3(256), 0, st, b->d_err, (int64_t)std::count_if(
-                               b->prob_index.begin(), b->prob_index.end(), [](int v) { return v >= 0; }));
         hipLaunchKernelGGL(bres_kernel, dim3((unsigned)b->n_res), dim3(256), 0, st, b->d_probs, b->d_res_work,
                            (int)b->cplx, b->opA, b->opB, b->d_err);
     }
@@ -593,14 +611,16 @@ hipError_t split_batch_enqueue(const SplitBatch *b, hipStream_t st, hipEvent_t e
 hipError_t split_batch_errors(const SplitBatch *b, double *out, int64_t count, hipStream_t st) {
     const int64_t np = (int64_t)std::count_if(b->prob_index.begin(), b->prob_index.end(), [](int v) { return v >= 0; });
     std::vector<unsigned long long> bits(np, 0);
-    if (np > 0 && b->check_rows > 0) {
+    if (np > 0) {
         hipError_t e = hipMemcpyAsync(bits.data(), b->d_err, np * sizeof(unsigned long long), hipMemcpyDeviceToHost, st);
         if (e == hipSuccess) e = hipStreamSynchronize(st);
         if (e != hipSuccess) return e;
     }
     for (int64_t i = 0; i < std::min<int64_t>(count, (int64_t)b->prob_index.size()); ++i) {
         const int k = b->prob_index[i];
-        out[i] = (k < 0 || b->check_rows <= 0) ? -1.0 : __builtin_bit_cast(double, bits[k]);
+        if (k < 0) out[i] = -1.0;
+        else if (bits[k] >= NONFINITE_BITS) out[i] = std::numeric_limits<double>::quiet_NaN();   // NaN / Inf operands
+        else out[i] = b->check_rows <= 0 ? -1.0 : __builtin_bit_cast(double, bits[k]);
     }
     return hipSuccess;
 }

@@ -26,6 +26,7 @@ struct SplitPlan {
     int S = 0, carrier = SPLIT_BF16;
     int64_t Mp = 0, Np = 0, Kp = 0;     // padded sizes of the slice arrays
     size_t off_slices_a = 0, off_slices_b = 0, off_exp_a = 0, off_exp_b = 0, off_err = 0, off_residual = 0, bytes = 0;
+    size_t off_flag = 0;                 // int: set by the exponent kernels when an operand holds NaN or Inf
     std::vector<SplitGroup> groups;     // in launch order (smallest weights first)
     int products() const { return S * (S + 1) / 2; }
 };
@@ -70,8 +71,9 @@ hipError_t launch_split_residual(const SplitPlan &pl, int opA, int opB, int64_t 
 
 // Row exponents of op(A) (eA[M]) and column exponents of op(B) (eB[N]): the
 // smallest E with |x| < 2^E over the line, INT_MIN for an all-zero line.
+// *nonfinite is cleared, then set to 1 if any entry is NaN or infinite.
 hipError_t launch_exponents(int opA, int opB, int64_t M, int64_t N, int64_t K, const double *A, int64_t lda,
-                            const double *B, int64_t ldb, int *eA, int *eB, hipStream_t stream);
+                            const double *B, int64_t ldb, int *eA, int *eB, int *nonfinite, hipStream_t stream);
 
 // Complex form (interleaved data, op codes 0 = N, 1 = T, 2 = C): |z| = |re| + |im|,
 // an entry's error is the larger of its two parts.

@@ -68,11 +68,18 @@ __device__ inline int exp_of(double v) {
 }
 
 // Lines contiguous in memory (stored row x of length K): one block per line.
-__global__ __launch_bounds__(256) void exp_lines_kernel(const double *X, int64_t ld, int64_t K, int *e) {
+// A NaN or infinite entry sets *nonfinite (the digit slices cannot carry it).
+__global__ __launch_bounds__(256) void exp_lines_kernel(const double *X, int64_t ld, int64_t K, int *e,
+                                                        int *nonfinite) {
     const int64_t x = blockIdx.x;
     const double *row = X + x * ld;
     int m = INT_MIN;
-    for (int64_t k = threadIdx.x; k < K; k += 256) m = max(m, exp_of(row[k]));
+    bool bad = false;
+    for (int64_t k = threadIdx.x; k < K; k += 256) {
+        bad |= !__builtin_isfinite(row[k]);
+        m = max(m, exp_of(row[k]));
+    }
+    if (bad) *nonfinite = 1;
     for (int off = 32; off > 0; off >>= 1) m = max(m, __shfl_xor(m, off));
     __shared__ int red[4];
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
@@ -83,12 +90,18 @@ __global__ __launch_bounds__(256) void exp_lines_kernel(const double *X, int64_t
 // Lines strided in memory (stored K x X, line x is column x): each thread
 // scans a k-chunk of one column and merges with atomicMax.
 __global__ __launch_bounds__(256) void exp_cols_kernel(const double *X, int64_t ld, int64_t K, int64_t ncols,
-                                                       int64_t kchunk, int *e) {
+                                                       int64_t kchunk, int *e, int *nonfinite) {
     const int64_t x = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (x >= ncols) return;
     const int64_t k0 = (int64_t)blockIdx.y * kchunk, k1 = min(K, k0 + kchunk);
     int m = INT_MIN;
-    for (int64_t k = k0; k < k1; ++k) m = max(m, exp_of(X[k * ld + x]));
+    bool bad = false;
+    for (int64_t k = k0; k < k1; ++k) {
+        const double v = X[k * ld + x];
+        bad |= !__builtin_isfinite(v);
+        m = max(m, exp_of(v));
+    }
+    if (bad) *nonfinite = 1;
     if (m != INT_MIN) atomicMax(e + x, m);
 }
 
@@ -423,6 +436,7 @@ SplitPlan plan_split(int64_t M, int64_t N, int64_t K, int slices, int carrier, i
     pl.off_exp_a = (slice_bytes + 255) & ~size_t(255);
     pl.off_exp_b = pl.off_exp_a + ((size_t)M * 4 + 255) / 256 * 256;
     pl.off_err = pl.off_exp_b + ((size_t)N * 4 + 255) / 256 * 256;
+    pl.off_flag = pl.off_err + 128;
     pl.off_residual = pl.off_err + 256;
     pl.bytes = pl.off_residual + split_residual_bytes(N, K, (int)std::min<int64_t>(check_rows, M));
 
@@ -439,7 +453,8 @@ hipError_t launch_split_gemm(const SplitPlan &pl, int opA, int opB, int64_t M, i
     const int S = pl.S;
     const bool fp16 = pl.carrier == SPLIT_FP16;
 
-    if (hipError_t e = launch_exponents(opA, opB, M, N, K, A, lda, B, ldb, ea, eb, stream); e != hipSuccess)
+    if (hipError_t e = launch_exponents(opA, opB, M, N, K, A, lda, B, ldb, ea, eb, (int *)(w + pl.off_flag), stream);
+        e != hipSuccess)
         return e;
 
     // digit slices
@@ -490,23 +505,24 @@ hipError_t launch_split_gemm(const SplitPlan &pl, int opA, int opB, int64_t M, i
 }
 
 hipError_t launch_exponents(int opA, int opB, int64_t M, int64_t N, int64_t K, const double *A, int64_t lda,
-                            const double *B, int64_t ldb, int *ea, int *eb, hipStream_t stream) {
+                            const double *B, int64_t ldb, int *ea, int *eb, int *nonfinite, hipStream_t stream) {
     hipLaunchKernelGGL(fill_int_kernel, dim3(64), dim3(256), 0, stream, ea, M, INT_MIN);
     hipLaunchKernelGGL(fill_int_kernel, dim3(64), dim3(256), 0, stream, eb, N, INT_MIN);
+    hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(256), 0, stream, nonfinite, 1, 0);
     const int64_t kchunk = 256;
     if (M > 0 && K > 0) {
         if (opA == 0)
-            hipLaunchKernelGGL(exp_lines_kernel, dim3((unsigned)M), dim3(256), 0, stream, A, lda, K, ea);
+            hipLaunchKernelGGL(exp_lines_kernel, dim3((unsigned)M), dim3(256), 0, stream, A, lda, K, ea, nonfinite);
         else
             hipLaunchKernelGGL(exp_cols_kernel, dim3((unsigned)((M + 255) / 256), (unsigned)((K + kchunk - 1) / kchunk)),
-                               dim3(256), 0, stream, A, lda, K, M, kchunk, ea);
+                               dim3(256), 0, stream, A, lda, K, M, kchunk, ea, nonfinite);
     }
     if (N > 0 && K > 0) {
         if (opB == 1)
-            hipLaunchKernelGGL(exp_lines_kernel, dim3((unsigned)N), dim3(256), 0, stream, B, ldb, K, eb);
+            hipLaunchKernelGGL(exp_lines_kernel, dim3((unsigned)N), dim3(256), 0, stream, B, ldb, K, eb, nonfinite);
         else
             hipLaunchKernelGGL(exp_cols_kernel, dim3((unsigned)((N + 255) / 256), (unsigned)((K + kchunk - 1) / kchunk)),
-                               dim3(256), 0, stream, B, ldb, K, N, kchunk, eb);
+                               dim3(256), 0, stream, B, ldb, K, N, kchunk, eb, nonfinite);
     }
     return hipGetLastError();
 }

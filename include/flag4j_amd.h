@@ -205,7 +205,9 @@ int f4j_grouped_plan_create_split(int is_complex, int opA, int opB, int64_t coun
                                   double *const *C, const int64_t *ldc, int precision, int slices,
                                   int use3m, f4j_grouped_plan **plan);
 /* After a run of a split plan: err[p] = max |C - AB| / (|A||B|) over the checked rows of problem p
- * (-1 if none), *products = real slice products per run.  Waits for the last run. */
+ * (-1 if none; NaN if problem p's operands hold NaN or Inf, which digit slices cannot carry: its C is
+ * then not meaningful, recompute it in FP64), *products = real slice products per run.  Waits for
+ * the last run. */
 int f4j_grouped_plan_split_report(f4j_grouped_plan *plan, double *err, int64_t count, int *products);
 int f4j_grouped_plan_run(f4j_grouped_plan *plan, void *stream);
 int f4j_grouped_plan_destroy(f4j_grouped_plan *plan);

@@ -209,3 +209,16 @@ def test_create_with_split_precision_defaults():
             L.check(lib.f4j_grouped_plan_destroy(h))
         for b in dA + dB + dC:
             b.close()
+
+
+def test_nonfinite_operands_are_reported():
+    """A product whose operands hold NaN or Inf is reported with a NaN error (its
+    digit-slice result is not meaningful); the other products are unaffected."""
+    As, Bs = problems(700, 3, 32, 96, "N", "N", False)
+    As[1] = As[1].copy()
+    As[1][0, 0] = np.nan
+    Cs, (errs, _) = grouped.mult_all(As, Bs, "N", "N", "split_bf16", 7, report=True)
+    assert np.isnan(errs[1])
+    assert np.all(np.isfinite(errs[[0, 2]]))
+    for t in (0, 2):
+        assert max_err(Cs[t], As[t], Bs[t], "N", "N", False) <= 1e-15

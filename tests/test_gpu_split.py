@@ -368,3 +368,45 @@ def test_complex_crt_gaussian_integers_exact():
     assert np.array_equal(C.real, np.array(re, dtype=np.float64))
     assert np.array_equal(C.imag, np.array(im, dtype=np.float64))
     assert rep.precision == L.PREC["split_int8"]
+
+
+def same_nonfinite(C, ref):
+    """C and ref have NaN, +Inf and -Inf in exactly the same places."""
+    return (np.array_equal(np.isnan(C), np.isnan(ref)) and np.array_equal(np.isposinf(C), np.isposinf(ref))
+            and np.array_equal(np.isneginf(C), np.isneginf(ref)))
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+@pytest.mark.parametrize("prec", ["split_int8", "split_bf16", "split_fp16"])
+def test_nonfinite_inputs_take_the_fp64_path(prec, cplx):
+    """NaN and Inf entries (outside the rows the residual check samples) cannot be
+    carried by residues or digit slices: the exponent pass flags them and the
+    product is computed in FP64, whose NaN / +-Inf pattern is the reference
+    arithmetic's; finite entries within the FP64 bound 2 K u (|A||B|)."""
+    r = np.random.default_rng(17)
+    M, N, K = 300, 257, 400
+    A, B = r.standard_normal((M, K)), r.standard_normal((K, N))
+    if cplx:
+        A = A + 1j * r.standard_normal((M, K))
+        B = B + 1j * r.standard_normal((K, N))
+    A[123, 45] = np.inf
+    A[201, 7] = np.nan
+    B[300, 77] = -np.inf
+    if cplx:
+        C = f4j.CMatrix(A).mult(f4j.CMatrix(B), precision=prec).numpy()
+        ref = o.cplx_gemm(A, B)
+        parts = [(C.real, ref.real), (C.imag, ref.imag)]
+        absA, absB = np.abs(A.real) + np.abs(A.imag), np.abs(B.real) + np.abs(B.imag)
+    else:
+        C, _ = split_mult(A, B, prec, 0)
+        ref = o.real_reordered(A, B)
+        parts = [(C, ref)]
+        absA, absB = np.abs(A), np.abs(B)
+    assert L.last_split_report().precision == L.PREC["fp64"]
+    with np.errstate(invalid="ignore"):
+        bound = (2 if cplx else 1) * 2 * K * 2.0**-53 * (np.where(np.isfinite(absA), absA, 0) @
+                                                          np.where(np.isfinite(absB), absB, 0))
+    for c, rf in parts:
+        assert same_nonfinite(c, rf)
+        fin = np.isfinite(rf)
+        assert np.all(np.abs(c[fin] - rf[fin]) <= bound[fin])
