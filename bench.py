@@ -161,6 +161,16 @@ def bench_batched(np, torch, f4j, rng, dev, count=10000, reps=10):
     out["split_bf16"] = {"slices": 7, "ms": round(ms_s, 4), "tflops_fp64_accurate": round(flops / ms_s / 1e9, 2),
                          "real_slice_products": prods, "max_err_checked_rows": float(np.max(errs)),
                          "products_over_1e-15": int(np.sum(errs > 1e-15)), "checked_rows_per_product": 1}
+    f4j._native.config_set("splitCheckRows", 1)
+    plan = grouped.GroupedPlan(*args, precision="split_int8")
+    ms_i = timed(plan)
+    errs, prods = plan.split_report()
+    plan.close()
+    f4j._native.config_set("splitCheckRows", 4)
+    out["split_int8"] = {"moduli": prods // count, "ms": round(ms_i, 4),
+                         "tflops_fp64_accurate": round(flops / ms_i / 1e9, 2), "real_residue_products": prods,
+                         "max_err_checked_rows": float(np.max(errs)), "products_over_1e-15": int(np.sum(errs > 1e-15)),
+                         "checked_rows_per_product": 1}
     return out
 
 
@@ -209,6 +219,10 @@ def bench_complex_batched(np, torch, f4j, rng, dev, count=2000, reps=3, slices=8
     ms_s = timed(plan)
     errs, prods = plan.split_report()
     plan.close()
+    plan = grouped.GroupedPlan(*args, precision="split_int8", slices=18, use3m=True)   # t >= 64: the wide range
+    ms_i = timed(plan)
+    errs_i, prods_i = plan.split_report()
+    plan.close()
     f4j._native.config_set("splitCheckRows", 4)
     plan = grouped.GroupedPlan(*args)
     plan.tune(1)
@@ -221,7 +235,11 @@ def bench_complex_batched(np, torch, f4j, rng, dev, count=2000, reps=3, slices=8
             "tflops_classical": round(flops / ms_s / 1e9, 2), "real_slice_products": prods,
             "max_err_checked_rows": float(np.max(errs)), "products_over_1e-15": int(np.sum(errs > 1e-15)),
             "checked_rows_per_product": 1, "fp64_grouped_ms": round(ms_d, 3),
-            "fp64_grouped_tflops": round(flops / ms_d / 1e9, 2)}
+            "fp64_grouped_tflops": round(flops / ms_d / 1e9, 2),
+            "split_int8": {"form": "3M on INT8 residues", "moduli": prods_i // (3 * count), "ms": round(ms_i, 3),
+                           "tflops_classical": round(flops / ms_i / 1e9, 2), "real_residue_products": prods_i,
+                           "max_err_checked_rows": float(np.max(errs_i)),
+                           "products_over_1e-15": int(np.sum(errs_i > 1e-15)), "checked_rows_per_product": 1}}
 
 
 def bench_gemv(np, torch, f4j, dA, x, stream, reps=50):

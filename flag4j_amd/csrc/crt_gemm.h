@@ -34,6 +34,39 @@ struct CrtPlan {
     size_t off_flag = 0;                 // int: set by the exponent kernels when an operand holds NaN or Inf
 };
 
+// C' of one entry from its chunk sums S_p = sum_l r_l w[l][p] (exact FP64
+// integers): q = rint(sum_p S_p c_p) is the multiple of M to remove and
+// D_p = S_p - q mu_p are exact; D_p scale_p are exact too, and the lower chunks
+// are summed in double-double (TwoSum) because they cancel against the top
+// chunk, so C' is rounded once, relative to itself.
+static_assert(CRT_MAX_CHUNKS == 4, "crt_recombine takes four chunk sums");
+__device__ inline double crt_recombine(double s0, double s1, double s2, double s3, const CrtTables &tb) {
+    const double S[CRT_MAX_CHUNKS] = {s0, s1, s2, s3};
+    double approx = 0.0;
+#pragma unroll
+    for (int p = 0; p < CRT_MAX_CHUNKS; ++p) approx = fma(S[p], tb.c[p], approx);
+    const double q = rint(approx);
+    double hi = 0.0, lo = 0.0;
+#pragma unroll
+    for (int p = CRT_MAX_CHUNKS - 1; p >= 1; --p) {
+        const double v = fma(-q, tb.mu[p], S[p]) * tb.scale[p];
+        const double sm = hi + v, z = sm - hi;
+        lo += (hi - (sm - z)) + (v - z);
+        hi = sm;
+    }
+    const double top = fma(-q, tb.mu[0], S[0]) * tb.scale[0];
+    const double sm = top + hi, z = sm - top;
+    return sm + (((top - (sm - z)) + (hi - z)) + lo);
+}
+
+// Bits t of the scaled integer operands for n moduli and inner size K: the
+// largest t with K 2^(2t) < M/4 (below 1: too few moduli for this K).
+int crt_bits(int n, int64_t K);
+
+// Reconstruction tables for n moduli with t = crt_bits(n, K); false if n is
+// outside [2, CRT_MAX_MODULI] or t < 1.
+bool crt_tables(int n, int64_t K, CrtTables *tb);
+
 // Smallest number of moduli whose product gives t >= 53 bits for inner size K
 // (every entry that is within a factor 2 of its line's largest keeps all its bits).
 int crt_default_moduli(int64_t K);

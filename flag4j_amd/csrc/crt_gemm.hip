@@ -143,11 +143,8 @@ int t_for(int n, int64_t K) {
 // Residue slices: A'(x, k) mod m_l as signed bytes, fragment-major (below).
 // ------------------------------------------------------------------------
 
-__device__ inline double scaled_int(double v, int e, int t) { return e == INT_MIN ? 0.0 : rint(ldexp(v, t - e)); }
-
-__device__ inline unsigned pack4(int a, int b, int c, int d) {
-    return (unsigned)(a & 255) | ((unsigned)(b & 255) << 8) | ((unsigned)(c & 255) << 16) | ((unsigned)d << 24);
-}
+using crt::pack4;
+using crt::scaled_int;
 
 __device__ inline double opnd_value(const CrtOperand &o, int64_t x, int64_t k) {
     const double *p = o.X + x * o.sx + k * o.sk;
@@ -526,27 +523,8 @@ __global__ __launch_bounds__(256) void crt_reconstruct_kernel(const unsigned cha
         if (e < cnt) {
             const int ea = ea_e[e], eb = eb_e[e];
             double out = 0.0;
-            if (ea != INT_MIN && eb != INT_MIN) {
-                double approx = 0.0;
-#pragma unroll
-                for (int p = 0; p < CRT_MAX_CHUNKS; ++p) approx = fma(S[p][e], tb.c[p], approx);
-                const double q = rint(approx);
-                // D_p scale_p are exact; the lower chunks are summed in
-                // double-double (TwoSum) because they cancel against the top
-                // chunk, so C' is rounded once, relative to itself.
-                double hi = 0.0, lo = 0.0;
-#pragma unroll
-                for (int p = CRT_MAX_CHUNKS - 1; p >= 1; --p) {
-                    const double v = fma(-q, tb.mu[p], S[p][e]) * tb.scale[p];
-                    const double sm = hi + v, z = sm - hi;
-                    lo += (hi - (sm - z)) + (v - z);
-                    hi = sm;
-                }
-                const double top = fma(-q, tb.mu[0], S[0][e]) * tb.scale[0];
-                const double sm = top + hi, z = sm - top;
-                const double acc = sm + (((top - (sm - z)) + (hi - z)) + lo);
-                out = ldexp(acc, ea + eb - 2 * tb.t);
-            }
+            if (ea != INT_MIN && eb != INT_MIN)
+                out = ldexp(crt_recombine(S[0][e], S[1][e], S[2][e], S[3][e], tb), ea + eb - 2 * tb.t);
             if (cplx) {
                 double *c = C + 2 * (i * ldc + j);
                 c[0] = first ? cre * out : c[0] + cre * out;
@@ -570,12 +548,13 @@ int crt_default_moduli(int64_t K) {
     return CRT_MAX_MODULI;
 }
 
-bool plan_crt(int64_t M, int64_t N, int64_t K, int n, int check_rows, CrtPlan *plan) {
+int crt_bits(int n, int64_t K) { return t_for(n, K); }
+
+bool crt_tables(int n, int64_t K, CrtTables *out) {
     if (n < 2 || n > CRT_MAX_MODULI) return false;
     const int t = t_for(n, K);
     if (t < 1) return false;
-    CrtPlan pl;
-    CrtTables &tb = pl.tb;
+    CrtTables tb{};
     tb.n = n;
     tb.t = t;
     Big Mb = big_from(1);
@@ -609,6 +588,13 @@ bool plan_crt(int64_t M, int64_t N, int64_t K, int n, int check_rows, CrtPlan *p
         const Big W = big_mul_small(Ml, (uint32_t)inv_mod((int)rem, MODULI[l]));
         for (int p = 0; p < tb.P; ++p) tb.w[l][p] = (double)big_bits40(W, 40 * (tb.P - 1 - p) - s);
     }
+    *out = tb;
+    return true;
+}
+
+bool plan_crt(int64_t M, int64_t N, int64_t K, int n, int check_rows, CrtPlan *plan) {
+    CrtPlan pl;
+    if (!crt_tables(n, K, &pl.tb)) return false;
     pl.Mp = (M + CBM - 1) / CBM * CBM;
     pl.Np = (N + CBN - 1) / CBN * CBN;
     pl.Kp = (K + CBK - 1) / CBK * CBK;

@@ -23,6 +23,7 @@
 // 1/m): r in [0, m), or r = m when a is a multiple of m.  The value m is kept:
 // the reconstruction only uses r modulo m (m W_l = 0 modulo M).
 #pragma once
+#include <climits>
 #include <cmath>
 #include <cstdint>
 
@@ -52,10 +53,20 @@ F4J_HD int residue_wide(double v, double m, double minv, double c26) {
     return residue_of(0.0, v1, m, minv, c26);
 }
 
-// Product residue in [0, m].
-F4J_HD int product_residue(int a, double m, double minv) {
+// Product residue in [0, m], as a double (exact integer) and as an int.
+F4J_HD double product_residue_d(int a, double m, double minv) {
     const double d = (double)a;
-    return (int)fma(-floor(d * minv), m, d);
+    return fma(-floor(d * minv), m, d);
+}
+F4J_HD int product_residue(int a, double m, double minv) { return (int)product_residue_d(a, m, minv); }
+
+// Entry v of a line with exponent e (|v| < 2^e; INT_MIN for an all-zero line)
+// as a t-bit integer: rint(v 2^(t - e)), |result| <= 2^t.
+F4J_HD double scaled_int(double v, int e, int t) { return e == INT_MIN ? 0.0 : rint(ldexp(v, t - e)); }
+
+// Four residues as the bytes of one 32-bit word (first in the lowest byte).
+F4J_HD unsigned pack4(int a, int b, int c, int d) {
+    return (unsigned)(a & 255) | ((unsigned)(b & 255) << 8) | ((unsigned)(c & 255) << 16) | ((unsigned)d << 24);
 }
 
 }  // namespace crt

@@ -38,6 +38,7 @@
 #include <vector>
 
 #include "../../include/flag4j_amd.h"
+#include "crt_gemm.h"
 #include "f4j_internal.h"
 #include "gemm_tile.h"
 #include "split_batch.h"
@@ -306,11 +307,11 @@ int f4j_grouped_plan_create(int is_complex, int opA, int opB, int64_t count, con
     if (!out) return fail(F4J_ERR_INVALID_ARG, "out is null");
     *out = nullptr;
     const bool cplx = is_complex != 0;
-    if (precision == F4J_PREC_SPLIT_BF16 || precision == F4J_PREC_SPLIT_FP16)
+    if (precision == F4J_PREC_SPLIT_BF16 || precision == F4J_PREC_SPLIT_FP16 || precision == F4J_PREC_SPLIT_INT8)
         return f4j_grouped_plan_create_split(is_complex, opA, opB, count, M, N, K, A, lda, B, ldb, C, ldc, precision,
                                              0, 1, out);
     if (precision != F4J_PREC_FP64 && precision != F4J_PREC_AUTO)
-        return fail(F4J_ERR_UNSUPPORTED, "grouped products: precision must be FP64, SPLIT_BF16 or SPLIT_FP16");
+        return fail(F4J_ERR_UNSUPPORTED, "grouped products: precision must be FP64, SPLIT_BF16, SPLIT_FP16 or SPLIT_INT8");
     if (const int rc = check_problems(cplx, opA, opB, count, M, N, K, A, lda, B, ldb, C, ldc); rc != F4J_OK)
         return rc;
     if (const int rc = runtime_ready(); rc != F4J_OK) return rc;
@@ -460,17 +461,21 @@ int f4j_grouped_plan_create_split(int is_complex, int opA, int opB, int64_t coun
     if (!out) return fail(F4J_ERR_INVALID_ARG, "out is null");
     *out = nullptr;
     const bool cplx = is_complex != 0;
-    if (precision != F4J_PREC_SPLIT_BF16 && precision != F4J_PREC_SPLIT_FP16)
-        return fail(F4J_ERR_INVALID_ARG, "split plans take precision SPLIT_BF16 or SPLIT_FP16");
-    if (slices < 0 || slices > SPLIT_MAX_SLICES) return fail(F4J_ERR_INVALID_ARG, "slices must be in 0..8");
+    if (precision != F4J_PREC_SPLIT_BF16 && precision != F4J_PREC_SPLIT_FP16 && precision != F4J_PREC_SPLIT_INT8)
+        return fail(F4J_ERR_INVALID_ARG, "split plans take precision SPLIT_BF16, SPLIT_FP16 or SPLIT_INT8");
+    const bool int8 = precision == F4J_PREC_SPLIT_INT8;
+    if (!int8 && (slices < 0 || slices > SPLIT_MAX_SLICES)) return fail(F4J_ERR_INVALID_ARG, "slices must be in 0..8");
+    if (int8 && (slices < 0 || slices > CRT_MAX_MODULI))
+        return fail(F4J_ERR_INVALID_ARG, "moduli must be 0 or in 2.." + std::to_string(CRT_MAX_MODULI));
     if (const int rc = check_problems(cplx, opA, opB, count, M, N, K, A, lda, B, ldb, C, ldc); rc != F4J_OK)
         return rc;
     if (const int rc = runtime_ready(); rc != F4J_OK) return rc;
     char buf[32];
     int check_rows = 4;
     if (f4j_config_get("splitCheckRows", buf, sizeof(buf)) == F4J_OK) check_rows = std::atoi(buf);
-    const int carrier = precision == F4J_PREC_SPLIT_FP16 ? SPLIT_FP16 : SPLIT_BF16;
-    const int S = slices > 0 ? slices : (carrier == SPLIT_FP16 ? 5 : 6);
+    const int carrier = int8 ? SPLIT_INT8 : precision == F4J_PREC_SPLIT_FP16 ? SPLIT_FP16 : SPLIT_BF16;
+    // digit forms: 0 = 6 (BF16) / 5 (FP16) digits; modular form: 0 = chosen from the largest K
+    const int S = (slices > 0 || int8) ? slices : (carrier == SPLIT_FP16 ? 5 : 6);
     std::vector<BatchProblem> probs(count);
     f4j_grouped_plan *pl = new f4j_grouped_plan;
     pl->is_complex = is_complex;

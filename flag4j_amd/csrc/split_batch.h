@@ -24,9 +24,11 @@ struct BatchProblem {
 struct SplitBatch;
 
 // Plan a batch on `carrier` (SPLIT_BF16 / SPLIT_FP16) with `slices` digits per
-// operand.  Complex products use the 3M form (three real products) when
-// use3m, else four.  `check_rows` rows of every product are verified by the
-// residual check (0 = off).  On failure returns nullptr with *status and *err set.
+// operand, or on SPLIT_INT8 (the modular form) with `slices` moduli (0: the
+// fewest that give t >= 53 bits at the batch's largest K).  Complex products
+// use the 3M form (three real products) when use3m, else four.  `check_rows`
+// rows of every product are verified by the residual check (0 = off).  On
+// failure returns nullptr with *status and *err set.
 SplitBatch *split_batch_create(bool cplx, int opA, int opB, int carrier, int slices, bool use3m, int check_rows,
                                const std::vector<BatchProblem> &probs, std::string *err, int *status);
 // Enqueue the whole batch on `stream` (no allocation, no memset: capturable).
@@ -37,7 +39,7 @@ hipError_t split_batch_enqueue(const SplitBatch *b, hipStream_t stream, hipEvent
 // (-1 where nothing was checked; NaN where the problem's operands hold NaN or
 // Inf, which the digit slices cannot carry); synchronises `stream`.
 hipError_t split_batch_errors(const SplitBatch *b, double *out, int64_t count, hipStream_t stream);
-int split_batch_real_products(const SplitBatch *b);     // real slice products per run (jobs x pairs)
+int split_batch_real_products(const SplitBatch *b);     // real slice products per run (jobs x pairs, or jobs x moduli)
 int split_batch_checked_rows(const SplitBatch *b);
 size_t split_batch_bytes(const SplitBatch *b);          // device memory held
 void split_batch_destroy(SplitBatch *b);

@@ -25,6 +25,16 @@
 //                  each into C with the job's coefficients, so no two
 //                  workgroups write the same element
 //   bres_kernel    compensated (Dot2) check of sampled rows, per product
+//
+// Modular form (carrier SPLIT_INT8, the arithmetic of crt_gemm.hip): lines are
+// scaled to t-bit integers (t from each product's K, at most 58) and stored as
+// residues modulo n coprime moduli <= 256 in the fragment-major layout of the
+// INT8 MFMA (crt_layout.h).  bcrt_split_kernel writes the residues of every
+// side; bcrt_kernel runs every 64 x 64 output tile of every product: per job
+// and modulus an exact int32 product on v_mfma_i32_32x32x32_i8, reduced mod m
+// and accumulated straight into the entry's CRT chunk sums (exact FP64
+// integers), recombined after the last modulus and added into C.  The product
+// residues never leave the registers.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -36,6 +46,9 @@
 #include <vector>
 
 #include "../../include/flag4j_amd.h"
+#include "crt_gemm.h"
+#include "crt_layout.h"
+#include "crt_residue.h"
 #include "f4j_internal.h"
 #include "split_batch.h"
 #include "split_gemm.h"
@@ -55,8 +68,10 @@ struct Side {                 // one operand of one real job
     int cplx, contig;         // contig: consecutive k are adjacent elements
     int64_t nx, K, Xp, Kp;    // lines, inner size, padded to the tile / the k step
     int64_t dig;              // offset (carrier elements) of the digit slices [S][Xp][Kp]
+                              // (modular form: bytes of the residues [n][Xp][Kp], fragment-major)
     int64_t eoff;             // offset of the line exponents [Xp]
     int prob;                 // the product's slot in the per-product error array
+    int t;                    // modular form: bits of the scaled integers
 };
 
 // Error-array value that marks a product whose operands hold NaN or Inf: the
@@ -297,6 +312,218 @@ __global__ __launch_bounds__(256) void bres_kernel(const Prob *__restrict__ prob
     atomicMax(err + w.x, __builtin_bit_cast(unsigned long long, e));
 }
 
+// ---- modular (INT8) form ----
+
+typedef int i32x4 __attribute__((ext_vector_type(4)));
+typedef int i32x16 __attribute__((ext_vector_type(16)));
+
+// Work item (side, x0, k0, -): the 32-line x 32-k fragment of every modulus,
+// one wave per item, four per block.  WIDE (some t > 58): residues by the
+// exact reduction step for |v| up to 2^75 (crt::residue_wide).  Each wave stages its 32 x 32 values
+// through LDS (reads along k for contiguous sides, along the lines for strided
+// ones); lane L then takes line L % 32 and k 16 (L / 32) .. + 15 and writes its
+// 16 bytes of the fragment of every modulus (crt::frag_offset).
+template <bool WIDE>
+__global__ __launch_bounds__(256) void bcrt_split_kernel(const Side *__restrict__ sides, const int4 *__restrict__ work,
+                                                         int64_t nwork, const int *__restrict__ epool,
+                                                         signed char *__restrict__ rpool, CrtTables tb) {
+    __shared__ double tile[4][32][33];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t wi = (int64_t)blockIdx.x * 4 + wave;
+    const bool active = wi < nwork;
+    const int4 w = work[active ? wi : 0];
+    const Side s = sides[w.x];
+    const int64_t x0 = w.y, k0 = w.z;
+    if (active) {
+        if (s.contig) {
+            const int c = lane & 31;
+            for (int r = lane >> 5; r < 32; r += 2) {
+                const int64_t x = x0 + r, k = k0 + c;
+                tile[wave][r][c] = (x < s.nx && k < s.K) ? side_value(s, x, k) : 0.0;
+            }
+        } else {
+            const int r = lane & 31;
+            for (int c = lane >> 5; c < 32; c += 2) {
+                const int64_t x = x0 + r, k = k0 + c;
+                tile[wave][r][c] = (x < s.nx && k < s.K) ? side_value(s, x, k) : 0.0;
+            }
+        }
+    }
+    __syncthreads();
+    if (!active) return;
+    const int r = lane & 31, kb = 16 * (lane >> 5);
+    const int64_t x = x0 + r;
+    const int ex = epool[s.eoff + x];
+    double vh[16], vl[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+        const double v = crt::scaled_int(tile[wave][r][kb + u], ex, s.t);
+        if (WIDE) {
+            vh[u] = v;                             // kept whole: the wide step reduces it per modulus
+            vl[u] = 0.0;
+        } else {
+            crt::residue_split(v, vh[u], vl[u]);
+        }
+    }
+    int4 *dst = (int4 *)(rpool + s.dig + crt::frag_offset(x, k0 + kb, s.Kp >> 5));
+    const int64_t lstride = s.Xp * s.Kp / 16;
+    for (int l = 0; l < tb.n; ++l) {
+        const double m = (double)tb.m[l], mi = tb.minv[l], c26 = tb.c26[l];
+        int rs[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u)
+            rs[u] = WIDE ? crt::residue_wide(vh[u], m, mi, c26) : crt::residue_of(vh[u], vl[u], m, mi, c26);
+        int4 v;
+        v.x = (int)crt::pack4(rs[0], rs[1], rs[2], rs[3]);
+        v.y = (int)crt::pack4(rs[4], rs[5], rs[6], rs[7]);
+        v.z = (int)crt::pack4(rs[8], rs[9], rs[10], rs[11]);
+        v.w = (int)crt::pack4(rs[12], rs[13], rs[14], rs[15]);
+        dst[l * lstride] = v;
+    }
+}
+
+// Persistent residue-product kernel with the reconstruction fused: workgroup
+// w runs output tiles begin[w] .. begin[w+1] (64 x 64, 2 x 2 waves of 32 x 32).
+// For each job of the tile's product and each modulus l, a wave forms the
+// exact int32 product of its 32 x 32 block over k, reduces it mod m_l
+// (crt::product_residue_d: r in [0, m_l]) and adds r w[l][p] into the entry's
+// chunk sums S_p (exact: below 2^48 per term and 2^53 over 20 moduli).  After
+// the last modulus each entry is recombined (crt_recombine), scaled by
+// 2^(eA + eB - 2t) and added into C with the job's coefficients; no two
+// workgroups write one element.
+//
+// Operands: a job is a run of n KC k blocks ("steps": modulus-major, k
+// blocks within), each one A and one B fragment of 1 KB, contiguous in the
+// fragment-major residues.  Every wave streams its own fragments through a
+// private LDS ring of RING steps by buffer loads that write LDS directly, so it
+// only waits for its own loads (vmcnt) and the block needs no barrier.  Every
+// step issues the same two loads (past the job's last step they repeat it into
+// the slot just read) so the wait count stays exact.  The k blocks alternate
+// between two accumulators, so consecutive MFMAs do not depend on each other.
+constexpr int RING = 8;                       // steps in flight per wave
+constexpr int STEP = 2 * crt::FRAG;           // LDS bytes of one step (A, then B fragment)
+typedef __attribute__((address_space(3))) void lds_void;
+#define F4J_WAIT_VMCNT(n) __builtin_amdgcn_s_waitcnt(((n) & 15) | (7 << 4) | (15 << 8) | (((n) >> 4) << 14))
+
+__global__ __launch_bounds__(256, 2) void bcrt_kernel(const Item *__restrict__ items, const int *__restrict__ begin,
+                                                      const Prob *__restrict__ probs, const Job *__restrict__ jobs,
+                                                      const Side *__restrict__ sides,
+                                                      const signed char *__restrict__ rpool,
+                                                      const int *__restrict__ epool, CrtTables tb, int cplx) {
+    __shared__ __attribute__((aligned(1024))) signed char lds[4 * RING * STEP];
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    signed char *ring = lds + wave * RING * STEP;
+    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+    const int it1 = begin[blockIdx.x + 1];
+    for (int it = begin[blockIdx.x]; it < it1; ++it) {
+        const Item item = items[it];
+        const Prob &p = probs[item.prob];
+        const int64_t M = p.M, N = p.N, ldc = p.ldc;
+        double *C = p.C;
+        const int64_t m0 = (int64_t)item.tm * BTile::BM, n0 = (int64_t)item.tn * BTile::BN;
+        for (int jb = 0; jb < p.njobs; ++jb) {
+            const Job jo = jobs[p.job0 + jb];
+            const Side &sa = sides[jo.sa];
+            const Side &sb = sides[jo.sb];
+            const int KC = (int)(sa.Kp >> 5);     // k blocks per modulus (even: Kp is a multiple of 64)
+            const int total = tb.n * KC;
+            // this wave's line block of modulus 0; planes of successive moduli follow
+            const signed char *abase = rpool + sa.dig + ((((m0 + wm) >> 5) * KC) << 10);
+            const signed char *bbase = rpool + sb.dig + ((((n0 + wn) >> 5) * KC) << 10);
+            const int64_t la = sa.Xp * sa.Kp, lb = sb.Xp * sb.Kp;
+            const signed char *acur = abase, *bcur = bbase;   // modulus of the next step to load
+            int iks = 0, issued = 0;
+            auto issue = [&](int slot) {
+                const __amdgpu_buffer_rsrc_t ra =
+                    __builtin_amdgcn_make_buffer_rsrc((void *)acur, (short)0, 0x7FFFFFF0, 0x00020000);
+                const __amdgpu_buffer_rsrc_t rb =
+                    __builtin_amdgcn_make_buffer_rsrc((void *)bcur, (short)0, 0x7FFFFFF0, 0x00020000);
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(ra, (lds_void *)(ring + slot * STEP), 16, 16 * lane,
+                                                         iks << 10, 0, 0);
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(rb, (lds_void *)(ring + slot * STEP + crt::FRAG), 16,
+                                                         16 * lane, iks << 10, 0, 0);
+                if (++issued < total && ++iks == KC) {   // past the last step: repeat it
+                    iks = 0;
+                    acur += la;
+                    bcur += lb;
+                }
+            };
+            auto frag = [&](int slot, int part) { return *(const i32x4 *)(ring + slot * STEP + part * crt::FRAG + 16 * lane); };
+#pragma unroll
+            for (int j = 0; j < RING - 1; ++j) issue(j);
+            double S0[16], S1[16], S2[16], S3[16];
+#pragma unroll
+            for (int r = 0; r < 16; ++r) S0[r] = S1[r] = S2[r] = S3[r] = 0.0;
+            i32x16 acc0, acc1;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc0[r] = acc1[r] = 0;
+            // Step s: load step s + RING - 1 into the slot read one step ago, wait
+            // for step s + 1 and read its fragments, then run step s's MFMA on the
+            // fragments read one step earlier (LDS latency beside the MFMA).
+            F4J_WAIT_VMCNT(2 * (RING - 2));
+            i32x4 fa = frag(0, 0), fb = frag(0, 1);
+            int cks = 0, cl = 0;                   // step being multiplied
+            for (int st = 0; st < total; st += 2) {
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const int s1 = st + h + 1;
+                    // the previous MFMA (and the wait for its fragments' reads) stays
+                    // above the load that overwrites their slot
+                    __builtin_amdgcn_sched_barrier(0);
+                    issue((s1 + RING - 2) & (RING - 1));
+                    F4J_WAIT_VMCNT(2 * (RING - 2));  // step s + 1's loads are in LDS
+                    const i32x4 na = frag(s1 & (RING - 1), 0), nb = frag(s1 & (RING - 1), 1);
+                    if (h == 0)
+                        acc0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa, fb, acc0, 0, 0, 0);
+                    else
+                        acc1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa, fb, acc1, 0, 0, 0);
+                    fa = na;
+                    fb = nb;
+                }
+                cks += 2;
+                if (cks == KC) {                   // modulus cl done: fold its residues into S
+                    const double m = (double)tb.m[cl], minv = tb.minv[cl];
+                    const double w0 = tb.w[cl][0], w1 = tb.w[cl][1], w2 = tb.w[cl][2], w3 = tb.w[cl][3];
+#pragma unroll
+                    for (int r = 0; r < 16; ++r) {
+                        const double rr = crt::product_residue_d(acc0[r] + acc1[r], m, minv);
+                        S0[r] = fma(rr, w0, S0[r]);
+                        S1[r] = fma(rr, w1, S1[r]);
+                        S2[r] = fma(rr, w2, S2[r]);
+                        S3[r] = fma(rr, w3, S3[r]);
+                        acc0[r] = acc1[r] = 0;
+                    }
+                    cks = 0;
+                    ++cl;
+                }
+            }
+            F4J_WAIT_VMCNT(0);                     // the repeated loads have landed: the ring is free
+            const int64_t col = n0 + wn + crt::acc_col(0, lane);
+            if (col < N) {
+                const int ebv = epool[sb.eoff + col];
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+                    const int64_t row = m0 + wm + crt::acc_row(0, r, lane);
+                    if (row >= M) continue;
+                    const int eav = epool[sa.eoff + row];
+                    double v = 0.0;
+                    if (eav != INT_MIN && ebv != INT_MIN)
+                        v = ldexp(crt_recombine(S0[r], S1[r], S2[r], S3[r], tb), eav + ebv - sa.t - sb.t);
+                    if (cplx) {
+                        double *c = C + 2 * (row * ldc + col);
+                        c[0] = jb == 0 ? jo.cre * v : c[0] + jo.cre * v;
+                        c[1] = jb == 0 ? jo.cim * v : c[1] + jo.cim * v;
+                    } else {
+                        double *c = C + row * ldc + col;
+                        *c = jb == 0 ? v : *c + v;
+                    }
+                }
+            }
+        }
+    }
+}
+
 template <int R, class T>
 void launch_bsplit(int S, dim3 grid, hipStream_t st, const Side *sides, const int4 *work, const int *epool, T *dpool) {
     switch (S) {
@@ -315,7 +542,9 @@ void launch_bsplit(int S, dim3 grid, hipStream_t st, const Side *sides, const in
 
 struct SplitBatch {
     bool cplx = false;
-    int opA = 0, opB = 0, carrier = SPLIT_BF16, S = 0, check_rows = 0;
+    int opA = 0, opB = 0, carrier = SPLIT_BF16, S = 0, check_rows = 0;   // S: digits, or moduli (INT8)
+    CrtTables tb{};                        // modular form: reconstruction tables for S moduli
+    bool wide = false;                     // modular form: some product has t > 58
     int64_t count = 0;
     int real_products = 0;
     int workers = 0;
@@ -357,8 +586,22 @@ SplitBatch *split_batch_create(bool cplx, int opA, int opB, int carrier, int sli
         *err = m;
         return nullptr;
     };
-    if (slices < 1 || slices > SPLIT_MAX_SLICES) return bad(F4J_ERR_INVALID_ARG, "slices must be in 1..8");
-    const int R = split_radix(carrier);
+    const bool int8 = carrier == SPLIT_INT8;
+    int64_t kmax = 1;
+    for (const BatchProblem &q : probs)
+        if (q.M > 0 && q.N > 0) kmax = std::max(kmax, q.K);
+    CrtTables tb{};
+    if (int8) {
+        // |residue| <= 128, so the int32 products are exact for K < 2^17
+        if (kmax >= (int64_t(1) << 17)) return bad(F4J_ERR_UNSUPPORTED, "modular form: K must be below 131072");
+        if (slices == 0) slices = crt_default_moduli(kmax);
+        if (slices < 2 || slices > CRT_MAX_MODULI)
+            return bad(F4J_ERR_INVALID_ARG, "moduli must be in 2.." + std::to_string(CRT_MAX_MODULI));
+        if (!crt_tables(slices, kmax, &tb))
+            return bad(F4J_ERR_INVALID_ARG, std::to_string(slices) + " moduli leave no bits for K = " + std::to_string(kmax));
+    } else if (slices < 1 || slices > SPLIT_MAX_SLICES) {
+        return bad(F4J_ERR_INVALID_ARG, "slices must be in 1..8");
+    }
     const int w = cplx ? 2 : 1;
     const double sga = (opA == F4J_OP_C) ? -1.0 : 1.0, sgb = (opB == F4J_OP_C) ? -1.0 : 1.0;
 
@@ -368,6 +611,7 @@ SplitBatch *split_batch_create(bool cplx, int opA, int opB, int carrier, int sli
     b->opB = opB;
     b->carrier = carrier;
     b->S = slices;
+    b->tb = tb;
     b->check_rows = check_rows;
     b->count = (int64_t)probs.size();
     b->prob_index.assign(probs.size(), -1);
@@ -397,9 +641,15 @@ SplitBatch *split_batch_create(bool cplx, int opA, int opB, int carrier, int sli
         }
         const int64_t Mp = (q.M + BTile::BM - 1) / BTile::BM * BTile::BM;
         const int64_t Np = (q.N + BTile::BN - 1) / BTile::BN * BTile::BN;
-        const int64_t Kp = (q.K + slice::BK - 1) / slice::BK * slice::BK;
+        const int64_t Kp = (q.K + slice::BK - 1) / slice::BK * slice::BK;   // 64: whole pairs of k blocks
+        // modular form: t from this product's K (more bits for short products), at most
+        // 75, the range of the residue steps (crt_residue.h)
+        const int t = int8 ? std::min(75, crt_bits(slices, q.K)) : 0;
+        b->wide |= t > 58;
         auto git = group_of_kp.find(Kp);
-        if (git == group_of_kp.end()) {
+        if (int8) {
+            git = group_of_kp.emplace(0, std::make_pair(0, 0)).first;     // no level groups
+        } else if (git == group_of_kp.end()) {
             const std::vector<SplitGroup> g = split_groups(slices, carrier, Kp);
             git = group_of_kp.emplace(Kp, std::make_pair((int)groups.size(), (int)g.size())).first;
             groups.insert(groups.end(), g.begin(), g.end());
@@ -443,6 +693,7 @@ SplitBatch *split_batch_create(bool cplx, int opA, int opB, int carrier, int sli
             c.sk = (c.contig ? 1 : q.ldb) * w;
             for (Side *s : {&a, &c}) {
                 s->prob = (int)dprobs.size();
+                s->t = t;
                 s->dig = dig;
                 dig += (int64_t)slices * s->Xp * s->Kp;
                 s->eoff = eoff;
@@ -462,7 +713,7 @@ SplitBatch *split_batch_create(bool cplx, int opA, int opB, int carrier, int sli
         b->prob_index[pi] = (int)dprobs.size();
         dprobs.push_back(pr);
     }
-    b->real_products = (int)jobs.size() * slices * (slices + 1) / 2;
+    b->real_products = (int)jobs.size() * (int8 ? slices : slices * (slices + 1) / 2);
 
     // Work lists.
     std::vector<int2> exp_work;
@@ -471,8 +722,9 @@ SplitBatch *split_batch_create(bool cplx, int opA, int opB, int carrier, int sli
         const Side &s = sides[si];
         const int step = s.contig ? 4 : 256;
         for (int64_t x = 0; x < s.Xp; x += step) exp_work.push_back(make_int2((int)si, (int)x));
+        const int64_t kstep = int8 ? 32 : slice::BK;
         for (int64_t x = 0; x < s.Xp; x += 32)
-            for (int64_t k = 0; k < s.Kp; k += slice::BK) split_work.push_back(make_int4((int)si, (int)x, (int)k, 0));
+            for (int64_t k = 0; k < s.Kp; k += kstep) split_work.push_back(make_int4((int)si, (int)x, (int)k, 0));
     }
     std::vector<int4> res_work;
     if (check_rows > 0)
@@ -490,7 +742,10 @@ SplitBatch *split_batch_create(bool cplx, int opA, int opB, int carrier, int sli
         const Side &sa = sides[jobs[p.job0].sa];
         const int ngrp = jobs[p.job0].ngrp;
         int64_t cost = 0;
-        for (int gi = 0; gi < ngrp; ++gi) cost += (int64_t)groups[jobs[p.job0].grp0 + gi].nseg * (sa.Kp / slice::BK) + 4;
+        if (int8)
+            cost = (int64_t)slices * (sa.Kp / 32 + 4);
+        else
+            for (int gi = 0; gi < ngrp; ++gi) cost += (int64_t)groups[jobs[p.job0].grp0 + gi].nseg * (sa.Kp / slice::BK) + 4;
         cost *= p.njobs;
         const int tmn = (int)((p.M + BTile::BM - 1) / BTile::BM), tnn = (int)((p.N + BTile::BN - 1) / BTile::BN);
         for (int tm = 0; tm < tmn; ++tm)
@@ -503,24 +758,67 @@ SplitBatch *split_batch_create(bool cplx, int opA, int opB, int carrier, int sli
         return bad(F4J_ERR_HIP, "device query failed");
     }
     int occ = 0;
-    const hipError_t oe = carrier == SPLIT_FP16
-                              ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, bgemm_kernel<_Float16, 64>, 256, 0)
-                              : hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, bgemm_kernel<__bf16, 256>, 256, 0);
+    const hipError_t oe =
+        int8 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, bcrt_kernel, 256, 0)
+        : carrier == SPLIT_FP16 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, bgemm_kernel<_Float16, 64>, 256, 0)
+                                : hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, bgemm_kernel<__bf16, 256>, 256, 0);
     if (oe != hipSuccess || occ < 1) occ = 1;
     b->workers = (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)prop.multiProcessorCount * occ,
                                                              (int64_t)items.size()));
+    // Schedule.  Workgroup w of the persistent launch runs on XCD w % 8 (the
+    // hardware deals workgroup ids round-robin over the 8 XCDs, each with its
+    // own L2).  Each product's tiles go to one XCD, so its operand slices are
+    // fetched into that L2 once rather than into several; a product above a
+    // quarter of an XCD's fair share is spread tile by tile instead.  Units
+    // (products, or tiles of the large ones) are dealt longest first to the
+    // least-loaded XCD, then each XCD's tiles longest first to its
+    // least-loaded workgroup.
     std::vector<std::vector<Item>> per(b->workers);
     {
-        std::sort(items.begin(), items.end(), [](const Item &x, const Item &y) { return x.cost > y.cost; });
         using Load = std::pair<int64_t, int>;
-        std::priority_queue<Load, std::vector<Load>, std::greater<Load>> heap;
-        for (int wk = 0; wk < b->workers; ++wk) heap.push({0, wk});
-        for (const Item &it : items) {
-            Load l = heap.top();
-            heap.pop();
-            per[l.second].push_back(it);
-            l.first += it.cost;
-            heap.push(l);
+        auto lpt = [](std::vector<std::pair<int64_t, std::vector<int>>> &units, int bins,
+                      std::vector<std::vector<int>> &out) {
+            std::stable_sort(units.begin(), units.end(), [](const auto &x, const auto &y) { return x.first > y.first; });
+            std::priority_queue<Load, std::vector<Load>, std::greater<Load>> heap;
+            for (int k = 0; k < bins; ++k) heap.push({0, k});
+            out.assign(bins, {});
+            for (const auto &u : units) {
+                Load l = heap.top();
+                heap.pop();
+                out[l.second].insert(out[l.second].end(), u.second.begin(), u.second.end());
+                l.first += u.first;
+                heap.push(l);
+            }
+        };
+        const int nxcd = std::min(8, b->workers);
+        std::map<int, std::vector<int>> of_prob;
+        int64_t total = 0;
+        for (size_t i = 0; i < items.size(); ++i) {
+            of_prob[items[i].prob].push_back((int)i);
+            total += items[i].cost;
+        }
+        const int64_t share = total / (4 * nxcd) + 1;
+        std::vector<std::pair<int64_t, std::vector<int>>> units;
+        for (auto &kv : of_prob) {
+            int64_t c = 0;
+            for (int i : kv.second) c += items[i].cost;
+            if (c <= share) {
+                units.push_back({c, kv.second});
+            } else {
+                for (int i : kv.second) units.push_back({items[i].cost, {i}});
+            }
+        }
+        std::vector<std::vector<int>> xcd_items;
+        lpt(units, nxcd, xcd_items);
+        for (int x = 0; x < nxcd; ++x) {
+            std::vector<int> wk;                   // workgroups on XCD x
+            for (int w = x; w < b->workers; w += nxcd) wk.push_back(w);
+            std::vector<std::pair<int64_t, std::vector<int>>> tiles;
+            for (int i : xcd_items[x]) tiles.push_back({items[i].cost, {i}});
+            std::vector<std::vector<int>> per_wk;
+            lpt(tiles, (int)wk.size(), per_wk);
+            for (size_t k = 0; k < wk.size(); ++k)
+                for (int i : per_wk[k]) per[wk[k]].push_back(items[i]);
         }
     }
     std::vector<Item> sched;
@@ -552,8 +850,9 @@ SplitBatch *split_batch_create(bool cplx, int opA, int opB, int carrier, int sli
     if (e == hipSuccess) e = upload(&b->d_split_work, split_work, &by);
     if (e == hipSuccess) e = upload(&b->d_res_work, res_work, &by);
     if (e == hipSuccess && dig > 0) {
-        e = hipMalloc(&b->d_digits, (size_t)dig * 2);
-        by += (size_t)dig * 2;
+        const size_t db = (size_t)dig * (int8 ? 1 : 2);     // residue bytes, or 2-byte digits
+        e = hipMalloc(&b->d_digits, db);
+        by += db;
     }
     if (e == hipSuccess && eoff > 0) {
         e = hipMalloc((void **)&b->d_exps, (size_t)eoff * sizeof(int));
@@ -569,7 +868,6 @@ SplitBatch *split_batch_create(bool cplx, int opA, int opB, int carrier, int sli
         return bad((e == hipErrorOutOfMemory) ? F4J_ERR_OUT_OF_MEMORY : F4J_ERR_HIP,
                    std::string("batched split plan: ") + hipGetErrorString(e));
     }
-    (void)R;
     return b;
 }
 
@@ -585,6 +883,24 @@ hipError_t split_batch_enqueue(const SplitBatch *b, hipStream_t st, hipEvent_t e
     hipLaunchKernelGGL(fill_u64_kernel, dim3(64), dim3(256), 0, st, b->d_err, nprob);
     hipLaunchKernelGGL(bexp_kernel, dim3((unsigned)b->n_exp), dim3(256), 0, st, b->d_sides, b->d_exp_work, b->d_exps,
                        b->d_err);
+    if (b->carrier == SPLIT_INT8) {
+        const dim3 grid((unsigned)((b->n_split + 3) / 4));
+        if (b->wide)
+            hipLaunchKernelGGL(bcrt_split_kernel<true>, grid, dim3(256), 0, st, b->d_sides, b->d_split_work,
+                               b->n_split, b->d_exps, (signed char *)b->d_digits, b->tb);
+        else
+            hipLaunchKernelGGL(bcrt_split_kernel<false>, grid, dim3(256), 0, st, b->d_sides, b->d_split_work,
+                               b->n_split, b->d_exps, (signed char *)b->d_digits, b->tb);
+        if (ev_begin) (void)hipEventRecord(ev_begin, st);
+        hipLaunchKernelGGL(bcrt_kernel, dim3((unsigned)b->workers), dim3(256), 0, st, b->d_items, b->d_begin,
+                           b->d_probs, b->d_jobs, b->d_sides, (const signed char *)b->d_digits, b->d_exps, b->tb,
+                           (int)b->cplx);
+        if (ev_end) (void)hipEventRecord(ev_end, st);
+        if (b->n_res > 0)
+            hipLaunchKernelGGL(bres_kernel, dim3((unsigned)b->n_res), dim3(256), 0, st, b->d_probs, b->d_res_work,
+                               (int)b->cplx, b->opA, b->opB, b->d_err);
+        return hipGetLastError();
+    }
     if (fp16)
         launch_bsplit<10, _Float16>(b->S, dim3((unsigned)b->n_split), st, b->d_sides, b->d_split_work, b->d_exps,
                                     (_Float16 *)b->d_digits);

@@ -11,7 +11,9 @@ namespace f4j {
 constexpr int SPLIT_MAX_SLICES = 8;
 constexpr int SPLIT_MAX_SEGS = SPLIT_MAX_SLICES * (SPLIT_MAX_SLICES + 1) / 2;
 
-enum SplitCarrier { SPLIT_BF16 = 0, SPLIT_FP16 = 1 };
+// Number carrier of the split forms: BF16 or FP16 digits, or INT8 residues
+// modulo coprime moduli (the modular form; batched plans only here).
+enum SplitCarrier { SPLIT_BF16 = 0, SPLIT_FP16 = 1, SPLIT_INT8 = 2 };
 
 // One launch of the slice-product GEMM: the digit pairs (s, t) whose levels
 // L = s + t lie in [Llo, Lhi], summed exactly in FP64 with weights

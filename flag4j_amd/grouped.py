@@ -19,9 +19,9 @@ class GroupedPlan:
 
     def __init__(self, shapes, a_ptrs, b_ptrs, c_ptrs, lda, ldb, ldc, opA="N", opB="N", complex_=False,
                  precision="fp64", slices=0, use3m=True):
-        """precision "fp64", or "split_bf16" / "split_fp16" for the batched split
-        form (`slices` digits, 0 = default; complex products as three real
-        products when `use3m`, else four)."""
+        """precision "fp64", or "split_bf16" / "split_fp16" / "split_int8" for the
+        batched split forms (`slices` digits or moduli, 0 = default; complex
+        products as three real products when `use3m`, else four)."""
         n = len(shapes)
         M = np.array([s[0] for s in shapes], dtype=np.int64)
         N = np.array([s[1] for s in shapes], dtype=np.int64)
@@ -33,7 +33,7 @@ class GroupedPlan:
         self._keep += [A, B, C, la, lb, lc]
         p = lambda a: ctypes.c_void_p(a.ctypes.data) if a.size else None
         h = ctypes.c_void_p()
-        self.split = precision in ("split_bf16", "split_fp16")
+        self.split = precision in ("split_bf16", "split_fp16", "split_int8")
         if self.split:
             _native.check(_native.load().f4j_grouped_plan_create_split(
                 int(complex_), _native.OP[opA], _native.OP[opB], n, p(M), p(N), p(K), p(A), p(la), p(B), p(lb),

@@ -140,7 +140,8 @@ def test_empty_and_zero_k_problems():
     assert errs[0] == -1.0 and errs[1] == -1.0 and errs[3] == -1.0 and 0 <= errs[2] <= 1e-15
 
 
-def test_plan_replay_is_stable():
+@pytest.mark.parametrize("prec,S", [("split_fp16", 6), ("split_int8", 0)])
+def test_plan_replay_is_stable(prec, S):
     """The captured graph replayed several times, with other device work and
     syncs in between, gives the same bits every time (no stale scratch state)."""
     from flag4j_amd import device
@@ -153,7 +154,7 @@ def test_plan_replay_is_stable():
     dX, dY = device.DeviceBuffer.from_array(X), device.DeviceBuffer(X.nbytes)
     plan = grouped.GroupedPlan(shapes, [a.ptr for a in dA], [b.ptr for b in dB], [c.ptr for c in dC],
                                [s[2] for s in shapes], [s[1] for s in shapes], [s[1] for s in shapes],
-                               "N", "N", True, "split_fp16", 6, True)
+                               "N", "N", True, prec, S, True)
     outs = []
     nan = np.full(max(m * n for m, n, _ in shapes), np.nan + 1j * np.nan)
     try:
@@ -222,3 +223,78 @@ def test_nonfinite_operands_are_reported():
     assert np.all(np.isfinite(errs[[0, 2]]))
     for t in (0, 2):
         assert max_err(Cs[t], As[t], Bs[t], "N", "N", False) <= 1e-15
+
+
+# ---- modular form (SPLIT_INT8): residues modulo coprime moduli on the INT8 matrix cores ----
+
+def moduli_of(prods, jobs):
+    assert prods % jobs == 0
+    return prods // jobs
+
+
+@pytest.mark.parametrize("opA,opB", [("N", "N"), ("T", "N"), ("N", "T"), ("T", "T")])
+def test_int8_real_batch_accuracy(opA, opB):
+    As, Bs = problems(800, 60, 16, 300, opA, opB, False)
+    Cs, (errs, prods) = grouped.mult_all(As, Bs, opA, opB, precision="split_int8", report=True)
+    assert 14 <= moduli_of(prods, 60) <= 16           # the fewest giving t >= 53 at K <= 300
+    for i, (A, B, C) in enumerate(zip(As, Bs, Cs)):
+        assert max_err(C, A, B, opA, opB, False) <= 1e-15, i
+        assert 0.0 <= errs[i] <= 1e-15, (i, errs[i])
+
+
+@pytest.mark.parametrize("opA,opB", [("C", "N"), ("N", "N"), ("T", "C")])
+@pytest.mark.parametrize("use3m", [True, False])
+def test_int8_complex_batch_accuracy(opA, opB, use3m):
+    As, Bs = problems(900, 40, 16, 260, opA, opB, True)
+    Cs, (errs, prods) = grouped.mult_all(As, Bs, opA, opB, precision="split_int8", use3m=use3m, report=True)
+    assert 14 <= moduli_of(prods, 40 * (3 if use3m else 4)) <= 16
+    for i, (A, B, C) in enumerate(zip(As, Bs, Cs)):
+        assert max_err(C, A, B, opA, opB, True) <= 1e-15, i
+        assert 0.0 <= errs[i] <= 1e-15, (i, errs[i])
+
+
+def test_int8_complex_wide_range_3m():
+    """A^H B with parts (2r-1) 10^(4 (2r'-1)), 3M: the small entries of a line keep
+    t - 27 bits, so the wide range takes t = 58 (17 moduli) to reach the bar, as
+    the digit form takes 8 slices instead of its default."""
+    As, Bs = problems(400, 30, 32, 300, "C", "N", True, phi=4)
+    Cs, (errs, _) = grouped.mult_all(As, Bs, "C", "N", precision="split_int8", slices=17, report=True)
+    worst = max(max_err(C, A, B, "C", "N", True) for A, B, C in zip(As, Bs, Cs))
+    assert worst <= 1e-15, worst
+    assert np.all((errs >= 0) & (errs <= 1e-15))
+
+
+def test_int8_small_integers_exact():
+    """Integers |x| <= 127: scaling by a power of two and the residue products are
+    exact, and the reconstruction returns the exact integer product."""
+    r = np.random.default_rng(6)
+    As, Bs = [], []
+    for M, N, K in [(17, 33, 65), (64, 64, 64), (130, 70, 200), (1, 1, 1), (3, 200, 5)]:
+        As.append(r.integers(-127, 128, (M, K)).astype(np.float64))
+        Bs.append(r.integers(-127, 128, (K, N)).astype(np.float64))
+    Cs = grouped.mult_all(As, Bs, precision="split_int8")
+    for A, B, C in zip(As, Bs, Cs):
+        exact = (A.astype(np.int64) @ B.astype(np.int64)).astype(np.float64)
+        assert np.array_equal(C, exact)
+
+
+def test_int8_moduli_bounds():
+    A, B = np.ones((4, 4)), np.ones((4, 4))
+    for bad in (1, 21):
+        with pytest.raises(L.NativeError):
+            grouped.mult_all([A], [B], precision="split_int8", slices=bad)
+    Cs, (_, prods) = grouped.mult_all([A], [B], precision="split_int8", slices=20, report=True)
+    assert prods == 20 and np.array_equal(Cs[0], A @ B)
+
+
+def test_int8_empty_zero_k_and_nonfinite():
+    g = np.random.default_rng(3)
+    As = [np.ones((0, 5)), np.ones((3, 0)), g.random((40, 50)), g.random((70, 20)), g.random((9, 9))]
+    Bs = [np.ones((5, 2)), np.ones((0, 4)), g.random((50, 30)), g.random((20, 65)), g.random((9, 9))]
+    As[4][2, 3] = np.inf
+    Cs, (errs, _) = grouped.mult_all(As, Bs, precision="split_int8", report=True)
+    assert Cs[0].shape == (0, 2) and np.all(Cs[1] == 0.0)
+    for t in (2, 3):
+        assert max_err(Cs[t], As[t], Bs[t], "N", "N", False) <= 1e-15
+        assert 0 <= errs[t] <= 1e-15
+    assert errs[0] == -1.0 and errs[1] == -1.0 and np.isnan(errs[4])
