@@ -194,7 +194,9 @@ int f4j_grouped_plan_create(int is_complex, int opA, int opB, int64_t count,
                             f4j_grouped_plan **plan);
 /* Batched split-precision products (no reference counterpart: Flag4j is FP64-only; the batch
  * replaces a loop of Matrix.mult / A.H().mult(B) calls).  precision F4J_PREC_SPLIT_BF16 or
- * F4J_PREC_SPLIT_FP16; slices 0 = 6 (BF16) / 5 (FP16); complex products run as three real
+ * F4J_PREC_SPLIT_FP16 (slices = digits per operand, 0 = 6 (BF16) / 5 (FP16)), or
+ * F4J_PREC_SPLIT_INT8 (slices = moduli, 2..20; 0 = the fewest giving 53-bit operands at the
+ * batch's largest K; inner sizes below 131072); complex products run as three real
  * products (3M, Gauss) when use3m != 0, else four.  splitCheckRows rows of every product are
  * verified after each run.  f4j_grouped_plan_create with a SPLIT_* precision is this call with
  * slices 0 and use3m 1.  Plans run, tune (a no-op: one tile shape) and destroy like FP64 plans. */
