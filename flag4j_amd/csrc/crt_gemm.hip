@@ -30,6 +30,7 @@
 #include <climits>
 #include <cmath>
 #include <cstdint>
+#include <type_traits>
 #include <vector>
 
 #include "crt_gemm.h"
@@ -323,8 +324,9 @@ typedef __attribute__((address_space(3))) void lds_void;
 // vmcnt(n) only (gfx9 encoding: expcnt and lgkmcnt at their maxima).
 #define F4J_WAIT_VMCNT(n) __builtin_amdgcn_s_waitcnt(((n) & 15) | (7 << 4) | (15 << 8) | (((n) >> 4) << 14))
 
-// Block tile 256 x 256, eight waves of 64 x 128 (2 x 4 MFMA blocks, 128 int32
-// accumulators per lane): wave w takes row group w % 4 and column half w / 4.
+// Block tile BM x 256 (BM = 128 RB), eight waves of 32 RB x 128 (RB x 4 MFMA
+// blocks, 64 RB int32 accumulators per lane): wave w takes row group w % 4 and
+// column half w / 4.
 // Both operands reach LDS by buffer loads that write LDS directly, each
 // fragment once per block (waves 0-3 load A's, 4-7 B's), through a ring of
 // STAGES steps of SK k bytes, loaded up to STAGES steps ahead of the MFMAs,
@@ -343,9 +345,9 @@ __global__ __launch_bounds__(512, 2) void crt_gemm_kernel(CrtGemmArgs a, CrtTabl
     const int64_t m0 = (int64_t)__builtin_amdgcn_readfirstlane((int)tm) * CBM;
     const int64_t n0 = (int64_t)__builtin_amdgcn_readfirstlane((int)tn) * CBN;
 
-    i32x16 acc[2][4];
+    i32x16 acc[crt::RB][4];
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < crt::RB; ++i)
 #pragma unroll
         for (int j = 0; j < 4; ++j)
 #pragma unroll
@@ -353,18 +355,24 @@ __global__ __launch_bounds__(512, 2) void crt_gemm_kernel(CrtGemmArgs a, CrtTabl
 
     const int KC = (int)(a.Kp >> 5);
     // this wave's operand: A's slice (waves 0-3) or B's (waves 4-7) of modulus l
-    const bool is_a = wave < crt::WAVES / 2;
+    const bool is_a = crt::loads_a(wave);
     const signed char *src = is_a ? a.A + (int64_t)l * a.Mp * a.Kp : a.B + (int64_t)l * a.Np * a.Kp;
     const int64_t x0 = is_a ? m0 : n0;
     const int nt = (int)(a.Kp / crt::SK);
 
-    auto issue = [&](int slot, int t) {          // step t into stage slot % STAGES: LOADS fragments per wave
+    auto issue = [&](int slot, int t) {          // step t into stage slot % STAGES
         const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
             (void *)(src + crt::load_base(x0, wave, t, KC)), (short)0, 0x7FFFFFF0, 0x00020000);
+        auto loads = [&](auto count) {
 #pragma unroll
-        for (int p = 0; p < crt::LOADS; ++p)
-            __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, (lds_void *)(lds + crt::load_lds(slot, wave, p)), 16,
-                                                     crt::load_voff(lane), crt::load_soff(p, KC), 0, 0);
+            for (int p = 0; p < decltype(count)::value; ++p)
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, (lds_void *)(lds + crt::load_lds(slot, wave, p)), 16,
+                                                         crt::load_voff(lane), crt::load_soff(p, KC), 0, 0);
+        };
+        if (is_a)                                // uniform: A waves load LOADS_A fragments, B waves LOADS_B
+            loads(std::integral_constant<int, crt::LOADS_A>{});
+        else
+            loads(std::integral_constant<int, crt::LOADS_B>{});
     };
     // Pipelined across steps: the MFMAs of a step's last k step run while the
     // waves meet at the barrier, issue the loads STAGES steps ahead and read the
@@ -372,26 +380,26 @@ __global__ __launch_bounds__(512, 2) void crt_gemm_kernel(CrtGemmArgs a, CrtTabl
     // operands ready.  At the barrier of step t every wave has read all of stage
     // t % STAGES and has its own loads of step t + 1 retired, so after it step
     // t + 1 may be read and step t + STAGES may overwrite stage t % STAGES.
-    // Every step issues the same LOADS loads (past the last step they repeat
-    // step nt - 1 into the free stage), so at that point the LOADS (STAGES - 2)
-    // most recent loads are steps t + 2 .. t + STAGES - 1 and that vmcnt
-    // retires step t + 1.
+    // Every step issues the same loads (past the last step they repeat step
+    // nt - 1 into the free stage); with two stages the wave's loads of step
+    // t + 1 are its most recent ones, so vmcnt(0) retires them.
     static_assert(crt::KB % 2 == 0, "even k steps per stage");
-    i32x4 fa[2][2], fb[2][4];
+    static_assert(crt::STAGES == 2, "the waits below assume two stages");
+    i32x4 fa[2][crt::RB], fb[2][4];
 #pragma unroll
     for (int t = 0; t < crt::STAGES - 1; ++t) issue(t, min(t, nt - 1));
-    F4J_WAIT_VMCNT(crt::LOADS * (crt::STAGES - 2));
+    F4J_WAIT_VMCNT(0);
     __builtin_amdgcn_s_barrier();
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-    for (int i = 0; i < 2; ++i) fa[0][i] = *(const i32x4 *)(lds + crt::a_frag(0, wave, i, 0, lane));
+    for (int i = 0; i < crt::RB; ++i) fa[0][i] = *(const i32x4 *)(lds + crt::a_frag(0, wave, i, 0, lane));
 #pragma unroll
     for (int j = 0; j < 4; ++j) fb[0][j] = *(const i32x4 *)(lds + crt::b_frag(0, wave, j, 0, lane));
     __builtin_amdgcn_sched_barrier(0);
     issue(crt::STAGES - 1, min(crt::STAGES - 1, nt - 1));
     auto mfmas = [&](int buf) {
 #pragma unroll
-        for (int i = 0; i < 2; ++i)
+        for (int i = 0; i < crt::RB; ++i)
 #pragma unroll
             for (int j = 0; j < 4; ++j)
                 acc[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[buf][i], fb[buf][j], acc[i][j], 0, 0, 0);
@@ -401,7 +409,7 @@ __global__ __launch_bounds__(512, 2) void crt_gemm_kernel(CrtGemmArgs a, CrtTabl
         for (int ks = 0; ks + 1 < crt::KB; ++ks) {
             __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-            for (int i = 0; i < 2; ++i)
+            for (int i = 0; i < crt::RB; ++i)
                 fa[(ks + 1) & 1][i] = *(const i32x4 *)(lds + crt::a_frag(t, wave, i, ks + 1, lane));
 #pragma unroll
             for (int j = 0; j < 4; ++j)
@@ -409,7 +417,7 @@ __global__ __launch_bounds__(512, 2) void crt_gemm_kernel(CrtGemmArgs a, CrtTabl
             mfmas(ks & 1);                        // (the compiler interleaves them with those reads)
         }
         __builtin_amdgcn_sched_barrier(0);
-        F4J_WAIT_VMCNT(crt::LOADS * (crt::STAGES - 2));   // this wave's loads of step t + 1 are in LDS
+        F4J_WAIT_VMCNT(0);                        // this wave's loads of step t + 1 are in LDS
         __builtin_amdgcn_s_waitcnt(0xC07F);      // lgkmcnt(0): its reads of stage t % STAGES are done
         __builtin_amdgcn_s_barrier();
         __builtin_amdgcn_sched_barrier(0);
@@ -417,7 +425,7 @@ __global__ __launch_bounds__(512, 2) void crt_gemm_kernel(CrtGemmArgs a, CrtTabl
         __builtin_amdgcn_sched_barrier(0);
         // step t + 1's first fragments (past the last step: the repeated stage, never used)
 #pragma unroll
-        for (int i = 0; i < 2; ++i) fa[0][i] = *(const i32x4 *)(lds + crt::a_frag(t + 1, wave, i, 0, lane));
+        for (int i = 0; i < crt::RB; ++i) fa[0][i] = *(const i32x4 *)(lds + crt::a_frag(t + 1, wave, i, 0, lane));
 #pragma unroll
         for (int j = 0; j < 4; ++j) fb[0][j] = *(const i32x4 *)(lds + crt::b_frag(t + 1, wave, j, 0, lane));
         __builtin_amdgcn_sched_barrier(0);
@@ -431,7 +439,7 @@ __global__ __launch_bounds__(512, 2) void crt_gemm_kernel(CrtGemmArgs a, CrtTabl
     unsigned char *R = a.R + (int64_t)l * a.plane;
     const int64_t rw = m0 + crt::wave_row(wave), cw = n0 + crt::wave_col(wave);
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < crt::RB; ++i)
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const int64_t col = cw + crt::acc_col(j, lane);

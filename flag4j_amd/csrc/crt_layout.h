@@ -19,15 +19,17 @@
 namespace f4j {
 namespace crt {
 
-constexpr int BM = 256, BN = 256;             // product block tile (rows, columns)
+constexpr int RB = 2;                         // 32-row line blocks per wave (wave tile 32 RB x 128)
+constexpr int BM = 4 * 32 * RB, BN = 256;     // product block tile (rows, columns)
 constexpr int BK = 128;                       // k padding unit of the residue slices, bytes
 constexpr int FRAG = 1024;                    // bytes of one 32 x 32 fragment
-constexpr int WAVES = 8;                      // product block: 4 row groups of 64 x 2 column halves of 128
+constexpr int WAVES = 8;                      // product block: 4 row groups of 32 RB x 2 column halves of 128
 constexpr int SK = 128;                       // k bytes per product step
 constexpr int KB = SK / 32;                   // 32-byte MFMA k steps (k blocks) per product step
 constexpr int STAGES = 2;                     // steps in the LDS ring
 constexpr int AHEAD = STAGES - 1;             // steps in flight ahead of the MFMAs
-constexpr int LOADS = 2 * KB;                 // fragment loads per wave per step
+constexpr int LOADS_A = RB * KB;              // fragment loads per step of an A wave (waves 0-3)
+constexpr int LOADS_B = 2 * KB;               // and of a B wave (waves 4-7)
 constexpr int STAGE = (BM + BN) * SK;         // LDS bytes of one step: its A fragments, then its B fragments
 
 F4J_HD int64_t frag_offset(int64_t x, int64_t k, int64_t KC) {
@@ -45,31 +47,36 @@ F4J_HD int split_k(int t) { return 16 * ((t >> 5) & 1) + 32 * (t >> 6); }
 F4J_HD int split_cols_line(int t) { return t & 127; }
 F4J_HD int split_cols_k(int t) { return 16 * (t >> 7); }
 
-// Product kernel.  A block computes a 256 x 256 tile.  Every step t (k bytes
+// Product kernel.  A block computes a BM x 256 tile.  Every step t (k bytes
 // SK t .. SK t + SK - 1) both operands' fragments go straight from the residue
 // slices into LDS stage t % STAGES, each fragment loaded once per block:
-// stage fragment g < 8 KB is A's line block g / KB, k block g % KB (of the
-// step); g >= 8 KB is B's line block (g - 8 KB) / KB, k block g % KB.  Wave w
-// loads fragments LOADS w .. LOADS w + LOADS - 1 (waves 0-3 from A, 4-7 from B):
-// line blocks 2 (w % 4) + p / KB and k blocks p % KB of its operand, by buffer
-// loads that write lane L's 16 bytes at +16 L.
+// stage fragment g < 4 RB KB is A's line block g / KB, k block g % KB (of the
+// step); g >= 4 RB KB is B's line block (g - 4 RB KB) / KB, k block g % KB.
+// A wave w < 4 loads A's line blocks RB w + p / KB (p < LOADS_A); a wave
+// w >= 4 loads B's line blocks 2 (w % 4) + p / KB (p < LOADS_B); k block p % KB,
+// by buffer loads that write lane L's 16 bytes at +16 L.
+F4J_HD bool loads_a(int w) { return w < WAVES / 2; }
 // Base (bytes into the operand's slice, 64-bit) of wave w's loads at step t for a block
 // whose operand lines start at x0 (m0 for A, n0 for B), and the per-load offset.
 F4J_HD int64_t load_base(int64_t x0, int w, int t, int KC) {
-    return ((((x0 >> 5) + 2 * (w & 3)) * KC + KB * (int64_t)t) << 10);
+    const int lb = loads_a(w) ? RB * (w & 3) : 2 * (w & 3);
+    return ((((x0 >> 5) + lb) * KC + KB * (int64_t)t) << 10);
 }
 F4J_HD int load_soff(int p, int KC) { return (((p / KB) * KC) + (p % KB)) << 10; }
 F4J_HD int load_voff(int lane) { return 16 * lane; }
-F4J_HD int load_lds(int slot, int w, int p) { return (slot % STAGES) * STAGE + (LOADS * w + p) * FRAG; }
-// Wave w computes rows 64 (w % 4) .. + 63 (line blocks i = 0, 1) and columns
+F4J_HD int load_lds(int slot, int w, int p) {
+    const int g = loads_a(w) ? LOADS_A * w + p : 4 * LOADS_A + LOADS_B * (w & 3) + p;
+    return (slot % STAGES) * STAGE + g * FRAG;
+}
+// Wave w computes rows 32 RB (w % 4) .. + 32 RB - 1 (line blocks i = 0 .. RB - 1) and columns
 // 128 (w / 4) .. + 127 (line blocks j = 0..3) of the tile.  Fragment reads at k step ks (0 .. KB - 1):
 F4J_HD int a_frag(int slot, int w, int i, int ks, int lane) {
-    return (slot % STAGES) * STAGE + ((2 * (w & 3) + i) * KB + ks) * FRAG + 16 * lane;
+    return (slot % STAGES) * STAGE + ((RB * (w & 3) + i) * KB + ks) * FRAG + 16 * lane;
 }
 F4J_HD int b_frag(int slot, int w, int j, int ks, int lane) {
-    return (slot % STAGES) * STAGE + (8 * KB + (4 * (w >> 2) + j) * KB + ks) * FRAG + 16 * lane;
+    return (slot % STAGES) * STAGE + (4 * RB * KB + (4 * (w >> 2) + j) * KB + ks) * FRAG + 16 * lane;
 }
-F4J_HD int wave_row(int w) { return 64 * (w & 3); }
+F4J_HD int wave_row(int w) { return 32 * RB * (w & 3); }
 F4J_HD int wave_col(int w) { return 128 * (w >> 2); }
 // MFMA result item r of `lane` in block (i, j): row and column within the wave tile.
 F4J_HD int acc_row(int i, int r, int lane) { return 32 * i + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3); }

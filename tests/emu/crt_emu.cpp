@@ -94,13 +94,13 @@ int main(int argc, char **argv) {
         ++tile_hits[tm * tiles_n + tn];
         const int64_t m0 = tm * crt::BM, n0 = tn * crt::BN;
         std::fill(tag.begin(), tag.end(), -1);
-        // every wave's 4 loads of step t into stage slot (waves 0-3 from A, 4-7 from B)
+        // every wave's loads of step t into stage slot (waves 0-3 from A, 4-7 from B)
         auto issue = [&](int slot, int t) {
             for (int w = 0; w < crt::WAVES; ++w) {
-                const bool is_a = w < crt::WAVES / 2;
+                const bool is_a = w < crt::WAVES / 2;          // restated, not crt::loads_a
                 const std::vector<signed char> &src = is_a ? Ar : Br;
                 const int64_t x0 = is_a ? m0 : n0, size = is_a ? Mp * Kp : Np * Kp;
-                for (int p = 0; p < crt::LOADS; ++p)
+                for (int p = 0; p < (is_a ? crt::LOADS_A : crt::LOADS_B); ++p)
                     for (int lane = 0; lane < 64; ++lane)
                         for (int v = 0; v < 16; ++v) {
                             const int64_t off = crt::load_base(x0, w, t, KC) + crt::load_soff(p, KC) +
@@ -121,21 +121,21 @@ int main(int argc, char **argv) {
             return lds[off];
         };
         // acc[w][i][j][lane][r]
-        std::vector<long long> acc(crt::WAVES * 2 * 4 * 64 * 16, 0);
+        std::vector<long long> acc(crt::WAVES * crt::RB * 4 * 64 * 16, 0);
         for (int t = 0; t < crt::AHEAD; ++t) issue(t, std::min(t, nt - 1));
         for (int t = 0; t < nt; ++t) {
             issue(t + crt::AHEAD, std::min(t + crt::AHEAD, nt - 1));   // into the stage no wave reads in step t
             for (int w = 0; w < crt::WAVES; ++w)
                 for (int ks = 0; ks < crt::KB; ++ks) {
                     // operands as the lanes hold them
-                    int a[2][64][16], b[4][64][16];
+                    int a[crt::RB][64][16], b[4][64][16];
                     for (int lane = 0; lane < 64; ++lane)
                         for (int v = 0; v < 16; ++v) {
-                            for (int i = 0; i < 2; ++i) a[i][lane][v] = read(crt::a_frag(t, w, i, ks, lane) + v, t);
+                            for (int i = 0; i < crt::RB; ++i) a[i][lane][v] = read(crt::a_frag(t, w, i, ks, lane) + v, t);
                             for (int j = 0; j < 4; ++j) b[j][lane][v] = read(crt::b_frag(t, w, j, ks, lane) + v, t);
                         }
                     // documented MFMA semantics
-                    for (int i = 0; i < 2; ++i)
+                    for (int i = 0; i < crt::RB; ++i)
                         for (int j = 0; j < 4; ++j)
                             for (int lane = 0; lane < 64; ++lane)
                                 for (int r = 0; r < 16; ++r) {
@@ -145,20 +145,20 @@ int main(int argc, char **argv) {
                                         const int la = row + 32 * (kk / 16), lb = col + 32 * (kk / 16);
                                         sum += (long long)a[i][la][kk % 16] * b[j][lb][kk % 16];
                                     }
-                                    acc[(((w * 2 + i) * 4 + j) * 64 + lane) * 16 + r] += sum;
+                                    acc[(((w * crt::RB + i) * 4 + j) * 64 + lane) * 16 + r] += sum;
                                 }
                 }
         }
         // epilogue
         for (int w = 0; w < crt::WAVES; ++w)
-            for (int i = 0; i < 2; ++i)
+            for (int i = 0; i < crt::RB; ++i)
                 for (int j = 0; j < 4; ++j)
                     for (int lane = 0; lane < 64; ++lane)
                         for (int r = 0; r < 16; ++r) {
                             const int64_t row = m0 + crt::wave_row(w) + crt::acc_row(i, r, lane);
                             const int64_t col = n0 + crt::wave_col(w) + crt::acc_col(j, lane);
                             if (row >= M || col >= N) continue;
-                            long long v = acc[(((w * 2 + i) * 4 + j) * 64 + lane) * 16 + r] % m;
+                            long long v = acc[(((w * crt::RB + i) * 4 + j) * 64 + lane) * 16 + r] % m;
                             if (v < 0) v += m;
                             R[row * N + col] = (int)v;
                             ++Rhits[row * N + col];
