@@ -110,8 +110,9 @@ __device__ inline int exp_of(double v) {
 }
 
 // Work item (side, first line): contiguous sides take 4 lines per block (a
-// wave per line, lanes along k); strided sides 256 lines (a thread per line,
-// lanes along the lines).  Lines past nx (padding) get INT_MIN.
+// wave per line, lanes along k); strided sides 64 lines (lanes along the
+// lines, so each k reads 64 adjacent entries; the block's four waves take
+// every fourth k and merge in LDS).  Lines past nx (padding) get INT_MIN.
 // A NaN or infinite entry marks the side's product in err (NONFINITE_BITS).
 __global__ __launch_bounds__(256) void bexp_kernel(const Side *__restrict__ sides, const int2 *__restrict__ work,
                                                    int *__restrict__ epool, unsigned long long *__restrict__ err) {
@@ -131,16 +132,21 @@ __global__ __launch_bounds__(256) void bexp_kernel(const Side *__restrict__ side
         for (int off = 32; off > 0; off >>= 1) m = max(m, __shfl_xor(m, off));
         if (lane == 0 && x < s.Xp) epool[s.eoff + x] = m;
     } else {
-        const int64_t x = w.y + threadIdx.x;
-        if (x >= s.Xp) return;
+        __shared__ int red[4][64];
+        const int xl = threadIdx.x & 63, ph = threadIdx.x >> 6;
+        const int64_t x = w.y + xl;                  // < Xp: Xp is a multiple of 64
         int m = INT_MIN;
-        if (x < s.nx)
-            for (int64_t k = 0; k < s.K; ++k) {
+        if (x < s.nx) {
+#pragma unroll 4
+            for (int64_t k = ph; k < s.K; k += 4) {
                 const double v = side_value(s, x, k);
                 bad |= !__builtin_isfinite(v);
                 m = max(m, exp_of(v));
             }
-        epool[s.eoff + x] = m;
+        }
+        red[ph][xl] = m;
+        __syncthreads();                             // uniform: the whole block takes this branch
+        if (ph == 0) epool[s.eoff + x] = max(max(red[0][xl], red[1][xl]), max(red[2][xl], red[3][xl]));
     }
     if (bad) atomicMax(err + s.prob, NONFINITE_BITS);
 }
@@ -720,7 +726,7 @@ SplitBatch *split_batch_create(bool cplx, int opA, int opB, int carrier, int sli
     std::vector<int4> split_work;
     for (size_t si = 0; si < sides.size(); ++si) {
         const Side &s = sides[si];
-        const int step = s.contig ? 4 : 256;
+        const int step = s.contig ? 4 : 64;
         for (int64_t x = 0; x < s.Xp; x += step) exp_work.push_back(make_int2((int)si, (int)x));
         const int64_t kstep = int8 ? 32 : slice::BK;
         for (int64_t x = 0; x < s.Xp; x += 32)
