@@ -14,7 +14,8 @@
 // rather than 128 x 128.  Each bucket runs one persistent launch of the FP64
 // MFMA tile kernel with the bucket's tile configuration; its tiles are assigned to workgroups by
 // longest-processing-time-first list scheduling on an estimated cost, so the
-// workgroups finish together.  The launches are captured into one hipGraph at
+// workgroups finish together, with each product's tiles kept on one XCD (one
+// L2 fetch of its operands).  The launches are captured into one hipGraph at
 // plan creation; running the plan replays the graph.  The autotuner times each
 // candidate tile configuration per bucket on the plan's own problems and
 // records the winner per (device, operand kinds, bucket) in a cache that can
@@ -43,6 +44,7 @@
 #include "gemm_tile.h"
 #include "split_batch.h"
 #include "split_gemm.h"
+#include "xcd_schedule.h"
 
 namespace f4j {
 namespace {
@@ -192,17 +194,17 @@ int schedule_bucket(f4j_grouped_plan &pl, int b, int cfg) {
     if (items.empty()) return F4J_OK;
     const int per_cu = grouped_wgs_per_cu(pl.ka, pl.kb, cfg);
     const int wgs = (int)std::min<int64_t>((int64_t)items.size(), (int64_t)pl.cus * std::max(per_cu, 1));
-    std::stable_sort(items.begin(), items.end(), [](const Cand &x, const Cand &y) { return x.cost > y.cost; });
-    using Slot = std::pair<double, int>;   // (load, wg)
-    std::priority_queue<Slot, std::vector<Slot>, std::greater<Slot>> heap;
-    for (int w = 0; w < wgs; ++w) heap.push({0.0, w});
-    std::vector<std::vector<GroupedItem>> per(wgs);
-    for (const Cand &c : items) {
-        Slot s = heap.top();
-        heap.pop();
-        per[s.second].push_back(c.it);
-        heap.push({s.first + c.cost, s.second});
+    // tiles dealt per XCD (each product's tiles to one L2), longest first (xcd_schedule.h)
+    std::vector<double> cost(items.size());
+    std::vector<int> group(items.size());
+    for (size_t i = 0; i < items.size(); ++i) {
+        cost[i] = items[i].cost;
+        group[i] = items[i].it.prob;
     }
+    const std::vector<std::vector<int>> idx = xcd_schedule(cost, group, wgs);
+    std::vector<std::vector<GroupedItem>> per(wgs);
+    for (int w = 0; w < wgs; ++w)
+        for (int i : idx[w]) per[w].push_back(items[i].it);
     std::vector<GroupedItem> flat;
     std::vector<int> begin(wgs + 1, 0);
     flat.reserve(items.size());

@@ -53,6 +53,7 @@
 #include "split_batch.h"
 #include "split_gemm.h"
 #include "split_tile.h"
+#include "xcd_schedule.h"
 
 namespace f4j {
 namespace {
@@ -771,61 +772,18 @@ SplitBatch *split_batch_create(bool cplx, int opA, int opB, int carrier, int sli
     if (oe != hipSuccess || occ < 1) occ = 1;
     b->workers = (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)prop.multiProcessorCount * occ,
                                                              (int64_t)items.size()));
-    // Schedule.  Workgroup w of the persistent launch runs on XCD w % 8 (the
-    // hardware deals workgroup ids round-robin over the 8 XCDs, each with its
-    // own L2).  Each product's tiles go to one XCD, so its operand slices are
-    // fetched into that L2 once rather than into several; a product above a
-    // quarter of an XCD's fair share is spread tile by tile instead.  Units
-    // (products, or tiles of the large ones) are dealt longest first to the
-    // least-loaded XCD, then each XCD's tiles longest first to its
-    // least-loaded workgroup.
+    // Tiles dealt per XCD, longest first (xcd_schedule.h).
     std::vector<std::vector<Item>> per(b->workers);
     {
-        using Load = std::pair<int64_t, int>;
-        auto lpt = [](std::vector<std::pair<int64_t, std::vector<int>>> &units, int bins,
-                      std::vector<std::vector<int>> &out) {
-            std::stable_sort(units.begin(), units.end(), [](const auto &x, const auto &y) { return x.first > y.first; });
-            std::priority_queue<Load, std::vector<Load>, std::greater<Load>> heap;
-            for (int k = 0; k < bins; ++k) heap.push({0, k});
-            out.assign(bins, {});
-            for (const auto &u : units) {
-                Load l = heap.top();
-                heap.pop();
-                out[l.second].insert(out[l.second].end(), u.second.begin(), u.second.end());
-                l.first += u.first;
-                heap.push(l);
-            }
-        };
-        const int nxcd = std::min(8, b->workers);
-        std::map<int, std::vector<int>> of_prob;
-        int64_t total = 0;
+        std::vector<double> cost(items.size());
+        std::vector<int> group(items.size());
         for (size_t i = 0; i < items.size(); ++i) {
-            of_prob[items[i].prob].push_back((int)i);
-            total += items[i].cost;
+            cost[i] = (double)items[i].cost;
+            group[i] = items[i].prob;
         }
-        const int64_t share = total / (4 * nxcd) + 1;
-        std::vector<std::pair<int64_t, std::vector<int>>> units;
-        for (auto &kv : of_prob) {
-            int64_t c = 0;
-            for (int i : kv.second) c += items[i].cost;
-            if (c <= share) {
-                units.push_back({c, kv.second});
-            } else {
-                for (int i : kv.second) units.push_back({items[i].cost, {i}});
-            }
-        }
-        std::vector<std::vector<int>> xcd_items;
-        lpt(units, nxcd, xcd_items);
-        for (int x = 0; x < nxcd; ++x) {
-            std::vector<int> wk;                   // workgroups on XCD x
-            for (int w = x; w < b->workers; w += nxcd) wk.push_back(w);
-            std::vector<std::pair<int64_t, std::vector<int>>> tiles;
-            for (int i : xcd_items[x]) tiles.push_back({items[i].cost, {i}});
-            std::vector<std::vector<int>> per_wk;
-            lpt(tiles, (int)wk.size(), per_wk);
-            for (size_t k = 0; k < wk.size(); ++k)
-                for (int i : per_wk[k]) per[wk[k]].push_back(items[i]);
-        }
+        const std::vector<std::vector<int>> idx = xcd_schedule(cost, group, b->workers);
+        for (int w = 0; w < b->workers; ++w)
+            for (int i : idx[w]) per[w].push_back(items[i]);
     }
     std::vector<Item> sched;
     std::vector<int> begin(b->workers + 1, 0);
