@@ -7,8 +7,10 @@
 // fetched into one L2 instead of several; a product above a quarter of an
 // XCD's fair share is spread tile by tile instead, so one large product cannot
 // unbalance the XCDs.  Units (products, or tiles of the large ones) go longest
-// first to the least-loaded XCD, then each XCD's tiles longest first to its
-// least-loaded workgroup (list scheduling, Graham's LPT rule).
+// first to the XCD with the least load per workgroup, then each XCD's tiles
+// longest first to its least-loaded workgroup (list scheduling, Graham's LPT
+// rule).  Every workgroup's load is then at most
+//     total / workers + max tile cost + max unit cost / (workgroups of the smallest XCD).
 #pragma once
 #include <algorithm>
 #include <cstdint>
@@ -24,18 +26,20 @@ namespace f4j {
 inline std::vector<std::vector<int>> xcd_schedule(const std::vector<double> &cost, const std::vector<int> &group,
                                                   int workers) {
     using Unit = std::pair<double, std::vector<int>>;
-    using Load = std::pair<double, int>;
-    auto lpt = [](std::vector<Unit> &units, int bins) {
+    using Load = std::pair<double, int>;           // (load / capacity, bin)
+    // units longest first, each to the bin with the least load per unit of capacity
+    auto lpt = [](std::vector<Unit> &units, const std::vector<int> &cap) {
         std::stable_sort(units.begin(), units.end(), [](const Unit &x, const Unit &y) { return x.first > y.first; });
         std::priority_queue<Load, std::vector<Load>, std::greater<Load>> heap;
-        for (int k = 0; k < bins; ++k) heap.push({0.0, k});
-        std::vector<std::vector<int>> out(bins);
+        for (int k = 0; k < (int)cap.size(); ++k) heap.push({0.0, k});
+        std::vector<std::vector<int>> out(cap.size());
+        std::vector<double> load(cap.size(), 0.0);
         for (const Unit &u : units) {
-            Load l = heap.top();
+            const int k = heap.top().second;
             heap.pop();
-            out[l.second].insert(out[l.second].end(), u.second.begin(), u.second.end());
-            l.first += u.first;
-            heap.push(l);
+            out[k].insert(out[k].end(), u.second.begin(), u.second.end());
+            load[k] += u.first;
+            heap.push({load[k] / cap[k], k});
         }
         return out;
     };
@@ -58,13 +62,15 @@ inline std::vector<std::vector<int>> xcd_schedule(const std::vector<double> &cos
         else
             for (int i : kv.second) units.push_back({cost[i], {i}});
     }
-    const std::vector<std::vector<int>> xcd = lpt(units, nxcd);
+    std::vector<int> cap(nxcd);                    // workgroups per XCD (they differ by at most one)
+    for (int x = 0; x < nxcd; ++x) cap[x] = (workers - x + nxcd - 1) / nxcd;
+    const std::vector<std::vector<int>> xcd = lpt(units, cap);
     for (int x = 0; x < nxcd; ++x) {
         std::vector<int> wk;                       // workgroups on XCD x
         for (int w = x; w < workers; w += nxcd) wk.push_back(w);
         std::vector<Unit> tiles;
         for (int i : xcd[x]) tiles.push_back({cost[i], {i}});
-        const std::vector<std::vector<int>> pw = lpt(tiles, (int)wk.size());
+        const std::vector<std::vector<int>> pw = lpt(tiles, std::vector<int>(wk.size(), 1));
         for (size_t k = 0; k < wk.size(); ++k) per[wk[k]] = pw[k];
     }
     return per;

@@ -79,3 +79,20 @@ def test_crt_residue_arithmetic():
     binary = build_once(RES_SRC, RES_BIN, [os.path.join(ROOT, "flag4j_amd", "csrc", "crt_residue.h")])
     r = subprocess.run([binary], capture_output=True, text=True, timeout=300)
     assert r.returncode == 0 and r.stdout.strip().startswith("OK"), r.stdout
+
+
+SCHED_SRC = os.path.join(ROOT, "tests", "emu", "sched_emu.cpp")
+SCHED_BIN = os.path.join(ROOT, "_build", "sched_emu")
+
+
+@pytest.fixture(scope="module")
+def sched_emu():
+    """The XCD-aware tile schedule of the persistent launches, on the host."""
+    return build_once(SCHED_SRC, SCHED_BIN, [os.path.join(ROOT, "flag4j_amd", "csrc", "xcd_schedule.h")])
+
+
+@pytest.mark.parametrize("nprod,workers,seed", [(1, 1, 1), (5, 3, 2), (200, 512, 3), (3000, 768, 4), (40, 8, 5),
+                                                (1000, 100, 6)])
+def test_xcd_schedule(sched_emu, nprod, workers, seed):
+    r = subprocess.run([sched_emu, str(nprod), str(workers), str(seed)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0 and r.stdout.strip().startswith("OK"), r.stdout
