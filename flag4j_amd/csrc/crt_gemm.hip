@@ -33,6 +33,7 @@
 #include <type_traits>
 #include <vector>
 
+#include "amdgcn.h"
 #include "crt_gemm.h"
 #include "crt_layout.h"
 #include "crt_residue.h"
@@ -307,8 +308,6 @@ __global__ __launch_bounds__(256) void crt_split_cols_kernel(CrtOperand o, int64
 // item r of lane l is row 8(r/4) + 4(l/32) + r%4, column l%32.
 // ------------------------------------------------------------------------
 
-typedef int i32x4 __attribute__((ext_vector_type(4)));
-typedef int i32x16 __attribute__((ext_vector_type(16)));
 
 constexpr int CBM = crt::BM, CBN = crt::BN, CBK = crt::BK;
 
@@ -318,11 +317,6 @@ struct CrtGemmArgs {
     unsigned char *R;                             // [n][plane], each plane M x N
     int64_t plane;
 };
-
-typedef __attribute__((address_space(3))) void lds_void;
-
-// vmcnt(n) only (gfx9 encoding: expcnt and lgkmcnt at their maxima).
-#define F4J_WAIT_VMCNT(n) __builtin_amdgcn_s_waitcnt(((n) & 15) | (7 << 4) | (15 << 8) | (((n) >> 4) << 14))
 
 // Block tile BM x 256 (BM = 128 RB), eight waves of 32 RB x 128 (RB x 4 MFMA
 // blocks, 64 RB int32 accumulators per lane): wave w takes row group w % 4 and

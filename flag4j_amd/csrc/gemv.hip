@@ -33,6 +33,7 @@
 // reference's bit for bit.  The kernels are bound by reading A once.
 #include <type_traits>
 
+#include "amdgcn.h"
 #include "f4j_internal.h"
 
 #pragma clang fp contract(off)
@@ -289,8 +290,6 @@ __global__ __launch_bounds__(64) void gemv_kernel(GemvArgs g) {
 // of A) and only that share of its lanes carries a chain; the other lanes
 // repeat those chains and store nothing.  Fewer rows per wave put more waves
 // (and more loads in flight) on each CU when M is a few thousand rows.
-typedef __attribute__((address_space(3))) void lds_void;
-#define F4J_GEMV_WAIT_VMCNT(n) __builtin_amdgcn_s_waitcnt(((n) & 15) | (7 << 4) | (15 << 8) | (((n) >> 4) << 14))
 
 // Bytes per stage slot: NL KB of A, then 1 KB of x.
 constexpr int slot_bytes(int nl) { return 1024 * nl + 1024; }
@@ -476,7 +475,7 @@ __global__ __launch_bounds__(64) void gemv_dma_kernel(GemvArgs g) {
     const int span = KIND == GEMV_BLOCKED ? 64 * 7 + 16 : 16;
     for (int64_t s = 0; s < g.nstage; ++s) {
         __builtin_amdgcn_sched_barrier(0);
-        F4J_GEMV_WAIT_VMCNT((D - 2) * L);
+        F4J_WAIT_VMCNT((D - 2) * L);
         __builtin_amdgcn_sched_barrier(0);
         issue(s + D - 1);                 // past the last stage: clamped reloads, never read
         __builtin_amdgcn_sched_barrier(0);
@@ -492,7 +491,7 @@ __global__ __launch_bounds__(64) void gemv_dma_kernel(GemvArgs g) {
             }
         }
     }
-    F4J_GEMV_WAIT_VMCNT(0);               // no LDS write in flight when the wave ends
+    F4J_WAIT_VMCNT(0);               // no LDS write in flight when the wave ends
 
     if constexpr (KIND == GEMV_BLOCKED) {
         const int r = l >> 3;

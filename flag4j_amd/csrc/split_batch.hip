@@ -45,6 +45,7 @@
 #include <queue>
 #include <vector>
 
+#include "amdgcn.h"
 #include "../../include/flag4j_amd.h"
 #include "crt_gemm.h"
 #include "crt_layout.h"
@@ -321,8 +322,6 @@ __global__ __launch_bounds__(256) void bres_kernel(const Prob *__restrict__ prob
 
 // ---- modular (INT8) form ----
 
-typedef int i32x4 __attribute__((ext_vector_type(4)));
-typedef int i32x16 __attribute__((ext_vector_type(16)));
 
 // Work item (side, x0, k0, -): the 32-line x 32-k fragment of every modulus,
 // one wave per item, four per block.  WIDE (some t > 58): residues by the
@@ -409,8 +408,6 @@ __global__ __launch_bounds__(256) void bcrt_split_kernel(const Side *__restrict_
 // between two accumulators, so consecutive MFMAs do not depend on each other.
 constexpr int RING = 8;                       // steps in flight per wave
 constexpr int STEP = 2 * crt::FRAG;           // LDS bytes of one step (A, then B fragment)
-typedef __attribute__((address_space(3))) void lds_void;
-#define F4J_WAIT_VMCNT(n) __builtin_amdgcn_s_waitcnt(((n) & 15) | (7 << 4) | (15 << 8) | (((n) >> 4) << 14))
 
 __global__ __launch_bounds__(256, 2) void bcrt_kernel(const Item *__restrict__ items, const int *__restrict__ begin,
                                                       const Prob *__restrict__ probs, const Job *__restrict__ jobs,
