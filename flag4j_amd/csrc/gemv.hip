@@ -120,29 +120,24 @@ __device__ inline void x_slot(int h, int l, int conjx, int &kl, int &part, bool 
     if constexpr (KIND == GEMV_BLOCKED) {
         const int j = 4 * h + (l >> 4);                // block j: element e at lane position (e + j) & 15
         kl = 64 * j + (((l & 15) - j) & 15);
-    } else if constexpr (KIND == GEMV_CHAIN) {
-        kl = l & 15;
     } else {
-        // Two copies of the stage's 16 elements b' (b' = conj(b) when conjx):
-        // lanes 0-31 hold (b're, -b'im), lanes 32-63 hold (b'im, b're).
-        const int copy = l >> 5, d = l & 31, p = d & 1;
-        kl = d >> 1;
-        part = copy == 0 ? p : 1 - p;
-        neg = part == 1 && ((copy == 0) != (conjx != 0));
+        static_assert(KIND == GEMV_CHAIN, "register-staged form: real A and x");
+        kl = l & 15;
     }
 }
 
+// Register-staged form of the real kinds (A not 16-byte aligned).
 template <int KIND, int VEC, int D>
 __global__ __launch_bounds__(64) void gemv_kernel(GemvArgs g) {
+    static_assert(KIND == GEMV_BLOCKED || KIND == GEMV_CHAIN, "real A and x");
     constexpr int R = rows_per_wave<KIND>(), XN = x_loads<KIND>(), NA = 16 / VEC;
     static_assert(D * (NA + XN) <= 63, "loads in flight must fit the vmcnt counter");
     __shared__ __attribute__((aligned(16))) double lds[1024 + 64 * XN];
     const int l = threadIdx.x;
     const int64_t row0 = (int64_t)blockIdx.x * R;
     const int rows = (int)((g.M - row0) < R ? (g.M - row0) : R);
-    const int ew = KIND == GEMV_CPLX ? 2 : 1;
     const __amdgpu_buffer_rsrc_t ra = rsrc(g.A + row0 * g.lda, (uint32_t)((int64_t)rows * g.lda * 8));
-    const __amdgpu_buffer_rsrc_t rx = rsrc(g.x, (uint32_t)(((g.K - 1) * g.incx + ew) * 8));
+    const __amdgpu_buffer_rsrc_t rx = rsrc(g.x, (uint32_t)(((g.K - 1) * g.incx + 1) * 8));
 
     typedef typename std::conditional<VEC == 2, f64x2, double>::type AV;
     // Per-lane load geometry: k offset within a stage (a row past the slab
@@ -156,7 +151,7 @@ __global__ __launch_bounds__(64) void gemv_kernel(GemvArgs g) {
         int r, kl;
         a_slot<KIND, VEC>(i, l, r, kl);
         akl[i] = r < rows ? kl : K;
-        aoff[i] = (uint32_t)(((int64_t)r * g.lda + (int64_t)kl * ew) * 8);
+        aoff[i] = (uint32_t)(((int64_t)r * g.lda + (int64_t)kl) * 8);
     }
 #pragma unroll
     for (int h = 0; h < XN; ++h) {
@@ -168,7 +163,7 @@ __global__ __launch_bounds__(64) void gemv_kernel(GemvArgs g) {
     double vx[D][XN];
     auto issue = [&](AV (&a)[NA], double (&xv)[XN], int64_t s) {
         const int ks = kstage<KIND>(s);
-        const uint32_t ka = (uint32_t)ks * (uint32_t)(8 * ew), kx = (uint32_t)ks * (uint32_t)incx * 8u;
+        const uint32_t ka = (uint32_t)ks * 8u, kx = (uint32_t)ks * (uint32_t)incx * 8u;
 #pragma unroll
         for (int i = 0; i < NA; ++i) {
             const uint32_t off = ks + akl[i] < K ? aoff[i] + ka : OOR;
@@ -187,12 +182,7 @@ __global__ __launch_bounds__(64) void gemv_kernel(GemvArgs g) {
 
     // Where this lane's chain finds its elements in LDS.
     int a_base, a_rot, x_base, x_rot = 0;
-    if constexpr (KIND == GEMV_CPLX) {
-        const int r = l & 31;
-        a_base = 128 * (r >> 2) + 32 * (r & 3);
-        a_rot = r;
-        x_base = 1024 + 32 * (l >> 5);
-    } else {
+    {
         if constexpr (VEC == 2) {
             a_base = 128 * (l >> 3) + 16 * (l & 7);
             a_rot = (l >> 3) + (l & 7);
@@ -227,15 +217,7 @@ __global__ __launch_bounds__(64) void gemv_kernel(GemvArgs g) {
 #pragma unroll
             for (int h = 0; h < XN; ++h) lds[1024 + 64 * h + l] = vx[d][h];
             issue(va[d], vx[d], s + D);       // past the last stage the offsets are out of range: zeros
-            if constexpr (KIND == GEMV_CPLX) {
-#pragma unroll
-                for (int e = 0; e < 16; ++e) {
-                    const f64x2 a = *(const f64x2 *)&lds[a_base + 2 * ((e + a_rot) & 15)];
-                    const f64x2 w = *(const f64x2 *)&lds[x_base + 2 * e];
-                    const double p = __dadd_rn(__dmul_rn(a.x, w.x), __dmul_rn(a.y, w.y));
-                    s_acc = __dadd_rn(s_acc, p);
-                }
-            } else if constexpr (VEC == 2) {
+            if constexpr (VEC == 2) {
 #pragma unroll
                 for (int p = 0; p < 8; ++p) {
                     const f64x2 a = *(const f64x2 *)&lds[a_base + 2 * ((p + a_rot) & 7)];
@@ -296,10 +278,11 @@ constexpr int slot_bytes(int nl) { return 1024 * nl + 1024; }
 
 // x loads per stage: XV 2 = 16-byte pairs (x aligned, unit stride, K even), XV 1 = dwords.
 template <int KIND, int XV>
-constexpr int x_dma_loads() { return KIND == GEMV_CPLX ? 2 : (KIND == GEMV_BLOCKED && XV == 1 ? 4 : 1); }
+constexpr int x_dma_loads() { return KIND == GEMV_BLOCKED && XV == 1 ? 4 : 1; }
 
 template <int KIND, int XV, int D, int NL>
 __global__ __launch_bounds__(64) void gemv_dma_kernel(GemvArgs g) {
+    static_assert(KIND != GEMV_CPLX, "complex A x runs gemv_cplx_split_kernel");
     static_assert(NL == 8 || (KIND != GEMV_BLOCKED && (NL == 2 || NL == 1)), "rows per wave");
     constexpr int R = rows_per_wave<KIND>() * NL / 8, XL = x_dma_loads<KIND, XV>(), L = NL + XL;
     constexpr int SLOT = slot_bytes(NL), XOFF = 1024 * NL;
@@ -329,11 +312,7 @@ __global__ __launch_bounds__(64) void gemv_dma_kernel(GemvArgs g) {
     int xkl[XL], xadd[XL];
 #pragma unroll
     for (int h = 0; h < XL; ++h) {
-        if constexpr (KIND == GEMV_CPLX) {
-            const int m = l >> 1, part = m & 1;          // copy h, double m = (element m / 2, part)
-            xkl[h] = m >> 1;
-            xadd[h] = 8 * (h == 0 ? part : 1 - part) + 4 * (l & 1);
-        } else if constexpr (KIND == GEMV_RX) {
+        if constexpr (KIND == GEMV_RX) {
             xkl[h] = l >> 2;                             // complex element l / 4, dword l % 4
             xadd[h] = 4 * (l & 3);
         } else if constexpr (KIND == GEMV_CX) {
@@ -372,7 +351,7 @@ __global__ __launch_bounds__(64) void gemv_dma_kernel(GemvArgs g) {
         for (int h = 0; h < XL; ++h) {
             const int k = min(ks + xkl[h], kx_max);
             const uint32_t off = (uint32_t)k * (uint32_t)incx * 8u + xadd[h];   // incx in doubles
-            if constexpr (XV == 2 && KIND != GEMV_CPLX)
+            if constexpr (XV == 2)
                 __builtin_amdgcn_raw_ptr_buffer_load_lds(rx, (lds_void *)(slot + XOFF), 16, off, 0, 0, 0);
             else
                 __builtin_amdgcn_raw_ptr_buffer_load_lds(rx, (lds_void *)(slot + XOFF + 256 * h), 4, off, 0, 0, 0);
@@ -382,20 +361,10 @@ __global__ __launch_bounds__(64) void gemv_dma_kernel(GemvArgs g) {
     // reader geometry (bytes within a slot)
     // (lanes past the wave's R chains read the chain of row lane % R)
     int a_base, a_rot, x_base = XOFF, x_rot = 0, kb = 0;
-    uint32_t sg1 = 0, sg2 = 0;                        // complex: sign masks of the two products
     if constexpr (KIND == GEMV_CX) {
         const int r = (l & 31) % R;
         a_base = 1024 * (r >> 2) + 256 * (r & 3);
         a_rot = r;
-    } else if constexpr (KIND == GEMV_CPLX) {
-        const int r = (l & 31) % R;
-        a_base = 1024 * (r >> 2) + 256 * (r & 3);
-        a_rot = r;
-        const bool im = l >= 32, cj = g.conjx != 0;
-        x_base = XOFF + (im ? 256 : 0);
-        // re = t1 - s t2, im = s t1 + t2 with t1 = ar w0, t2 = ai w1, s = -1 for conj(x)
-        sg1 = (im && cj) ? 0x80000000u : 0u;
-        sg2 = (!im && !cj) ? 0x80000000u : 0u;
     } else {
         const int c = KIND == GEMV_BLOCKED ? l : l % R;   // blocked: chain (row l / 8, block l % 8)
         a_base = 1024 * (c >> 3) + 128 * (c & 7);
@@ -434,17 +403,6 @@ __global__ __launch_bounds__(64) void gemv_dma_kernel(GemvArgs g) {
                 const bool ok = !masked || ks + 2 * p < K;
                 s_acc = ok ? vr : s_acc;
                 s_im = ok ? vi : s_im;
-            }
-        } else if constexpr (KIND == GEMV_CPLX) {
-#pragma unroll
-            for (int e = 0; e < 16; ++e) {
-                const f64x2 a = *(const f64x2 *)(slot + a_base + 16 * ((e + a_rot) & 15));
-                const f64x2 w = *(const f64x2 *)(slot + x_base + 16 * e);
-                double t1 = __dmul_rn(a.x, w.x), t2 = __dmul_rn(a.y, w.y);
-                t1 = __builtin_bit_cast(double, __builtin_bit_cast(uint64_t, t1) ^ ((uint64_t)sg1 << 32));
-                t2 = __builtin_bit_cast(double, __builtin_bit_cast(uint64_t, t2) ^ ((uint64_t)sg2 << 32));
-                const double v = __dadd_rn(s_acc, __dadd_rn(t1, t2));
-                s_acc = (!masked || ks + e < K) ? v : s_acc;
             }
         } else {
 #pragma unroll
@@ -515,6 +473,81 @@ void launch_dma(GemvArgs g, int64_t stages, hipStream_t stream) {
     g.nstage = stages;
     hipLaunchKernelGGL((gemv_dma_kernel<KIND, XV, D, NL>), dim3((unsigned)((g.M + R - 1) / R)), dim3(64), 0, stream,
                        g);
+}
+
+// Complex A x (GEMV_CPLX) with the products spread over the block and only the
+// additions in chains.  A chain's serial work is its additions: the products
+// of one k chunk are independent.  A block takes R rows; every chunk of
+// KC = 64 k, all 256 threads form the chunk's R x 64 Complex128.mult products
+// (E = R / 4 consecutive k of one row per thread; re = ar xr - ai xi,
+// im = ar xi + ai xr, each product and sum rounded, as the reference's) into
+// an LDS buffer, while the first 2R lanes of wave 0 (row l % R, component
+// l / R) add the previous chunk's products in k order.  The buffers alternate,
+// one barrier per chunk, and every thread's loads for a chunk are issued one
+// chunk ahead (measured at M = K = 8192: 205 us; 8 or 16 rows per block alike;
+// 128-k chunks, or loads two chunks ahead, 252 us).  With one lane per chain
+// (32 rows per wave, one wave per CU at M = 8K) the chains' own instruction
+// latency bound it at 343 us.
+template <int R>
+__global__ __launch_bounds__(256) void gemv_cplx_split_kernel(GemvArgs g) {
+    constexpr int KC = 64, E = KC * R / 256, TPR = KC / E;   // k per chunk; products per thread; threads per row
+    static_assert(E * 256 == KC * R && E >= 1 && E <= 8, "whole products per thread");
+    __shared__ f64x2 prod[2][R][KC + 1];         // +1: the chain lanes read distinct banks
+    const int t = threadIdx.x;
+    const int r = t / TPR, q = t % TPR;
+    const int64_t row0 = (int64_t)blockIdx.x * R;
+    const int64_t row = row0 + r;
+    const bool row_ok = row < g.M;
+    const double *arow = g.A + (row_ok ? row : 0) * g.lda;
+    const int64_t nchunk = (g.K + KC - 1) / KC;
+    const double xs = g.conjx ? -1.0 : 1.0;      // conj(x): the imaginary part's sign flipped exactly
+
+    f64x2 a[E], x[E];
+    auto load = [&](int64_t c) {                 // chunk c's A and x elements of this thread
+#pragma unroll
+        for (int u = 0; u < E; ++u) {
+            const int64_t k = c * KC + E * q + u;
+            const bool ok = row_ok && k < g.K;
+            a[u] = ok ? *(const f64x2 *)(arow + 2 * k) : f64x2{0.0, 0.0};
+            x[u] = k < g.K ? *(const f64x2 *)(g.x + k * g.incx) : f64x2{0.0, 0.0};
+        }
+    };
+    auto products = [&](int buf) {
+#pragma unroll
+        for (int u = 0; u < E; ++u) {
+            const double xr = x[u].x, xi = xs * x[u].y;
+            const double re = __dsub_rn(__dmul_rn(a[u].x, xr), __dmul_rn(a[u].y, xi));
+            const double im = __dadd_rn(__dmul_rn(a[u].x, xi), __dmul_rn(a[u].y, xr));
+            prod[buf][r][E * q + u] = f64x2{re, im};
+        }
+    };
+    // chain lanes: wave 0, lanes 0 .. 2R - 1
+    const bool chain = t < 2 * R;
+    const int cr = t % R, comp = t / R;
+    double s = 0.0;
+
+    load(0);
+    products(0);
+    if (nchunk > 1) load(1);
+    __syncthreads();
+    for (int64_t c = 0; c < nchunk; ++c) {
+        if (c + 1 < nchunk) {
+            products((int)((c + 1) & 1));        // chunk c + 1, loaded one chunk ago
+            if (c + 2 < nchunk) load(c + 2);
+        }
+        if (chain) {
+            const int kn = (int)min<int64_t>(KC, g.K - c * KC);
+            const double *pc = (const double *)&prod[c & 1][cr][0] + comp;
+            if (kn == KC) {                      // full chunk: unrolled, the reads run ahead of the adds
+#pragma unroll
+                for (int k = 0; k < KC; ++k) s = __dadd_rn(s, pc[2 * k]);
+            } else {
+                for (int k = 0; k < kn; ++k) s = __dadd_rn(s, pc[2 * k]);
+            }
+        }
+        __syncthreads();
+    }
+    if (chain && row0 + cr < g.M) g.y[(row0 + cr) * g.incy + comp] = s;
 }
 
 // One-chain-per-row kinds: rows per wave and ring depth, from a sweep on
@@ -588,7 +621,8 @@ hipError_t launch_gemv(int kind, int64_t M, int64_t K, const double *A, int64_t 
     } else if (kind == GEMV_CX) {
         launch_rows<GEMV_CX, 1>(g, (K + 15) / 16, stream);
     } else {
-        launch_rows<GEMV_CPLX, 1>(g, (K + 15) / 16, stream);
+        constexpr int R = 16;
+        hipLaunchKernelGGL((gemv_cplx_split_kernel<R>), dim3((unsigned)((M + R - 1) / R)), dim3(256), 0, stream, g);
     }
     return hipGetLastError();
 }
