@@ -18,12 +18,14 @@
 //   sum from dest, so y_i is one k-ascending sum of Complex128.mult products
 //   (M:numbers/Complex128.java:276-278, :336-341).
 //
-// Every result is a chain of dependent additions.  A lane runs one chain:
-// a (row, 64-block) chain for the blocked form (8 rows x 8 blocks per wave, the
-// block sums then added in block order across lanes), a row for the plain real
-// form (64 rows per wave) and a (row, component) pair for complex (32 rows per
-// wave; real A below about 32K rows takes 8 or 16 rows per wave, with the
-// other lanes idle, to spread the rows over more waves).  The wave streams
+// Every result is a chain of dependent additions.  The plain real chain and
+// complex A x form their products with the whole block and leave only the
+// additions to one lane per chain (gemv_split_kernel, below).  The other
+// kinds run their products inside the chains: a (row, 64-block) chain for the
+// blocked form (8 rows x 8 blocks per wave, the block sums then added in block
+// order across lanes) and a (row, component) pair for the mixed kinds (real A
+// below about 32K rows takes 8 or 16 rows per wave, with the other lanes idle,
+// to spread the rows over more waves).  The wave streams
 // its slab of A in stages of 16 elements per chain (8 KB): coalesced 8-byte buffer loads into registers D stages ahead, then a
 // write into LDS rotated so that the 64 lanes read their own chains' elements
 // without bank conflicts.  Loads past K or past the last row are given an
@@ -282,7 +284,7 @@ constexpr int x_dma_loads() { return KIND == GEMV_BLOCKED && XV == 1 ? 4 : 1; }
 
 template <int KIND, int XV, int D, int NL>
 __global__ __launch_bounds__(64) void gemv_dma_kernel(GemvArgs g) {
-    static_assert(KIND != GEMV_CPLX, "complex A x runs gemv_cplx_split_kernel");
+    static_assert(KIND != GEMV_CPLX, "complex A x runs gemv_split_kernel");
     static_assert(NL == 8 || (KIND != GEMV_BLOCKED && (NL == 2 || NL == 1)), "rows per wave");
     constexpr int R = rows_per_wave<KIND>() * NL / 8, XL = x_dma_loads<KIND, XV>(), L = NL + XL;
     constexpr int SLOT = slot_bytes(NL), XOFF = 1024 * NL;
@@ -475,24 +477,31 @@ void launch_dma(GemvArgs g, int64_t stages, hipStream_t stream) {
                        g);
 }
 
-// Complex A x (GEMV_CPLX) with the products spread over the block and only the
-// additions in chains.  A chain's serial work is its additions: the products
-// of one k chunk are independent.  A block takes R rows; every chunk of
-// KC = 64 k, all 256 threads form the chunk's R x 64 Complex128.mult products
-// (E = R / 4 consecutive k of one row per thread; re = ar xr - ai xi,
-// im = ar xi + ai xr, each product and sum rounded, as the reference's) into
-// an LDS buffer, while the first 2R lanes of wave 0 (row l % R, component
-// l / R) add the previous chunk's products in k order.  The buffers alternate,
-// one barrier per chunk, and every thread's loads for a chunk are issued one
-// chunk ahead (measured at M = K = 8192: 205 us; 8 or 16 rows per block alike;
-// 128-k chunks, or loads two chunks ahead, 252 us).  With one lane per chain
-// (32 rows per wave, one wave per CU at M = 8K) the chains' own instruction
-// latency bound it at 343 us.
-template <int R>
-__global__ __launch_bounds__(256) void gemv_cplx_split_kernel(GemvArgs g) {
+// One-chain-per-row products with the products spread over the block and only
+// the additions in chains: complex A x (GEMV_CPLX: Complex128.mult products,
+// one chain per row and component) and the real plain chain (GEMV_CHAIN).  A
+// chain's serial work is its additions: the products of one k chunk are
+// independent.  A block takes R rows; every chunk of KC = 64 k, all 256
+// threads form the chunk's R x 64 products (E = R / 4 consecutive k of one row
+// per thread; complex re = ar xr - ai xi, im = ar xi + ai xr; every product
+// and sum rounded, as the reference's) into an LDS buffer, while the first C R
+// lanes of wave 0 (row l % R, component l / R) add the previous chunk's
+// products in k order.  The buffers alternate, one barrier per chunk, and
+// every thread's loads for a chunk are issued one chunk ahead (complex, M = K =
+// 8192: 205 us; 8 or 16 rows per block alike; 128-k chunks, or loads two
+// chunks ahead, 252 us; real chain: 8192^2 150 us, 32768 x 2048 111 us, with
+// 128-k chunks 152 / 168 us).  With one lane per chain (complex: 32 rows per
+// wave, one wave per CU at M = 8K) the chains' own instruction latency bound
+// it (complex 343 us, real chain 8192^2 191 us).
+template <int KIND, int R>
+__global__ __launch_bounds__(256) void gemv_split_kernel(GemvArgs g) {
+    static_assert(KIND == GEMV_CPLX || KIND == GEMV_CHAIN, "complex A x, or the real plain chain");
+    constexpr bool CPLX = KIND == GEMV_CPLX;
+    constexpr int C = CPLX ? 2 : 1;                           // components (doubles per element)
     constexpr int KC = 64, E = KC * R / 256, TPR = KC / E;   // k per chunk; products per thread; threads per row
     static_assert(E * 256 == KC * R && E >= 1 && E <= 8, "whole products per thread");
-    __shared__ f64x2 prod[2][R][KC + 1];         // +1: the chain lanes read distinct banks
+    typedef typename std::conditional<CPLX, f64x2, double>::type V;
+    __shared__ V prod[2][R][KC + 1];             // +1: the chain lanes read distinct banks
     const int t = threadIdx.x;
     const int r = t / TPR, q = t % TPR;
     const int64_t row0 = (int64_t)blockIdx.x * R;
@@ -502,27 +511,31 @@ __global__ __launch_bounds__(256) void gemv_cplx_split_kernel(GemvArgs g) {
     const int64_t nchunk = (g.K + KC - 1) / KC;
     const double xs = g.conjx ? -1.0 : 1.0;      // conj(x): the imaginary part's sign flipped exactly
 
-    f64x2 a[E], x[E];
+    V a[E], x[E];
     auto load = [&](int64_t c) {                 // chunk c's A and x elements of this thread
 #pragma unroll
         for (int u = 0; u < E; ++u) {
             const int64_t k = c * KC + E * q + u;
             const bool ok = row_ok && k < g.K;
-            a[u] = ok ? *(const f64x2 *)(arow + 2 * k) : f64x2{0.0, 0.0};
-            x[u] = k < g.K ? *(const f64x2 *)(g.x + k * g.incx) : f64x2{0.0, 0.0};
+            a[u] = ok ? *(const V *)(arow + C * k) : V{};
+            x[u] = k < g.K ? *(const V *)(g.x + k * g.incx) : V{};
         }
     };
     auto products = [&](int buf) {
 #pragma unroll
         for (int u = 0; u < E; ++u) {
-            const double xr = x[u].x, xi = xs * x[u].y;
-            const double re = __dsub_rn(__dmul_rn(a[u].x, xr), __dmul_rn(a[u].y, xi));
-            const double im = __dadd_rn(__dmul_rn(a[u].x, xi), __dmul_rn(a[u].y, xr));
-            prod[buf][r][E * q + u] = f64x2{re, im};
+            if constexpr (CPLX) {
+                const double xr = x[u].x, xi = xs * x[u].y;
+                const double re = __dsub_rn(__dmul_rn(a[u].x, xr), __dmul_rn(a[u].y, xi));
+                const double im = __dadd_rn(__dmul_rn(a[u].x, xi), __dmul_rn(a[u].y, xr));
+                prod[buf][r][E * q + u] = f64x2{re, im};
+            } else {
+                prod[buf][r][E * q + u] = __dmul_rn(a[u], x[u]);
+            }
         }
     };
-    // chain lanes: wave 0, lanes 0 .. 2R - 1
-    const bool chain = t < 2 * R;
+    // chain lanes: wave 0, lanes 0 .. C R - 1
+    const bool chain = t < C * R;
     const int cr = t % R, comp = t / R;
     double s = 0.0;
 
@@ -540,9 +553,9 @@ __global__ __launch_bounds__(256) void gemv_cplx_split_kernel(GemvArgs g) {
             const double *pc = (const double *)&prod[c & 1][cr][0] + comp;
             if (kn == KC) {                      // full chunk: unrolled, the reads run ahead of the adds
 #pragma unroll
-                for (int k = 0; k < KC; ++k) s = __dadd_rn(s, pc[2 * k]);
+                for (int k = 0; k < KC; ++k) s = __dadd_rn(s, pc[C * k]);
             } else {
-                for (int k = 0; k < kn; ++k) s = __dadd_rn(s, pc[2 * k]);
+                for (int k = 0; k < kn; ++k) s = __dadd_rn(s, pc[C * k]);
             }
         }
         __syncthreads();
@@ -612,17 +625,17 @@ hipError_t launch_gemv(int kind, int64_t M, int64_t K, const double *A, int64_t 
         else if (a16) launch_dma<GEMV_BLOCKED, 1, 4>(g, stages, stream);
         else launch<GEMV_BLOCKED, 1, 3>(g, stages, stream);
     } else if (kind == GEMV_CHAIN) {
-        const int64_t stages = (K + 15) / 16;
-        if (a16 && x16) launch_rows<GEMV_CHAIN, 2>(g, stages, stream);
-        else if (a16) launch_rows<GEMV_CHAIN, 1>(g, stages, stream);
-        else launch<GEMV_CHAIN, 1, 3>(g, stages, stream);
+        constexpr int R = 16;
+        hipLaunchKernelGGL((gemv_split_kernel<GEMV_CHAIN, R>), dim3((unsigned)((M + R - 1) / R)), dim3(256), 0,
+                           stream, g);
     } else if (kind == GEMV_RX) {
         launch_rows<GEMV_RX, 1>(g, (K + 15) / 16, stream);
     } else if (kind == GEMV_CX) {
         launch_rows<GEMV_CX, 1>(g, (K + 15) / 16, stream);
     } else {
         constexpr int R = 16;
-        hipLaunchKernelGGL((gemv_cplx_split_kernel<R>), dim3((unsigned)((M + R - 1) / R)), dim3(256), 0, stream, g);
+        hipLaunchKernelGGL((gemv_split_kernel<GEMV_CPLX, R>), dim3((unsigned)((M + R - 1) / R)), dim3(256), 0,
+                           stream, g);
     }
     return hipGetLastError();
 }
