@@ -69,20 +69,18 @@ constexpr bool a_cplx() { return KIND == GEMV_CPLX || KIND == GEMV_CX; }
 template <int KIND>
 constexpr bool x_cplx() { return KIND == GEMV_CPLX || KIND == GEMV_RX; }
 
-// Rows of A per wave, A loads (VEC doubles each) and x loads (one double each) per lane per stage.
+// The blocked form's streaming kernels (gemv_dma_kernel, gemv_kernel): a wave
+// runs 64 chains, (row, 64-block) for 8 rows x 8 blocks; per stage 2 x loads
+// per lane in the register-staged form.
 template <int KIND>
-constexpr int rows_per_wave() { return KIND == GEMV_BLOCKED ? 8 : (a_cplx<KIND>() ? 32 : 64); }
+constexpr int rows_per_wave() { static_assert(KIND == GEMV_BLOCKED, "blocked form"); return 8; }
 template <int KIND>
-constexpr int x_loads() { return KIND == GEMV_BLOCKED ? 2 : 1; }
+constexpr int x_loads() { return 2; }
 
-// Stage s covers k = kstage(s) + [0, 64 J) with J = 8 blocks (blocked form) or
-// 1 (otherwise); within it, the k a lane loads depends only on the lane and
-// the load index.
+// Stage s covers k = kstage(s) + [0, 512): 16 elements of each of the 8 blocks;
+// within it, the k a lane loads depends only on the lane and the load index.
 template <int KIND>
-__device__ inline int kstage(int64_t s) {
-    if constexpr (KIND == GEMV_BLOCKED) return (int)(512 * (s >> 2) + 16 * (s & 3));
-    else return (int)(16 * s);
-}
+__device__ inline int kstage(int64_t s) { return (int)(512 * (s >> 2) + 16 * (s & 3)); }
 
 // Row (within the wave's slab) and k offset (within the stage) of A load i of
 // lane l.  A stage holds 16 elements of each of the 64 chains.  Load i of lane l
@@ -90,28 +88,18 @@ __device__ inline int kstage(int64_t s) {
 // each lane fetches is rotated so that the reading lanes spread over the banks.
 template <int KIND, int VEC>
 __device__ inline void a_slot(int i, int l, int &r, int &kl) {
-    if constexpr (a_cplx<KIND>()) {
-        r = 4 * i + (l >> 4);                          // rows 4i..4i+3, one complex element per lane
-        kl = ((l & 15) - r) & 15;                      // reader of row r finds element e at (e + r) & 15
+    int ch, el;
+    if constexpr (VEC == 2) {
+        const int b = l >> 3;
+        ch = 8 * i + b;                                // reader of chain c finds pair p at (p + c/8 + c%8) & 7
+        el = 2 * (((l & 7) - i - b) & 7);
     } else {
-        int ch, el;
-        if constexpr (VEC == 2) {
-            const int b = l >> 3;
-            ch = 8 * i + b;                            // reader of chain c finds pair p at (p + c/8 + c%8) & 7
-            el = 2 * (((l & 7) - i - b) & 7);
-        } else {
-            const int b = l >> 4;
-            ch = 4 * i + b;                            // reader of chain c finds element e at (e + c/4 + c%4) & 15
-            el = ((l & 15) - i - b) & 15;
-        }
-        if constexpr (KIND == GEMV_BLOCKED) {
-            r = ch >> 3;                               // chain = (row ch / 8, block ch % 8)
-            kl = 64 * (ch & 7) + el;
-        } else {
-            r = ch;                                    // chain = row
-            kl = el;
-        }
+        const int b = l >> 4;
+        ch = 4 * i + b;                                // reader of chain c finds element e at (e + c/4 + c%4) & 15
+        el = ((l & 15) - i - b) & 15;
     }
+    r = ch >> 3;                                       // chain = (row ch / 8, block ch % 8)
+    kl = 64 * (ch & 7) + el;
 }
 
 // k offset (within the stage), component and sign of x load h of lane l.
@@ -119,19 +107,14 @@ template <int KIND>
 __device__ inline void x_slot(int h, int l, int conjx, int &kl, int &part, bool &neg) {
     part = 0;
     neg = false;
-    if constexpr (KIND == GEMV_BLOCKED) {
-        const int j = 4 * h + (l >> 4);                // block j: element e at lane position (e + j) & 15
-        kl = 64 * j + (((l & 15) - j) & 15);
-    } else {
-        static_assert(KIND == GEMV_CHAIN, "register-staged form: real A and x");
-        kl = l & 15;
-    }
+    const int j = 4 * h + (l >> 4);                    // block j: element e at lane position (e + j) & 15
+    kl = 64 * j + (((l & 15) - j) & 15);
 }
 
-// Register-staged form of the real kinds (A not 16-byte aligned).
+// Register-staged form of the blocked kind (A not 16-byte aligned, or odd K).
 template <int KIND, int VEC, int D>
 __global__ __launch_bounds__(64) void gemv_kernel(GemvArgs g) {
-    static_assert(KIND == GEMV_BLOCKED || KIND == GEMV_CHAIN, "real A and x");
+    static_assert(KIND == GEMV_BLOCKED, "the blocked form (other kinds: gemv_split_kernel)");
     constexpr int R = rows_per_wave<KIND>(), XN = x_loads<KIND>(), NA = 16 / VEC;
     static_assert(D * (NA + XN) <= 63, "loads in flight must fit the vmcnt counter");
     __shared__ __attribute__((aligned(16))) double lds[1024 + 64 * XN];
@@ -192,12 +175,10 @@ __global__ __launch_bounds__(64) void gemv_kernel(GemvArgs g) {
             a_base = 64 * (l >> 2) + 16 * (l & 3);
             a_rot = (l >> 2) + (l & 3);
         }
-        if constexpr (KIND == GEMV_BLOCKED) {
+        {
             const int j = l & 7;
             x_base = 1024 + 64 * (j >> 2) + 16 * (j & 3);
             x_rot = j;
-        } else {
-            x_base = 1024;
         }
     }
 
@@ -236,28 +217,19 @@ __global__ __launch_bounds__(64) void gemv_kernel(GemvArgs g) {
                     s_acc = __dadd_rn(s_acc, __dmul_rn(a, xv));
                 }
             }
-            if constexpr (KIND == GEMV_BLOCKED) {
-                if ((s & 3) == 3) {
-                    // the 8 block sums of row l / 8 sit in lanes 8 (l / 8) + j: add them in block order
-                    double t = dest;
+            if ((s & 3) == 3) {
+                // the 8 block sums of row l / 8 sit in lanes 8 (l / 8) + j: add them in block order
+                double t = dest;
 #pragma unroll
-                    for (int j = 0; j < 8; ++j) t = __dadd_rn(t, __shfl(s_acc, (l & ~7) + j));
-                    dest = t;
-                    s_acc = 0.0;
-                }
+                for (int j = 0; j < 8; ++j) t = __dadd_rn(t, __shfl(s_acc, (l & ~7) + j));
+                dest = t;
+                s_acc = 0.0;
             }
         }
     }
 
-    if constexpr (KIND == GEMV_BLOCKED) {
-        const int r = l >> 3;
-        if ((l & 7) == 0 && r < rows) g.y[(row0 + r) * g.incy] = dest;
-    } else if constexpr (KIND == GEMV_CHAIN) {
-        if (l < rows) g.y[(row0 + l) * g.incy] = s_acc;
-    } else {
-        const int r = l & 31;
-        if (r < rows) g.y[(row0 + r) * g.incy + (l >> 5)] = s_acc;
-    }
+    const int r = l >> 3;
+    if ((l & 7) == 0 && r < rows) g.y[(row0 + r) * g.incy] = dest;
 }
 
 // ---- direct-to-LDS form (16-byte aligned A with even K and lda; complex always) ----
@@ -269,11 +241,8 @@ __global__ __launch_bounds__(64) void gemv_kernel(GemvArgs g) {
 // s vmcnt((D - 2) L) retires stage s.  Elements past K are skipped by the
 // reader (a uniform test picks the masked path for the last stages only).
 //
-// NL (the one-chain-per-row kinds): A loads per stage, 8 for a full wave of
-// chains.  With NL < 8 a wave takes NL / 8 of the rows (a stage holds NL KB
-// of A) and only that share of its lanes carries a chain; the other lanes
-// repeat those chains and store nothing.  Fewer rows per wave put more waves
-// (and more loads in flight) on each CU when M is a few thousand rows.
+// It serves the blocked form (the other kinds run gemv_split_kernel): NL = 8
+// A loads per stage, 8 rows x 8 blocks of chains per wave.
 
 // Bytes per stage slot: NL KB of A, then 1 KB of x.
 constexpr int slot_bytes(int nl) { return 1024 * nl + 1024; }
@@ -284,8 +253,7 @@ constexpr int x_dma_loads() { return KIND == GEMV_BLOCKED && XV == 1 ? 4 : 1; }
 
 template <int KIND, int XV, int D, int NL>
 __global__ __launch_bounds__(64) void gemv_dma_kernel(GemvArgs g) {
-    static_assert(KIND != GEMV_CPLX, "complex A x runs gemv_split_kernel");
-    static_assert(NL == 8 || (KIND != GEMV_BLOCKED && (NL == 2 || NL == 1)), "rows per wave");
+    static_assert(KIND == GEMV_BLOCKED && NL == 8, "the blocked form (other kinds: gemv_split_kernel)");
     constexpr int R = rows_per_wave<KIND>() * NL / 8, XL = x_dma_loads<KIND, XV>(), L = NL + XL;
     constexpr int SLOT = slot_bytes(NL), XOFF = 1024 * NL;
     constexpr int ew = a_cplx<KIND>() ? 2 : 1, xw = x_cplx<KIND>() ? 2 : 1;
@@ -314,28 +282,14 @@ __global__ __launch_bounds__(64) void gemv_dma_kernel(GemvArgs g) {
     int xkl[XL], xadd[XL];
 #pragma unroll
     for (int h = 0; h < XL; ++h) {
-        if constexpr (KIND == GEMV_RX) {
-            xkl[h] = l >> 2;                             // complex element l / 4, dword l % 4
-            xadd[h] = 4 * (l & 3);
-        } else if constexpr (KIND == GEMV_CX) {
-            xkl[h] = (l >> 1) & 15;                      // double l / 2 (lanes 32-63 repeat 0-31)
-            xadd[h] = 4 * (l & 1);
-        } else if constexpr (XV == 2) {
-            if constexpr (KIND == GEMV_BLOCKED) {
-                const int j = l >> 3;                    // block j: pair p at lane position (p + j) & 7
-                xkl[h] = 64 * j + 2 * (((l & 7) - j) & 7);
-            } else {
-                xkl[h] = 2 * (l & 7);
-            }
+        if constexpr (XV == 2) {
+            const int j = l >> 3;                        // block j: pair p at lane position (p + j) & 7
+            xkl[h] = 64 * j + 2 * (((l & 7) - j) & 7);
             xadd[h] = 0;
         } else {
             const int m = 32 * h + (l >> 1);             // double m of the stage's x area
-            if constexpr (KIND == GEMV_BLOCKED) {
-                const int j = m >> 4;                    // block j: element e at (e + j) & 15
-                xkl[h] = 64 * j + (((m & 15) - j) & 15);
-            } else {
-                xkl[h] = m & 15;
-            }
+            const int j = m >> 4;                        // block j: element e at (e + j) & 15
+            xkl[h] = 64 * j + (((m & 15) - j) & 15);
             xadd[h] = 4 * (l & 1);
         }
     }
@@ -360,68 +314,27 @@ __global__ __launch_bounds__(64) void gemv_dma_kernel(GemvArgs g) {
         }
     };
 
-    // reader geometry (bytes within a slot)
-    // (lanes past the wave's R chains read the chain of row lane % R)
-    int a_base, a_rot, x_base = XOFF, x_rot = 0, kb = 0;
-    if constexpr (KIND == GEMV_CX) {
-        const int r = (l & 31) % R;
-        a_base = 1024 * (r >> 2) + 256 * (r & 3);
-        a_rot = r;
-    } else {
-        const int c = KIND == GEMV_BLOCKED ? l : l % R;   // blocked: chain (row l / 8, block l % 8)
-        a_base = 1024 * (c >> 3) + 128 * (c & 7);
-        a_rot = (c >> 3) + (c & 7);
-        if constexpr (KIND == GEMV_BLOCKED) {
-            const int j = l & 7;
-            kb = 64 * j;
-            x_base = XOFF + 128 * j;
-            x_rot = j;
-        }
-    }
+    // reader geometry (bytes within a slot): lane l runs chain (row l / 8, block l % 8)
+    const int j = l & 7;
+    const int a_base = 1024 * (l >> 3) + 128 * j, a_rot = (l >> 3) + j;
+    const int kb = 64 * j, x_base = XOFF + 128 * j, x_rot = j;
 
-    double s_acc = 0.0, dest = 0.0, s_im = 0.0;   // s_im: second chain of the real-A, complex-x form
+    double s_acc = 0.0, dest = 0.0;
     auto compute = [&](int64_t s, bool masked) {
         const char *slot = lds + (int)(s % D) * SLOT;
         const int ks = kstage<KIND>(s);
-        if constexpr (KIND == GEMV_CX) {
-            // y_c = sum_k a_c x_k: Complex128.mult(double) (M:numbers/Complex128.java:402-404)
-            const bool im = l >= 32;
-#pragma unroll
-            for (int e = 0; e < 16; ++e) {
-                const f64x2 a = *(const f64x2 *)(slot + a_base + 16 * ((e + a_rot) & 15));
-                const double w = *(const double *)(slot + XOFF + 8 * e);
-                const double v = __dadd_rn(s_acc, __dmul_rn(im ? a.y : a.x, w));
-                s_acc = (!masked || ks + e < K) ? v : s_acc;
-            }
-        } else if constexpr (KIND == GEMV_RX) {
-            // y = sum_k x_k a_k, both components: x.mult(a) = (x.re a, x.im a)
-#pragma unroll
-            for (int p = 0; p < 8; ++p) {
-                const f64x2 a = *(const f64x2 *)(slot + a_base + 16 * ((p + a_rot) & 7));
-                const f64x2 w0 = *(const f64x2 *)(slot + XOFF + 32 * p);
-                const f64x2 w1 = *(const f64x2 *)(slot + XOFF + 32 * p + 16);
-                const double vr = __dadd_rn(__dadd_rn(s_acc, __dmul_rn(w0.x, a.x)), __dmul_rn(w1.x, a.y));
-                const double vi = __dadd_rn(__dadd_rn(s_im, __dmul_rn(w0.y, a.x)), __dmul_rn(w1.y, a.y));
-                const bool ok = !masked || ks + 2 * p < K;
-                s_acc = ok ? vr : s_acc;
-                s_im = ok ? vi : s_im;
-            }
-        } else {
+        {
 #pragma unroll
             for (int p = 0; p < 8; ++p) {
                 const f64x2 a = *(const f64x2 *)(slot + a_base + 16 * ((p + a_rot) & 7));
                 double x0, x1;
-                if constexpr (XV == 2 && KIND == GEMV_BLOCKED) {
+                if constexpr (XV == 2) {
                     const f64x2 w = *(const f64x2 *)(slot + x_base + 16 * ((p + x_rot) & 7));
                     x0 = w.x;
                     x1 = w.y;
-                } else if constexpr (KIND == GEMV_BLOCKED) {
+                } else {
                     x0 = *(const double *)(slot + x_base + 8 * ((2 * p + x_rot) & 15));
                     x1 = *(const double *)(slot + x_base + 8 * ((2 * p + 1 + x_rot) & 15));
-                } else {
-                    const f64x2 w = *(const f64x2 *)(slot + x_base + 16 * p);
-                    x0 = w.x;
-                    x1 = w.y;
                 }
                 const double v = __dadd_rn(__dadd_rn(s_acc, __dmul_rn(a.x, x0)), __dmul_rn(a.y, x1));
                 s_acc = (!masked || ks + kb + 2 * p < K) ? v : s_acc;
@@ -432,7 +345,7 @@ __global__ __launch_bounds__(64) void gemv_dma_kernel(GemvArgs g) {
 #pragma unroll
     for (int d = 0; d < D - 1; ++d) issue(d);
     // a stage is complete when every chain's 16 elements lie below K
-    const int span = KIND == GEMV_BLOCKED ? 64 * 7 + 16 : 16;
+    const int span = 64 * 7 + 16;
     for (int64_t s = 0; s < g.nstage; ++s) {
         __builtin_amdgcn_sched_barrier(0);
         F4J_WAIT_VMCNT((D - 2) * L);
@@ -441,32 +354,18 @@ __global__ __launch_bounds__(64) void gemv_dma_kernel(GemvArgs g) {
         __builtin_amdgcn_sched_barrier(0);
         if (kstage<KIND>(s) + span <= K) compute(s, false);
         else compute(s, true);
-        if constexpr (KIND == GEMV_BLOCKED) {
-            if ((s & 3) == 3) {
-                double t = dest;
+        if ((s & 3) == 3) {                // the row's 8 block sums, added in block order
+            double t = dest;
 #pragma unroll
-                for (int j = 0; j < 8; ++j) t = __dadd_rn(t, __shfl(s_acc, (l & ~7) + j));
-                dest = t;
-                s_acc = 0.0;
-            }
+            for (int b = 0; b < 8; ++b) t = __dadd_rn(t, __shfl(s_acc, (l & ~7) + b));
+            dest = t;
+            s_acc = 0.0;
         }
     }
     F4J_WAIT_VMCNT(0);               // no LDS write in flight when the wave ends
 
-    if constexpr (KIND == GEMV_BLOCKED) {
-        const int r = l >> 3;
-        if ((l & 7) == 0 && r < rows) g.y[(row0 + r) * g.incy] = dest;
-    } else if constexpr (KIND == GEMV_CHAIN) {
-        if (l < rows) g.y[(row0 + l) * g.incy] = s_acc;
-    } else if constexpr (KIND == GEMV_RX) {
-        if (l < rows) {
-            g.y[(row0 + l) * g.incy] = s_acc;
-            g.y[(row0 + l) * g.incy + 1] = s_im;
-        }
-    } else {
-        const int r = l & 31;
-        if (r < rows) g.y[(row0 + r) * g.incy + (l >> 5)] = s_acc;
-    }
+    const int r = l >> 3;
+    if ((l & 7) == 0 && r < rows) g.y[(row0 + r) * g.incy] = dest;
 }
 
 template <int KIND, int XV, int D, int NL = 8>
@@ -479,7 +378,9 @@ void launch_dma(GemvArgs g, int64_t stages, hipStream_t stream) {
 
 // One-chain-per-row products with the products spread over the block and only
 // the additions in chains: complex A x (GEMV_CPLX: Complex128.mult products,
-// one chain per row and component) and the real plain chain (GEMV_CHAIN).  A
+// one chain per row and component), the mixed kinds (GEMV_RX: x_k.re a_ik and
+// x_k.im a_ik; GEMV_CX: a_ik.re x_k and a_ik.im x_k, per component) and the
+// real plain chain (GEMV_CHAIN).  A
 // chain's serial work is its additions: the products of one k chunk are
 // independent.  A block takes R rows; every chunk of KC = 64 k, all 256
 // threads form the chunk's R x 64 products (E = R / 4 consecutive k of one row
@@ -495,12 +396,14 @@ void launch_dma(GemvArgs g, int64_t stages, hipStream_t stream) {
 // it (complex 343 us, real chain 8192^2 191 us).
 template <int KIND, int R>
 __global__ __launch_bounds__(256) void gemv_split_kernel(GemvArgs g) {
-    static_assert(KIND == GEMV_CPLX || KIND == GEMV_CHAIN, "complex A x, or the real plain chain");
-    constexpr bool CPLX = KIND == GEMV_CPLX;
-    constexpr int C = CPLX ? 2 : 1;                           // components (doubles per element)
+    static_assert(KIND != GEMV_BLOCKED, "one chain per row and component");
+    constexpr bool CPLX = KIND == GEMV_CPLX, AC = a_cplx<KIND>(), XC = x_cplx<KIND>();
+    constexpr int C = (AC || XC) ? 2 : 1;                     // components of y
     constexpr int KC = 64, E = KC * R / 256, TPR = KC / E;   // k per chunk; products per thread; threads per row
     static_assert(E * 256 == KC * R && E >= 1 && E <= 8, "whole products per thread");
-    typedef typename std::conditional<CPLX, f64x2, double>::type V;
+    typedef typename std::conditional<AC, f64x2, double>::type VA;
+    typedef typename std::conditional<XC, f64x2, double>::type VX;
+    typedef typename std::conditional<C == 2, f64x2, double>::type V;
     __shared__ V prod[2][R][KC + 1];             // +1: the chain lanes read distinct banks
     const int t = threadIdx.x;
     const int r = t / TPR, q = t % TPR;
@@ -511,14 +414,15 @@ __global__ __launch_bounds__(256) void gemv_split_kernel(GemvArgs g) {
     const int64_t nchunk = (g.K + KC - 1) / KC;
     const double xs = g.conjx ? -1.0 : 1.0;      // conj(x): the imaginary part's sign flipped exactly
 
-    V a[E], x[E];
+    VA a[E];
+    VX x[E];
     auto load = [&](int64_t c) {                 // chunk c's A and x elements of this thread
 #pragma unroll
         for (int u = 0; u < E; ++u) {
             const int64_t k = c * KC + E * q + u;
             const bool ok = row_ok && k < g.K;
-            a[u] = ok ? *(const V *)(arow + C * k) : V{};
-            x[u] = k < g.K ? *(const V *)(g.x + k * g.incx) : V{};
+            a[u] = ok ? *(const VA *)(arow + (AC ? 2 : 1) * k) : VA{};
+            x[u] = k < g.K ? *(const VX *)(g.x + k * g.incx) : VX{};
         }
     };
     auto products = [&](int buf) {
@@ -529,6 +433,10 @@ __global__ __launch_bounds__(256) void gemv_split_kernel(GemvArgs g) {
                 const double re = __dsub_rn(__dmul_rn(a[u].x, xr), __dmul_rn(a[u].y, xi));
                 const double im = __dadd_rn(__dmul_rn(a[u].x, xi), __dmul_rn(a[u].y, xr));
                 prod[buf][r][E * q + u] = f64x2{re, im};
+            } else if constexpr (KIND == GEMV_RX) {     // x.mult(a) = (x.re a, x.im a)
+                prod[buf][r][E * q + u] = f64x2{__dmul_rn(x[u].x, a[u]), __dmul_rn(x[u].y, a[u])};
+            } else if constexpr (KIND == GEMV_CX) {     // a.mult(x) = (a.re x, a.im x)
+                prod[buf][r][E * q + u] = f64x2{__dmul_rn(a[u].x, x[u]), __dmul_rn(a[u].y, x[u])};
             } else {
                 prod[buf][r][E * q + u] = __dmul_rn(a[u], x[u]);
             }
@@ -561,22 +469,6 @@ __global__ __launch_bounds__(256) void gemv_split_kernel(GemvArgs g) {
         __syncthreads();
     }
     if (chain && row0 + cr < g.M) g.y[(row0 + cr) * g.incy + comp] = s;
-}
-
-// One-chain-per-row kinds: rows per wave and ring depth, from a sweep on
-// MI355X over M = 8K..32K rows (A of 512 MB real / 1 GB complex) of 8 / 16 /
-// 32-64 rows per wave against ring depths 4..17.  Complex A: full waves with
-// a 4-stage ring (best or within 8% of best at every M).  Real A: 8 rows per
-// wave, 9 stages, below 16K rows; 16 rows, 5 stages, above — deep rings on
-// many small waves leave too few of them resident per CU.
-template <int KIND, int XV>
-void launch_rows(GemvArgs g, int64_t stages, hipStream_t stream) {
-    if constexpr (a_cplx<KIND>()) {
-        launch_dma<KIND, XV, 4, 8>(g, stages, stream);
-    } else {
-        if (g.M >= 16384) launch_dma<KIND, XV, 5, 2>(g, stages, stream);
-        else launch_dma<KIND, XV, 9, 1>(g, stages, stream);
-    }
 }
 
 template <int KIND, int VEC, int D>
@@ -629,9 +521,13 @@ hipError_t launch_gemv(int kind, int64_t M, int64_t K, const double *A, int64_t 
         hipLaunchKernelGGL((gemv_split_kernel<GEMV_CHAIN, R>), dim3((unsigned)((M + R - 1) / R)), dim3(256), 0,
                            stream, g);
     } else if (kind == GEMV_RX) {
-        launch_rows<GEMV_RX, 1>(g, (K + 15) / 16, stream);
+        constexpr int R = 16;
+        hipLaunchKernelGGL((gemv_split_kernel<GEMV_RX, R>), dim3((unsigned)((M + R - 1) / R)), dim3(256), 0, stream,
+                           g);
     } else if (kind == GEMV_CX) {
-        launch_rows<GEMV_CX, 1>(g, (K + 15) / 16, stream);
+        constexpr int R = 16;
+        hipLaunchKernelGGL((gemv_split_kernel<GEMV_CX, R>), dim3((unsigned)((M + R - 1) / R)), dim3(256), 0, stream,
+                           g);
     } else {
         constexpr int R = 16;
         hipLaunchKernelGGL((gemv_split_kernel<GEMV_CPLX, R>), dim3((unsigned)((M + R - 1) / R)), dim3(256), 0,
