@@ -398,27 +398,33 @@ __global__ __launch_bounds__(256) void bcrt_split_kernel(const Side *__restrict_
 // 2^(eA + eB - 2t) and added into C with the job's coefficients; no two
 // workgroups write one element.
 //
-// Operands: a job is a run of n KC k blocks ("steps": modulus-major, k
-// blocks within), each one A and one B fragment of 1 KB, contiguous in the
-// fragment-major residues.  Every wave streams its own fragments through a
-// private LDS ring of RING steps by buffer loads that write LDS directly, so it
-// only waits for its own loads (vmcnt) and the block needs no barrier.  Every
-// step issues the same two loads (past the job's last step they repeat it into
-// the slot just read) so the wait count stays exact.  The k blocks alternate
-// between two accumulators, so consecutive MFMAs do not depend on each other.
-constexpr int RING = 8;                       // steps in flight per wave
-constexpr int STEP = 2 * crt::FRAG;           // LDS bytes of one step (A, then B fragment)
+// Operands: a job is a run of n KC / 2 steps (modulus-major), each step two k
+// blocks of the block's 2 A and 2 B line blocks: 8 fragments of 1 KB, each
+// contiguous in the fragment-major residues and loaded once per block (wave w
+// loads line block w % 2 of A (w < 2) or of B (w >= 2), both k blocks, by
+// buffer loads that write LDS directly) into a ring of RING steps.  At step s
+// every wave waits for its own loads of step s (vmcnt), the block meets at a
+// barrier, and each wave reads its A and B fragments and runs two MFMAs (one
+// accumulator per k block parity); then it loads step s + RING - 1 into the
+// slot every wave finished reading before that barrier.  Every step issues
+// the same loads (past the job's last step they repeat it) so the wait count
+// stays exact.  Loading each fragment once per block, rather than once per
+// wave that needs it, halves the kernel's fragment traffic.
+constexpr int RING = 4;                       // steps in flight per block
+constexpr int STEP = 8 * crt::FRAG;           // LDS bytes of one step: A lb 0, 1 then B lb 0, 1; k blocks 2s, 2s + 1
 
 __global__ __launch_bounds__(256, 2) void bcrt_kernel(const Item *__restrict__ items, const int *__restrict__ begin,
                                                       const Prob *__restrict__ probs, const Job *__restrict__ jobs,
                                                       const Side *__restrict__ sides,
                                                       const signed char *__restrict__ rpool,
                                                       const int *__restrict__ epool, CrtTables tb, int cplx) {
-    __shared__ __attribute__((aligned(1024))) signed char lds[4 * RING * STEP];
+    __shared__ __attribute__((aligned(1024))) signed char ring[RING * STEP];
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    signed char *ring = lds + wave * RING * STEP;
     const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+    // fragment slots within a step: A line block i at 2 i + kb, B line block j at 4 + 2 j + kb
+    const int ld_base = (wave < 2 ? 2 * wave : 4 + 2 * (wave - 2)) * crt::FRAG;   // this wave's two loads
+    const int rd_a = 2 * (wm >> 5) * crt::FRAG, rd_b = (4 + 2 * (wn >> 5)) * crt::FRAG;
     const int it1 = begin[blockIdx.x + 1];
     for (int it = begin[blockIdx.x]; it < it1; ++it) {
         const Item item = items[it];
@@ -431,29 +437,25 @@ __global__ __launch_bounds__(256, 2) void bcrt_kernel(const Item *__restrict__ i
             const Side &sa = sides[jo.sa];
             const Side &sb = sides[jo.sb];
             const int KC = (int)(sa.Kp >> 5);     // k blocks per modulus (even: Kp is a multiple of 64)
-            const int total = tb.n * KC;
-            // this wave's line block of modulus 0; planes of successive moduli follow
-            const signed char *abase = rpool + sa.dig + ((((m0 + wm) >> 5) * KC) << 10);
-            const signed char *bbase = rpool + sb.dig + ((((n0 + wn) >> 5) * KC) << 10);
-            const int64_t la = sa.Xp * sa.Kp, lb = sb.Xp * sb.Kp;
-            const signed char *acur = abase, *bcur = bbase;   // modulus of the next step to load
-            int iks = 0, issued = 0;
+            const int total = tb.n * (KC >> 1);   // steps
+            // this wave's line block of modulus 0 (A for waves 0-1, B for 2-3); moduli follow
+            const Side &so = wave < 2 ? sa : sb;
+            const int64_t x0 = (wave < 2 ? m0 : n0) + 32 * (wave & 1);
+            const signed char *cur = rpool + so.dig + (((x0 >> 5) * KC) << 10);
+            const int64_t plane = so.Xp * so.Kp;
+            int iks = 0, issued = 0;              // next step to load: k blocks 2 iks, 2 iks + 1
             auto issue = [&](int slot) {
-                const __amdgpu_buffer_rsrc_t ra =
-                    __builtin_amdgcn_make_buffer_rsrc((void *)acur, (short)0, 0x7FFFFFF0, 0x00020000);
-                const __amdgpu_buffer_rsrc_t rb =
-                    __builtin_amdgcn_make_buffer_rsrc((void *)bcur, (short)0, 0x7FFFFFF0, 0x00020000);
-                __builtin_amdgcn_raw_ptr_buffer_load_lds(ra, (lds_void *)(ring + slot * STEP), 16, 16 * lane,
-                                                         iks << 10, 0, 0);
-                __builtin_amdgcn_raw_ptr_buffer_load_lds(rb, (lds_void *)(ring + slot * STEP + crt::FRAG), 16,
-                                                         16 * lane, iks << 10, 0, 0);
-                if (++issued < total && ++iks == KC) {   // past the last step: repeat it
+                const __amdgpu_buffer_rsrc_t rs =
+                    __builtin_amdgcn_make_buffer_rsrc((void *)cur, (short)0, 0x7FFFFFF0, 0x00020000);
+#pragma unroll
+                for (int kb = 0; kb < 2; ++kb)
+                    __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, (lds_void *)(ring + slot * STEP + ld_base + kb * crt::FRAG),
+                                                             16, 16 * lane, (2 * iks + kb) << 10, 0, 0);
+                if (++issued < total && ++iks == (KC >> 1)) {   // past the last step: repeat it
                     iks = 0;
-                    acur += la;
-                    bcur += lb;
+                    cur += plane;
                 }
             };
-            auto frag = [&](int slot, int part) { return *(const i32x4 *)(ring + slot * STEP + part * crt::FRAG + 16 * lane); };
 #pragma unroll
             for (int j = 0; j < RING - 1; ++j) issue(j);
             double S0[16], S1[16], S2[16], S3[16];
@@ -462,31 +464,20 @@ __global__ __launch_bounds__(256, 2) void bcrt_kernel(const Item *__restrict__ i
             i32x16 acc0, acc1;
 #pragma unroll
             for (int r = 0; r < 16; ++r) acc0[r] = acc1[r] = 0;
-            // Step s: load step s + RING - 1 into the slot read one step ago, wait
-            // for step s + 1 and read its fragments, then run step s's MFMA on the
-            // fragments read one step earlier (LDS latency beside the MFMA).
-            F4J_WAIT_VMCNT(2 * (RING - 2));
-            i32x4 fa = frag(0, 0), fb = frag(0, 1);
-            int cks = 0, cl = 0;                   // step being multiplied
-            for (int st = 0; st < total; st += 2) {
-#pragma unroll
-                for (int h = 0; h < 2; ++h) {
-                    const int s1 = st + h + 1;
-                    // the previous MFMA (and the wait for its fragments' reads) stays
-                    // above the load that overwrites their slot
-                    __builtin_amdgcn_sched_barrier(0);
-                    issue((s1 + RING - 2) & (RING - 1));
-                    F4J_WAIT_VMCNT(2 * (RING - 2));  // step s + 1's loads are in LDS
-                    const i32x4 na = frag(s1 & (RING - 1), 0), nb = frag(s1 & (RING - 1), 1);
-                    if (h == 0)
-                        acc0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa, fb, acc0, 0, 0, 0);
-                    else
-                        acc1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa, fb, acc1, 0, 0, 0);
-                    fa = na;
-                    fb = nb;
-                }
-                cks += 2;
-                if (cks == KC) {                   // modulus cl done: fold its residues into S
+            int cks = 0, cl = 0;                   // step being multiplied (within modulus cl)
+            for (int st = 0; st < total; ++st) {
+                const int slot = st & (RING - 1);
+                F4J_WAIT_VMCNT(2 * (RING - 2));      // this wave's loads of step st are in LDS
+                __builtin_amdgcn_s_barrier();        // ... and every wave's
+                __builtin_amdgcn_sched_barrier(0);
+                const signed char *sp = ring + slot * STEP + 16 * lane;
+                const i32x4 fa0 = *(const i32x4 *)(sp + rd_a), fa1 = *(const i32x4 *)(sp + rd_a + crt::FRAG);
+                const i32x4 fb0 = *(const i32x4 *)(sp + rd_b), fb1 = *(const i32x4 *)(sp + rd_b + crt::FRAG);
+                acc0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa0, fb0, acc0, 0, 0, 0);
+                acc1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa1, fb1, acc1, 0, 0, 0);
+                __builtin_amdgcn_sched_barrier(0);
+                issue((st + RING - 1) & (RING - 1)); // the slot every wave read before this step's barrier
+                if (++cks == (KC >> 1)) {          // modulus cl done: fold its residues into S
                     const double m = (double)tb.m[cl], minv = tb.minv[cl];
                     const double w0 = tb.w[cl][0], w1 = tb.w[cl][1], w2 = tb.w[cl][2], w3 = tb.w[cl][3];
 #pragma unroll
@@ -502,7 +493,8 @@ __global__ __launch_bounds__(256, 2) void bcrt_kernel(const Item *__restrict__ i
                     ++cl;
                 }
             }
-            F4J_WAIT_VMCNT(0);                     // the repeated loads have landed: the ring is free
+            F4J_WAIT_VMCNT(0);                     // the repeated loads have landed
+            __syncthreads();                       // and every wave is done with the ring
             const int64_t col = n0 + wn + crt::acc_col(0, lane);
             if (col < N) {
                 const int ebv = epool[sb.eoff + col];
