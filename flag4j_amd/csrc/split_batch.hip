@@ -292,7 +292,8 @@ __global__ __launch_bounds__(256) void bres_kernel(const Prob *__restrict__ prob
     double e = 0.0;
     if (!cplx) {
         double s = 0.0, c = 0.0, sa = 0.0;
-        for (int64_t k = 0; k < p.K; ++k) {
+#pragma unroll 8
+        for (int64_t k = 0; k < p.K; ++k) {         // (unrolled: the loads of 8 k run ahead of the sums)
             const double a = opA ? p.A[k * p.lda + i] : p.A[i * p.lda + k];
             const double b = opB ? p.B[j * p.ldb + k] : p.B[k * p.ldb + j];
             dot2_add(s, c, a, b);
@@ -303,6 +304,7 @@ __global__ __launch_bounds__(256) void bres_kernel(const Prob *__restrict__ prob
     } else {
         const double sga = opA == 2 ? -1.0 : 1.0, sgb = opB == 2 ? -1.0 : 1.0;
         double rs = 0.0, rc = 0.0, is = 0.0, ic = 0.0, sa = 0.0;
+#pragma unroll 8
         for (int64_t k = 0; k < p.K; ++k) {
             const double *pa = opA ? p.A + 2 * (k * p.lda + i) : p.A + 2 * (i * p.lda + k);
             const double *pb = opB ? p.B + 2 * (j * p.ldb + k) : p.B + 2 * (k * p.ldb + j);
