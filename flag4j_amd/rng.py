@@ -11,12 +11,16 @@ Python-level loop per element.  Flag4j draws its random matrices from
 java.util.Random (M:rng/RandomComplex.java:69, M:rng/RandomDenseTensor.java:94-97),
 so a Java harness seeded the same way produces the same matrices.
 """
+import os
+from concurrent.futures import ThreadPoolExecutor
+
 import numpy as np
 
 _A = 0x5DEECE66D
 _C = 0xB
 _MASK = (1 << 48) - 1
-_BLOCK = 1 << 14
+_BLOCK = 1 << 16
+_STEP_TABLE = []
 
 
 def _affine_pow(n):
@@ -31,17 +35,24 @@ def _affine_pow(n):
     return a, c
 
 
+def _step_table():
+    """(A_k, C_k) for k = 1.._BLOCK: the states of one block relative to its origin."""
+    if not _STEP_TABLE:
+        a1 = np.empty(_BLOCK, dtype=np.uint64)
+        c1 = np.empty(_BLOCK, dtype=np.uint64)
+        a, c = 1, 0
+        for k in range(_BLOCK):
+            a, c = (_A * a) & _MASK, (_A * c + _C) & _MASK
+            a1[k], c1[k] = a, c
+        _STEP_TABLE.extend([a1, c1])
+    return _STEP_TABLE[0], _STEP_TABLE[1]
+
+
 def _states(s0, start, count):
     """LCG states s_{start+1} .. s_{start+count} (the values next() shifts from)."""
     L = _BLOCK
     nblk = (count + L - 1) // L
-    # states within one block, relative to its origin: s_{k} for k = 1..L
-    a1 = np.empty(L, dtype=np.uint64)
-    c1 = np.empty(L, dtype=np.uint64)
-    a, c = 1, 0
-    for k in range(L):
-        a, c = (_A * a) & _MASK, (_A * c + _C) & _MASK
-        a1[k], c1[k] = a, c
+    a1, c1 = _step_table()
     out = np.empty(nblk * L, dtype=np.uint64)
     mask = np.uint64(_MASK)
     ab, cb = _affine_pow(start)
@@ -63,12 +74,32 @@ def _next_doubles(seed, start_double, n):
     return ((hi << 27) + lo).astype(np.float64) * 2.0**-53
 
 
-def next_doubles(seed, n, chunk=1 << 22):
-    """The first n values of new java.util.Random(seed).nextDouble()."""
+def _threads():
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def next_doubles(seed, n, chunk=1 << 22, start=0):
+    """Values start .. start + n - 1 of new java.util.Random(seed).nextDouble().
+    Chunks are independent (each jumps ahead to its own origin), so they are
+    generated on a thread pool: numpy releases the interpreter lock inside its
+    array loops.  The stream is the same whatever the thread count."""
     out = np.empty(n, dtype=np.float64)
-    for s in range(0, n, chunk):
-        e = min(n, s + chunk)
-        out[s:e] = _next_doubles(seed, s, e - s)
+    spans = [(s, min(n, s + chunk)) for s in range(0, n, chunk)]
+
+    def fill(span):
+        s, e = span
+        out[s:e] = _next_doubles(seed, start + s, e - s)
+
+    if len(spans) <= 1:
+        for sp in spans:
+            fill(sp)
+    else:
+        with ThreadPoolExecutor(max_workers=min(_threads(), len(spans))) as ex:
+            list(ex.map(fill, spans))
     return out
 
 
@@ -90,24 +121,36 @@ def scaled_uniform(seed, rows, cols, phi):
     return ((2.0 * r - 1.0) * 10.0 ** (phi * (2.0 * rp - 1.0))).reshape(rows, cols)
 
 
-def next_gaussians(seed, n):
+def _polar(u):
+    """Accepted polar-method pairs of consecutive doubles u[0::2], u[1::2], in order."""
+    v1, v2 = 2 * u[0::2] - 1, 2 * u[1::2] - 1
+    s = v1 * v1 + v2 * v2
+    ok = (s < 1) & (s != 0)
+    mul = np.sqrt(-2 * np.log(s[ok]) / s[ok])
+    g = np.empty(2 * int(ok.sum()))
+    g[0::2], g[1::2] = v1[ok] * mul, v2[ok] * mul
+    return g
+
+
+def next_gaussians(seed, n, chunk=1 << 22):
     """The first n values of new java.util.Random(seed).nextGaussian(): the
     polar method on consecutive nextDouble() pairs, rejected pairs skipped,
-    each accepted pair giving v1*mul then v2*mul (log/sqrt from numpy)."""
-    out = np.empty(0)
-    start, need = 0, n
+    each accepted pair giving v1*mul then v2*mul (log/sqrt from numpy).
+    Chunks of pairs are transformed on a thread pool and joined in order."""
+    parts, start, need = [], 0, n
     while need > 0:
         m = max(64, int(need * 0.7) + 64)            # pairs to draw (acceptance ~ pi/4)
-        u = _next_doubles(seed, start, 2 * m)
+        u = next_doubles(seed, 2 * m, start=start)
         start += 2 * m
-        v1, v2 = 2 * u[0::2] - 1, 2 * u[1::2] - 1
-        s = v1 * v1 + v2 * v2
-        ok = (s < 1) & (s != 0)
-        mul = np.sqrt(-2 * np.log(s[ok]) / s[ok])
-        g = np.empty(2 * int(ok.sum()))
-        g[0::2], g[1::2] = v1[ok] * mul, v2[ok] * mul
-        out = np.concatenate([out, g])
-        need = n - out.size
+        spans = [u[s:min(2 * m, s + chunk)] for s in range(0, 2 * m, chunk)]
+        if len(spans) <= 1:
+            got = [_polar(sp) for sp in spans]
+        else:
+            with ThreadPoolExecutor(max_workers=min(_threads(), len(spans))) as ex:
+                got = list(ex.map(_polar, spans))
+        parts.extend(got)
+        need -= sum(g.size for g in got)
+    out = np.concatenate(parts) if parts else np.empty(0)
     return out[:n]
 
 

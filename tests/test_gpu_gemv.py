@@ -65,6 +65,21 @@ def test_golden_vectors_through_the_api(golden):
     assert np.array_equal(bits(C.data), bits(real_mat(g["c"])))
 
 
+def test_real_blocked_exact_cases():
+    """The blocked kernel on rows whose blocked-order results are fixed by hand in exact
+    arithmetic (test_oracle.blocked_vector_cases): 40 copies of each row, sign
+    alternating, so M K + K is above the small path and the blocked form runs."""
+    from test_oracle import blocked_vector_cases
+    for a, x, blocked, _ in blocked_vector_cases():
+        sgn = np.where(np.arange(40) % 2, -1.0, 1.0)
+        A = np.array([a] * 40) * sgn[:, None]
+        y = dgemv(A, np.array(x))
+        exp = sgn * blocked
+        if blocked == 0.0:
+            exp = np.zeros(40)          # +0.0 + (-0.0) = +0.0 in every row
+        assert np.array_equal(bits(y), bits(exp)), (len(a), y[:2], blocked)
+
+
 @pytest.mark.parametrize("M,K", [(1000, 1000), (257, 4097), (64, 512), (8, 513), (3, 1000), (100, 64),
                                  (100, 65), (9, 2001), (31, 1001), (2, 100000)])
 def test_real_blocked_bitwise(M, K):

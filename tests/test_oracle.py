@@ -57,24 +57,57 @@ def test_real_column_vector_golden(golden):
     assert np.array_equal(bits(o.real_standard(A, x.reshape(-1, 1)).reshape(-1)), bits(exp))
 
 
-def test_blocked_vector_partial_sums():
-    """The N = 1 blocked kernel (RealDenseMatMult.java:458-491) sums each 64-k
-    block from +0.0 and adds the block sums: check against a literal
-    restatement of that rule at K > 64, and that it differs from the k-ordered
-    form on an input built to separate them."""
-    rng = np.random.default_rng(3)
-    A = rng.standard_normal((5, 200)) * 10.0 ** rng.integers(-8, 8, (5, 200))
-    x = rng.standard_normal(200)
-    y = o.real_blocked_vector(A, x, 64)
-    for i in range(5):
-        tot = 0.0
-        for kk in range(0, 200, 64):
-            s = 0.0
-            for k in range(kk, min(kk + 64, 200)):
-                s += A[i, k] * x[k]
-            tot += s
-        assert y[i] == tot
-    assert not np.array_equal(y, o.real_standard(A, x.reshape(-1, 1)).reshape(-1))
+def blocked_vector_cases():
+    """Rows whose results under the N = 1 blocked rule (RealDenseMatMult.java:458-491:
+    each 64-wide k block summed from +0.0, the block sums added in k order into
+    dest = +0.0) are fixed by hand in exact binary arithmetic.  u = 2^-53 is half
+    an ulp of 1, so 1 + u rounds (to even) back to 1, while a sum of tiny terms
+    alone is exact.  Each case: (row a, vector x, blocked result, k-ordered result)."""
+    u = 2.0**-53
+    cases = []
+    # K = 128: block 0 = 1 (its 63 tiny terms vanish one by one), block 1 = 64u
+    # = 2^-47 exactly, so the blocked result is 1 + 2^-47; one k-ordered chain
+    # loses every tiny term and gives 1.  A block size of 32 would give
+    # ((1 + 2^-48) + 2^-48) + 2^-48 = 1 + 3 * 2^-48, of 128 the chain's 1.
+    cases.append(([1.0] + [u] * 127, [1.0] * 128, 1.0 + 2.0**-47, 1.0))
+    # K = 100: the ragged last block holds k = 64..99, 36 terms: 1 + 36u = 1 + 9 * 2^-51.
+    cases.append(([1.0] + [u] * 99, [1.0] * 100, 1.0 + 9 * 2.0**-51, 1.0))
+    # The tiny terms as products: a = 2^-27, x = 2^-26 (each product is exactly u).
+    cases.append(([1.0] + [2.0**-27] * 127, [1.0] + [2.0**-26] * 127, 1.0 + 2.0**-47, 1.0))
+    # K = 192, block sums 1, -1, 2^-60 added in k order: (1 - 1) + 2^-60 = 2^-60;
+    # in reverse order (2^-60 - 1) + 1 = 0.  The chain gives 2^-60 too.
+    a = [0.0] * 192
+    a[0], a[64], a[128] = 1.0, -1.0, 2.0**-60
+    cases.append((a, [1.0] * 192, 2.0**-60, 2.0**-60))
+    # K = 130: tiny terms ahead of the 1 in the last block: block 0 = 64u,
+    # block 1 = 64u, block 2 = 1 + u + u... (k = 128: 1, k = 129: u): 1;
+    # total ((0 + 2^-47) + 2^-47) + 1 = 1 + 2^-46; the chain: 128u = 2^-46, + 1, + u -> 1 + 2^-46.
+    cases.append(([u] * 128 + [1.0, u], [1.0] * 130, 1.0 + 2.0**-46, 1.0 + 2.0**-46))
+    # Signed zero: every product -0.0; sums start from +0.0 and +0.0 + -0.0 = +0.0.
+    cases.append(([-0.0] * 70, [1.0] * 70, 0.0, 0.0))
+    return cases
+
+
+def test_blocked_vector_exact_cases():
+    """The N = 1 blocked kernel against results fixed by hand in exact arithmetic
+    (not a restatement of its loop): the 64-wide blocks, the ragged last block,
+    block order and the +0.0 starts each decide one case."""
+    for a, x, blocked, chain in blocked_vector_cases():
+        A = np.array([a], dtype=np.float64)
+        xv = np.array(x, dtype=np.float64)
+        y = o.real_blocked_vector(A, xv, 64)
+        assert bits(y)[0] == bits(np.array([blocked]))[0], (len(a), y[0], blocked)
+        ys = o.real_standard(A, xv.reshape(-1, 1)).reshape(-1)
+        assert bits(ys)[0] == bits(np.array([chain]))[0], (len(a), ys[0], chain)
+    # a plausible mistake, a block size of 32, fails the first case
+    a, x, blocked, _ = blocked_vector_cases()[0]
+    y32 = o.real_blocked_vector(np.array([a]), np.array(x), 32)
+    assert y32[0] == 1.0 + 3 * 2.0**-48 and y32[0] != blocked
+    # rows are independent (the 64-row blocks of the i loop change nothing):
+    # 130 copies of the first case, negated on odd rows, in one call
+    A = np.array([a] * 130) * np.where(np.arange(130) % 2, -1.0, 1.0)[:, None]
+    y = o.real_blocked_vector(A, np.array(x), 64)
+    assert np.array_equal(y, np.where(np.arange(130) % 2, -blocked, blocked))
 
 
 def test_real_api_mult_and_pow(golden):
@@ -309,6 +342,88 @@ def test_real_dispatch_selections_for_the_workloads():
     assert o.select_real(100, 100, 1) == "BLK_VEC"
     assert o.select_cplx(300, 300, 1) == "MT_STRD_VEC"
     assert o.select_cplx(2048, 1536, 2048) == "MT_REORD"
+
+
+# (m, n = inner size, k = B.cols) -> kernel, on both sides of every threshold of
+# ReDeMatMultDispatcher.dispatch/getFunc (ReDeMatMultDispatcher.java:157-237, defaults of
+# R:configs/ReDeMatMult.properties: smallThreshold 2450, aspectThreshold 2.0,
+# squareMtStandardThreshold 50, squareMtReorderedThreshold 2048, minDimSmallThreshold 10,
+# wideMtReorderedThreshold 25, squareStandardVecThreshold 256), read off the source text.
+# Each pair differs in its result, so a "<" written as "<=" (or a branch taken in
+# another order) at any threshold flips one entry.
+REAL_BOUNDARIES = [
+    ((40, 31, 39), "SMALL_STANDARD"),   # 31 * 40 + 31 * 39 = 2449 < 2450
+    ((35, 35, 35), "MT_STRD"),          # 2 * 35^2 = 2450: not small; square, n = 35 < 50
+    ((255, 255, 1), "BLK_VEC"),         # vector, aspect 1 <= 8, m = 255 < 256
+    ((256, 256, 1), "MT_BLK_VEC"),      # m = 256
+    ((2048, 256, 1), "MT_BLK_VEC"),     # aspect exactly 8.0 = 4 * aspectThreshold
+    ((2049, 256, 1), "BLK_VEC"),        # aspect 8.004 > 8
+    ((256, 2048, 1), "MT_BLK_VEC"),     # aspect 8.0 with n the larger
+    ((256, 2049, 1), "BLK_VEC"),
+    ((49, 49, 49), "MT_STRD"),          # square, n = 49 < 50
+    ((50, 50, 50), "MT_REORD"),         # n = 50
+    ((2047, 2047, 2047), "MT_REORD"),   # n = 2047 < 2048
+    ((2048, 2048, 2048), "MT_BLK_REORD"),
+    ((98, 49, 98), "MT_STRD"),          # aspect exactly 2.0: still "square"
+    ((99, 49, 98), "MT_REORD"),         # aspect 2.02: m == maxDim, minDim 49 >= 10
+    ((4096, 2048, 4096), "MT_BLK_REORD"),
+    ((4097, 2048, 4096), "MT_REORD"),
+    ((1000, 9, 1000), "STRD"),          # m == maxDim (ties with k: m is tested first), minDim 9 < 10
+    ((1000, 10, 1000), "MT_REORD"),
+    ((24, 1000, 600), "MT_STRD"),       # n == maxDim, m = 24 < 25
+    ((25, 1000, 600), "MT_REORD"),
+    ((9, 1000, 600), "STRD"),           # n == maxDim, minDim 9 < 10
+    ((10, 1000, 600), "MT_STRD"),       # minDim 10, m = 10 < 25
+    ((5, 1000, 1000), "STRD"),          # n and k tie for maxDim: the n branch is tested first
+    ((9, 600, 1000), "REORD"),          # k == maxDim, minDim 9
+    ((10, 600, 1000), "MT_REORD"),
+]
+
+# Cm128DeMatMultDispatcher.dispatch/getFunc (Cm128DeMatMultDispatcher.java:155-226): the same
+# tree for N > 1; every N = 1 product (above the small path) is MT_STRD_VEC, with no
+# squareStandardVecThreshold.
+CPLX_BOUNDARIES = [
+    ((40, 31, 39), "SMALL_STANDARD"),
+    ((35, 35, 35), "MT_STRD"),
+    ((255, 255, 1), "MT_STRD_VEC"),
+    ((2049, 256, 1), "MT_STRD_VEC"),
+    ((1224, 1, 1), "SMALL_STANDARD"),   # 1224 + 1 < 2450
+    ((49, 49, 49), "MT_STRD"),
+    ((50, 50, 50), "MT_REORD"),
+    ((2047, 2047, 2047), "MT_REORD"),
+    ((2048, 2048, 2048), "MT_BLK_REORD"),
+    ((98, 49, 98), "MT_STRD"),
+    ((99, 49, 98), "MT_REORD"),
+    ((1000, 9, 1000), "STRD"),
+    ((24, 1000, 600), "MT_STRD"),
+    ((25, 1000, 600), "MT_REORD"),
+    ((5, 1000, 1000), "STRD"),
+    ((9, 600, 1000), "REORD"),
+    ((10, 600, 1000), "MT_REORD"),
+]
+
+
+@pytest.mark.parametrize("shape,kernel", REAL_BOUNDARIES)
+def test_real_selector_threshold_boundaries(shape, kernel):
+    assert o.select_real(*shape) == kernel
+
+
+@pytest.mark.parametrize("shape,kernel", CPLX_BOUNDARIES)
+def test_cplx_selector_threshold_boundaries(shape, kernel):
+    assert o.select_cplx(*shape) == kernel
+
+
+def test_selector_boundaries_follow_configured_thresholds():
+    """The thresholds come from the configuration, not constants: moving one moves
+    its boundary (the reference reads them from R:configs/*.properties,
+    ReDeMatMultDispatchConfigs.java:52-117)."""
+    assert o.select_real(35, 35, 35, small_threshold=2451) == "SMALL_STANDARD"
+    assert o.select_real(49, 49, 49, sq_mt_standard=49) == "MT_REORD"
+    assert o.select_real(99, 49, 98, aspect_threshold=2.1) == "MT_STRD"
+    assert o.select_real(255, 255, 1, sq_seq_vec=255) == "MT_BLK_VEC"
+    assert o.select_real(10, 600, 1000, min_dim_small=11) == "REORD"
+    assert o.select_real(25, 1000, 600, wide_mt_reordered=26) == "MT_STRD"
+    assert o.select_real(2047, 2047, 2047, sq_mt_reordered=2047) == "MT_BLK_REORD"
 
 
 def test_java_random_known_answers():
