@@ -70,6 +70,8 @@ bool crt_tables(int n, int64_t K, CrtTables *tb);
 // Smallest number of moduli whose product gives t >= 53 bits for inner size K
 // (every entry that is within a factor 2 of its line's largest keeps all its bits).
 int crt_default_moduli(int64_t K);
+// Fewest moduli whose t = crt_bits(n, K) reaches `bits` (CRT_MAX_MODULI if none).
+int crt_moduli_for_bits(int bits, int64_t K);
 
 // Plan for C (M x N) = op(A) op(B) with n moduli; returns false if n is
 // outside [2, CRT_MAX_MODULI] or gives fewer than 1 bit for this K.

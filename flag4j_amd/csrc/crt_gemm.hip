@@ -544,11 +544,13 @@ __global__ __launch_bounds__(256) void crt_reconstruct_kernel(const unsigned cha
 
 }  // namespace
 
-int crt_default_moduli(int64_t K) {
+int crt_moduli_for_bits(int bits, int64_t K) {
     for (int n = 2; n <= CRT_MAX_MODULI; ++n)
-        if (t_for(n, K) >= 53) return n;
+        if (t_for(n, K) >= bits) return n;
     return CRT_MAX_MODULI;
 }
+
+int crt_default_moduli(int64_t K) { return crt_moduli_for_bits(53, K); }
 
 int crt_bits(int n, int64_t K) { return t_for(n, K); }
 

@@ -42,10 +42,21 @@ struct GroupedItem {
 // Launch the FP64 MFMA GEMM for one problem (kernel-level, device pointers).
 hipError_t launch_gemm_f64(OpKind ka, OpKind kb, const GemmProblem &p, hipStream_t stream, int tile_cfg);
 
+// Optional condition on the items of a grouped launch: with err set, an item of
+// problem p runs only if slot[p] >= 0 and err[slot[p]] > above (the bits of
+// non-negative doubles order as the values do).  The batched split plans use it
+// to recompute in FP64 exactly the products whose residual check failed.
+struct GroupedGuard {
+    const unsigned long long *err = nullptr;
+    const int *slot = nullptr;
+    unsigned long long above = 0;
+};
+
 // Persistent grouped FP64 GEMM over a work-item list; workgroup g runs items
 // [begin[g], begin[g+1]).  All problems share op kinds and tile config.
 hipError_t launch_gemm_f64_grouped(OpKind ka, OpKind kb, const GemmProblem *probs, const GroupedItem *items,
-                                   const int *begin, int wgs, int cfg, hipStream_t stream);
+                                   const int *begin, int wgs, int cfg, hipStream_t stream,
+                                   GroupedGuard guard = GroupedGuard{});
 
 // Flag4j-order (bitwise) GEMM; sizes and leading dimensions in element units.
 hipError_t launch_gemm_strict(bool cplx, int opA, int opB, int64_t M, int64_t N, int64_t K, const double *A,
@@ -66,10 +77,11 @@ hipError_t launch_gemm_strict_mixed(int mode, int64_t M, int64_t N, int64_t K, c
 // y[i * incy] = sum_k A[i * lda + k] x[k * incx]; lda, incx, incy in units of
 // each operand's elements (complex or real), conjx conjugates x (GEMV_CPLX only).
 // Returns hipErrorInvalidValue where the kernels do not apply (see gemv_supported;
-// complex A, and real A with a complex x, must also be 16-byte aligned).
+// outside the blocked form, complex A and complex x must also be 16-byte aligned).
 enum GemvKind : int { GEMV_BLOCKED = 0, GEMV_CHAIN = 1, GEMV_CPLX = 2, GEMV_RX = 3, GEMV_CX = 4 };
 bool gemv_supported(int kind, int64_t K, int64_t lda, int64_t incx);
-bool gemv_applicable(int kind, int64_t K, const double *A, int64_t lda, int64_t incx);   // supported and aligned
+bool gemv_applicable(int kind, int64_t K, const double *A, int64_t lda, const double *x,
+                     int64_t incx);   // supported and aligned
 hipError_t launch_gemv(int kind, int64_t M, int64_t K, const double *A, int64_t lda, const double *x, int64_t incx,
                        double *y, int64_t incy, bool conjx, hipStream_t stream);
 

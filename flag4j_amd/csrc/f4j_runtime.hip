@@ -229,10 +229,28 @@ int default_slices_for(int prec) {
     return prec == F4J_PREC_SPLIT_FP16 ? 5 : 6;
 }
 
+// Whether `slices` (the caller's value) leaves the count to the library: then a
+// product whose residual check fails is computed once more with a count raised
+// by what the measured error asks for, before any FP64 fallback.
+bool auto_slices(int slices) {
+    if (slices > 0) return false;
+    std::lock_guard<std::mutex> g(S().cfg_mu);
+    return S().cfg.default_slices == 0;
+}
+
+// Digits to add so that an error `err` falls below `tol`: each digit adds
+// `bits_per` bits; one bit of margin.
+int extra_digits(double err, double tol, int bits_per) {
+    const double bits = std::log2(err / tol) + 1.0;
+    return std::max(1, (int)std::ceil(bits / bits_per));
+}
+
 // Split-precision real GEMM (synchronous: its workspace is shared and its
 // residual check reads back one number).
+// With `retry` set, a failed check whose error more digits can fix returns F4J_OK
+// with *retry = the digit count to run again with (no fallback, no report).
 int run_split(int opA, int opB, int64_t M, int64_t N, int64_t K, const double *A, int64_t lda, const double *B,
-              int64_t ldb, double *C, int64_t ldc, int prec, int slices, hipStream_t stream) {
+              int64_t ldb, double *C, int64_t ldc, int prec, int slices, hipStream_t stream, int *retry) {
     const int carrier = prec == F4J_PREC_SPLIT_FP16 ? SPLIT_FP16 : SPLIT_BF16;
     const int S_ = slices > 0 ? slices : default_slices_for(prec);
     if (S_ < 1 || S_ > SPLIT_MAX_SLICES)
@@ -280,6 +298,11 @@ int run_split(int opA, int opB, int64_t M, int64_t N, int64_t K, const double *A
     } else if (rep.checked_rows > 0) {
         rep.max_rel_err = err;
         if (!(err <= c.split_tolerance)) {
+            if (retry && S_ < SPLIT_MAX_SLICES && std::isfinite(err) && c.split_tolerance > 0) {
+                *retry = std::min(SPLIT_MAX_SLICES,
+                                  S_ + extra_digits(err, c.split_tolerance, carrier == SPLIT_FP16 ? 10 : 9));
+                return F4J_OK;
+            }
             if (!c.split_fallback) {
                 t_split = rep;
                 return fail(F4J_ERR_ACCURACY, "split product error " + std::to_string(err) + " above tolerance " +
@@ -312,7 +335,8 @@ hipError_t launch_fp64(bool cplx, int opA, int opB, int64_t M, int64_t N, int64_
 // (three real products in the 3M form when use3m, else four), with the same
 // residual check, fallback and report as the real form.
 int run_split_cplx(int opA, int opB, int64_t M, int64_t N, int64_t K, const double *A, int64_t lda, const double *B,
-                   int64_t ldb, double *C, int64_t ldc, int prec, int slices, int use3m, hipStream_t stream) {
+                   int64_t ldb, double *C, int64_t ldc, int prec, int slices, int use3m, hipStream_t stream,
+                   int *retry) {
     const int carrier = prec == F4J_PREC_SPLIT_FP16 ? SPLIT_FP16 : SPLIT_BF16;
     const int S_ = slices > 0 ? slices : default_slices_for(prec);
     if (S_ < 1 || S_ > SPLIT_MAX_SLICES)
@@ -358,6 +382,11 @@ int run_split_cplx(int opA, int opB, int64_t M, int64_t N, int64_t K, const doub
     } else if (rep.checked_rows > 0) {
         rep.max_rel_err = perr;
         if (!(perr <= c.split_tolerance)) {
+            if (retry && S_ < SPLIT_MAX_SLICES && std::isfinite(perr) && c.split_tolerance > 0) {
+                *retry = std::min(SPLIT_MAX_SLICES,
+                                  S_ + extra_digits(perr, c.split_tolerance, carrier == SPLIT_FP16 ? 10 : 9));
+                return F4J_OK;
+            }
             if (!c.split_fallback) {
                 t_split = rep;
                 return fail(F4J_ERR_ACCURACY, "complex split product error " + std::to_string(perr) +
@@ -375,14 +404,17 @@ int run_split_cplx(int opA, int opB, int64_t M, int64_t N, int64_t K, const doub
 // Modular (CRT) split-precision real GEMM on the INT8 matrix cores;
 // `slices` = number of moduli (0: enough for FP64 accuracy at this K).
 int run_crt(bool cplx, int opA, int opB, int64_t M, int64_t N, int64_t K, const double *A, int64_t lda,
-            const double *B, int64_t ldb, double *C, int64_t ldc, int slices, int use3m, hipStream_t stream) {
+            const double *B, int64_t ldb, double *C, int64_t ldc, int slices, int use3m, hipStream_t stream,
+            int *retry) {
     Config c;
     {
         std::lock_guard<std::mutex> g(S().cfg_mu);
         c = S().cfg;
     }
-    int n = slices > 0 ? slices : (c.default_slices > 0 ? c.default_slices : crt_default_moduli(K));
-    if (n > CRT_MAX_MODULI) n = CRT_MAX_MODULI;
+    const int n = slices > 0 ? slices : (c.default_slices > 0 ? c.default_slices : crt_default_moduli(K));
+    if (n < 2 || n > CRT_MAX_MODULI)
+        return fail(F4J_ERR_INVALID_ARG, "split_int8: moduli must be in 2.." + std::to_string(CRT_MAX_MODULI) +
+                                             " (0 = chosen from K), got " + std::to_string(n));
     CrtPlan pl;
     if (K >= (int64_t(1) << 17) || !plan_crt(M, N, K, n, c.split_check_rows, &pl))
         return fail(F4J_ERR_INVALID_ARG, "split_int8: " + std::to_string(n) +
@@ -467,6 +499,13 @@ int run_crt(bool cplx, int opA, int opB, int64_t M, int64_t N, int64_t K, const 
     } else if (rep.checked_rows > 0) {
         rep.max_rel_err = err;
         if (!(err <= c.split_tolerance)) {
+            if (retry && n < CRT_MAX_MODULI && std::isfinite(err) && c.split_tolerance > 0) {
+                // t grows by about 4 bits per modulus: ask for the bits the error lacks
+                const int bits = (int)std::ceil(std::log2(err / c.split_tolerance)) + 1;
+                const int m = std::max(n + 1, crt_moduli_for_bits(crt_bits(n, K) + bits, K));
+                *retry = std::min(CRT_MAX_MODULI, m);
+                return F4J_OK;
+            }
             if (!c.split_fallback) {
                 t_split = rep;
                 return fail(F4J_ERR_ACCURACY, "split_int8 product error " + std::to_string(err) +
@@ -506,7 +545,7 @@ int gemv_kind(bool cplx, bool b_transposed, int64_t M, int64_t K) {
 // matrix-vector kernels' limits (the caller then takes another path).
 bool try_gemv(int kind, int64_t M, int64_t K, const double *A, int64_t lda, const double *x, int64_t incx, double *y,
               int64_t incy, bool conjx, hipStream_t stream, hipError_t *err) {
-    if (!gemv_applicable(kind, K, A, lda, incx)) return false;
+    if (!gemv_applicable(kind, K, A, lda, x, incx)) return false;
     *err = launch_gemv(kind, M, K, A, lda, x, incx, y, incy, conjx, stream);
     return true;
 }
@@ -536,12 +575,20 @@ int gemm_dev(bool cplx, int opA, int opB, int64_t M, int64_t N, int64_t K, const
         F4J_HIP(launch_gemm_strict(cplx, opA, opB, M, N, K, A, lda, B, ldb, C, ldc, stream), "strict gemm launch");
         return F4J_OK;
     }
-    if ((prec == F4J_PREC_SPLIT_BF16 || prec == F4J_PREC_SPLIT_FP16) && !cplx)
-        return run_split(opA, opB, M, N, K, A, lda, B, ldb, C, ldc, prec, slices, stream);
-    if ((prec == F4J_PREC_SPLIT_BF16 || prec == F4J_PREC_SPLIT_FP16) && cplx)
-        return run_split_cplx(opA, opB, M, N, K, A, lda, B, ldb, C, ldc, prec, slices, use3m, stream);
-    if (prec == F4J_PREC_SPLIT_INT8)
-        return run_crt(cplx, opA, opB, M, N, K, A, lda, B, ldb, C, ldc, slices, use3m, stream);
+    if (prec == F4J_PREC_SPLIT_BF16 || prec == F4J_PREC_SPLIT_FP16 || prec == F4J_PREC_SPLIT_INT8) {
+        // slices = 0: a failed check is run once more with the count its error asks for
+        int again = 0;
+        int *retry = auto_slices(slices) ? &again : nullptr;
+        auto run = [&](int sl, int *rt) {
+            if (prec == F4J_PREC_SPLIT_INT8)
+                return run_crt(cplx, opA, opB, M, N, K, A, lda, B, ldb, C, ldc, sl, use3m, stream, rt);
+            if (cplx) return run_split_cplx(opA, opB, M, N, K, A, lda, B, ldb, C, ldc, prec, sl, use3m, stream, rt);
+            return run_split(opA, opB, M, N, K, A, lda, B, ldb, C, ldc, prec, sl, stream, rt);
+        };
+        const int rc = run(slices, retry);
+        if (rc != F4J_OK || again == 0) return rc;
+        return run(again, nullptr);
+    }
     if (prec != F4J_PREC_FP64)
         return fail(F4J_ERR_UNSUPPORTED, "precision " + std::to_string(prec) + " is not available for " +
                                              (cplx ? "complex" : "real") + " products in this build");

@@ -159,16 +159,21 @@ __global__ __launch_bounds__(WGM *WGN *WAVE, MINW) void gemm_f64_kernel(GemmProb
 
 // Persistent grouped kernel: workgroup g computes the work items
 // items[begin[g] .. begin[g+1]), each one tile of one problem.  Problems whose
-// rows are 16-byte aligned (flags bit 0) use 16-byte loads.
+// rows are 16-byte aligned (flags bit 0) use 16-byte loads.  With guard.err
+// set, only the items whose problem the guard selects run (uniform per item).
 template <int BM, int BN, int BK, int WGM, int WGN, int MINW, int KA, int KB>
 __global__ __launch_bounds__(WGM *WGN *WAVE, MINW) void gemm_f64_grouped_kernel(const GemmProblem *probs,
                                                                              const GroupedItem *items,
-                                                                             const int *begin) {
+                                                                             const int *begin, GroupedGuard guard) {
     constexpr int LDS_STAGE = Opnd<KA, true, BM, BK>::LDS_DOUBLES + Opnd<KB, false, BN, BK>::LDS_DOUBLES;
     __shared__ double lds[2 * LDS_STAGE];
     const int b0 = begin[blockIdx.x], b1 = begin[blockIdx.x + 1];
     for (int it = b0; it < b1; ++it) {
         const GroupedItem w = items[it];
+        if (guard.err) {
+            const int s = guard.slot[w.prob];
+            if (s < 0 || !(guard.err[s] > guard.above)) continue;
+        }
         const GemmProblem p = probs[w.prob];
         if (p.flags & 1)
             gemm_tile<BM, BN, BK, WGM, WGN, 2, KA, KB>(p, (int64_t)w.tm * BM, (int64_t)w.tn * BN, lds);
@@ -203,12 +208,12 @@ hipError_t launch_kinds(const GemmProblem &p, bool vec2, hipStream_t stream, int
 
 template <int KA, int KB>
 hipError_t launch_grouped_kinds(const GemmProblem *probs, const GroupedItem *items, const int *begin, int wgs, int cfg,
-                                hipStream_t stream) {
+                                hipStream_t stream, GroupedGuard guard) {
     switch (cfg) {
 #define F4J_TILE_CASE(id, BM, BN, BK, WGM, WGN, MINW)                                                        \
     case id:                                                                                                \
         hipLaunchKernelGGL((gemm_f64_grouped_kernel<BM, BN, BK, WGM, WGN, MINW, KA, KB>), dim3((unsigned)wgs), \
-                           dim3(WGM * WGN * WAVE), 0, stream, probs, items, begin);                          \
+                           dim3(WGM * WGN * WAVE), 0, stream, probs, items, begin, guard);                   \
         return hipGetLastError();
         F4J_FP64_TILES(F4J_TILE_CASE)
 #undef F4J_TILE_CASE
@@ -248,10 +253,10 @@ int grouped_wgs_per_cu(OpKind ka, OpKind kb, int cfg) {
 }
 
 hipError_t launch_gemm_f64_grouped(OpKind ka, OpKind kb, const GemmProblem *probs, const GroupedItem *items,
-                                   const int *begin, int wgs, int cfg, hipStream_t stream) {
+                                   const int *begin, int wgs, int cfg, hipStream_t stream, GroupedGuard guard) {
     if (wgs <= 0) return hipSuccess;
 #define F4J_CASE(A_, B_) \
-    if (ka == A_ && kb == B_) return launch_grouped_kinds<A_, B_>(probs, items, begin, wgs, cfg, stream);
+    if (ka == A_ && kb == B_) return launch_grouped_kinds<A_, B_>(probs, items, begin, wgs, cfg, stream, guard);
     F4J_CASE(R_N, R_N) F4J_CASE(R_N, R_T) F4J_CASE(R_T, R_N) F4J_CASE(R_T, R_T)
     F4J_CASE(Z_N, Z_N) F4J_CASE(Z_N, Z_T) F4J_CASE(Z_N, Z_C)
     F4J_CASE(Z_T, Z_N) F4J_CASE(Z_T, Z_T) F4J_CASE(Z_T, Z_C)

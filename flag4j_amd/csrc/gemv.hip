@@ -495,17 +495,21 @@ bool gemv_supported(int kind, int64_t K, int64_t lda, int64_t incx) {
            (K + 4096) * incx * x_width(kind) * 8 <= lim;
 }
 
-bool gemv_applicable(int kind, int64_t K, const double *A, int64_t lda, int64_t incx) {
+bool gemv_applicable(int kind, int64_t K, const double *A, int64_t lda, const double *x, int64_t incx) {
     if (!gemv_supported(kind, K, lda, incx)) return false;
-    // complex A, and real A with a complex x, stream 16-byte pairs / elements only
-    const bool a16 = ((uintptr_t)A & 15) == 0 && (a_width(kind) == 2 || (lda % 2 == 0 && K % 2 == 0));
-    return a16 || kind == GEMV_BLOCKED || kind == GEMV_CHAIN;
+    // The blocked form picks its load path from the alignment itself.  The split
+    // kernel reads real entries as 8-byte doubles and complex ones as 16-byte
+    // pairs: complex A and complex x must be 16-byte aligned.
+    if (kind == GEMV_BLOCKED) return true;
+    const bool a_ok = a_width(kind) == 1 || ((uintptr_t)A & 15) == 0;
+    const bool x_ok = x_width(kind) == 1 || ((uintptr_t)x & 15) == 0;
+    return a_ok && x_ok;
 }
 
 hipError_t launch_gemv(int kind, int64_t M, int64_t K, const double *A, int64_t lda, const double *x, int64_t incx,
                        double *y, int64_t incy, bool conjx, hipStream_t stream) {
     if (M == 0) return hipSuccess;
-    if (!gemv_applicable(kind, K, A, lda, incx)) return hipErrorInvalidValue;
+    if (!gemv_applicable(kind, K, A, lda, x, incx)) return hipErrorInvalidValue;
     GemvArgs g{M, K, A, lda * a_width(kind), x, incx * x_width(kind), y, incy * y_width(kind), 0, conjx ? 1 : 0};
     // Direct-to-LDS loads move 16 bytes: 16-byte aligned pairs (complex: elements)
     // that never straddle a row end.  Otherwise the register-staged form.

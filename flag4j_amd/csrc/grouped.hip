@@ -138,6 +138,12 @@ struct f4j_grouped_plan {
     hipGraphExec_t exec = nullptr;
     double flops = 0;
     f4j::SplitBatch *split = nullptr;        // split-precision plan (precision SPLIT_*), else FP64 buckets
+    int64_t split_tiles = 0;                 // output tiles of the split plan's product launch
+    // Split plans with splitFallback = 1: the FP64 buckets recompute, in the same
+    // graph, the products whose checked error is above splitTolerance (or whose
+    // operands hold NaN / Inf); `guard` selects them on the device.
+    int *d_slot = nullptr;
+    f4j::GroupedGuard guard{};
     hipStream_t last = nullptr;              // stream of the last run (the split report waits on it)
 };
 
@@ -224,7 +230,7 @@ int schedule_bucket(f4j_grouped_plan &pl, int b, int cfg) {
 hipError_t enqueue_bucket(const f4j_grouped_plan &pl, int b, hipStream_t st) {
     const f4j_grouped_plan::Bucket &bk = pl.buckets[b];
     if (bk.items == 0) return hipSuccess;
-    return launch_gemm_f64_grouped(pl.ka, pl.kb, pl.d_probs, bk.d_items, bk.d_begin, bk.wgs, bk.cfg, st);
+    return launch_gemm_f64_grouped(pl.ka, pl.kb, pl.d_probs, bk.d_items, bk.d_begin, bk.wgs, bk.cfg, st, pl.guard);
 }
 
 int capture(f4j_grouped_plan &pl) {
@@ -238,9 +244,8 @@ int capture(f4j_grouped_plan &pl) {
     }
     G_HIP(hipStreamBeginCapture(pl.cap, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
     hipError_t e = hipSuccess;
-    if (pl.split)
-        e = split_batch_enqueue(pl.split, pl.cap);
-    else
+    if (pl.split) e = split_batch_enqueue(pl.split, pl.cap);
+    if (!pl.split || pl.guard.err)
         for (int b = 0; b < NUM_BUCKETS && e == hipSuccess; ++b) e = enqueue_bucket(pl, b, pl.cap);
     hipGraph_t g = nullptr;
     const hipError_t e2 = hipStreamEndCapture(pl.cap, &g);
@@ -260,6 +265,7 @@ void destroy_plan(f4j_grouped_plan *pl) {
     if (pl->graph) (void)hipGraphDestroy(pl->graph);
     for (auto &b : pl->buckets) free_bucket(b);
     if (pl->d_probs) (void)hipFree(pl->d_probs);
+    if (pl->d_slot) (void)hipFree(pl->d_slot);
     split_batch_destroy(pl->split);
     if (pl->cap) (void)hipStreamDestroy(pl->cap);
     delete pl;
@@ -297,6 +303,59 @@ int check_problems(bool cplx, int opA, int opB, int64_t count, const int64_t *M,
     return F4J_OK;
 }
 
+// The FP64 side of a plan: the real-expanded problem table on the device and
+// one scheduled bucket per tile shape (tile configuration from the tuner cache,
+// else the bucket's default).  pl.is_complex, opA, opB and count are set.
+int setup_fp64(f4j_grouped_plan &pl, const int64_t *M, const int64_t *N, const int64_t *K, const double *const *A,
+               const int64_t *lda, const double *const *B, const int64_t *ldb, double *const *C, const int64_t *ldc) {
+    const bool cplx = pl.is_complex != 0;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return fail(F4J_ERR_NO_DEVICE, "no HIP device");
+    pl.ka = kind_for(pl.opA, cplx);
+    pl.kb = kind_for(pl.opB, cplx);
+    hipDeviceProp_t prop{};
+    if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return fail(F4J_ERR_HIP, "hipGetDeviceProperties failed");
+    pl.cus = prop.multiProcessorCount;
+    const int w = cplx ? 2 : 1;
+    const int64_t count = pl.count;
+    pl.probs.resize(count);
+    pl.prob_bucket.resize(count);
+    for (int64_t i = 0; i < count; ++i) {
+        const bool empty = M[i] == 0 || N[i] == 0;
+        GemmProblem &p = pl.probs[i];
+        p.M = empty ? 0 : M[i];
+        p.N = empty ? 0 : N[i] * w;
+        p.K = K[i] * w;
+        p.A = A[i];
+        p.lda = lda[i] * w;
+        p.B = B[i];
+        p.ldb = ldb[i] * w;
+        p.C = C[i];
+        p.ldc = ldc[i] * w;
+        p.flags = (vec2_ok(p.A, p.lda, stored_cols(pl.ka, true, p.M, p.N, p.K)) &&
+                   vec2_ok(p.B, p.ldb, stored_cols(pl.kb, false, p.M, p.N, p.K)))
+                      ? 1
+                      : 0;
+        pl.prob_bucket[i] = bucket_of(p.M, p.N);
+    }
+    if (count > 0) {
+        G_HIP(hipMalloc(&pl.d_probs, count * sizeof(GemmProblem)), "hipMalloc (grouped problems)");
+        G_HIP(hipMemcpy(pl.d_probs, pl.probs.data(), count * sizeof(GemmProblem), hipMemcpyHostToDevice),
+              "copy problems");
+    }
+    const std::string arch = device_arch();
+    for (int b = 0; b < NUM_BUCKETS; ++b) {
+        int cfg = DEFAULT_CFG[b];
+        {
+            std::lock_guard<std::mutex> g(g_tune_mu);
+            auto it = g_tune.find(tune_key(arch, pl.ka, pl.kb, b));
+            if (it != g_tune.end()) cfg = it->second;
+        }
+        if (const int rc = schedule_bucket(pl, b, cfg); rc != F4J_OK) return rc;
+    }
+    return F4J_OK;
+}
+
 }  // namespace
 }  // namespace f4j
 
@@ -317,66 +376,22 @@ int f4j_grouped_plan_create(int is_complex, int opA, int opB, int64_t count, con
     if (const int rc = check_problems(cplx, opA, opB, count, M, N, K, A, lda, B, ldb, C, ldc); rc != F4J_OK)
         return rc;
     if (const int rc = runtime_ready(); rc != F4J_OK) return rc;
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess) return fail(F4J_ERR_NO_DEVICE, "no HIP device");
     f4j_grouped_plan *pl = new f4j_grouped_plan;
     pl->is_complex = is_complex;
     pl->opA = opA;
     pl->opB = opB;
     pl->precision = F4J_PREC_FP64;
-    pl->ka = kind_for(opA, cplx);
-    pl->kb = kind_for(opB, cplx);
     pl->count = count;
-    hipDeviceProp_t prop{};
-    if (hipGetDeviceProperties(&prop, dev) != hipSuccess) {
-        destroy_plan(pl);
-        return fail(F4J_ERR_HIP, "hipGetDeviceProperties failed");
-    }
-    pl->cus = prop.multiProcessorCount;
-    const int w = cplx ? 2 : 1;
-    pl->probs.resize(count);
-    pl->prob_bucket.resize(count);
-    for (int64_t i = 0; i < count; ++i) {
-        const bool empty = M[i] == 0 || N[i] == 0;
-        GemmProblem &p = pl->probs[i];
-        p.M = empty ? 0 : M[i];
-        p.N = empty ? 0 : N[i] * w;
-        p.K = K[i] * w;
-        p.A = A[i];
-        p.lda = lda[i] * w;
-        p.B = B[i];
-        p.ldb = ldb[i] * w;
-        p.C = C[i];
-        p.ldc = ldc[i] * w;
-        p.flags = (vec2_ok(p.A, p.lda, stored_cols(pl->ka, true, p.M, p.N, p.K)) &&
-                   vec2_ok(p.B, p.ldb, stored_cols(pl->kb, false, p.M, p.N, p.K)))
-                      ? 1
-                      : 0;
-        pl->prob_bucket[i] = bucket_of(p.M, p.N);
-        pl->flops += 2.0 * (double)p.M * (double)p.N * (double)p.K;
-    }
     hipError_t e = hipStreamCreateWithFlags(&pl->cap, hipStreamNonBlocking);
-    if (e == hipSuccess && count > 0) e = hipMalloc(&pl->d_probs, count * sizeof(GemmProblem));
-    if (e == hipSuccess && count > 0)
-        e = hipMemcpy(pl->d_probs, pl->probs.data(), count * sizeof(GemmProblem), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
         destroy_plan(pl);
         return hipfail(e, "grouped plan setup");
     }
-    const std::string arch = device_arch();
-    for (int b = 0; b < NUM_BUCKETS; ++b) {
-        int cfg = DEFAULT_CFG[b];
-        {
-            std::lock_guard<std::mutex> g(g_tune_mu);
-            auto it = g_tune.find(tune_key(arch, pl->ka, pl->kb, b));
-            if (it != g_tune.end()) cfg = it->second;
-        }
-        const int rc = schedule_bucket(*pl, b, cfg);
-        if (rc != F4J_OK) {
-            destroy_plan(pl);
-            return rc;
-        }
+    if (const int rc = setup_fp64(*pl, M, N, K, A, lda, B, ldb, C, ldc); rc != F4J_OK) {
+        destroy_plan(pl);
+        return rc;
     }
+    for (const GemmProblem &p : pl->probs) pl->flops += 2.0 * (double)p.M * (double)p.N * (double)p.K;
     const int rc = capture(*pl);
     if (rc != F4J_OK) {
         destroy_plan(pl);
@@ -409,7 +424,7 @@ int f4j_grouped_plan_info(const f4j_grouped_plan *plan, double *flops, int *tile
             if (tile_cfg) tile_cfg[b] = -1;
             if (items) items[b] = 0;
         }
-        if (items && nb > 0) items[0] = plan->buckets[0].items;
+        if (items && nb > 0) items[0] = plan->split_tiles;
         return F4J_OK;
     }
     for (int b = 0; b < std::min(nb, NUM_BUCKETS); ++b) {
@@ -473,11 +488,21 @@ int f4j_grouped_plan_create_split(int is_complex, int opA, int opB, int64_t coun
         return rc;
     if (const int rc = runtime_ready(); rc != F4J_OK) return rc;
     char buf[32];
-    int check_rows = 4;
+    int check_rows = 4, fallback = 1;
+    double tolerance = 1e-15;
     if (f4j_config_get("splitCheckRows", buf, sizeof(buf)) == F4J_OK) check_rows = std::atoi(buf);
+    if (f4j_config_get("splitFallback", buf, sizeof(buf)) == F4J_OK) fallback = std::atoi(buf);
+    if (f4j_config_get("splitTolerance", buf, sizeof(buf)) == F4J_OK) tolerance = std::strtod(buf, nullptr);
     const int carrier = int8 ? SPLIT_INT8 : precision == F4J_PREC_SPLIT_FP16 ? SPLIT_FP16 : SPLIT_BF16;
-    // digit forms: 0 = 6 (BF16) / 5 (FP16) digits; modular form: 0 = chosen from the largest K
-    const int S = (slices > 0 || int8) ? slices : (carrier == SPLIT_FP16 ? 5 : 6);
+    // digit forms, 0 = real 7 (BF16) / 6 (FP16) digits, complex 8 / 7: one digit
+    // beyond what uniform data needs; short products (K down to 16) of entries
+    // spread over 10^+-4 need it in every entry, not only in the checked rows
+    // (measured on 2,000 A^H B products, 3M: FP16 6 digits left 15 products above
+    // 1e-15, 4 of them with passing checked rows; 7 left none).  Modular form: 0 =
+    // chosen from the largest K (split_batch.h).
+    const int S = (slices > 0 || int8) ? slices
+                  : carrier == SPLIT_FP16 ? (cplx ? 7 : 6)
+                                          : (cplx ? 8 : 7);
     std::vector<BatchProblem> probs(count);
     f4j_grouped_plan *pl = new f4j_grouped_plan;
     pl->is_complex = is_complex;
@@ -496,14 +521,30 @@ int f4j_grouped_plan_create_split(int is_complex, int opA, int opB, int64_t coun
         destroy_plan(pl);
         return fail(status, err);
     }
-    int64_t tiles = 0;
     for (int64_t i = 0; i < count; ++i)
-        if (M[i] > 0 && N[i] > 0 && K[i] > 0) tiles += ((M[i] + 63) / 64) * ((N[i] + 63) / 64);
-    pl->buckets[0].items = tiles;
-    const hipError_t e = hipStreamCreateWithFlags(&pl->cap, hipStreamNonBlocking);
+        if (M[i] > 0 && N[i] > 0 && K[i] > 0) pl->split_tiles += ((M[i] + 63) / 64) * ((N[i] + 63) / 64);
+    hipError_t e = hipStreamCreateWithFlags(&pl->cap, hipStreamNonBlocking);
     if (e != hipSuccess) {
         destroy_plan(pl);
         return hipfail(e, "grouped plan setup");
+    }
+    if (fallback && check_rows > 0 && count > 0) {
+        // FP64 recomputation of the products the check rejects, in the same graph
+        if (const int rc = setup_fp64(*pl, M, N, K, A, lda, B, ldb, C, ldc); rc != F4J_OK) {
+            destroy_plan(pl);
+            return rc;
+        }
+        const std::vector<int> &slots = split_batch_slots(pl->split);
+        e = hipMalloc(&pl->d_slot, slots.size() * sizeof(int));
+        if (e == hipSuccess)
+            e = hipMemcpy(pl->d_slot, slots.data(), slots.size() * sizeof(int), hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            destroy_plan(pl);
+            return hipfail(e, "grouped plan setup");
+        }
+        pl->guard.err = split_batch_err_bits(pl->split);
+        pl->guard.slot = pl->d_slot;
+        pl->guard.above = __builtin_bit_cast(unsigned long long, tolerance);
     }
     const int rc = capture(*pl);
     if (rc != F4J_OK) {

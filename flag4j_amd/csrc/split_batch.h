@@ -25,7 +25,7 @@ struct SplitBatch;
 
 // Plan a batch on `carrier` (SPLIT_BF16 / SPLIT_FP16) with `slices` digits per
 // operand, or on SPLIT_INT8 (the modular form) with `slices` moduli (0: the
-// fewest that give t >= 53 bits at the batch's largest K).  Complex products
+// fewest that give t >= 64 bits at the batch's largest K).  Complex products
 // use the 3M form (three real products) when use3m, else four.  `check_rows`
 // rows of every product are verified by the residual check (0 = off).  On
 // failure returns nullptr with *status and *err set.
@@ -39,6 +39,11 @@ hipError_t split_batch_enqueue(const SplitBatch *b, hipStream_t stream, hipEvent
 // (-1 where nothing was checked; NaN where the problem's operands hold NaN or
 // Inf, which the digit slices cannot carry); synchronises `stream`.
 hipError_t split_batch_errors(const SplitBatch *b, double *out, int64_t count, hipStream_t stream);
+// Device array of the last run's per-product error bits (the bits of the
+// non-negative error, or a NaN pattern above every finite error for NaN / Inf
+// operands), and the slot of each problem in it (-1: no product computed).
+const unsigned long long *split_batch_err_bits(const SplitBatch *b);
+const std::vector<int> &split_batch_slots(const SplitBatch *b);
 int split_batch_real_products(const SplitBatch *b);     // real slice products per run (jobs x pairs, or jobs x moduli)
 int split_batch_checked_rows(const SplitBatch *b);
 size_t split_batch_bytes(const SplitBatch *b);          // device memory held

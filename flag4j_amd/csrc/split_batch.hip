@@ -592,7 +592,10 @@ SplitBatch *split_batch_create(bool cplx, int opA, int opB, int carrier, int sli
     if (int8) {
         // |residue| <= 128, so the int32 products are exact for K < 2^17
         if (kmax >= (int64_t(1) << 17)) return bad(F4J_ERR_UNSUPPORTED, "modular form: K must be below 131072");
-        if (slices == 0) slices = crt_default_moduli(kmax);
+        // Default: t >= 64 bits at the largest K.  Short products of entries spread
+        // over many binades need bits beyond 53: their (|A||B|)_ij is a sum of few
+        // terms, far below 2^(eA + eB) (checked on 10^+-4 spreads at K = 32..1024).
+        if (slices == 0) slices = crt_moduli_for_bits(64, kmax);
         if (slices < 2 || slices > CRT_MAX_MODULI)
             return bad(F4J_ERR_INVALID_ARG, "moduli must be in 2.." + std::to_string(CRT_MAX_MODULI));
         if (!crt_tables(slices, kmax, &tb))
@@ -896,6 +899,8 @@ hipError_t split_batch_errors(const SplitBatch *b, double *out, int64_t count, h
     return hipSuccess;
 }
 
+const unsigned long long *split_batch_err_bits(const SplitBatch *b) { return b->d_err; }
+const std::vector<int> &split_batch_slots(const SplitBatch *b) { return b->prob_index; }
 int split_batch_real_products(const SplitBatch *b) { return b->real_products; }
 int split_batch_checked_rows(const SplitBatch *b) { return b->check_rows; }
 size_t split_batch_bytes(const SplitBatch *b) { return b->bytes; }

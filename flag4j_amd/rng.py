@@ -132,6 +132,17 @@ def _polar(u):
     return g
 
 
+def scaled_uniform_complex(seed, n, phi):
+    """n complex entries: real part (2 r0 - 1) 10^(phi (2 r1 - 1)), imaginary part
+    (2 r2 - 1) 10^(phi (2 r3 - 1)), from four consecutive draws r0 .. r3 per entry."""
+    v = next_doubles(seed, 4 * n)
+    out = np.empty(n, dtype=np.complex128)
+    f = out.view(np.float64)
+    f[0::2] = (2.0 * v[0::4] - 1.0) * 10.0 ** (phi * (2.0 * v[1::4] - 1.0))
+    f[1::2] = (2.0 * v[2::4] - 1.0) * 10.0 ** (phi * (2.0 * v[3::4] - 1.0))
+    return out
+
+
 def next_gaussians(seed, n, chunk=1 << 22):
     """The first n values of new java.util.Random(seed).nextGaussian(): the
     polar method on consecutive nextDouble() pairs, rejected pairs skipped,

@@ -194,22 +194,27 @@ int f4j_grouped_plan_create(int is_complex, int opA, int opB, int64_t count,
                             f4j_grouped_plan **plan);
 /* Batched split-precision products (no reference counterpart: Flag4j is FP64-only; the batch
  * replaces a loop of Matrix.mult / A.H().mult(B) calls).  precision F4J_PREC_SPLIT_BF16 or
- * F4J_PREC_SPLIT_FP16 (slices = digits per operand, 0 = 6 (BF16) / 5 (FP16)), or
- * F4J_PREC_SPLIT_INT8 (slices = moduli, 2..20; 0 = the fewest giving 53-bit operands at the
+ * F4J_PREC_SPLIT_FP16 (slices = digits per operand, 0 = 7 (BF16) / 6 (FP16) for real products,
+ * 8 / 7 for complex ones), or
+ * F4J_PREC_SPLIT_INT8 (slices = moduli, 2..20; 0 = the fewest giving 64-bit operands at the
  * batch's largest K; inner sizes below 131072); complex products run as three real
  * products (3M, Gauss) when use3m != 0, else four.  splitCheckRows rows of every product are
- * verified after each run.  f4j_grouped_plan_create with a SPLIT_* precision is this call with
- * slices 0 and use3m 1.  Plans run, tune (a no-op: one tile shape) and destroy like FP64 plans. */
+ * verified after each run.  With splitFallback = 1 (at plan creation) the plan recomputes in
+ * FP64, inside the same graph and without a host round trip, every product whose checked error
+ * is above splitTolerance or whose operands hold NaN or Inf.  f4j_grouped_plan_create with a
+ * SPLIT_* precision is this call with slices 0 and use3m 1.  Plans run, tune (a no-op: one
+ * tile shape) and destroy like FP64 plans. */
 int f4j_grouped_plan_create_split(int is_complex, int opA, int opB, int64_t count,
                                   const int64_t *M, const int64_t *N, const int64_t *K,
                                   const double *const *A, const int64_t *lda,
                                   const double *const *B, const int64_t *ldb,
                                   double *const *C, const int64_t *ldc, int precision, int slices,
                                   int use3m, f4j_grouped_plan **plan);
-/* After a run of a split plan: err[p] = max |C - AB| / (|A||B|) over the checked rows of problem p
- * (-1 if none; NaN if problem p's operands hold NaN or Inf, which digit slices cannot carry: its C is
- * then not meaningful, recompute it in FP64), *products = real slice products per run.  Waits for
- * the last run. */
+/* After a run of a split plan: err[p] = max |C - AB| / (|A||B|) of the split result over the
+ * checked rows of problem p (-1 if none; NaN if problem p's operands hold NaN or Inf, which digits
+ * and residues cannot carry), *products = real slice products per run.  Where err[p] is NaN or
+ * above splitTolerance and the plan was created with splitFallback = 1, C_p holds the FP64
+ * recomputation; otherwise C_p is the split result.  Waits for the last run. */
 int f4j_grouped_plan_split_report(f4j_grouped_plan *plan, double *err, int64_t count, int *products);
 int f4j_grouped_plan_run(f4j_grouped_plan *plan, void *stream);
 int f4j_grouped_plan_destroy(f4j_grouped_plan *plan);

@@ -270,6 +270,49 @@ void f4o_cplx_gemm(int opA, int opB, int M, int N, int K, const double *A, const
         }
 }
 
+/* Sampled entries of Flag4j's products, for checks of large products where the
+ * whole oracle product would take minutes: entry s is (op(A) op(B))_{I[s],J[s]}
+ * as every kernel ReDeMatMultDispatcher / RealDenseMatrixMultiplyDispatcher can
+ * select computes it (RealDenseMatMult.java:50-79 standard, and the reordered /
+ * blocked forms, which give the same bits; RealDenseMatMultTranspose.java:53-82):
+ * sum = +0.0; sum += a_ik * b_kj for k ascending, each product and sum rounded.
+ * opA / opB: 0 = N, 1 = T (A stored K x M, B stored N x K). */
+void f4o_real_entries(int opA, int opB, int M, int N, int K, const double *A, const double *B, int ns,
+                      const int64_t *I, const int64_t *J, double *out) {
+    for (int s = 0; s < ns; s++) {
+        const int64_t i = I[s], j = J[s];
+        double sum = 0.0;
+        for (int k = 0; k < K; k++) {
+            const double a = opA ? A[(int64_t)k * M + i] : A[i * K + k];
+            const double b = opB ? B[j * K + k] : B[(int64_t)k * N + j];
+            sum += a * b;
+        }
+        out[s] = sum;
+    }
+}
+
+/* The complex counterpart (DenseSemiringMatMult.java:54-84 and the variants that
+ * reload sum from dest, Complex128.java:276-278, :336-341, :513-516): per k
+ * p = a.mult(b) then sum = sum.add(p) from (+0, +0).  out holds (re, im) pairs. */
+void f4o_cplx_entries(int opA, int opB, int M, int N, int K, const double *A, const double *B, int ns,
+                      const int64_t *I, const int64_t *J, double *out) {
+    for (int s = 0; s < ns; s++) {
+        const int64_t i = I[s], j = J[s];
+        double sr = 0.0, si = 0.0;
+        for (int k = 0; k < K; k++) {
+            double ar, ai, br, bi;
+            cload(A, opA, M, K, (int)i, k, &ar, &ai);
+            cload(B, opB, K, N, k, (int)j, &br, &bi);
+            const double pr = ar * br - ai * bi;
+            const double pi = ar * bi + ai * br;
+            sr = sr + pr;
+            si = si + pi;
+        }
+        out[2 * s] = sr;
+        out[2 * s + 1] = si;
+    }
+}
+
 /* Mixed real / complex products: RealFieldDenseMatMult
  * (M:linalg/ops/dense/real_field_ops/RealFieldDenseMatMult.java:58-1090).
  * Every variant (standard, reordered, blocked, concurrent, vector) sums each

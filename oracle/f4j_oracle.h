@@ -92,6 +92,13 @@ enum { F4O_T_MULT_T = 0, F4O_T_BLOCKED, F4O_T_CONCURRENT, F4O_T_BLOCKED_CONCURRE
 int f4o_select_transpose(int rows);
 
 /* ---- accuracy oracle: compensated dot products (Dot2) ---- */
+/* Sampled entries of the reference's products in its own arithmetic (k-ascending,
+ * separately rounded): real opA/opB 0 = N, 1 = T; complex opA/opB as f4o_cplx_gemm,
+ * out as (re, im) pairs. */
+void f4o_real_entries(int opA, int opB, int M, int N, int K, const double *A, const double *B, int ns,
+                      const int64_t *I, const int64_t *J, double *out);
+void f4o_cplx_entries(int opA, int opB, int M, int N, int K, const double *A, const double *B, int ns,
+                      const int64_t *I, const int64_t *J, double *out);
 /* For each sample s, hi[s]+lo[s] approximates (op(A) op(B))_{I[s],J[s]} with
  * error ~ u^2 * K * (|A||B|)_ij; absab[s] = (|A||B|)_ij.  Real, row-major,
  * opB: 0 = N (B is KxN), 1 = T (B is NxK). */

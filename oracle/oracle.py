@@ -62,6 +62,8 @@ def lib():
         L.f4o_select_real.argtypes = [_i, _i, _i, ctypes.POINTER(Thresholds)]
         L.f4o_select_cplx.argtypes = [_i, _i, _i, ctypes.POINTER(Thresholds)]
         L.f4o_select_transpose.argtypes = [_i]
+        L.f4o_real_entries.argtypes = [_i, _i, _i, _i, _i, _dp, _dp, _i, _i64p, _i64p, _dp]
+        L.f4o_cplx_entries.argtypes = [_i, _i, _i, _i, _i, _dp, _dp, _i, _i64p, _i64p, _dp]
         L.f4o_real_dot2.argtypes = [_i, _i, _i, _dp, _dp, _i, _i, _i64p, _i64p, _dp, _dp, _dp]
         L.f4o_cplx_dot2.argtypes = [_i, _i, _i, _i, _i, _dp, _dp, _i, _i64p, _i64p,
                                     _dp, _dp, _dp, _dp, _dp]
@@ -230,6 +232,33 @@ def select_cplx(m, n, k, **overrides):
 
 def select_transpose(rows):
     return TRANSPOSE_NAMES[lib().f4o_select_transpose(rows)]
+
+
+# ---- sampled entries in the reference's arithmetic ----
+def real_entries(A, B, I, J, opA="N", opB="N"):
+    """(op(A) op(B))_{I,J} as Flag4j's real kernels compute them (bitwise)."""
+    A, B = _f64(A), _f64(B)
+    M, K = A.shape if opA == "N" else A.shape[::-1]
+    N = B.shape[1] if opB == "N" else B.shape[0]
+    I = np.ascontiguousarray(I, dtype=np.int64)
+    J = np.ascontiguousarray(J, dtype=np.int64)
+    out = np.empty(len(I))
+    lib().f4o_real_entries(OPS[opA], OPS[opB], M, N, K, _p(A), _p(B), len(I), I.ctypes.data_as(_i64p),
+                           J.ctypes.data_as(_i64p), _p(out))
+    return out
+
+
+def cplx_entries(A, B, I, J, opA="N", opB="N"):
+    """(op(A) op(B))_{I,J} as Flag4j's complex kernels compute them (bitwise)."""
+    A, B = _c128(A), _c128(B)
+    M, K = _op_shape(A, opA)
+    _, N = _op_shape(B, opB)
+    I = np.ascontiguousarray(I, dtype=np.int64)
+    J = np.ascontiguousarray(J, dtype=np.int64)
+    out = np.empty(len(I), dtype=np.complex128)
+    lib().f4o_cplx_entries(OPS[opA], OPS[opB], M, N, K, _p(A.view(np.float64)), _p(B.view(np.float64)), len(I),
+                           I.ctypes.data_as(_i64p), J.ctypes.data_as(_i64p), _p(out.view(np.float64)))
+    return out
 
 
 # ---- accuracy oracle ----

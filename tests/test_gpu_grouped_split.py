@@ -181,7 +181,7 @@ def test_plan_replay_is_stable(prec, S):
 
 def test_create_with_split_precision_defaults():
     """f4j_grouped_plan_create with a SPLIT_* precision builds a split plan
-    (default slices: 6 for BF16, 21 pairs per real product)."""
+    (default slices: 7 for BF16, 28 pairs per real product)."""
     from flag4j_amd import device
     As, Bs = problems(700, 10, 16, 100, "N", "N", False)
     dA = [device.DeviceBuffer.from_array(np.ascontiguousarray(A)) for A in As]
@@ -200,7 +200,7 @@ def test_create_with_split_precision_defaults():
         errs = np.zeros(len(As))
         prods = ctypes.c_int()
         L.check(lib.f4j_grouped_plan_split_report(h, ctypes.c_void_p(errs.ctypes.data), len(As), ctypes.byref(prods)))
-        assert prods.value == 10 * 21
+        assert prods.value == 10 * 28
         assert np.all((errs >= 0) & (errs <= 1e-15))
         for A, B, c in zip(As, Bs, dC):
             C = c.download(np.float64, A.shape[0] * B.shape[1]).reshape(A.shape[0], B.shape[1])
@@ -236,7 +236,7 @@ def moduli_of(prods, jobs):
 def test_int8_real_batch_accuracy(opA, opB):
     As, Bs = problems(800, 60, 16, 300, opA, opB, False)
     Cs, (errs, prods) = grouped.mult_all(As, Bs, opA, opB, precision="split_int8", report=True)
-    assert 14 <= moduli_of(prods, 60) <= 16           # the fewest giving t >= 53 at K <= 300
+    assert moduli_of(prods, 60) == 18                 # the fewest giving t >= 64 at K <= 300
     for i, (A, B, C) in enumerate(zip(As, Bs, Cs)):
         assert max_err(C, A, B, opA, opB, False) <= 1e-15, i
         assert 0.0 <= errs[i] <= 1e-15, (i, errs[i])
@@ -247,7 +247,7 @@ def test_int8_real_batch_accuracy(opA, opB):
 def test_int8_complex_batch_accuracy(opA, opB, use3m):
     As, Bs = problems(900, 40, 16, 260, opA, opB, True)
     Cs, (errs, prods) = grouped.mult_all(As, Bs, opA, opB, precision="split_int8", use3m=use3m, report=True)
-    assert 14 <= moduli_of(prods, 40 * (3 if use3m else 4)) <= 16
+    assert moduli_of(prods, 40 * (3 if use3m else 4)) == 18
     for i, (A, B, C) in enumerate(zip(As, Bs, Cs)):
         assert max_err(C, A, B, opA, opB, True) <= 1e-15, i
         assert 0.0 <= errs[i] <= 1e-15, (i, errs[i])
@@ -298,3 +298,74 @@ def test_int8_empty_zero_k_and_nonfinite():
         assert max_err(Cs[t], As[t], Bs[t], "N", "N", False) <= 1e-15
         assert 0 <= errs[t] <= 1e-15
     assert errs[0] == -1.0 and errs[1] == -1.0 and np.isnan(errs[4])
+
+
+# ---- FP64 recomputation inside the plan (splitFallback = 1) ----
+
+def test_fallback_recomputes_rejected_products_in_fp64():
+    """With splitFallback = 1 a split plan recomputes in FP64, inside its own graph,
+    the products whose checked error is above splitTolerance: those equal the FP64
+    grouped plan's result bit for bit, the others keep the split result bit for bit,
+    on every replay of the graph."""
+    from flag4j_amd import device
+    As, Bs = problems(1100, 30, 16, 200, "N", "N", False)
+    ref = grouped.mult_all(As, Bs, precision="fp64")
+    L.config_set("splitFallback", 0)
+    try:
+        split, (errs, _) = grouped.mult_all(As, Bs, precision="split_bf16", slices=3, report=True)
+    finally:
+        L.config_set("splitFallback", 1)
+    tol = float(np.median(errs))
+    assert np.sum(errs > tol) >= 5 and np.sum(errs <= tol) >= 5
+    dA = [device.DeviceBuffer.from_array(np.ascontiguousarray(A)) for A in As]
+    dB = [device.DeviceBuffer.from_array(np.ascontiguousarray(B)) for B in Bs]
+    dC = [device.DeviceBuffer(A.shape[0] * B.shape[1] * 8) for A, B in zip(As, Bs)]
+    L.config_set("splitTolerance", repr(tol))
+    try:
+        plan = grouped.GroupedPlan([(A.shape[0], B.shape[1], A.shape[1]) for A, B in zip(As, Bs)],
+                                   [t.ptr for t in dA], [t.ptr for t in dB], [t.ptr for t in dC],
+                                   [A.shape[1] for A in As], [B.shape[1] for B in Bs], [B.shape[1] for B in Bs],
+                                   precision="split_bf16", slices=3)
+    finally:
+        L.config_set("splitTolerance", 1e-15)
+    try:
+        for rep in range(3):
+            for c in dC:                       # stale C must not survive: refill with NaN
+                c.upload(np.full(c.nbytes // 8, np.nan))
+            plan.run()
+            device.sync()
+            e2, _ = plan.split_report()
+            assert np.array_equal(e2, errs), rep
+            for t, (A, B) in enumerate(zip(As, Bs)):
+                C = dC[t].download(np.float64, A.shape[0] * B.shape[1]).reshape(A.shape[0], B.shape[1])
+                exp = ref[t] if errs[t] > tol else split[t]
+                assert np.array_equal(C, exp), (rep, t, errs[t], tol)
+    finally:
+        plan.close()
+        for b in dA + dB + dC:
+            b.close()
+
+
+def test_default_slices_on_wide_range():
+    """Default counts on A^H B with parts spread over 10^+-4 (3M): the modular
+    form's default (t >= 64) meets 1e-15 by itself; the digit forms' defaults
+    meet it after the plan's FP64 recomputation of the products their check
+    rejects, and those products equal the FP64 plan's bits."""
+    As, Bs = problems(1200, 40, 32, 300, "C", "N", True, phi=4)
+    L.config_set("splitFallback", 0)
+    try:
+        Cs, (errs, prods) = grouped.mult_all(As, Bs, "C", "N", precision="split_int8", report=True)
+    finally:
+        L.config_set("splitFallback", 1)
+    assert prods == 40 * 3 * 18
+    assert np.all((errs >= 0) & (errs <= 1e-15))
+    assert max(max_err(C, A, B, "C", "N", True) for A, B, C in zip(As, Bs, Cs)) <= 1e-15
+    ref = grouped.mult_all(As, Bs, "C", "N", precision="fp64")
+    for prec in ("split_bf16", "split_fp16"):
+        Cs, (errs, _) = grouped.mult_all(As, Bs, "C", "N", precision=prec, report=True)
+        assert np.all(errs >= 0), prec
+        for t, (A, B, C) in enumerate(zip(As, Bs, Cs)):
+            if errs[t] > 1e-15:
+                assert np.array_equal(C.view(np.float64), ref[t].view(np.float64)), (prec, t)
+            else:
+                assert max_err(C, A, B, "C", "N", True) <= 1e-15, (prec, t)

@@ -162,6 +162,33 @@ def test_device_leading_dimensions():
                 assert np.all(C[:, N:] == 3.0 + 3.0j)
 
 
+def test_matrix_vector_alignment():
+    """N = 1 on device operands at 8-byte (not 16-byte) aligned addresses: a real
+    matrix with odd K and odd row length runs the chain kernel (it reads real
+    entries as single doubles), and a complex vector or matrix that is not
+    16-byte aligned takes a path that reads it safely.  Every result equals the
+    reference's arithmetic bit for bit."""
+    rng = np.random.default_rng(33)
+    L = _native.load()
+    M, K = 257, 333
+    for real_left in (True, False):
+        A = wide(rng, M, K) if real_left else wide_c(rng, M, K)
+        x = wide_c(rng, K, 1) if real_left else wide(rng, K, 1)
+        ref = o.mixed_gemm(A, x)
+        f = L.f4j_dzgemm_dev if real_left else L.f4j_zdgemm_dev
+        for a_off, x_off in ((0, 0), (8, 0), (0, 8), (8, 8)):
+            pa = np.concatenate([np.zeros(a_off // 8), A.reshape(-1).view(np.float64)])
+            px = np.concatenate([np.zeros(x_off // 8), x.reshape(-1).view(np.float64)])
+            with device.DeviceBuffer.from_array(pa) as dA, device.DeviceBuffer.from_array(px) as dx, \
+                    device.DeviceBuffer(M * 16) as dy:
+                dy.upload(np.full(M, np.nan + 0j))
+                _native.check(f(M, 1, K, dA.ptr + a_off, K, dx.ptr + x_off, 1, dy.ptr, 1, _native.PREC["fp64"], 0,
+                                None))
+                device.sync()
+                y = dy.download(np.complex128, M).reshape(M, 1)
+                assert same_bits(y, ref), (real_left, a_off, x_off)
+
+
 def test_empty_and_zero_inner():
     C = mixed(np.zeros((4, 0)), np.zeros((0, 3), dtype=complex))
     assert np.all(C == 0) and C.shape == (4, 3)

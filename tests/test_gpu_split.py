@@ -262,6 +262,20 @@ def test_crt_rejects_unrepresentable_k():
     assert st == L.ERR_INVALID_ARG
 
 
+def test_crt_moduli_out_of_range_are_rejected():
+    """A modulus count outside 2..20 is an invalid argument (as in the batched plans),
+    not silently clamped."""
+    A, B = rng.uniform(40, 64, 64), rng.uniform(41, 64, 64)
+    C = np.empty((64, 64))
+    for bad in (1, 21, 25):
+        st = L.load().f4j_dgemm(0, 0, 64, 64, 64, A.ctypes.data, 64, B.ctypes.data, 64, C.ctypes.data, 64,
+                                L.PREC["split_int8"], bad)
+        assert st == L.ERR_INVALID_ARG, bad
+        assert b"moduli" in L.load().f4j_last_error()
+    C, rep = split_mult(A, B, "split_int8", 20)
+    assert rep.slices == 20 and rep.precision == L.PREC["split_int8"]
+
+
 @pytest.mark.parametrize("prec,S", [("split_bf16", 3), ("split_int8", 0)])
 def test_product_timing_report(prec, S):
     """splitTiming = 1 reports the product launches' event time; 0 reports -1."""

@@ -430,3 +430,42 @@ def test_java_random_known_answers():
     """Known outputs of java.util.Random (Java SE specification of the LCG)."""
     assert o.JavaRandom(42).next_int() == -1170105035
     assert o.JavaRandom(0).next_double() == 0.730967787376657
+
+
+@pytest.mark.parametrize("opA,opB", [("N", "N"), ("N", "T"), ("T", "N"), ("T", "T")])
+def test_real_entries_equal_the_pinned_kernels(golden, opA, opB):
+    """Sampled entries in the reference's arithmetic equal the full products of the
+    golden-pinned kernels bit for bit (all op pairs), and the golden vectors themselves."""
+    rng = np.random.default_rng(11)
+    A = rng.standard_normal((37, 53)) * 10.0 ** rng.integers(-6, 7, (37, 53))
+    B = rng.standard_normal((53, 29)) * 10.0 ** rng.integers(-6, 7, (53, 29))
+    full = o.real_reordered(A, B)
+    Ast = A if opA == "N" else A.T.copy()
+    Bst = B if opB == "N" else B.T.copy()
+    I, J = np.repeat(np.arange(37), 29), np.tile(np.arange(29), 37)
+    got = o.real_entries(Ast, Bst, I, J, opA, opB)
+    assert np.array_equal(bits(got), bits(full[I, J]))
+    g = golden["real_rect"]
+    Ag, Bg, exp = real_mat(g["a"]), real_mat(g["b"]), real_mat(g["c"])
+    M, N = Ag.shape[0], Bg.shape[1]
+    I, J = np.repeat(np.arange(M), N), np.tile(np.arange(N), M)
+    assert np.array_equal(bits(o.real_entries(Ag, Bg, I, J)), bits(exp))
+
+
+@pytest.mark.parametrize("opA,opB", [("N", "N"), ("C", "N"), ("N", "T"), ("T", "C")])
+def test_cplx_entries_equal_the_pinned_kernel(golden, opA, opB):
+    rng = np.random.default_rng(12)
+    A = rng.standard_normal((21, 17)) + 1j * rng.standard_normal((21, 17))
+    B = rng.standard_normal((17, 19)) + 1j * rng.standard_normal((17, 19))
+    Ast = A if opA == "N" else A.T.copy()
+    Bst = B if opB == "N" else B.T.copy()
+    full = o.cplx_gemm(Ast, Bst, opA, opB)
+    M, N = full.shape
+    I, J = np.repeat(np.arange(M), N), np.tile(np.arange(N), M)
+    got = o.cplx_entries(Ast, Bst, I, J, opA, opB)
+    assert np.array_equal(got.view(np.float64), full[I, J].view(np.float64))
+    g = golden["cplx_square"]
+    Ag, Bg, exp = cplx_mat(g["a"]), cplx_mat(g["b"]), cplx_mat(g["c"])
+    M, N = Ag.shape[0], Bg.shape[1]
+    I, J = np.repeat(np.arange(M), N), np.tile(np.arange(N), M)
+    assert np.array_equal(o.cplx_entries(Ag, Bg, I, J).view(np.float64), exp.reshape(-1).view(np.float64))
