@@ -56,6 +56,7 @@ SIGNATURES = {
     "f4j_stream_sync": ([_vp], _i),
     "f4j_dgemm": ([_i, _i, _i64, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _i, _i], _i),
     "f4j_dgemm_dev": ([_i, _i, _i64, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _i, _i, _vp], _i),
+    "f4j_dgemm_dev_noalloc": ([_i, _i, _i64, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _i, _i, _vp], _i),
     "f4j_zgemm": ([_i, _i, _i64, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _i, _i, _i], _i),
     "f4j_zgemm_dev": ([_i, _i, _i64, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _i, _i, _i, _vp], _i),
     "f4j_dzgemm": ([_i64, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _i, _i], _i),

@@ -50,6 +50,16 @@ struct Config {
     int split_timing = 0;             // 1: time the slice-product launches with HIP events (report product_ms)
 };
 
+// Scratch memory of the allocation-free entry (f4j_dgemm_dev_noalloc): the
+// modular form's residues, exponents and check partials live here instead of in
+// the grown workspace.  Each GPU the library runs on holds its own copy until the
+// process ends.  In host code `scratch` names no device address: the address of
+// a device's copy comes from hipGetSymbolAddress at initialisation
+// (State::scratch_ptr), and only that address is handed to kernels.
+constexpr size_t NOALLOC_SCRATCH_BYTES = size_t(64) << 20;
+constexpr int NOALLOC_MAX_DEVICES = 64;
+__device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
 struct State {
     std::mutex mu;                    // guards init state and the staging buffers
     std::mutex cfg_mu;                // guards cfg
@@ -67,6 +77,7 @@ struct State {
     hipStream_t side = nullptr;             // B's split and the residual sums beside A's split and the
                                             // modular products (guarded by ws_mu)
     hipEvent_t fork_ev = nullptr, join_ev = nullptr, split_fork_ev = nullptr, split_join_ev = nullptr;
+    void *scratch_ptr[NOALLOC_MAX_DEVICES] = {};   // per device: its copy of `scratch` (set at init)
 };
 
 State &S() {
@@ -102,6 +113,11 @@ int init_locked(State &s, int device) {
         return fail(F4J_ERR_INVALID_ARG, "device " + std::to_string(device) + " out of range [0," + std::to_string(n) + ")");
     F4J_HIP(hipSetDevice(device), "hipSetDevice");
     F4J_HIP(hipGetDeviceProperties(&s.prop, device), "hipGetDeviceProperties");
+    if (device < NOALLOC_MAX_DEVICES && !s.scratch_ptr[device]) {
+        void *p = nullptr;        // this device's copy of the scratch array (the device is current)
+        F4J_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress (allocation-free scratch)");
+        s.scratch_ptr[device] = p;
+    }
     F4J_HIP(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking), "hipStreamCreate");
     for (hipEvent_t &e : s.tev) F4J_HIP(hipEventCreate(&e), "hipEventCreate");
     F4J_HIP(hipStreamCreateWithFlags(&s.side, hipStreamNonBlocking), "hipStreamCreate");
@@ -403,9 +419,11 @@ int run_split_cplx(int opA, int opB, int64_t M, int64_t N, int64_t K, const doub
 
 // Modular (CRT) split-precision real GEMM on the INT8 matrix cores;
 // `slices` = number of moduli (0: enough for FP64 accuracy at this K).
+// With `fixed` set the workspace is that buffer (no allocation): products whose
+// workspace does not fit are rejected, and a retry is only asked for if it fits.
 int run_crt(bool cplx, int opA, int opB, int64_t M, int64_t N, int64_t K, const double *A, int64_t lda,
             const double *B, int64_t ldb, double *C, int64_t ldc, int slices, int use3m, hipStream_t stream,
-            int *retry) {
+            int *retry, const DevBuf *fixed) {
     Config c;
     {
         std::lock_guard<std::mutex> g(S().cfg_mu);
@@ -431,13 +449,20 @@ int run_crt(bool cplx, int opA, int opB, int64_t M, int64_t N, int64_t K, const 
     if (!cplx) jobs = {{1, 0, 1, 0, 1, 0}};
     else if (use3m) jobs = {{1, 0, 1, 0, 1, -1}, {0, sa, 0, sb, -1, -1}, {1, sa, 1, sb, 0, 1}};
     else jobs = {{1, 0, 1, 0, 1, 0}, {0, sa, 0, sb, -1, 0}, {1, 0, 0, sb, 0, 1}, {0, sa, 1, 0, 0, 1}};
+    if (fixed && pl.bytes > fixed->cap)
+        return fail(F4J_ERR_INVALID_ARG, "split_int8 without allocation: this product needs " +
+                                             std::to_string(pl.bytes) + " bytes of scratch, the limit is " +
+                                             std::to_string(fixed->cap) + " (about 1024^3 with 16 moduli)");
     State &s = S();
     std::lock_guard<std::mutex> g(s.ws_mu);
-    int rc = grow(s.ws, pl.bytes);
-    if (rc != F4J_OK) return rc;
+    if (!fixed) {
+        const int rc = grow(s.ws, pl.bytes);
+        if (rc != F4J_OK) return rc;
+    }
+    void *wsp = fixed ? fixed->p : s.ws.p;
     const bool timing = c.split_timing != 0;
     const int check_rows = (int)std::min<int64_t>(c.split_check_rows, M);
-    char *w = (char *)s.ws.p;
+    char *w = (char *)wsp;
     double *part = (double *)(w + pl.off_part);
     // Whatever path leaves this function, the side stream is idle first: its
     // work writes the shared workspace, which the next call may grow.
@@ -459,7 +484,7 @@ int run_crt(bool cplx, int opA, int opB, int64_t M, int64_t N, int64_t K, const 
         out.first = jb == 0;
         // B's exponents and residues on the side stream, beside A's on this one
         const CrtSide side{s.side, s.split_fork_ev, s.split_join_ev};
-        F4J_HIP(launch_crt_gemm(pl, oa, ob, M, N, K, C, ldc, out, s.ws.p, stream, timing ? s.tev[2 * jb] : nullptr,
+        F4J_HIP(launch_crt_gemm(pl, oa, ob, M, N, K, C, ldc, out, wsp, stream, timing ? s.tev[2 * jb] : nullptr,
                                 timing ? s.tev[2 * jb + 1] : nullptr, &side),
                 "crt gemm launch");
         if (jb == 0 && check_rows > 0) {
@@ -502,9 +527,12 @@ int run_crt(bool cplx, int opA, int opB, int64_t M, int64_t N, int64_t K, const 
             if (retry && n < CRT_MAX_MODULI && std::isfinite(err) && c.split_tolerance > 0) {
                 // t grows by about 4 bits per modulus: ask for the bits the error lacks
                 const int bits = (int)std::ceil(std::log2(err / c.split_tolerance)) + 1;
-                const int m = std::max(n + 1, crt_moduli_for_bits(crt_bits(n, K) + bits, K));
-                *retry = std::min(CRT_MAX_MODULI, m);
-                return F4J_OK;
+                const int m = std::min(CRT_MAX_MODULI, std::max(n + 1, crt_moduli_for_bits(crt_bits(n, K) + bits, K)));
+                CrtPlan p2;
+                if (!fixed || (plan_crt(M, N, K, m, c.split_check_rows, &p2) && p2.bytes <= fixed->cap)) {
+                    *retry = m;
+                    return F4J_OK;
+                }
             }
             if (!c.split_fallback) {
                 t_split = rep;
@@ -551,9 +579,10 @@ bool try_gemv(int kind, int64_t M, int64_t K, const double *A, int64_t lda, cons
 }
 
 // Device-level GEMM, all pointers on the device, async on `stream`.
+// `fixed` (optional): the modular form's workspace, instead of the grown one.
 int gemm_dev(bool cplx, int opA, int opB, int64_t M, int64_t N, int64_t K, const double *A, int64_t lda,
              const double *B, int64_t ldb, double *C, int64_t ldc, int precision, int slices, int use3m,
-             hipStream_t stream) {
+             hipStream_t stream, const DevBuf *fixed = nullptr) {
     if (M == 0 || N == 0) return F4J_OK;
     const int w = cplx ? 2 : 1;
     if (K == 0) {
@@ -581,7 +610,7 @@ int gemm_dev(bool cplx, int opA, int opB, int64_t M, int64_t N, int64_t K, const
         int *retry = auto_slices(slices) ? &again : nullptr;
         auto run = [&](int sl, int *rt) {
             if (prec == F4J_PREC_SPLIT_INT8)
-                return run_crt(cplx, opA, opB, M, N, K, A, lda, B, ldb, C, ldc, sl, use3m, stream, rt);
+                return run_crt(cplx, opA, opB, M, N, K, A, lda, B, ldb, C, ldc, sl, use3m, stream, rt, fixed);
             if (cplx) return run_split_cplx(opA, opB, M, N, K, A, lda, B, ldb, C, ldc, prec, sl, use3m, stream, rt);
             return run_split(opA, opB, M, N, K, A, lda, B, ldb, C, ldc, prec, sl, stream, rt);
         };
@@ -978,6 +1007,25 @@ int f4j_dgemm_dev(int opA, int opB, int64_t M, int64_t N, int64_t K, const doubl
     if (rc != F4J_OK) return rc;
     if ((rc = ensure_ready()) != F4J_OK) return rc;
     return gemm_dev(false, opA, opB, M, N, K, A, lda, B, ldb, C, ldc, precision, slices, 0, pick_stream(stream));
+}
+
+int f4j_dgemm_dev_noalloc(int opA, int opB, int64_t M, int64_t N, int64_t K, const double *A, int64_t lda,
+                          const double *B, int64_t ldb, double *C, int64_t ldc, int precision, int slices,
+                          void *stream) {
+    int rc = validate(false, opA, opB, M, N, K, A, lda, B, ldb, C, ldc, precision);
+    if (rc != F4J_OK) return rc;
+    if ((rc = ensure_ready()) != F4J_OK) return rc;
+    State &s = S();
+    const int prec = resolve_precision(precision, M, N, K, opA, opB);
+    if (prec == F4J_PREC_SPLIT_BF16 || prec == F4J_PREC_SPLIT_FP16)
+        return fail(F4J_ERR_UNSUPPORTED, "f4j_dgemm_dev_noalloc: the digit forms need a workspace; use FP64, "
+                                         "STRICT or SPLIT_INT8");
+    DevBuf fixed;
+    fixed.p = s.device < NOALLOC_MAX_DEVICES ? s.scratch_ptr[s.device] : nullptr;
+    fixed.cap = NOALLOC_SCRATCH_BYTES;
+    if (!fixed.p) return fail(F4J_ERR_UNSUPPORTED, "f4j_dgemm_dev_noalloc: no scratch on device " +
+                                                       std::to_string(s.device));
+    return gemm_dev(false, opA, opB, M, N, K, A, lda, B, ldb, C, ldc, prec, slices, 0, pick_stream(stream), &fixed);
 }
 
 int f4j_zgemm(int opA, int opB, int64_t M, int64_t N, int64_t K, const double *A, int64_t lda, const double *B,

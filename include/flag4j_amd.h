@@ -105,6 +105,21 @@ int f4j_dgemm_dev(int opA, int opB, int64_t M, int64_t N, int64_t K,
                   const double *A, int64_t lda, const double *B, int64_t ldb,
                   double *C, int64_t ldc, int precision, int slices, void *stream);
 
+/* ---- real GEMM without allocation ----
+ * No counterpart in the reference: this entry is for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * f4j_dgemm_dev with the same arguments and results, except that it never calls hipMalloc,
+ * hipFree or an allocating routine: the modular form (SPLIT_INT8) keeps its residues,
+ * exponents and check partials in a fixed 64 MiB device array of the library (one per GPU,
+ * held until the process ends; about 1024^3 with 16 moduli fits), FP64 and STRICT need no
+ * scratch, and the digit forms (SPLIT_BF16/FP16) return F4J_ERR_UNSUPPORTED.  A product whose
+ * scratch would exceed the array returns F4J_ERR_INVALID_ARG.  Calls are serialised inside
+ * the library and synchronous like f4j_dgemm_dev's split calls; the array keeps the last
+ * call's contents.  Call f4j_init at set-up: the first call of the library initialises it. */
+int f4j_dgemm_dev_noalloc(int opA, int opB, int64_t M, int64_t N, int64_t K,
+                          const double *A, int64_t lda, const double *B, int64_t ldb,
+                          double *C, int64_t ldc, int precision, int slices, void *stream);
+
 /* ---- complex128 GEMM ----
  * Replaces, by (opA, opB):
  *   (N, N): CMatrix.mult(CMatrix)         M:arrays/dense/CMatrix.java:463-465 ->
