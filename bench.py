@@ -13,21 +13,27 @@ slices; FP16 digits, 5 slices); and the modular form (INT8 residues modulo
 covers exponents, splitting, slice products, recombination and the FP64
 residual check.  Each form's error is measured against exact sums on sampled
 entries.  `value` is the fastest form whose sampled error is <= 1e-15 of
-(|A||B|)_ij.  Reported beside it: the matrix-vector product on the same
-operand (Matrix.mult(Vector), and CMatrix.mult(CVector) at the same size, both
-in the reference's summation order, against the HBM roofline); the batched workload (10,000 products with
-M, N, K log-uniform on [16, 512], N(0,1) entries; FP64 and split-BF16 grouped
-plans) and the complex batched workload (2,000 A^H B with M, N, K log-uniform
-on [32, 1024], parts spread over 10^+-4; 3M on split-FP16 and the FP64 grouped
-plan).
+(|A||B|)_ij.
+
+Reported beside it, each with a CPU leg (the oracle's C restatement of the
+kernel the reference's dispatcher selects, on this host's threads):
+  configs          Matrix.mult 512^3 FP64; CMatrix.mult 2048x1536x2048 (FP64 and
+                   the modular form, 3M); Matrix.mult 16384x4096x16384 (FP64 and
+                   the modular form);
+  batched          10,000 real products, M, N, K log-uniform on [16, 512], N(0,1):
+                   the FP64 grouped plan and the BF16 / INT8 split plans;
+  complex_batched  2,000 complex A^H B, M, N, K log-uniform on [32, 1024], parts
+                   spread over 10^+-4: FP64 grouped plan, 3M on FP16 digits and on
+                   INT8 residues;
+  gemv             Matrix.mult(Vector) and CMatrix.mult(CVector) at 8192^2, in the
+                   reference's summation orders, against the HBM roofline.
+All inputs come from java.util.Random streams (flag4j_amd.rng).
 
 Prints one JSON line.  `roofline` is for the dominant kernel of the form that
-gives `value` (the FP64 GEMM, or the slice-product launch of a split form),
-timed with HIP events on the stream it is launched on; `roofline_fp64` is the
-plain FP64 kernel's.  `cpu_baseline` times the oracle (a C restatement of the
-kernel Flag4j's dispatcher selects at this shape,
-RealDenseMatMult.concurrentBlockedReordered) on a bounded sample of the same
-product on this host's cores.
+gives `value`, timed with HIP events on the stream it is launched on;
+`roofline_fp64` is the plain FP64 kernel's.  `cpu_baseline` times the oracle's
+restatement of RealDenseMatMult.concurrentBlockedReordered (what the reference
+selects at 8192^3) on a bounded sample of the same product.
 """
 import argparse
 import json
@@ -52,10 +58,12 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--size", type=int, default=8192)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--cpu-threads", type=int, default=0, help="host threads of the CPU legs (0: the CPUs this "
+                    "process may use, at most 16)")
     ap.add_argument("--no-split", action="store_true", help="plain FP64 only")
     ap.add_argument("--forms", default=",".join(FORMS), help="comma-separated subset of " + ",".join(FORMS))
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-configs", action="store_true", help="skip the 512^3, complex and 16384x4096x16384 lines")
     ap.add_argument("--no-gemv", action="store_true")
     ap.add_argument("--profile-copy", action="store_true",
                     help="after the timed region, copy 512 MiB on the device once (calibrates HBM byte counters)")
@@ -108,138 +116,319 @@ def exact_entries(np, A, B, I, J):
     return out, ab
 
 
-def bench_batched(np, torch, f4j, rng, dev, count=10000, reps=10):
+def host_threads():
+    """Host threads for the CPU legs: the CPUs this process may run on, at most 16
+    (the host share of one GPU on the benchmark machines)."""
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except AttributeError:
+        aff = os.cpu_count() or 1
+    return max(1, min(aff, 16))
+
+
+def host_info():
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except AttributeError:
+        aff = None
+    return {"cpu_model": model, "nproc": os.cpu_count(), "affinity_cpus": aff}
+
+
+def real_kernel(o, name, A, B, T):
+    """Run the oracle's restatement of the RealDenseMatMult kernel the reference's
+    dispatcher names (ReDeMatMultDispatcher.java:183-237)."""
+    if name in ("SMALL_STANDARD", "STRD"):
+        return o.real_standard(A, B)
+    if name == "REORD":
+        return o.real_reordered(A, B)
+    if name == "MT_STRD":
+        return o.real_mt_standard(A, B, threads=T)
+    if name == "MT_REORD":
+        return o.real_mt_reordered(A, B, threads=T)
+    if name == "MT_BLK_REORD":
+        return o.real_mt_blocked_reordered(A, B, threads=T, bs=64)
+    return o.real_blocked_reordered(A, B, 64)
+
+
+def cplx_kernel(o, name, A, B, T):
+    """The same for DenseSemiringMatMult on primitive doubles (Cm128DeMatMultDispatcher.java:182-226)."""
+    if name in ("SMALL_STANDARD", "STRD"):
+        return o.cplx_gemm(A, B)
+    if name == "REORD":
+        return o.cplx_mt_reordered(A, B, threads=1)
+    if name == "MT_STRD":
+        return o.cplx_mt_standard(A, B, threads=T)
+    return o.cplx_mt_reordered(A, B, threads=T)
+
+
+def cpu_loop(np, o, shapes, get, cplx, T, budget_s=6.0):
+    """A Java loop of Matrix.mult / A.H().mult(B) calls, restated: each product runs
+    the kernel the reference's dispatcher selects for its shape (A.H() materialised
+    first, as the reference does).  Products are timed in order until `budget_s`
+    has passed; the rate over that prefix (flop-weighted) is the loop's rate."""
+    done, flops, t = 0, 0.0, 0.0
+    for i, (M, N, K) in enumerate(shapes):
+        A, B = get(i)
+        t0 = time.perf_counter()
+        if cplx:
+            Ah = np.ascontiguousarray(A.conj().T)
+            cplx_kernel(o, o.select_cplx(M, K, N), Ah, B, T)
+        else:
+            real_kernel(o, o.select_real(M, K, N), A, B, T)
+        t += time.perf_counter() - t0
+        flops += (8.0 if cplx else 2.0) * M * N * K
+        done += 1
+        if t >= budget_s:
+            break
+    return done, flops, t
+
+
+def timed_plan(torch, plan, st, reps):
+    plan.run(st.cuda_stream)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(st)
+    for _ in range(reps):
+        plan.run(st.cuda_stream)
+    e1.record(st)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def bench_batched(np, torch, f4j, rng, dev, T, cpu=True, count=10000, reps=10):
     """10,000 FP64 products, M, N, K log-uniform on [16, 512] (java.util.Random
-    seed 4), N(0,1) entries drawn on the device; one grouped plan (tuned), its
-    hipGraph replayed `reps` times.  The same batch again as a split-BF16 plan
-    (7 slices: K goes down to 16), with its per-product residual-check error."""
+    seed 4), N(0,1) entries from java.util.Random seeds 41 (A) and 42 (B) by the
+    polar method; one grouped plan (tuned), its hipGraph replayed `reps` times.
+    The same batch as split plans with their default counts (BF16 7 digits; INT8
+    residues with t >= 64 bits, 18 moduli) and with 15 moduli (t >= 53, what
+    N(0,1) data needs), each with its per-product residual-check error."""
     from flag4j_amd import grouped
+    from oracle import oracle as o
     dims = rng.log_uniform_ints(4, 3 * count, 16, 512).reshape(count, 3)
     M, N, K = dims[:, 0], dims[:, 1], dims[:, 2]
     oa = np.cumsum(np.r_[0, M * K])
     ob = np.cumsum(np.r_[0, K * N])
     oc = np.cumsum(np.r_[0, M * N])
-    g = torch.Generator(device=dev)
-    g.manual_seed(1)
-    dA = torch.randn(int(oa[-1]), dtype=torch.float64, device=dev, generator=g)
-    dB = torch.randn(int(ob[-1]), dtype=torch.float64, device=dev, generator=g)
+    Ah = rng.next_gaussians(41, int(oa[-1]))
+    Bh = rng.next_gaussians(42, int(ob[-1]))
+    dA = torch.from_numpy(Ah).to(dev)
+    dB = torch.from_numpy(Bh).to(dev)
     dC = torch.empty(int(oc[-1]), dtype=torch.float64, device=dev)
     args = (list(zip(M.tolist(), N.tolist(), K.tolist())), [dA.data_ptr() + 8 * int(x) for x in oa[:-1]],
             [dB.data_ptr() + 8 * int(x) for x in ob[:-1]], [dC.data_ptr() + 8 * int(x) for x in oc[:-1]],
             K.tolist(), N.tolist(), N.tolist())
     st = torch.cuda.Stream(device=dev)
-
-    def timed(plan):
-        plan.run(st.cuda_stream)
-        torch.cuda.synchronize()
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record(st)
-        for _ in range(reps):
-            plan.run(st.cuda_stream)
-        e1.record(st)
-        torch.cuda.synchronize()
-        return e0.elapsed_time(e1) / reps
-
     plan = grouped.GroupedPlan(*args)
     plan.tune(2)
-    ms = timed(plan)
+    ms = timed_plan(torch, plan, st, reps)
     flops = float(np.sum(2.0 * M * N * K))
     roof = float(np.sum(np.maximum(2.0 * M * N * K / (PEAK_FP64_TFLOPS * 1e12), 8.0 * (M * K + K * N + M * N)
                                    / (PEAK_HBM_GBS * 1e9))))
     info = plan.info()
     plan.close()
-    out = {"workload": "10000 FP64 products, M,N,K log-uniform on [16,512], N(0,1)", "ms": round(ms, 4),
-           "tflops": round(flops / ms / 1e9, 2), "per_problem_roofline_ms": round(roof * 1e3, 4),
+    out = {"workload": "10000 FP64 products, M,N,K log-uniform on [16,512], N(0,1) (java.util.Random seeds 4, 41, 42)",
+           "ms": round(ms, 4), "tflops": round(flops / ms / 1e9, 2), "per_problem_roofline_ms": round(roof * 1e3, 4),
            "roofline_frac": round(roof * 1e3 / ms, 4), "tile_cfg_per_bucket": info["tile_cfg"],
            "tiles_per_bucket": info["tiles"]}
     f4j._native.config_set("splitCheckRows", 1)
-    plan = grouped.GroupedPlan(*args, precision="split_bf16", slices=7)
-    ms_s = timed(plan)
-    errs, prods = plan.split_report()
-    plan.close()
+    for key, prec, S in (("split_bf16", "split_bf16", 0), ("split_int8", "split_int8", 0),
+                         ("split_int8_15", "split_int8", 15)):
+        plan = grouped.GroupedPlan(*args, precision=prec, slices=S)
+        ms_s = timed_plan(torch, plan, st, reps)
+        errs, prods = plan.split_report()
+        plan.close()
+        per = prods // count                      # digit pairs S (S + 1) / 2, or moduli
+        sl = per if prec == "split_int8" else int(round(((8 * per + 1) ** 0.5 - 1) / 2))
+        out[key] = {"slices": sl, "default": S == 0, "ms": round(ms_s, 4),
+                    "tflops_fp64_accurate": round(flops / ms_s / 1e9, 2), "real_products": prods,
+                    "max_err_checked_rows": float(np.max(errs)),
+                    "products_over_1e-15": int(np.sum(errs > 1e-15)), "checked_rows_per_product": 1}
     f4j._native.config_set("splitCheckRows", 4)
-    out["split_bf16"] = {"slices": 7, "ms": round(ms_s, 4), "tflops_fp64_accurate": round(flops / ms_s / 1e9, 2),
-                         "real_slice_products": prods, "max_err_checked_rows": float(np.max(errs)),
-                         "products_over_1e-15": int(np.sum(errs > 1e-15)), "checked_rows_per_product": 1}
-    f4j._native.config_set("splitCheckRows", 1)
-    plan = grouped.GroupedPlan(*args, precision="split_int8")
-    ms_i = timed(plan)
-    errs, prods = plan.split_report()
-    plan.close()
-    f4j._native.config_set("splitCheckRows", 4)
-    out["split_int8"] = {"moduli": prods // count, "ms": round(ms_i, 4),
-                         "tflops_fp64_accurate": round(flops / ms_i / 1e9, 2), "real_residue_products": prods,
-                         "max_err_checked_rows": float(np.max(errs)), "products_over_1e-15": int(np.sum(errs > 1e-15)),
-                         "checked_rows_per_product": 1}
+    del dA, dB, dC
+    torch.cuda.empty_cache()
+    if cpu:
+        shapes = list(zip(M.tolist(), N.tolist(), K.tolist()))
+        get = lambda i: (Ah[oa[i]:oa[i + 1]].reshape(M[i], K[i]), Bh[ob[i]:ob[i + 1]].reshape(K[i], N[i]))
+        done, fl, t = cpu_loop(np, o, shapes, get, False, T)
+        rate = fl / t / 1e9
+        out["cpu_baseline"] = {"value": round(rate, 3), "unit": "GFLOP/s", "cores": T, "kind": "port",
+                               "ms_whole_batch_extrapolated": round(flops / (rate * 1e9) * 1e3, 1),
+                               "sample": f"the first {done} of {count} products ({fl:.3g} flop, {t:.1f} s), each on "
+                                         f"the oracle kernel ReDeMatMultDispatcher selects for its shape"}
     return out
 
 
-def bench_complex_batched(np, torch, f4j, rng, dev, count=2000, reps=3, slices=8):
+def bench_complex_batched(np, torch, f4j, rng, dev, T, cpu=True, count=2000, reps=3):
     """2,000 complex products A^H B (A stored K x M, B K x N), M, N, K log-uniform
     on [32, 1024] (java.util.Random seed 5); real and imaginary parts
-    (2r-1) 10^(4 (2r'-1)) drawn on the device.  The 3M form on FP16 digits
-    (`slices`), one grouped split plan in a hipGraph, error from the residual
-    check (1 row of every product); the FP64 grouped plan on the same data."""
+    (2r-1) 10^(4 (2r'-1)) from java.util.Random seeds 51 (A) and 52 (B).  3M on
+    FP16 digits and on INT8 residues (default counts), one grouped split plan each
+    in a hipGraph, error from the residual check (1 row of every product); the FP64
+    grouped plan on the same data."""
     from flag4j_amd import grouped
+    from oracle import oracle as o
     dims = rng.log_uniform_ints(5, 3 * count, 32, 1024).reshape(count, 3)
     M, N, K = dims[:, 0], dims[:, 1], dims[:, 2]
     oa = np.cumsum(np.r_[0, K * M])
     ob = np.cumsum(np.r_[0, K * N])
     oc = np.cumsum(np.r_[0, M * N])
-    g = torch.Generator(device=dev)
-    g.manual_seed(2)
-
-    def wide(n):
-        r = torch.rand(n, dtype=torch.float64, device=dev, generator=g)
-        rp = torch.rand(n, dtype=torch.float64, device=dev, generator=g)
-        return (2 * r - 1) * torch.pow(10.0, 4 * (2 * rp - 1))
-
-    dA = torch.complex(wide(int(oa[-1])), wide(int(oa[-1])))
-    dB = torch.complex(wide(int(ob[-1])), wide(int(ob[-1])))
+    Ah = rng.scaled_uniform_complex(51, int(oa[-1]), 4)
+    Bh = rng.scaled_uniform_complex(52, int(ob[-1]), 4)
+    dA = torch.from_numpy(Ah).to(dev)
+    dB = torch.from_numpy(Bh).to(dev)
     dC = torch.empty(int(oc[-1]), dtype=torch.complex128, device=dev)
     args = (list(zip(M.tolist(), N.tolist(), K.tolist())), [dA.data_ptr() + 16 * int(x) for x in oa[:-1]],
             [dB.data_ptr() + 16 * int(x) for x in ob[:-1]], [dC.data_ptr() + 16 * int(x) for x in oc[:-1]],
             M.tolist(), N.tolist(), N.tolist(), "C", "N", True)
     st = torch.cuda.Stream(device=dev)
-
-    def timed(plan):
-        plan.run(st.cuda_stream)
-        torch.cuda.synchronize()
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record(st)
-        for _ in range(reps):
-            plan.run(st.cuda_stream)
-        e1.record(st)
-        torch.cuda.synchronize()
-        return e0.elapsed_time(e1) / reps
-
     flops = float(np.sum(8.0 * M * N * K))
-    f4j._native.config_set("splitCheckRows", 1)
-    plan = grouped.GroupedPlan(*args, precision="split_fp16", slices=slices, use3m=True)
-    ms_s = timed(plan)
-    errs, prods = plan.split_report()
-    plan.close()
-    plan = grouped.GroupedPlan(*args, precision="split_int8", slices=18, use3m=True)   # t >= 64: the wide range
-    ms_i = timed(plan)
-    errs_i, prods_i = plan.split_report()
-    plan.close()
-    f4j._native.config_set("splitCheckRows", 4)
+    roof = float(np.sum(np.maximum(8.0 * M * N * K / (PEAK_FP64_TFLOPS * 1e12), 16.0 * (M * K + K * N + M * N)
+                                   / (PEAK_HBM_GBS * 1e9))))
     plan = grouped.GroupedPlan(*args)
     plan.tune(1)
-    ms_d = timed(plan)
+    ms_d = timed_plan(torch, plan, st, reps)
     plan.close()
+    out = {"workload": "2000 complex128 A^H B, M,N,K log-uniform on [32,1024], parts (2r-1) 10^(4(2r'-1)) "
+                       "(java.util.Random seeds 5, 51, 52)",
+           "fp64_grouped_ms": round(ms_d, 3), "fp64_grouped_tflops": round(flops / ms_d / 1e9, 2),
+           "per_problem_roofline_ms": round(roof * 1e3, 4), "roofline_frac": round(roof * 1e3 / ms_d, 4)}
+    f4j._native.config_set("splitCheckRows", 1)
+    for key, prec in (("split_fp16", "split_fp16"), ("split_int8", "split_int8")):
+        plan = grouped.GroupedPlan(*args, precision=prec, use3m=True)
+        ms_s = timed_plan(torch, plan, st, reps)
+        errs, prods = plan.split_report()
+        plan.close()
+        per = prods // (3 * count)
+        sl = per if prec == "split_int8" else int(round(((8 * per + 1) ** 0.5 - 1) / 2))
+        out[key] = {"form": "3M", "slices": sl, "ms": round(ms_s, 3),
+                    "tflops_classical": round(flops / ms_s / 1e9, 2), "real_products": prods,
+                    "max_err_checked_rows": float(np.max(errs)), "products_over_1e-15": int(np.sum(errs > 1e-15)),
+                    "checked_rows_per_product": 1}
+    f4j._native.config_set("splitCheckRows", 4)
     del dA, dB, dC
     torch.cuda.empty_cache()
-    return {"workload": "2000 complex128 A^H B, M,N,K log-uniform on [32,1024], parts (2r-1) 10^(4(2r'-1))",
-            "form": "3M on split-FP16", "slices": slices, "ms": round(ms_s, 3),
-            "tflops_classical": round(flops / ms_s / 1e9, 2), "real_slice_products": prods,
-            "max_err_checked_rows": float(np.max(errs)), "products_over_1e-15": int(np.sum(errs > 1e-15)),
-            "checked_rows_per_product": 1, "fp64_grouped_ms": round(ms_d, 3),
-            "fp64_grouped_tflops": round(flops / ms_d / 1e9, 2),
-            "split_int8": {"form": "3M on INT8 residues", "moduli": prods_i // (3 * count), "ms": round(ms_i, 3),
-                           "tflops_classical": round(flops / ms_i / 1e9, 2), "real_residue_products": prods_i,
-                           "max_err_checked_rows": float(np.max(errs_i)),
-                           "products_over_1e-15": int(np.sum(errs_i > 1e-15)), "checked_rows_per_product": 1}}
+    if cpu:
+        shapes = list(zip(M.tolist(), N.tolist(), K.tolist()))
+        get = lambda i: (Ah[oa[i]:oa[i + 1]].reshape(K[i], M[i]), Bh[ob[i]:ob[i + 1]].reshape(K[i], N[i]))
+        done, fl, t = cpu_loop(np, o, shapes, get, True, T)
+        rate = fl / t / 1e9
+        out["cpu_baseline"] = {"value": round(rate, 3), "unit": "GFLOP/s (classical 8MNK)", "cores": T,
+                               "kind": "port", "ms_whole_batch_extrapolated": round(flops / (rate * 1e9) * 1e3, 1),
+                               "sample": f"the first {done} of {count} products ({fl:.3g} flop, {t:.1f} s): A.H() "
+                                         f"materialised, then the oracle kernel Cm128DeMatMultDispatcher selects, on "
+                                         f"primitive doubles"}
+    return out
+
+
+def time_dev(torch, stream, fn, reps):
+    for _ in range(2):
+        fn()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        fn()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def bench_single_configs(np, torch, f4j, rng, dev, stream, T, cpu=True):
+    """The single-product configurations besides 8192^3: real 512^3 (FP64), complex
+    2048 x 1536 x 2048 (FP64 and the modular form, 3M) and real 16384 x 4096 x 16384
+    (FP64 and the modular form), device-resident, timed with events on the launch
+    stream; each beside the oracle's restatement of the kernel the reference's
+    dispatcher selects, on this host."""
+    from oracle import oracle as o
+    sh = stream.cuda_stream
+    out = {}
+
+    def mfma(flops, ms):
+        tf = flops / (ms * 1e-3) / 1e12
+        return {"ms": round(ms, 4), "tflops": round(tf, 2), "frac_fp64_peak": round(tf / PEAK_FP64_TFLOPS, 4)}
+
+    # 512^3 real
+    n = 512
+    A, B = rng.uniform(1, n, n), rng.uniform(2, n, n)
+    dA, dB = torch.from_numpy(A).to(dev), torch.from_numpy(B).to(dev)
+    dC = torch.empty((n, n), dtype=torch.float64, device=dev)
+    fl = 2.0 * n ** 3
+    ms = time_dev(torch, stream, lambda: f4j.dgemm_dev("N", "N", n, n, n, dA.data_ptr(), n, dB.data_ptr(), n,
+                                                       dC.data_ptr(), n, precision="fp64", stream=sh), 200)
+    e = {"workload": "Matrix.mult 512x512x512 FP64, entries 2r-1 (seeds 1, 2)", "fp64": mfma(fl, ms)}
+    if cpu:
+        t0 = time.perf_counter()
+        for _ in range(5):
+            o.real_mt_reordered(A, B, threads=T)
+        t = (time.perf_counter() - t0) / 5
+        e["cpu_baseline"] = {"value": round(fl / t / 1e9, 3), "unit": "GFLOP/s", "cores": T, "kind": "port",
+                             "sample": f"whole product, 5 runs ({t * 1e3:.1f} ms each): "
+                                       "RealDenseMatMult.concurrentReordered (MT_REORD) restated"}
+    out["real_512"] = e
+    # complex 2048 x 1536 x 2048
+    M, K, N = 2048, 1536, 2048
+    Az, Bz = rng.uniform_complex(3, M, K), rng.uniform_complex(4, K, N)
+    dA, dB = torch.from_numpy(Az).to(dev), torch.from_numpy(Bz).to(dev)
+    dC = torch.empty((M, N), dtype=torch.complex128, device=dev)
+    fl = 8.0 * M * N * K
+    ms = time_dev(torch, stream, lambda: f4j.zgemm_dev("N", "N", M, N, K, dA.data_ptr(), K, dB.data_ptr(), N,
+                                                       dC.data_ptr(), N, precision="fp64", stream=sh), 20)
+    ms8 = time_dev(torch, stream, lambda: f4j.zgemm_dev("N", "N", M, N, K, dA.data_ptr(), K, dB.data_ptr(), N,
+                                                        dC.data_ptr(), N, precision="split_int8", use_3m=True,
+                                                        stream=sh), 5)
+    rep = f4j._native.last_split_report()
+    e = {"workload": "CMatrix.mult 2048x1536 * 1536x2048 complex128, parts 2r-1 (seeds 3, 4)", "fp64": mfma(fl, ms),
+         "split_int8_3m": {"ms": round(ms8, 4), "tflops_classical": round(fl / (ms8 * 1e-3) / 1e12, 2),
+                           "moduli": rep.slices, "delivered": "split" if rep.precision != 0 else "fp64 fallback",
+                           "residual_check_err": rep.max_rel_err}}
+    if cpu:
+        t0 = time.perf_counter()
+        o.cplx_mt_reordered(Az, Bz, threads=T)
+        t = time.perf_counter() - t0
+        e["cpu_baseline"] = {"value": round(fl / t / 1e9, 3), "unit": "GFLOP/s (classical 8MNK)",
+                             "cores": T, "kind": "port",
+                             "sample": f"whole product ({t:.2f} s): DenseSemiringMatMult.concurrentReordered (MT_REORD) "
+                                       "on primitive doubles (no Complex128 objects, so faster than the reference)"}
+    out["complex_2048x1536x2048"] = e
+    del dA, dB, dC
+    # 16384 x 4096 x 16384 real
+    M, K, N = 16384, 4096, 16384
+    A, B = rng.uniform(21, M, K), rng.uniform(22, K, N)
+    dA, dB = torch.from_numpy(A).to(dev), torch.from_numpy(B).to(dev)
+    dC = torch.empty((M, N), dtype=torch.float64, device=dev)
+    fl = 2.0 * M * N * K
+    ms = time_dev(torch, stream, lambda: f4j.dgemm_dev("N", "N", M, N, K, dA.data_ptr(), K, dB.data_ptr(), N,
+                                                       dC.data_ptr(), N, precision="fp64", stream=sh), 5)
+    ms8 = time_dev(torch, stream, lambda: f4j.dgemm_dev("N", "N", M, N, K, dA.data_ptr(), K, dB.data_ptr(), N,
+                                                        dC.data_ptr(), N, precision="split_int8", stream=sh), 5)
+    rep = f4j._native.last_split_report()
+    e = {"workload": "Matrix.mult 16384x4096 * 4096x16384 FP64, entries 2r-1 (seeds 21, 22)", "fp64": mfma(fl, ms),
+         "split_int8": {"ms": round(ms8, 4), "tflops_fp64_accurate": round(fl / (ms8 * 1e-3) / 1e12, 2),
+                        "moduli": rep.slices, "delivered": "split" if rep.precision != 0 else "fp64 fallback",
+                        "residual_check_err": rep.max_rel_err}}
+    if cpu:
+        rows = 64 * T
+        C = np.empty((rows, N))
+        t0 = time.perf_counter()
+        o.real_mt_reordered(A[:rows], B, C=C, threads=T)
+        t = time.perf_counter() - t0
+        rate = 2.0 * rows * N * K / t / 1e9
+        e["cpu_baseline"] = {"value": round(rate, 3), "unit": "GFLOP/s", "cores": T, "kind": "port",
+                             "whole_product_s_extrapolated": round(fl / (rate * 1e9), 1),
+                             "sample": f"rows 0..{rows - 1} ({t:.1f} s), extrapolated linearly in M (rows are "
+                                       "independent): RealDenseMatMult.concurrentReordered (MT_REORD) restated"}
+    out["real_16384x4096x16384"] = e
+    del dA, dB, dC
+    torch.cuda.empty_cache()
+    return out
 
 
 def bench_gemv(np, torch, f4j, dA, x, stream, reps=50):
@@ -414,11 +603,20 @@ def main():
         gemv = bench_gemv(np, torch, f4j, dA, dB[0].contiguous(), stream)
         log(f"[bench] gemv: {gemv}")
 
+    T = args.cpu_threads or host_threads()
+    want_cpu = not args.no_cpu_baseline
+    configs = None
+    if not args.no_configs:
+        log("[bench] single-product configurations")
+        configs = bench_single_configs(np, torch, f4j, rng, dev, stream, T, cpu=want_cpu)
+        log(f"[bench] configs: {configs}")
     batched = complex_batched = None
     if not args.no_batched:
-        batched = bench_batched(np, torch, f4j, rng, dev)
+        log("[bench] batched real")
+        batched = bench_batched(np, torch, f4j, rng, dev, T, cpu=want_cpu)
         log(f"[bench] batched: {batched}")
-        complex_batched = bench_complex_batched(np, torch, f4j, rng, dev)
+        log("[bench] batched complex")
+        complex_batched = bench_complex_batched(np, torch, f4j, rng, dev, T, cpu=want_cpu)
         log(f"[bench] complex batched: {complex_batched}")
 
     # End-to-end through the host-array ABI (PCIe included), one call.
@@ -429,10 +627,10 @@ def main():
     e2e_s = time.perf_counter() - t1
 
     cpu = None
-    if not args.no_cpu_baseline:
-        threads = args.cpu_threads or min(len(os.sched_getaffinity(0)), 16)
-        log(f"[bench] CPU baseline on {threads} threads")
-        cpu = cpu_baseline(A, B, n, threads)
+    if want_cpu:
+        log(f"[bench] CPU baseline on {T} threads")
+        cpu = cpu_baseline(A, B, n, T)
+        cpu.update(host_info())
 
     dtypes = {"fp64": "f64", "split_bf16": "bf16 digits, f32 accumulate, f64 recombine",
               "split_fp16": "f16 digits, f32 accumulate, f64 recombine",
@@ -458,6 +656,7 @@ def main():
         "forms": forms,
         "accuracy": {"max_err_over_absAabsB_sampled": forms[best[0]]["max_err"], "samples": 512,
                      "reference": "exact sums (TwoProduct + fsum) on sampled entries"},
+        "configs": configs,
         "batched": batched,
         "complex_batched": complex_batched,
         "gemv": gemv,

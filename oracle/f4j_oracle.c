@@ -119,7 +119,8 @@ void f4o_real_mult_transpose(int M, int N, int K, const double *A, const double 
 /* ------------------------------------------------------------------------ */
 
 typedef struct {
-    int kind;                    /* 0 real reordered, 1 real blocked, 2 complex reordered */
+    int kind;                    /* 0 real reordered, 1 real blocked, 2 complex reordered,
+                                    3 real standard, 4 complex standard */
     int M, N, K, bs;
     const double *A, *B;
     double *C;
@@ -166,6 +167,36 @@ static void cplx_reordered_rows(int N, int K, const double *A, const double *B, 
     }
 }
 
+static void real_standard_rows(int N, int K, const double *A, const double *B, double *C, int r0, int r1) {
+    /* RealDenseMatMult.java:243-272 (concurrentStandard) task body: one dot product
+     * per entry, B read with stride N. */
+    for (int i = r0; i < r1; i++)
+        for (int j = 0; j < N; j++) {
+            double sum = 0.0;
+            for (int k = 0; k < K; k++) sum += A[(int64_t)i * K + k] * B[(int64_t)k * N + j];
+            C[(int64_t)i * N + j] = 0.0 + sum;
+        }
+}
+
+static void cplx_standard_rows(int N, int K, const double *A, const double *B, double *C, int r0, int r1) {
+    /* DenseSemiringMatMult.java:247-287 (concurrentStandard) task body on primitive
+     * doubles: per entry sum = ZERO, sum = sum.add(a.mult(b)) for k ascending. */
+    for (int i = r0; i < r1; i++)
+        for (int j = 0; j < N; j++) {
+            double sr = 0.0, si = 0.0;
+            for (int k = 0; k < K; k++) {
+                const double ar = A[2 * ((int64_t)i * K + k)], ai = A[2 * ((int64_t)i * K + k) + 1];
+                const double br = B[2 * ((int64_t)k * N + j)], bi = B[2 * ((int64_t)k * N + j) + 1];
+                const double pr = ar * br - ai * bi;
+                const double pi = ar * bi + ai * br;
+                sr = sr + pr;
+                si = si + pi;
+            }
+            C[2 * ((int64_t)i * N + j)] = sr;
+            C[2 * ((int64_t)i * N + j) + 1] = si;
+        }
+}
+
 static void *pool_worker(void *arg) {
     pool_job *job = (pool_job *)arg;
     for (;;) {
@@ -181,8 +212,12 @@ static void *pool_worker(void *arg) {
         else if (job->kind == 1) {
             for (int i = s; i < e; i++) memset(job->C + (int64_t)i * job->N, 0, sizeof(double) * (size_t)job->N);
             blocked_reordered_rows(job->N, job->K, job->A, job->B, job->C, s, e, job->bs);
-        } else
+        } else if (job->kind == 2)
             cplx_reordered_rows(job->N, job->K, job->A, job->B, job->C, s, e);
+        else if (job->kind == 3)
+            real_standard_rows(job->N, job->K, job->A, job->B, job->C, s, e);
+        else
+            cplx_standard_rows(job->N, job->K, job->A, job->B, job->C, s, e);
     }
     return NULL;
 }
@@ -229,6 +264,32 @@ void f4o_cplx_mt_reordered(int M, int N, int K, const double *A, const double *B
     pool_job job = {0};
     const int R = row1 - row0, T = threads < 1 ? 1 : threads;
     job.kind = 2; job.N = N; job.K = K; job.A = A; job.B = B; job.C = C;
+    job.row0 = row0; job.row1 = row1;
+    job.chunk = (R + T - 1) / T; if (job.chunk < 1) job.chunk = 1;
+    job.ntasks = T;
+    run_pool(&job, T);
+}
+
+/* ThreadManager.java:155-168 partitioning of the concurrentStandard kernels
+ * (RealDenseMatMult.java:243-272; DenseSemiringMatMult.java:247-287, complex). */
+void f4o_real_mt_standard(int M, int N, int K, const double *A, const double *B, double *C,
+                          int row0, int row1, int threads) {
+    (void)M;
+    pool_job job = {0};
+    const int R = row1 - row0, T = threads < 1 ? 1 : threads;
+    job.kind = 3; job.N = N; job.K = K; job.A = A; job.B = B; job.C = C;
+    job.row0 = row0; job.row1 = row1;
+    job.chunk = (R + T - 1) / T; if (job.chunk < 1) job.chunk = 1;
+    job.ntasks = T;
+    run_pool(&job, T);
+}
+
+void f4o_cplx_mt_standard(int M, int N, int K, const double *A, const double *B, double *C,
+                          int row0, int row1, int threads) {
+    (void)M;
+    pool_job job = {0};
+    const int R = row1 - row0, T = threads < 1 ? 1 : threads;
+    job.kind = 4; job.N = N; job.K = K; job.A = A; job.B = B; job.C = C;
     job.row0 = row0; job.row1 = row1;
     job.chunk = (R + T - 1) / T; if (job.chunk < 1) job.chunk = 1;
     job.ntasks = T;

@@ -69,6 +69,12 @@ void f4o_cplx_gemm(int opA, int opB, int M, int N, int K, const double *A, const
 void f4o_mixed_gemm(int mode, int M, int N, int K, const double *A, const double *B, double *C);
 void f4o_cplx_mt_reordered(int M, int N, int K, const double *A, const double *B, double *C,
                            int row0, int row1, int threads);
+/* concurrentStandard (RealDenseMatMult.java:243-272, DenseSemiringMatMult.java:247-287):
+ * T tasks of ceil(R/T) rows, one k-ascending dot product per entry. */
+void f4o_real_mt_standard(int M, int N, int K, const double *A, const double *B, double *C,
+                          int row0, int row1, int threads);
+void f4o_cplx_mt_standard(int M, int N, int K, const double *A, const double *B, double *C,
+                          int row0, int row1, int threads);
 
 /* ---- dispatcher decision trees ---- */
 typedef struct {

@@ -59,6 +59,8 @@ def lib():
         L.f4o_cplx_gemm.argtypes = [_i, _i, _i, _i, _i, _dp, _dp, _dp]
         L.f4o_mixed_gemm.argtypes = [_i, _i, _i, _i, _dp, _dp, _dp]
         L.f4o_cplx_mt_reordered.argtypes = [_i, _i, _i, _dp, _dp, _dp, _i, _i, _i]
+        L.f4o_real_mt_standard.argtypes = [_i, _i, _i, _dp, _dp, _dp, _i, _i, _i]
+        L.f4o_cplx_mt_standard.argtypes = [_i, _i, _i, _dp, _dp, _dp, _i, _i, _i]
         L.f4o_select_real.argtypes = [_i, _i, _i, ctypes.POINTER(Thresholds)]
         L.f4o_select_cplx.argtypes = [_i, _i, _i, ctypes.POINTER(Thresholds)]
         L.f4o_select_transpose.argtypes = [_i]
@@ -172,6 +174,17 @@ def real_mt_blocked_reordered(A, B, C=None, rows=None, threads=1, bs=64):
     return C
 
 
+def real_mt_standard(A, B, C=None, rows=None, threads=1):
+    A, B = _f64(A), _f64(B)
+    M, K = A.shape
+    N = B.shape[1]
+    if C is None:
+        C = np.empty((M, N))
+    r0, r1 = rows if rows is not None else (0, M)
+    lib().f4o_real_mt_standard(M, N, K, _p(A), _p(B), _p(C), r0, r1, threads)
+    return C
+
+
 # ---- complex ----
 def _op_shape(X, op):
     r, c = X.shape
@@ -210,6 +223,18 @@ def cplx_mt_reordered(A, B, C=None, rows=None, threads=1):
     r0, r1 = rows if rows is not None else (0, M)
     lib().f4o_cplx_mt_reordered(M, N, K, _p(A.view(np.float64)), _p(B.view(np.float64)),
                                 _p(C.view(np.float64)), r0, r1, threads)
+    return C
+
+
+def cplx_mt_standard(A, B, C=None, rows=None, threads=1):
+    A, B = _c128(A), _c128(B)
+    M, K = A.shape
+    N = B.shape[1]
+    if C is None:
+        C = np.empty((M, N), dtype=np.complex128)
+    r0, r1 = rows if rows is not None else (0, M)
+    lib().f4o_cplx_mt_standard(M, N, K, _p(A.view(np.float64)), _p(B.view(np.float64)),
+                               _p(C.view(np.float64)), r0, r1, threads)
     return C
 
 

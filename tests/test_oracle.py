@@ -469,3 +469,25 @@ def test_cplx_entries_equal_the_pinned_kernel(golden, opA, opB):
     M, N = Ag.shape[0], Bg.shape[1]
     I, J = np.repeat(np.arange(M), N), np.tile(np.arange(N), M)
     assert np.array_equal(o.cplx_entries(Ag, Bg, I, J).view(np.float64), exp.reshape(-1).view(np.float64))
+
+
+def test_concurrent_standard_kernels(golden):
+    """concurrentStandard (RealDenseMatMult.java:243-272, DenseSemiringMatMult.java:247-287)
+    gives the golden vectors' bits and the pinned sequential kernels' bits for any thread count."""
+    g = golden["real_rect"]
+    A, B, exp = real_mat(g["a"]), real_mat(g["b"]), real_mat(g["c"])
+    for T in (1, 2, 5):
+        assert np.array_equal(bits(o.real_mt_standard(A, B, threads=T).reshape(-1)), bits(exp))
+    g = golden["cplx_rect"]
+    A, B, exp = cplx_mat(g["a"]), cplx_mat(g["b"]), cplx_mat(g["c"])
+    for T in (1, 3):
+        got = o.cplx_mt_standard(A, B, threads=T).reshape(exp.shape)
+        assert np.array_equal(got.view(np.float64), exp.view(np.float64))
+    rng = np.random.default_rng(13)
+    A = rng.standard_normal((40, 70)) * 10.0 ** rng.integers(-5, 6, (40, 70))
+    B = rng.standard_normal((70, 33)) * 10.0 ** rng.integers(-5, 6, (70, 33))
+    assert np.array_equal(bits(o.real_mt_standard(A, B, threads=4)), bits(o.real_standard(A, B)))
+    Az = A[:, :35] + 1j * A[:, 35:]
+    Bz = B[:35] + 1j * B[35:]
+    assert np.array_equal(o.cplx_mt_standard(Az, Bz, threads=3).view(np.float64),
+                          o.cplx_gemm(Az, Bz).view(np.float64))
