@@ -16,9 +16,12 @@
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
+#include <atomic>
 #include <mutex>
 #include <sstream>
 #include <string>
+#include <thread>
+#include <vector>
 
 #include "../../include/flag4j_amd.h"
 #include "f4j_internal.h"
@@ -60,15 +63,37 @@ constexpr size_t NOALLOC_SCRATCH_BYTES = size_t(64) << 20;
 constexpr int NOALLOC_MAX_DEVICES = 64;
 __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
 
+// Staging of the host-array calls.  The caller's (pageable) arrays move through
+// a ring of pinned chunks: the calling thread copies chunk i + 1 in while the DMA
+// engine moves chunk i, and on the way out the DMA of the next chunks runs while
+// it copies one out (pinned DMA reaches about 56 GB/s each way on MI355X hosts,
+// a pageable hipMemcpy 18-30 GB/s).  Each context owns its device buffers,
+// stream and ring, so up to STAGE_CTX host calls run at once; further callers
+// wait for a context.
+constexpr int STAGE_CTX = 4;
+constexpr int STAGE_SLOTS = 4;
+constexpr size_t STAGE_SLOT_BYTES = size_t(16) << 20;
+constexpr int STAGE_COPY_THREADS = 4;      // host copy of one chunk split over this many threads
+
+struct Stage {
+    std::mutex mu;                         // held by the call using the context
+    bool ready = false;
+    hipStream_t st = nullptr;              // copies and products of this context, in order
+    DevBuf A, B, C;
+    void *slot[STAGE_SLOTS] = {};          // pinned (hipHostMalloc)
+    hipEvent_t slot_ev[STAGE_SLOTS] = {};  // last DMA that used the slot
+    bool slot_used[STAGE_SLOTS] = {};
+};
+
 struct State {
-    std::mutex mu;                    // guards init state and the staging buffers
+    std::mutex mu;                    // guards init state
     std::mutex cfg_mu;                // guards cfg
     bool inited = false;
     int device = -1;
     hipStream_t stream = nullptr;
     hipDeviceProp_t prop{};
     Config cfg;
-    DevBuf bufA, bufB, bufC;
+    Stage stages[STAGE_CTX];
     std::mutex ws_mu;                 // guards ws and tev (split-precision workspace, product timing)
     DevBuf ws;
     std::mutex mix_mu;                // guards mixws (planes of complex x real products)
@@ -166,6 +191,145 @@ int grow(DevBuf &b, size_t bytes) {
     if (e != hipSuccess) return hip_fail(e, "hipMalloc (staging buffer)");
     b.p = p;
     b.cap = cap;
+    return F4J_OK;
+}
+
+// ---- host staging ----
+
+int stage_setup(Stage &g) {
+    if (g.ready) return F4J_OK;
+    F4J_HIP(hipStreamCreateWithFlags(&g.st, hipStreamNonBlocking), "hipStreamCreate (staging)");
+    for (int i = 0; i < STAGE_SLOTS; ++i) {
+        F4J_HIP(hipHostMalloc(&g.slot[i], STAGE_SLOT_BYTES, hipHostMallocDefault), "hipHostMalloc (staging ring)");
+        F4J_HIP(hipEventCreateWithFlags(&g.slot_ev[i], hipEventDisableTiming), "hipEventCreate (staging)");
+        g.slot_used[i] = false;
+    }
+    g.ready = true;
+    return F4J_OK;
+}
+
+void stage_release(Stage &g) {
+    if (!g.ready) return;
+    (void)hipStreamSynchronize(g.st);
+    for (int i = 0; i < STAGE_SLOTS; ++i) {
+        if (g.slot[i]) (void)hipHostFree(g.slot[i]);
+        if (g.slot_ev[i]) (void)hipEventDestroy(g.slot_ev[i]);
+        g.slot[i] = nullptr;
+        g.slot_ev[i] = nullptr;
+    }
+    for (DevBuf *b : {&g.A, &g.B, &g.C})
+        if (b->p) {
+            (void)hipFree(b->p);
+            b->p = nullptr;
+            b->cap = 0;
+        }
+    (void)hipStreamDestroy(g.st);
+    g.st = nullptr;
+    g.ready = false;
+}
+
+// A free context if there is one, else wait for one (round robin).
+Stage &stage_acquire(std::unique_lock<std::mutex> &lk) {
+    State &s = S();
+    for (Stage &g : s.stages) {
+        std::unique_lock<std::mutex> l(g.mu, std::try_to_lock);
+        if (l.owns_lock()) {
+            lk = std::move(l);
+            return g;
+        }
+    }
+    static std::atomic<unsigned> next{0};
+    Stage &g = s.stages[next++ % STAGE_CTX];
+    lk = std::unique_lock<std::mutex>(g.mu);
+    return g;
+}
+
+// Copy `rows` rows of `row_bytes` between a strided host array (pitch bytes) and
+// a packed buffer, on up to STAGE_COPY_THREADS threads for large chunks.
+void host_rows_copy(char *packed, const char *host_src, char *host_dst, int64_t rows, size_t row_bytes, size_t pitch) {
+    auto part = [&](int64_t r0, int64_t r1) {
+        if (host_src) {
+            if (pitch == row_bytes) std::memcpy(packed + r0 * row_bytes, host_src + r0 * pitch, (r1 - r0) * row_bytes);
+            else
+                for (int64_t r = r0; r < r1; ++r) std::memcpy(packed + r * row_bytes, host_src + r * pitch, row_bytes);
+        } else {
+            if (pitch == row_bytes) std::memcpy(host_dst + r0 * pitch, packed + r0 * row_bytes, (r1 - r0) * row_bytes);
+            else
+                for (int64_t r = r0; r < r1; ++r) std::memcpy(host_dst + r * pitch, packed + r * row_bytes, row_bytes);
+        }
+    };
+    const size_t bytes = (size_t)rows * row_bytes;
+    const int nt = (int)std::min<int64_t>(rows, bytes >= (size_t(4) << 20) ? STAGE_COPY_THREADS : 1);
+    if (nt <= 1) {
+        part(0, rows);
+        return;
+    }
+    std::vector<std::thread> th;
+    const int64_t per = (rows + nt - 1) / nt;
+    for (int t = 1; t < nt; ++t) {
+        const int64_t r0 = std::min(rows, t * per), r1 = std::min(rows, (t + 1) * per);
+        if (r0 < r1) th.emplace_back(part, r0, r1);
+    }
+    part(0, std::min(rows, per));
+    for (std::thread &x : th) x.join();
+}
+
+// Host (pageable, `pitch` bytes between rows) -> device (packed rows), through the ring.
+int stage_in(Stage &g, void *dev, const void *host, int64_t rows, size_t row_bytes, size_t pitch) {
+    if (rows <= 0 || row_bytes == 0) return F4J_OK;
+    if (row_bytes > STAGE_SLOT_BYTES) {       // rows longer than a chunk: the runtime's own path
+        F4J_HIP(hipMemcpy2DAsync(dev, row_bytes, host, pitch, row_bytes, rows, hipMemcpyHostToDevice, g.st),
+                "copy to device");
+        return F4J_OK;
+    }
+    const int64_t rpc = (int64_t)(STAGE_SLOT_BYTES / row_bytes);
+    int k = 0;
+    for (int64_t r0 = 0; r0 < rows; r0 += rpc, ++k) {
+        const int64_t n = std::min(rpc, rows - r0);
+        const int sl = k % STAGE_SLOTS;
+        if (g.slot_used[sl]) F4J_HIP(hipEventSynchronize(g.slot_ev[sl]), "staging wait");
+        host_rows_copy((char *)g.slot[sl], (const char *)host + r0 * pitch, nullptr, n, row_bytes, pitch);
+        F4J_HIP(hipMemcpyAsync((char *)dev + r0 * row_bytes, g.slot[sl], n * row_bytes, hipMemcpyHostToDevice, g.st),
+                "copy to device");
+        F4J_HIP(hipEventRecord(g.slot_ev[sl], g.st), "hipEventRecord (staging)");
+        g.slot_used[sl] = true;
+    }
+    return F4J_OK;
+}
+
+// Device (packed rows) -> host (pageable, `pitch` bytes between rows), through the
+// ring; waits for the work queued before it on the context's stream.  Synchronous.
+int stage_out(Stage &g, void *host, size_t pitch, const void *dev, int64_t rows, size_t row_bytes) {
+    if (rows <= 0 || row_bytes == 0) return F4J_OK;
+    if (row_bytes > STAGE_SLOT_BYTES) {
+        F4J_HIP(hipMemcpy2DAsync(host, pitch, dev, row_bytes, row_bytes, rows, hipMemcpyDeviceToHost, g.st),
+                "copy to host");
+        F4J_HIP(hipStreamSynchronize(g.st), "stream synchronize");
+        return F4J_OK;
+    }
+    const int64_t rpc = (int64_t)(STAGE_SLOT_BYTES / row_bytes);
+    const int64_t nchunk = (rows + rpc - 1) / rpc;
+    auto issue = [&](int64_t k) -> int {
+        const int sl = (int)(k % STAGE_SLOTS);
+        const int64_t r0 = k * rpc, n = std::min(rpc, rows - r0);
+        F4J_HIP(hipMemcpyAsync(g.slot[sl], (const char *)dev + r0 * row_bytes, n * row_bytes, hipMemcpyDeviceToHost,
+                               g.st),
+                "copy to host");
+        F4J_HIP(hipEventRecord(g.slot_ev[sl], g.st), "hipEventRecord (staging)");
+        g.slot_used[sl] = true;
+        return F4J_OK;
+    };
+    // keep STAGE_SLOTS - 1 chunks in flight ahead of the one being copied out
+    for (int64_t k = 0; k < std::min<int64_t>(nchunk, STAGE_SLOTS - 1); ++k)
+        if (const int rc = issue(k); rc != F4J_OK) return rc;
+    for (int64_t k = 0; k < nchunk; ++k) {
+        const int sl = (int)(k % STAGE_SLOTS);
+        F4J_HIP(hipEventSynchronize(g.slot_ev[sl]), "staging wait");
+        if (k + STAGE_SLOTS - 1 < nchunk)
+            if (const int rc = issue(k + STAGE_SLOTS - 1); rc != F4J_OK) return rc;
+        const int64_t r0 = k * rpc, n = std::min(rpc, rows - r0);
+        host_rows_copy((char *)g.slot[sl], nullptr, (char *)host + r0 * pitch, n, row_bytes, pitch);
+    }
     return F4J_OK;
 }
 
@@ -639,7 +803,8 @@ int gemm_dev(bool cplx, int opA, int opB, int64_t M, int64_t N, int64_t K, const
     return F4J_OK;
 }
 
-// Host-array GEMM: stage operands into the pooled device buffers, run, copy back.
+// Host-array GEMM: operands move in through a staging context's pinned ring, the
+// product runs on the context's stream, C moves out through the ring.
 int gemm_host(bool cplx, int opA, int opB, int64_t M, int64_t N, int64_t K, const double *A, int64_t lda,
               const double *B, int64_t ldb, double *C, int64_t ldc, int precision, int slices, int use3m) {
     int rc = validate(cplx, opA, opB, M, N, K, A, lda, B, ldb, C, ldc, precision);
@@ -647,29 +812,28 @@ int gemm_host(bool cplx, int opA, int opB, int64_t M, int64_t N, int64_t K, cons
     if (M == 0 || N == 0) return F4J_OK;
     rc = ensure_ready();
     if (rc != F4J_OK) return rc;
-    State &s = S();
-    std::lock_guard<std::mutex> g(s.mu);
+    std::unique_lock<std::mutex> lk;
+    Stage &g = stage_acquire(lk);
+    if ((rc = stage_setup(g)) != F4J_OK) return rc;
     const size_t es = cplx ? 16 : 8;
     const int64_t rA = opA == F4J_OP_N ? M : K, cA = opA == F4J_OP_N ? K : M;
     const int64_t rB = opB == F4J_OP_N ? K : N, cB = opB == F4J_OP_N ? N : K;
     if (K > 0) {
-        if ((rc = grow(s.bufA, (size_t)(rA * cA) * es)) != F4J_OK) return rc;
-        if ((rc = grow(s.bufB, (size_t)(rB * cB) * es)) != F4J_OK) return rc;
+        if ((rc = grow(g.A, (size_t)(rA * cA) * es)) != F4J_OK) return rc;
+        if ((rc = grow(g.B, (size_t)(rB * cB) * es)) != F4J_OK) return rc;
     }
-    if ((rc = grow(s.bufC, (size_t)(M * N) * es)) != F4J_OK) return rc;
+    if ((rc = grow(g.C, (size_t)(M * N) * es)) != F4J_OK) return rc;
     if (K > 0) {
-        F4J_HIP(hipMemcpy2DAsync(s.bufA.p, cA * es, A, lda * es, cA * es, rA, hipMemcpyHostToDevice, s.stream), "copy A");
-        F4J_HIP(hipMemcpy2DAsync(s.bufB.p, cB * es, B, ldb * es, cB * es, rB, hipMemcpyHostToDevice, s.stream), "copy B");
+        if ((rc = stage_in(g, g.B.p, B, rB, cB * es, ldb * es)) != F4J_OK) return rc;
+        if ((rc = stage_in(g, g.A.p, A, rA, cA * es, lda * es)) != F4J_OK) return rc;
     }
-    rc = gemm_dev(cplx, opA, opB, M, N, K, (const double *)s.bufA.p, cA, (const double *)s.bufB.p, cB,
-                  (double *)s.bufC.p, N, precision, slices, use3m, s.stream);
+    rc = gemm_dev(cplx, opA, opB, M, N, K, (const double *)g.A.p, cA, (const double *)g.B.p, cB, (double *)g.C.p, N,
+                  precision, slices, use3m, g.st);
     if (rc != F4J_OK) {
-        (void)hipStreamSynchronize(s.stream);
+        (void)hipStreamSynchronize(g.st);
         return rc;
     }
-    F4J_HIP(hipMemcpy2DAsync(C, ldc * es, s.bufC.p, N * es, N * es, M, hipMemcpyDeviceToHost, s.stream), "copy C");
-    F4J_HIP(hipStreamSynchronize(s.stream), "stream synchronize");
-    return F4J_OK;
+    return stage_out(g, C, ldc * es, g.C.p, M, N * es);
 }
 
 const char *prec_names[] = {"fp64", "strict", "split_bf16", "split_fp16", "split_int8"};
@@ -777,33 +941,32 @@ int mixed_dev(int mode, int64_t M, int64_t N, int64_t K, const double *A, int64_
     return F4J_OK;
 }
 
-// Host-array mixed product: stage A, B and C through the pooled device buffers.
+// Host-array mixed product: A, B and C move through a staging context.
 int mixed_host(int mode, int64_t M, int64_t N, int64_t K, const double *A, int64_t lda, const double *B, int64_t ldb,
                double *C, int64_t ldc, int precision, int slices) {
     int rc = validate(false, F4J_OP_N, F4J_OP_N, M, N, K, A, lda, B, ldb, C, ldc, precision);
     if (rc != F4J_OK || M == 0 || N == 0) return rc;
     if ((rc = ensure_ready()) != F4J_OK) return rc;
-    State &s = S();
-    std::lock_guard<std::mutex> g(s.mu);
+    std::unique_lock<std::mutex> lk;
+    Stage &g = stage_acquire(lk);
+    if ((rc = stage_setup(g)) != F4J_OK) return rc;
     const size_t ea = mode == 1 ? 16 : 8, eb = mode == 0 ? 16 : 8;
     if (K > 0) {
-        if ((rc = grow(s.bufA, (size_t)(M * K) * ea)) != F4J_OK) return rc;
-        if ((rc = grow(s.bufB, (size_t)(K * N) * eb)) != F4J_OK) return rc;
+        if ((rc = grow(g.A, (size_t)(M * K) * ea)) != F4J_OK) return rc;
+        if ((rc = grow(g.B, (size_t)(K * N) * eb)) != F4J_OK) return rc;
     }
-    if ((rc = grow(s.bufC, (size_t)(M * N) * 16)) != F4J_OK) return rc;
+    if ((rc = grow(g.C, (size_t)(M * N) * 16)) != F4J_OK) return rc;
     if (K > 0) {
-        F4J_HIP(hipMemcpy2DAsync(s.bufA.p, K * ea, A, lda * ea, K * ea, M, hipMemcpyHostToDevice, s.stream), "copy A");
-        F4J_HIP(hipMemcpy2DAsync(s.bufB.p, N * eb, B, ldb * eb, N * eb, K, hipMemcpyHostToDevice, s.stream), "copy B");
+        if ((rc = stage_in(g, g.B.p, B, K, N * eb, ldb * eb)) != F4J_OK) return rc;
+        if ((rc = stage_in(g, g.A.p, A, M, K * ea, lda * ea)) != F4J_OK) return rc;
     }
-    rc = mixed_dev(mode, M, N, K, (const double *)s.bufA.p, K, (const double *)s.bufB.p, N, (double *)s.bufC.p, N,
-                   precision, slices, s.stream);
+    rc = mixed_dev(mode, M, N, K, (const double *)g.A.p, K, (const double *)g.B.p, N, (double *)g.C.p, N, precision,
+                   slices, g.st);
     if (rc != F4J_OK) {
-        (void)hipStreamSynchronize(s.stream);
+        (void)hipStreamSynchronize(g.st);
         return rc;
     }
-    F4J_HIP(hipMemcpy2DAsync(C, ldc * 16, s.bufC.p, N * 16, N * 16, M, hipMemcpyDeviceToHost, s.stream), "copy C");
-    F4J_HIP(hipStreamSynchronize(s.stream), "stream synchronize");
-    return F4J_OK;
+    return stage_out(g, C, ldc * 16, g.C.p, M, N * 16);
 }
 
 // y = A x on the device (sizes and strides in elements).
@@ -848,33 +1011,31 @@ int validate_gemv(bool cplx, int64_t M, int64_t K, const void *A, int64_t lda, c
     return F4J_OK;
 }
 
-// Host-array y = A x: stage A, x and y through the pooled device buffers.
+// Host-array y = A x: A, x and y move through a staging context.
 int gemv_host(bool cplx, int64_t M, int64_t K, const double *A, int64_t lda, const double *x, int64_t incx, double *y,
               int64_t incy, int precision) {
     int rc = validate_gemv(cplx, M, K, A, lda, x, incx, y, incy, precision);
     if (rc != F4J_OK || M == 0) return rc;
     if ((rc = ensure_ready()) != F4J_OK) return rc;
-    State &s = S();
-    std::lock_guard<std::mutex> g(s.mu);
+    std::unique_lock<std::mutex> lk;
+    Stage &g = stage_acquire(lk);
+    if ((rc = stage_setup(g)) != F4J_OK) return rc;
     const size_t es = cplx ? 16 : 8;
     if (K > 0) {
-        if ((rc = grow(s.bufA, (size_t)(M * K) * es)) != F4J_OK) return rc;
-        if ((rc = grow(s.bufB, (size_t)K * es)) != F4J_OK) return rc;
+        if ((rc = grow(g.A, (size_t)(M * K) * es)) != F4J_OK) return rc;
+        if ((rc = grow(g.B, (size_t)K * es)) != F4J_OK) return rc;
     }
-    if ((rc = grow(s.bufC, (size_t)M * es)) != F4J_OK) return rc;
+    if ((rc = grow(g.C, (size_t)M * es)) != F4J_OK) return rc;
     if (K > 0) {
-        F4J_HIP(hipMemcpy2DAsync(s.bufA.p, K * es, A, lda * es, K * es, M, hipMemcpyHostToDevice, s.stream), "copy A");
-        F4J_HIP(hipMemcpy2DAsync(s.bufB.p, es, x, incx * es, es, K, hipMemcpyHostToDevice, s.stream), "copy x");
+        if ((rc = stage_in(g, g.B.p, x, K, es, incx * es)) != F4J_OK) return rc;
+        if ((rc = stage_in(g, g.A.p, A, M, K * es, lda * es)) != F4J_OK) return rc;
     }
-    rc = gemv_dev(cplx, M, K, (const double *)s.bufA.p, K, (const double *)s.bufB.p, 1, (double *)s.bufC.p, 1,
-                  precision, s.stream);
+    rc = gemv_dev(cplx, M, K, (const double *)g.A.p, K, (const double *)g.B.p, 1, (double *)g.C.p, 1, precision, g.st);
     if (rc != F4J_OK) {
-        (void)hipStreamSynchronize(s.stream);
+        (void)hipStreamSynchronize(g.st);
         return rc;
     }
-    F4J_HIP(hipMemcpy2DAsync(y, incy * es, s.bufC.p, es, es, M, hipMemcpyDeviceToHost, s.stream), "copy y");
-    F4J_HIP(hipStreamSynchronize(s.stream), "stream synchronize");
-    return F4J_OK;
+    return stage_out(g, y, incy * es, g.C.p, M, es);
 }
 
 int runtime_ready() { return ensure_ready(); }
@@ -902,7 +1063,11 @@ int f4j_shutdown(void) {
     F4J_HIP(hipSetDevice(s.device), "hipSetDevice");
     F4J_HIP(hipStreamSynchronize(s.stream), "stream synchronize");
     F4J_HIP(hipStreamSynchronize(s.side), "stream synchronize");
-    for (DevBuf *b : {&s.bufA, &s.bufB, &s.bufC, &s.ws, &s.mixws})
+    for (Stage &st : s.stages) {
+        std::lock_guard<std::mutex> sg(st.mu);
+        stage_release(st);
+    }
+    for (DevBuf *b : {&s.ws, &s.mixws})
         if (b->p) {
             F4J_HIP(hipFree(b->p), "hipFree");
             b->p = nullptr;
