@@ -134,8 +134,8 @@ def config_set(key, value):
 
 
 def config_get(key):
-    buf = ctypes.create_string_buffer(64)
-    check(load().f4j_config_get(key.encode(), buf, 64))
+    buf = ctypes.create_string_buffer(4096)
+    check(load().f4j_config_get(key.encode(), buf, 4096))
     return buf.value.decode()
 
 

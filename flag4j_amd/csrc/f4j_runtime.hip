@@ -51,6 +51,7 @@ struct Config {
     double split_tolerance = 1e-15;   // max |C - AB| / (|A||B|) accepted on the checked rows
     int split_fallback = 1;           // 1: recompute in FP64 when the check fails; 0: return F4J_ERR_ACCURACY
     int split_timing = 0;             // 1: time the slice-product launches with HIP events (report product_ms)
+    std::string tuner_cache;          // grouped-GEMM tuning cache file: read at init, written after tuning
 };
 
 // Scratch memory of the allocation-free entry (f4j_dgemm_dev_noalloc): the
@@ -155,6 +156,21 @@ int init_locked(State &s, int device) {
     return F4J_OK;
 }
 
+// Load the configured tuning cache, if the file exists (a missing file is the
+// first run; a malformed one leaves the cache as it was).
+void load_tuner_cache() {
+    std::string path;
+    {
+        std::lock_guard<std::mutex> g(S().cfg_mu);
+        path = S().cfg.tuner_cache;
+    }
+    if (path.empty()) return;
+    std::ifstream probe(path);
+    if (!probe) return;
+    const std::string keep = t_err;
+    if (f4j_tuner_load(path.c_str()) != F4J_OK) t_err = keep;
+}
+
 // Make sure the library is initialised and the calling thread's current
 // device is the library's device.
 int ensure_ready() {
@@ -164,6 +180,7 @@ int ensure_ready() {
         if (!s.inited) {
             const int rc = init_locked(s, 0);
             if (rc != F4J_OK) return rc;
+            load_tuner_cache();
         }
     }
     F4J_HIP(hipSetDevice(s.device), "hipSetDevice");
@@ -878,6 +895,8 @@ int config_set_locked(Config &c, const std::string &key, const std::string &valu
     } else if (key == "splitTiming") {
         if (value != "0" && value != "1") return fail(F4J_ERR_INVALID_ARG, "splitTiming: expected 0 or 1");
         c.split_timing = value == "1";
+    } else if (key == "tunerCache") {
+        c.tuner_cache = value;
     } else if (key == "defaultSlices") {
         const long v = std::strtol(value.c_str(), &end, 10);
         if (end == value.c_str() || *end || v < 0 || v > CRT_MAX_MODULI)
@@ -1053,7 +1072,10 @@ int f4j_abi_version(void) { return F4J_ABI_VERSION; }
 int f4j_init(int device) {
     State &s = S();
     std::lock_guard<std::mutex> g(s.mu);
-    return init_locked(s, device);
+    const bool was = s.inited;
+    const int rc = init_locked(s, device);
+    if (rc == F4J_OK && !was) load_tuner_cache();
+    return rc;
 }
 
 int f4j_shutdown(void) {
@@ -1267,8 +1289,13 @@ int f4j_last_split_report(f4j_split_report *out) {
 int f4j_config_set(const char *key, const char *value) {
     if (!key || !value) return fail(F4J_ERR_INVALID_ARG, "null key or value");
     State &s = S();
-    std::lock_guard<std::mutex> g(s.cfg_mu);
-    return config_set_locked(s.cfg, trim(key), trim(value));
+    int rc;
+    {
+        std::lock_guard<std::mutex> g(s.cfg_mu);
+        rc = config_set_locked(s.cfg, trim(key), trim(value));
+    }
+    if (rc == F4J_OK && trim(key) == "tunerCache") load_tuner_cache();   // a cache named after init is read now
+    return rc;
 }
 
 int f4j_config_get(const char *key, char *buf, size_t buflen) {
@@ -1289,6 +1316,7 @@ int f4j_config_get(const char *key, char *buf, size_t buflen) {
             v = b;
         } else if (k == "splitFallback") v = std::to_string(s.cfg.split_fallback);
         else if (k == "splitTiming") v = std::to_string(s.cfg.split_timing);
+        else if (k == "tunerCache") v = s.cfg.tuner_cache;
         else return fail(F4J_ERR_INVALID_ARG, "unknown configuration key '" + k + "'");
     }
     std::snprintf(buf, buflen, "%s", v.c_str());
@@ -1300,7 +1328,7 @@ int f4j_config_load(const char *path) {
     std::ifstream in(path);
     if (!in) return fail(F4J_ERR_INVALID_ARG, std::string("cannot open ") + path);
     State &s = S();
-    std::lock_guard<std::mutex> g(s.cfg_mu);
+    std::unique_lock<std::mutex> g(s.cfg_mu);
     Config c = s.cfg;       // all-or-nothing: apply only if every line parses
     std::string line;
     int lineno = 0;
@@ -1317,7 +1345,10 @@ int f4j_config_load(const char *path) {
             return rc;
         }
     }
+    const bool new_cache = c.tuner_cache != s.cfg.tuner_cache;
     s.cfg = c;
+    g.unlock();
+    if (new_cache) load_tuner_cache();
     return F4J_OK;
 }
 

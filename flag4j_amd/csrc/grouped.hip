@@ -468,6 +468,10 @@ int f4j_grouped_plan_tune(f4j_grouped_plan *plan, int reps) {
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    // persist the cache where the configuration names one (read back at the next init)
+    char path[4096];
+    if (f4j_config_get("tunerCache", path, sizeof(path)) == F4J_OK && path[0])
+        if (const int rc = f4j_tuner_save(path); rc != F4J_OK) return rc;
     return capture(*plan);
 }
 

@@ -268,7 +268,9 @@ int f4j_last_split_report(f4j_split_report *out);
  * tileConfig (-1 = per shape bucket, else a fixed FP64 tile id), splitCheckRows (rows of C the
  * FP64 residual check verifies after a split product; 0 = off), splitTolerance (largest accepted
  * |C - AB| / (|A||B|)), splitFallback (1 = recompute in FP64 when the check fails, 0 = return
- * F4J_ERR_ACCURACY), splitTiming (1 = report the slice-product time in f4j_split_report). */
+ * F4J_ERR_ACCURACY), splitTiming (1 = report the slice-product time in f4j_split_report),
+ * tunerCache (path of the grouped-GEMM tuning cache: read when the key is set and at f4j_init when
+ * the file exists, rewritten by f4j_grouped_plan_tune; empty = none). */
 int f4j_config_set(const char *key, const char *value);
 int f4j_config_get(const char *key, char *buf, size_t buflen);
 int f4j_config_load(const char *path);

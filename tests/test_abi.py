@@ -207,3 +207,27 @@ def test_grouped_plan_argument_checks():
                                      ctypes.byref(h)) == _native.ERR_INVALID_ARG
     assert L.f4j_grouped_plan_create(0, 0, 0, 1, None, None, None, None, None, None, None, None, None, 0,
                                      ctypes.byref(h)) == _native.ERR_INVALID_ARG
+
+
+def test_tuner_cache_config_key(tmp_path):
+    """tunerCache names a tuning cache that is read when the key is set (and at
+    initialisation): entries land in the process-wide cache; a missing file is a
+    first run, not an error; the key round-trips and loads from a properties file."""
+    L = _native.load()
+    _native.check(L.f4j_tuner_clear())
+    cache = tmp_path / "tune.txt"
+    cache.write_text("gfx950 1,1 0 8\n")
+    try:
+        _native.config_set("tunerCache", str(cache))
+        assert _native.config_get("tunerCache") == str(cache)
+        out = tmp_path / "out.txt"
+        _native.check(L.f4j_tuner_save(str(out).encode()))
+        assert "gfx950 1,1 0 8" in out.read_text().splitlines()
+        _native.config_set("tunerCache", str(tmp_path / "missing.txt"))      # not an error
+        props = tmp_path / "c.properties"
+        props.write_text(f"tunerCache = {cache}\n")
+        _native.check(L.f4j_config_load(str(props).encode()))
+        assert _native.config_get("tunerCache") == str(cache)
+    finally:
+        _native.config_set("tunerCache", "")
+        _native.check(L.f4j_tuner_clear())

@@ -117,3 +117,45 @@ def test_plan_replay_tune_and_refill():
     for i in (0, 17, 299):
         C = c1[oc[i]:oc[i + 1]].reshape(shapes[i][0], shapes[i][1])
         assert check_real(C, As[i], Bs[i])
+
+
+def test_tuner_cache_read_at_init_and_written_by_tune(tmp_path):
+    """A tuning cache named by tunerCache is read at f4j_init (a plan then takes
+    the cached tile configuration) and rewritten after f4j_grouped_plan_tune."""
+    As, Bs = rand_problems(410, 40, 16, 60)           # small products: the 64 x 64 bucket (0)
+    lib = L.load()
+    cache = tmp_path / "tune.txt"
+    cache.write_text("gfx950 0,0 0 7\n")
+    L.check(lib.f4j_tuner_clear())
+    try:
+        L.config_set("tunerCache", str(cache))
+        L.check(lib.f4j_shutdown())
+        L.check(lib.f4j_tuner_clear())
+        L.check(lib.f4j_init(0))                    # reads the cache
+        shapes = [(a.shape[0], b.shape[1], a.shape[1]) for a, b in zip(As, Bs)]
+        pa = np.concatenate([a.reshape(-1) for a in As])
+        pb = np.concatenate([b.reshape(-1) for b in Bs])
+        oa = np.cumsum([0] + [a.size for a in As])
+        ob = np.cumsum([0] + [b.size for b in Bs])
+        oc = np.cumsum([0] + [m * n for m, n, _ in shapes])
+        with device.DeviceBuffer.from_array(pa) as dA, device.DeviceBuffer.from_array(pb) as dB, \
+                device.DeviceBuffer(int(oc[-1]) * 8) as dC:
+            plan = grouped.GroupedPlan(shapes, [dA.ptr + 8 * int(x) for x in oa[:-1]],
+                                       [dB.ptr + 8 * int(x) for x in ob[:-1]],
+                                       [dC.ptr + 8 * int(x) for x in oc[:-1]],
+                                       [a.shape[1] for a in As], [b.shape[1] for b in Bs], [n for _, n, _ in shapes])
+            info = plan.info()
+            assert info["tiles"][0] > 0 and info["tile_cfg"][0] == 7
+            cache.write_text("# emptied\n")
+            plan.tune(reps=1)
+            lines = [l for l in cache.read_text().splitlines() if not l.startswith("#")]
+            assert any(l.startswith("gfx950 0,0 0 ") for l in lines), lines
+            plan.run()
+            device.sync()
+            C = dC.download(np.float64, int(oc[-1]))
+            plan.close()
+        for i in (0, 39):
+            assert check_real(C[oc[i]:oc[i + 1]].reshape(shapes[i][0], shapes[i][1]), As[i], Bs[i])
+    finally:
+        L.config_set("tunerCache", "")
+        L.check(lib.f4j_tuner_clear())
