@@ -408,7 +408,16 @@ __global__ __launch_bounds__(512, 2) void crt_gemm_kernel(CrtGemmArgs a, CrtTabl
 #pragma unroll
             for (int j = 0; j < 4; ++j)
                 fb[(ks + 1) & 1][j] = *(const i32x4 *)(lds + crt::b_frag(t, wave, j, ks + 1, lane));
-            mfmas(ks & 1);                        // (the compiler interleaves them with those reads)
+            mfmas(ks & 1);
+            // one fragment read after each of the first MFMAs: the reads of k step ks + 1
+            // then have the rest of the k step's MFMAs to arrive (2-3% over letting
+            // the compiler gather them after the MFMAs)
+#pragma unroll
+            for (int q = 0; q < crt::RB + 4; ++q) {
+                __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);     // MFMA
+                __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);     // LDS read
+            }
+            __builtin_amdgcn_sched_group_barrier(0x008, 4 * crt::RB - (crt::RB + 4), 0);
         }
         __builtin_amdgcn_sched_barrier(0);
         F4J_WAIT_VMCNT(0);                        // this wave's loads of step t + 1 are in LDS
@@ -416,14 +425,22 @@ __global__ __launch_bounds__(512, 2) void crt_gemm_kernel(CrtGemmArgs a, CrtTabl
         __builtin_amdgcn_s_barrier();
         __builtin_amdgcn_sched_barrier(0);
         issue(t + crt::STAGES, min(t + crt::STAGES, nt - 1));   // into stage t % STAGES
-        __builtin_amdgcn_sched_barrier(0);
-        // step t + 1's first fragments (past the last step: the repeated stage, never used)
+        // step t + 1's first fragments (past the last step: the repeated stage, never used),
+        // then step t's last k step; reads and MFMAs alternate, the loads go after the first MFMA
 #pragma unroll
         for (int i = 0; i < crt::RB; ++i) fa[0][i] = *(const i32x4 *)(lds + crt::a_frag(t + 1, wave, i, 0, lane));
 #pragma unroll
         for (int j = 0; j < 4; ++j) fb[0][j] = *(const i32x4 *)(lds + crt::b_frag(t + 1, wave, j, 0, lane));
+        mfmas((crt::KB - 1) & 1);
+#pragma unroll
+        for (int q = 0; q < crt::RB + 4; ++q) {
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 1);         // MFMA
+            __builtin_amdgcn_sched_group_barrier(0x100, 1, 1);         // LDS read
+        }
+        __builtin_amdgcn_sched_group_barrier(0x008, 1, 1);
+        __builtin_amdgcn_sched_group_barrier(0x020, crt::LOADS_A, 1);  // global loads into LDS
+        __builtin_amdgcn_sched_group_barrier(0x008, 4 * crt::RB - (crt::RB + 4) - 1, 1);
         __builtin_amdgcn_sched_barrier(0);
-        mfmas((crt::KB - 1) & 1);                 // step t's last k step, beside the barrier and reads
     }
     F4J_WAIT_VMCNT(0);                            // no LDS write in flight when the block ends
 
