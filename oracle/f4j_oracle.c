@@ -572,3 +572,142 @@ void f4o_lcg_fill_uniform(int64_t seed, int64_t n, double *out) {
     f4o_lcg_init(&r, seed);
     for (int64_t i = 0; i < n; i++) out[i] = 2.0 * f4o_lcg_next_double(&r) - 1.0;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Least squares, RealLstsqSolver.solve(A, B): reduced Householder QR       */
+/* (M:linalg/decompositions/unitary/RealUnitaryDecomposition.java,          */
+/* UnitaryDecomposition.java reduce(); M:linalg/decompositions/qr/RealQR.java;*/
+/* M:linalg/transformations/Householder.java:156-188), X = back-solve of     */
+/* R X = Q^H B (M:linalg/solvers/lstsq/LstsqSolver.java:160-178,             */
+/* M:linalg/solvers/exact/triangular/RealBackSolver.java).                   */
+/* ------------------------------------------------------------------------ */
+
+static double jmax(double a, double b) {      /* java.lang.Math.max (NaN wins) */
+    if (a != a || b != b) return NAN;
+    return a > b ? a : b;
+}
+
+/* Householder.leftMultReflector: src -= (alpha v)(v^T src) on rows
+ * [r0, r1) and columns [c0, ncols), as w = v^T src, w *= alpha, then the
+ * rank-one update, every product and sum rounded. */
+static void left_mult_reflector(double *src, int ncols, const double *v, double alpha, int c0, int r0, int r1,
+                                double *w) {
+    const double v0 = v[r0];
+    for (int i = c0; i < ncols; i++) w[i] = v0 * src[(int64_t)r0 * ncols + i];
+    for (int k = r0 + 1; k < r1; k++) {
+        const double rv = v[k];
+        const double *row = src + (int64_t)k * ncols;
+        for (int i = c0; i < ncols; i++) w[i] += rv * row[i];
+    }
+    for (int i = c0; i < ncols; i++) w[i] = w[i] * alpha;            /* RealOps.scalMult */
+    for (int i = r0; i < r1; i++) {
+        const double rv = v[i];
+        double *row = src + (int64_t)i * ncols;
+        for (int j = c0; j < ncols; j++) row[j] -= rv * w[j];
+    }
+}
+
+/* Reduced QR of A (M x N): T holds R above the diagonal and the reflectors
+ * below it, qf the reflector factors; Q (M x min(M, N)) is formed as getQ does. */
+void f4o_qr(int M, int N, const double *A, double *T, double *qf, double *Q) {
+    const int mn = M < N ? M : N, mx = M > N ? M : N;
+    double *v = (double *)calloc((size_t)mx, sizeof(double));
+    double *w = (double *)calloc((size_t)mx, sizeof(double));
+    memcpy(T, A, sizeof(double) * (size_t)M * N);
+    const double eps = 0x1.0p-53;                                  /* Flag4jConstants.EPS_F64 */
+    for (int j = 0; j < mn; j++) {
+        double maxAbs = 0.0;                                       /* findMaxAndInit */
+        for (int i = j; i < M; i++) {
+            v[i] = T[(int64_t)i * N + j];
+            maxAbs = jmax(fabs(v[i]), maxAbs);
+        }
+        double norm = 0.0, factor;
+        const int apply = maxAbs >= eps;
+        if (!apply) {
+            factor = 0.0;
+        } else {                                                   /* computePhasedNorm */
+            const double maxInv = 1.0 / maxAbs;
+            for (int i = j; i < M; i++) {
+                v[i] *= maxInv;
+                const double sv = v[i];
+                norm += sv * sv;
+            }
+            norm = sqrt(norm);
+            if (v[j] < 0) norm = -norm;
+            const double shift = v[j] + norm;
+            factor = shift / norm;
+            norm *= maxAbs;
+            const double invShift = 1.0 / shift;
+            v[j] = 1.0;
+            for (int i = j + 1; i < M; i++) v[i] *= invShift;
+        }
+        qf[j] = factor;
+        if (apply) {                                               /* updateData */
+            left_mult_reflector(T, N, v, qf[j], j, j, M, w);
+            T[(int64_t)j * N + j] = -norm;
+            for (int i = j + 1; i < M; i++) T[(int64_t)i * N + j] = v[i];
+        }
+    }
+    if (Q) {                                                       /* getQ, reduced: I(M, mn) */
+        memset(Q, 0, sizeof(double) * (size_t)M * mn);
+        for (int i = 0; i < mn; i++) Q[(int64_t)i * mn + i] = 1.0;
+        for (int j = mn - 1; j >= 0; j--) {
+            v[j] = 1.0;
+            for (int i = j + 1; i < M; i++) v[i] = T[(int64_t)i * N + j];
+            if (qf[j] != 0) left_mult_reflector(Q, mn, v, qf[j], j, j, M, w);
+        }
+    }
+    free(v);
+    free(w);
+}
+
+/* X (N x R) for A (M x N), B (M x R): 0 on success, 1 when the back solver
+ * reports a singular R (|det R| <= EPS_F64 max(N, N) or NaN; det is the product
+ * of R's diagonal, last entry first), 2 when M < N (R is not square). */
+int f4o_lstsq(int M, int N, int R, const double *A, const double *B, double *X) {
+    if (M < N) return 2;
+    double *T = (double *)malloc(sizeof(double) * (size_t)M * N);
+    double *qf = (double *)malloc(sizeof(double) * (size_t)N);
+    double *Q = (double *)malloc(sizeof(double) * (size_t)M * N);
+    double *Y = (double *)malloc(sizeof(double) * (size_t)N * (R > 0 ? R : 1));
+    f4o_qr(M, N, A, T, qf, Q);
+    /* Y = Q^H B: Matrix.mult through ReDeMatMultDispatcher (or the matrix-vector
+     * dispatch for one column): k-ascending sums, except one column above the
+     * small path, which takes the 64-block rule (RealDenseMatMult.java:458-491). */
+    const int small = (int64_t)N * M + (int64_t)M * R < 2450;
+    for (int i = 0; i < N; i++)
+        for (int c = 0; c < R; c++) {
+            if (R == 1 && !small) {
+                double dest = 0.0;
+                for (int kk = 0; kk < M; kk += 64) {
+                    double sum = 0.0;
+                    for (int k = kk; k < M && k < kk + 64; k++) sum += Q[(int64_t)k * N + i] * B[k];
+                    dest += sum;
+                }
+                Y[i] = dest;
+            } else {
+                double sum = 0.0;
+                for (int k = 0; k < M; k++) sum += Q[(int64_t)k * N + i] * B[(int64_t)k * R + c];
+                Y[(int64_t)i * R + c] = 0.0 + sum;
+            }
+        }
+    /* RealBackSolver.solve(U, B) with U = R (N x N, the upper part of T) */
+    const int n = N;
+    double det = T[(int64_t)(n - 1) * N + (n - 1)];
+    for (int i = n - 2; i >= 0; i--) det *= T[(int64_t)i * N + i];
+    for (int c = 0; c < R; c++) {
+        X[(int64_t)(n - 1) * R + c] = Y[(int64_t)(n - 1) * R + c] / T[(int64_t)(n - 1) * N + (n - 1)];
+        for (int i = n - 2; i >= 0; i--) {
+            double sum = 0.0;
+            for (int k = i + 1; k < n; k++) sum += T[(int64_t)i * N + k] * X[(int64_t)k * R + c];
+            X[(int64_t)i * R + c] = (Y[(int64_t)i * R + c] - sum) / T[(int64_t)i * N + i];
+        }
+    }
+    const double ad = fabs(det);
+    const int singular = ad <= 0x1.0p-53 * (double)n || ad != ad;
+    free(T);
+    free(qf);
+    free(Q);
+    free(Y);
+    return singular ? 1 : 0;
+}

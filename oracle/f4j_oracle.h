@@ -125,6 +125,13 @@ double   f4o_lcg_next_gaussian(f4o_lcg *r);
 /* Fill with 2r-1 (uniform [-1,1)). */
 void     f4o_lcg_fill_uniform(int64_t seed, int64_t n, double *out);
 
+/* Least squares (RealLstsqSolver.solve): reduced Householder QR of A (M x N)
+ * into T (R above the diagonal, reflectors below) and qf (factors), with Q
+ * (M x min(M, N)) when Q is not null; f4o_lstsq returns 0, 1 (singular R) or
+ * 2 (M < N) and writes X (N x R) on 0. */
+void f4o_qr(int M, int N, const double *A, double *T, double *qf, double *Q);
+int  f4o_lstsq(int M, int N, int R, const double *A, const double *B, double *X);
+
 #ifdef __cplusplus
 }
 #endif

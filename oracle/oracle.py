@@ -66,6 +66,8 @@ def lib():
         L.f4o_select_transpose.argtypes = [_i]
         L.f4o_real_entries.argtypes = [_i, _i, _i, _i, _i, _dp, _dp, _i, _i64p, _i64p, _dp]
         L.f4o_cplx_entries.argtypes = [_i, _i, _i, _i, _i, _dp, _dp, _i, _i64p, _i64p, _dp]
+        L.f4o_qr.argtypes = [_i, _i, _dp, _dp, _dp, _dp]
+        L.f4o_lstsq.argtypes = [_i, _i, _i, _dp, _dp, _dp]
         L.f4o_real_dot2.argtypes = [_i, _i, _i, _dp, _dp, _i, _i, _i64p, _i64p, _dp, _dp, _dp]
         L.f4o_cplx_dot2.argtypes = [_i, _i, _i, _i, _i, _dp, _dp, _i, _i64p, _i64p,
                                     _dp, _dp, _dp, _dp, _dp]
@@ -284,6 +286,28 @@ def cplx_entries(A, B, I, J, opA="N", opB="N"):
     lib().f4o_cplx_entries(OPS[opA], OPS[opB], M, N, K, _p(A.view(np.float64)), _p(B.view(np.float64)), len(I),
                            I.ctypes.data_as(_i64p), J.ctypes.data_as(_i64p), _p(out.view(np.float64)))
     return out
+
+
+# ---- least squares ----
+def qr(A):
+    """(T, qf, Q): reduced Householder QR as RealQR computes it."""
+    A = _f64(A)
+    M, N = A.shape
+    mn = min(M, N)
+    T, qf, Q = np.empty((M, N)), np.empty(mn), np.empty((M, mn))
+    lib().f4o_qr(M, N, _p(A), _p(T), _p(qf), _p(Q))
+    return T, qf, Q
+
+
+def lstsq(A, B):
+    """(status, X): RealLstsqSolver.solve(A, B); status 0 ok, 1 singular R, 2 M < N."""
+    A = _f64(A)
+    B2 = _f64(B.reshape(B.shape[0], -1))
+    M, N = A.shape
+    R = B2.shape[1]
+    X = np.empty((N, R))
+    st = lib().f4o_lstsq(M, N, R, _p(A), _p(B2), _p(X))
+    return st, X.reshape((N,) + B.shape[1:])
 
 
 # ---- accuracy oracle ----

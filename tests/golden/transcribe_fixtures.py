@@ -22,6 +22,7 @@ Test files read (T: = src/test/java/org/flag4j/):
   T:arrays/dense/complex_matrix/CMatrixMultTests.java:20-62 (CMatrix.mult(Matrix))
   T:arrays/dense/matrix/MatrixVectorTests.java:27-98 (Matrix.mult(Vector), Matrix.mult(CVector))
   T:arrays/dense/complex_matrix/CMatrixMatVecMultTests.java:22-100 (CMatrix.mult(Vector / CVector))
+  T:linalg/solvers/RealLstsqSolverTests.java:17-98 (RealLstsqSolver.solve, vector and matrix right-hand sides)
 """
 import json
 import os
@@ -78,6 +79,14 @@ SPECS = [
      {"a": ("aEntries", 0), "b": ("bEntries", 0), "c": ("expEntries", 0), "b_bad": ("bEntries", 1)}),
     ("cplx_api_matvec", "arrays/dense/complex_matrix/CMatrixMatVecMultTests.java", "matMultComplexTestCase",
      {"a": ("aEntries", 0), "b": ("bEntries", 0), "c": ("expEntries", 0), "b_bad": ("bEntries", 1)}),
+    ("lstsq_vec_1", "linalg/solvers/RealLstsqSolverTests.java", "solveTestCase",
+     {"a": ("aEntries", 0), "b": ("bEntries", 0), "x": ("expEntries", 0)}),
+    ("lstsq_vec_2", "linalg/solvers/RealLstsqSolverTests.java", "solveTestCase",
+     {"a": ("aEntries", 1), "b": ("bEntries", 1), "x": ("expEntries", 1)}),
+    ("lstsq_mat_1", "linalg/solvers/RealLstsqSolverTests.java", "solveTestMatrix",
+     {"a": ("aEntries", 0), "b": ("bEntries", 0), "x": ("expEntries", 0)}),
+    ("lstsq_mat_2", "linalg/solvers/RealLstsqSolverTests.java", "solveTestMatrix",
+     {"a": ("aEntries", 1), "b": ("bEntries", 1), "x": ("expEntries", 1)}),
 ]
 
 

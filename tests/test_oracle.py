@@ -491,3 +491,38 @@ def test_concurrent_standard_kernels(golden):
     Bz = B[:35] + 1j * B[35:]
     assert np.array_equal(o.cplx_mt_standard(Az, Bz, threads=3).view(np.float64),
                           o.cplx_gemm(Az, Bz).view(np.float64))
+
+
+@pytest.mark.parametrize("case", ["lstsq_vec_1", "lstsq_vec_2", "lstsq_mat_1", "lstsq_mat_2"])
+def test_lstsq_golden(golden, case):
+    """T:linalg/solvers/RealLstsqSolverTests.java:17-98 (assertEquals on Vector / Matrix is
+    Arrays.equals: bitwise)."""
+    g = golden[case]
+    A, B, exp = real_mat(g["a"]), real_mat(g["b"]), real_mat(g["x"])
+    st, X = o.lstsq(A, B)
+    assert st == 0
+    assert np.array_equal(bits(X.reshape(-1)), bits(exp.reshape(-1))), (X, exp)
+
+
+def test_lstsq_properties():
+    """Q from the restated QR is orthonormal and Q R = A (to rounding); the least-squares
+    residual is orthogonal to the columns of A; a rank-deficient A is reported singular,
+    a wide A (M < N) rejected."""
+    rng = np.random.default_rng(21)
+    A = rng.standard_normal((40, 12))
+    T, qf, Q = o.qr(A)
+    Rm = np.triu(T[:12])
+    assert np.allclose(Q.T @ Q, np.eye(12), atol=1e-14)
+    assert np.allclose(Q @ Rm, A, atol=1e-13)
+    B = rng.standard_normal((40, 3))
+    st, X = o.lstsq(A, B)
+    assert st == 0
+    assert np.allclose(A.T @ (A @ X - B), 0, atol=1e-12)
+    assert np.allclose(X, np.linalg.lstsq(A, B, rcond=None)[0], atol=1e-12)
+    A2 = A.copy()
+    A2[:, 5] = 0.0                                        # R[5][5] = 0: det R = 0
+    assert o.lstsq(A2, B)[0] == 1
+    A3 = A.copy()
+    A3[:, 5] = A3[:, 4]                                   # R[5][5] ~ u only: det R is far above eps n,
+    assert o.lstsq(A3, B)[0] == 0                         # so the back solver does not flag it
+    assert o.lstsq(A[:10], B[:10])[0] == 2
