@@ -18,6 +18,7 @@ ERR_HIP = -4
 ERR_NO_DEVICE = -5
 ERR_UNSUPPORTED = -6
 ERR_ACCURACY = -7
+ERR_SINGULAR = -8
 
 OP = {"N": 0, "T": 1, "C": 2}
 PREC = {"auto": -1, "fp64": 0, "strict": 1, "split_bf16": 2, "split_fp16": 3, "split_int8": 4}
@@ -34,7 +35,7 @@ class DeviceInfo(ctypes.Structure):
                 ("peak_hbm_gbs", ctypes.c_double), ("name", ctypes.c_char * 64), ("arch", ctypes.c_char * 32)]
 
 
-ABI_VERSION = 3          # F4J_ABI_VERSION the structures below match
+ABI_VERSION = 4          # F4J_ABI_VERSION the structures below match
 
 
 class SplitReport(ctypes.Structure):
@@ -67,6 +68,8 @@ SIGNATURES = {
     "f4j_dgemv_dev": ([_i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _i, _vp], _i),
     "f4j_zgemv": ([_i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _i], _i),
     "f4j_zgemv_dev": ([_i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _i, _vp], _i),
+    "f4j_dlstsq": ([_i64, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _i], _i),
+    "f4j_dlstsq_dev": ([_i64, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _i, _vp], _i),
     "f4j_grouped_plan_create": ([_i, _i, _i, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i,
                                  ctypes.POINTER(_vp)], _i),
     "f4j_grouped_plan_create_split": ([_i, _i, _i, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i,

@@ -28,6 +28,11 @@ class IllegalArgumentException(ValueError):
     """java.lang.IllegalArgumentException as thrown by M:util/ValidateParameters.java."""
 
 
+class SingularMatrixException(ArithmeticError):
+    """M:util/exceptions/SingularMatrixException.java: thrown by the back solver when the
+    coefficient matrix is singular or near singular."""
+
+
 class Shape:
     """Row-major shape (M:arrays/Shape.java:98-105, :188-193, :397-412)."""
 
@@ -399,3 +404,42 @@ class CVector(_DenseVector):
 
     __eq__ = equals
     __hash__ = None
+
+
+class RealLstsqSolver:
+    """RealLstsqSolver (M:linalg/solvers/lstsq/RealLstsqSolver.java:131, LstsqSolver.java:157-181):
+    X minimising ||A X - B|| through the reduced Householder QR of A and back substitution,
+    run on the GPU in the reference's arithmetic (f4j_dlstsq).  ``precision`` applies to the
+    product Q^H B only ("auto" as Matrix.mult; "strict" makes the whole solve bitwise equal
+    to the reference's).
+
+    Errors as the reference raises them: LinearAlgebraException when B's rows differ from A's
+    (Qh.mult(B)), IllegalArgumentException when A has more columns than rows (R not square),
+    SingularMatrixException when |det R| <= 2^-53 n or det R is NaN."""
+
+    def __init__(self, precision="auto"):
+        self.precision = precision
+
+    def solve(self, A, b):
+        if not isinstance(A, Matrix):
+            raise TypeError("A must be a Matrix")
+        M, N = A.numRows, A.numCols
+        vector = isinstance(b, Vector)
+        if not vector and not isinstance(b, Matrix):
+            raise TypeError("b must be a Vector or a Matrix")
+        rows = b.size if vector else b.numRows
+        R = 1 if vector else b.numCols
+        mn = min(M, N)
+        if rows != M:                                      # Qh (mn x M) . b
+            raise LinearAlgebraException(_mat_mult_shape_err(Shape(mn, M), Shape(rows, R)))
+        if M < N or N == 0:                                # ensureSquare(R)
+            raise IllegalArgumentException(f"Expecting matrix to be square but got shape {Shape(mn, N)}.")
+        a = np.ascontiguousarray(A.data, dtype=np.float64).reshape(M, N)
+        bb = np.ascontiguousarray(b.data, dtype=np.float64).reshape(M, R)
+        x = np.empty((N, R))
+        lib = _native.load()
+        st = lib.f4j_dlstsq(M, N, R, _ptr(a), N, _ptr(bb), R, _ptr(x), R, _native.PREC[self.precision])
+        if st == _native.ERR_SINGULAR:
+            raise SingularMatrixException("Could not solve system.")
+        _native.check(st)
+        return Vector(x.reshape(-1)) if vector else Matrix(x)

@@ -99,6 +99,8 @@ struct State {
     DevBuf ws;
     std::mutex mix_mu;                // guards mixws (planes of complex x real products)
     DevBuf mixws;
+    std::mutex ls_mu;                 // guards lsws (least-squares factors and products)
+    DevBuf lsws;
     hipEvent_t tev[8] = {};                 // product timing: a begin / end pair per real product (up to 4)
     hipStream_t side = nullptr;             // B's split and the residual sums beside A's split and the
                                             // modular products (guarded by ws_mu)
@@ -1057,6 +1059,96 @@ int gemv_host(bool cplx, int64_t M, int64_t K, const double *A, int64_t lda, con
     return stage_out(g, y, incy * es, g.C.p, M, es);
 }
 
+int validate_lstsq(int64_t M, int64_t N, int64_t R, const void *A, int64_t lda, const void *B, int64_t ldb,
+                   const void *X, int64_t ldx, int precision) {
+    if (M < 0 || N < 0 || R < 0) return fail(F4J_ERR_INVALID_ARG, "negative dimension");
+    if (precision < F4J_PREC_AUTO || precision > F4J_PREC_SPLIT_INT8)
+        return fail(F4J_ERR_INVALID_ARG, "invalid precision code " + std::to_string(precision));
+    // R of the reduced QR is min(M, N) x N: the back solver needs it square
+    // (ValidateParameters.ensureSquare), and an empty one has no det.
+    if (N == 0 || M < N)
+        return fail(F4J_ERR_SHAPE, "least squares needs M >= N >= 1 (R of the reduced QR of a " + std::to_string(M) +
+                                       "x" + std::to_string(N) + " matrix must be square and non-empty)");
+    if (!A) return fail(F4J_ERR_INVALID_ARG, "A is null");
+    if (lda < N) return fail(F4J_ERR_INVALID_ARG, "lda < N");
+    // the update and transpose grids take 32 rows per workgroup row, at most 65535 of them
+    if (M > (int64_t)65535 * 32)
+        return fail(F4J_ERR_UNSUPPORTED, "least squares supports at most 2097120 rows");
+    if (R > 0) {
+        if (!B || !X) return fail(F4J_ERR_INVALID_ARG, "B or X is null");
+        if (ldb < R || ldx < R) return fail(F4J_ERR_INVALID_ARG, "ldb or ldx < R");
+    }
+    return F4J_OK;
+}
+
+// Least squares on device operands (RealLstsqSolver.solve): QR of A, Q, Qh = Q^T
+// materialised as the reference's Q.H(), Y = Qh B through gemm_dev at
+// `precision`, back substitution.  Waits for `stream` to check det R.
+int lstsq_dev(int64_t M, int64_t N, int64_t R, const double *A, int64_t lda, const double *B, int64_t ldb, double *X,
+              int64_t ldx, int precision, hipStream_t stream) {
+    State &s = S();
+    std::lock_guard<std::mutex> lk(s.ls_mu);
+    // T, Q, Qh, v, w, qf, diag, Y, det; each part starts on a 256-byte boundary
+    const int64_t parts[9] = {M * N, M * N, N * M, M, N, N, N, N * R, 1};
+    int64_t off[9], total = 0;
+    for (int i = 0; i < 9; ++i) {
+        off[i] = total;
+        total += (parts[i] + 31) & ~int64_t(31);
+    }
+    int rc = grow(s.lsws, (size_t)total * sizeof(double));
+    if (rc != F4J_OK) return rc;
+    double *base = (double *)s.lsws.p;
+    double *T = base + off[0], *Q = base + off[1], *Qh = base + off[2], *v = base + off[3], *w = base + off[4],
+           *qf = base + off[5], *diag = base + off[6], *Y = base + off[7], *det = base + off[8];
+    F4J_HIP(hipMemcpy2DAsync(T, N * sizeof(double), A, lda * sizeof(double), N * sizeof(double), M,
+                             hipMemcpyDeviceToDevice, stream),
+            "copy of A");
+    F4J_HIP(launch_householder_qr(T, M, N, N, v, w, qf, diag, stream), "Householder QR launch");
+    F4J_HIP(launch_householder_q(T, M, N, N, qf, Q, N, w, stream), "Q launch");
+    F4J_HIP(launch_transpose(Q, M, N, N, Qh, M, stream), "transpose launch");
+    if (R > 0) {
+        rc = gemm_dev(false, F4J_OP_N, F4J_OP_N, N, R, M, Qh, M, B, ldb, Y, R, precision, 0, 0, stream);
+        if (rc != F4J_OK) return rc;
+    }
+    F4J_HIP(launch_back_solve(T, N, N, Y, R, X, ldx, R, det, stream), "back substitution launch");
+    double d = 0.0;
+    F4J_HIP(hipMemcpyAsync(&d, det, sizeof(double), hipMemcpyDeviceToHost, stream), "copy of det R");
+    F4J_HIP(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    const double ad = std::fabs(d);
+    // RealBackSolver / BackSolver.checkSingular: |det| <= EPS_F64 max(rows, cols), or NaN
+    if (ad <= 0x1.0p-53 * (double)N || std::isnan(ad)) return fail(F4J_ERR_SINGULAR, "Could not solve system.");
+    return F4J_OK;
+}
+
+// Host-array least squares: A and B move in through a staging context, X out.
+int lstsq_host(int64_t M, int64_t N, int64_t R, const double *A, int64_t lda, const double *B, int64_t ldb, double *X,
+               int64_t ldx, int precision) {
+    int rc = validate_lstsq(M, N, R, A, lda, B, ldb, X, ldx, precision);
+    if (rc != F4J_OK) return rc;
+    if ((rc = ensure_ready()) != F4J_OK) return rc;
+    std::unique_lock<std::mutex> lk;
+    Stage &g = stage_acquire(lk);
+    if ((rc = stage_setup(g)) != F4J_OK) return rc;
+    const int64_t Rs = std::max<int64_t>(R, 1);
+    if ((rc = grow(g.A, (size_t)(M * N) * sizeof(double))) != F4J_OK) return rc;
+    if ((rc = grow(g.B, (size_t)(M * Rs) * sizeof(double))) != F4J_OK) return rc;
+    if ((rc = grow(g.C, (size_t)(N * Rs) * sizeof(double))) != F4J_OK) return rc;
+    if ((rc = stage_in(g, g.A.p, A, M, N * sizeof(double), lda * sizeof(double))) != F4J_OK) return rc;
+    if (R > 0 && (rc = stage_in(g, g.B.p, B, M, R * sizeof(double), ldb * sizeof(double))) != F4J_OK) return rc;
+    rc = lstsq_dev(M, N, R, (const double *)g.A.p, N, (const double *)g.B.p, Rs, (double *)g.C.p, Rs, precision, g.st);
+    if (rc != F4J_OK && rc != F4J_ERR_SINGULAR) {
+        (void)hipStreamSynchronize(g.st);
+        return rc;
+    }
+    if (R > 0) {
+        const std::string msg = t_err;
+        const int rc_out = stage_out(g, X, ldx * sizeof(double), g.C.p, N, R * sizeof(double));
+        if (rc_out != F4J_OK) return rc_out;
+        t_err = msg;
+    }
+    return rc;
+}
+
 int runtime_ready() { return ensure_ready(); }
 
 hipStream_t runtime_stream(void *stream) { return pick_stream(stream); }
@@ -1265,6 +1357,19 @@ int f4j_dgemv_dev(int64_t M, int64_t K, const double *A, int64_t lda, const doub
     if (rc != F4J_OK) return rc;
     if ((rc = ensure_ready()) != F4J_OK) return rc;
     return gemv_dev(false, M, K, A, lda, x, incx, y, incy, precision, pick_stream(stream));
+}
+
+int f4j_dlstsq(int64_t M, int64_t N, int64_t R, const double *A, int64_t lda, const double *B, int64_t ldb, double *X,
+               int64_t ldx, int precision) {
+    return lstsq_host(M, N, R, A, lda, B, ldb, X, ldx, precision);
+}
+
+int f4j_dlstsq_dev(int64_t M, int64_t N, int64_t R, const double *A, int64_t lda, const double *B, int64_t ldb,
+                   double *X, int64_t ldx, int precision, void *stream) {
+    int rc = validate_lstsq(M, N, R, A, lda, B, ldb, X, ldx, precision);
+    if (rc != F4J_OK) return rc;
+    if ((rc = ensure_ready()) != F4J_OK) return rc;
+    return lstsq_dev(M, N, R, A, lda, B, ldb, X, ldx, precision, pick_stream(stream));
 }
 
 int f4j_zgemv(int64_t M, int64_t K, const double *A, int64_t lda, const double *x, int64_t incx, double *y,

@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define F4J_ABI_VERSION 3
+#define F4J_ABI_VERSION 4
 
 typedef enum f4j_status {
     F4J_OK = 0,
@@ -41,7 +41,8 @@ typedef enum f4j_status {
     F4J_ERR_HIP = -4,             /* a HIP runtime call failed */
     F4J_ERR_NO_DEVICE = -5,
     F4J_ERR_UNSUPPORTED = -6,     /* valid request this build does not implement */
-    F4J_ERR_ACCURACY = -7         /* split-precision residual check failed */
+    F4J_ERR_ACCURACY = -7,        /* split-precision residual check failed */
+    F4J_ERR_SINGULAR = -8         /* least squares: R is singular (Flag4j: SingularMatrixException) */
 } f4j_status;
 
 typedef enum f4j_op { F4J_OP_N = 0, F4J_OP_T = 1, F4J_OP_C = 2 } f4j_op;
@@ -188,6 +189,29 @@ int f4j_zgemv(int64_t M, int64_t K, const double *A, int64_t lda, const double *
               double *y, int64_t incy, int precision);
 int f4j_zgemv_dev(int64_t M, int64_t K, const double *A, int64_t lda, const double *x, int64_t incx,
                   double *y, int64_t incy, int precision, void *stream);
+
+/* ---- Least squares: a caller of the product ----
+ * Replaces RealLstsqSolver.solve(A, B) and solve(A, b)
+ *   M:linalg/solvers/lstsq/LstsqSolver.java:157-181 (solve), :232-236 (decompose: Qh = Q.H(), R);
+ *   M:linalg/decompositions/unitary/UnitaryDecomposition.java (decompose) and
+ *   M:linalg/decompositions/unitary/RealUnitaryDecomposition.java (computeHouseholder,
+ *   updateData, getQ :147-168): reduced Householder QR;
+ *   M:linalg/transformations/Householder.java:156-188 (leftMultReflector);
+ *   M:linalg/solvers/exact/triangular/RealBackSolver.java (solve; BackSolver.java:111-134 checks).
+ * X (N x R, ldx) = argmin ||A X - B|| for A (M x N, lda) and B (M x R, ldb), M >= N >= 1; R = 1
+ * is the Vector form.  The QR of A, Q and the back substitution run on the device in the
+ * reference's order, bit for bit.  Y = Qh B is a library product at `precision`: with AUTO it
+ * dispatches as f4j_dgemm does (R = 1 and the small path run the reference's order; larger
+ * products run the configured default, FP64 MFMA), with STRICT the whole solve equals the
+ * reference's bit for bit.  F4J_ERR_SHAPE: M < N or N = 0 (R must be square: ensureSquare).
+ * F4J_ERR_SINGULAR: |det R| <= 2^-53 N or det R is NaN, det R being the product of R's diagonal
+ * from the last entry up (checkSingular); X then holds the back substitution's values.  The _dev
+ * form waits for `stream` before it returns (the check reads det R).  Library workspace: about
+ * 3 M N + N R doubles, grown and reused like the staging buffers. */
+int f4j_dlstsq(int64_t M, int64_t N, int64_t R, const double *A, int64_t lda, const double *B, int64_t ldb,
+               double *X, int64_t ldx, int precision);
+int f4j_dlstsq_dev(int64_t M, int64_t N, int64_t R, const double *A, int64_t lda, const double *B, int64_t ldb,
+                   double *X, int64_t ldx, int precision, void *stream);
 
 /* ---- grouped (batched) products ----
  * No reference counterpart: Flag4j multiplies one pair per call; a batch is a Java loop of

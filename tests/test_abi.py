@@ -30,7 +30,7 @@ def test_exports_every_declared_symbol():
 
 
 def test_abi_version():
-    assert _native.load().f4j_abi_version() == 3
+    assert _native.load().f4j_abi_version() == 4
 
 
 def _dgemm(*args):
@@ -50,6 +50,38 @@ def test_invalid_arguments_rejected_before_device_use():
     assert b"ldc" in _native.load().f4j_last_error()
     assert _dgemm(0, 0, 2, 2, 2, p, 2, p, 2, p, 2, 9, 0) == L.ERR_INVALID_ARG      # precision
     assert _native.load().f4j_zgemm(0, 3, 2, 2, 2, p, 2, p, 2, p, 2, 0, 0, 0) == L.ERR_INVALID_ARG
+
+
+def test_lstsq_arguments_rejected_before_device_use():
+    """f4j_dlstsq validates before it touches the device: R of the reduced QR must be
+    square (ensureSquare, as RealBackSolver requires: M >= N >= 1), sizes non-negative,
+    pointers present, leading dimensions large enough, the precision code known."""
+    a = np.zeros(64)
+    p = ctypes.c_void_p(a.ctypes.data)
+    L = _native
+    f = _native.load().f4j_dlstsq
+    assert f(3, 4, 1, p, 4, p, 1, p, 1, -1) == L.ERR_SHAPE                 # M < N
+    assert f(3, 0, 1, p, 1, p, 1, p, 1, -1) == L.ERR_SHAPE                 # N = 0
+    assert b"square" in _native.load().f4j_last_error()
+    assert f(-1, 2, 1, p, 2, p, 1, p, 1, -1) == L.ERR_INVALID_ARG
+    assert f(4, 2, 1, None, 2, p, 1, p, 1, -1) == L.ERR_INVALID_ARG        # null A
+    assert f(4, 2, 1, p, 1, p, 1, p, 1, -1) == L.ERR_INVALID_ARG           # lda < N
+    assert f(4, 2, 3, p, 2, p, 2, p, 3, -1) == L.ERR_INVALID_ARG           # ldb < R
+    assert f(4, 2, 1, p, 2, None, 1, p, 1, -1) == L.ERR_INVALID_ARG        # null B
+    assert f(4, 2, 1, p, 2, p, 1, p, 1, 9) == L.ERR_INVALID_ARG            # precision
+    assert _native.load().f4j_dlstsq_dev(3, 4, 1, p, 4, p, 1, p, 1, -1, None) == L.ERR_SHAPE
+
+
+def test_lstsq_solver_errors_match_reference_exceptions():
+    """RealLstsqSolver.solve: Qh.mult(b) with b's rows != A's rows -> LinearAlgebraException;
+    a wide A leaves R non-square -> IllegalArgumentException (ValidateParameters.ensureSquare)."""
+    s = f4j.RealLstsqSolver()
+    with pytest.raises(f4j.LinearAlgebraException):
+        s.solve(f4j.Matrix(np.ones((5, 3))), f4j.Vector(np.ones(4)))
+    with pytest.raises(f4j.LinearAlgebraException):
+        s.solve(f4j.Matrix(np.ones((5, 3))), f4j.Matrix(np.ones((3, 2))))
+    with pytest.raises(f4j.IllegalArgumentException):
+        s.solve(f4j.Matrix(np.ones((3, 5))), f4j.Vector(np.ones(3)))
 
 
 def test_empty_products_need_no_device():
@@ -178,6 +210,23 @@ def test_strict_kernel_has_no_fused_multiply_add():
     assert "v_fma_f64" not in asm
     gemv = open(os.path.join(ROOT, "_build", "obj", "gemv.s")).read()
     assert "gemv_dma_kernel" in gemv and "v_fma_f64" not in gemv
+    # The least-squares kernels: fused multiply-adds only inside the correctly rounded
+    # division (v_rcp_f64 .. v_div_fixup_f64) and square root (v_rsq_f64 .. v_ldexp_f64)
+    # expansions, never in the reference's products and sums.
+    lines = open(os.path.join(ROOT, "_build", "obj", "lstsq.s")).read().splitlines()
+    depth, outside, kernels = 0, [], set()
+    for ln in lines:
+        t = ln.strip()
+        if t.startswith("_ZN3f4j") and ":" in t:
+            kernels.add(t.split(":")[0])
+        if t.startswith("v_rcp_f64") or t.startswith("v_rsq_f64"):
+            depth += 1
+        elif (t.startswith("v_div_fixup_f64") or t.startswith("v_ldexp_f64")) and depth > 0:
+            depth -= 1
+        elif depth == 0 and (t.startswith("v_fma_f64") or t.startswith("v_fmac_f64")):
+            outside.append(t)
+    assert any("hh_dot_kernel" in k for k in kernels) and any("back_solve_kernel" in k for k in kernels)
+    assert not outside, outside
 
 
 def test_tuner_cache_file_roundtrip(tmp_path):
