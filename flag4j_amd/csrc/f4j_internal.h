@@ -98,16 +98,17 @@ hipError_t launch_zero_fill(double *C, int64_t rows, int64_t cols, int64_t ld, h
 // holds A on entry and, after launch_householder_qr, R above the diagonal and the
 // reflectors below it; v (M), w (N) are work vectors, qf (min(M, N)) the reflector
 // factors, diag (min(M, N)) R's diagonal.  launch_householder_q forms the reduced
-// Q (M x min(M, N), ld ldq) from T and qf.  launch_back_solve solves R X = Y
-// (R the upper n x n part of T; Y, X n x R) and writes det R to `det`.
+// Q (M x min(M, N), ld ldq) from T and qf (v, w: work vectors as above).
+// launch_back_solve solves R X = Y (R the upper n x n part of T; Y, X n x R; xs:
+// R n doubles of work space) and writes det R to `det`.
 hipError_t launch_householder_qr(double *T, int64_t M, int64_t N, int64_t ldt, double *v, double *w, double *qf,
                                  double *diag, hipStream_t stream);
 hipError_t launch_householder_q(const double *T, int64_t M, int64_t N, int64_t ldt, const double *qf, double *Q,
-                                int64_t ldq, double *w, hipStream_t stream);
+                                int64_t ldq, double *v, double *w, hipStream_t stream);
 hipError_t launch_transpose(const double *Q, int64_t M, int64_t N, int64_t ldq, double *Qh, int64_t ldh,
                             hipStream_t stream);
 hipError_t launch_back_solve(const double *T, int64_t ldt, int64_t n, const double *Y, int64_t ldy, double *X,
-                             int64_t ldx, int64_t R, double *det, hipStream_t stream);
+                             int64_t ldx, int64_t R, double *xs, double *det, hipStream_t stream);
 
 void set_error(const std::string &msg);
 

@@ -1088,10 +1088,10 @@ int lstsq_dev(int64_t M, int64_t N, int64_t R, const double *A, int64_t lda, con
               int64_t ldx, int precision, hipStream_t stream) {
     State &s = S();
     std::lock_guard<std::mutex> lk(s.ls_mu);
-    // T, Q, Qh, v, w, qf, diag, Y, det; each part starts on a 256-byte boundary
-    const int64_t parts[9] = {M * N, M * N, N * M, M, N, N, N, N * R, 1};
-    int64_t off[9], total = 0;
-    for (int i = 0; i < 9; ++i) {
+    // T, Q, Qh, v, w, qf, diag, Y, det, xs; each part starts on a 256-byte boundary
+    const int64_t parts[10] = {M * N, M * N, N * M, M, N, N, N, N * R, 1, N * R};
+    int64_t off[10], total = 0;
+    for (int i = 0; i < 10; ++i) {
         off[i] = total;
         total += (parts[i] + 31) & ~int64_t(31);
     }
@@ -1099,18 +1099,18 @@ int lstsq_dev(int64_t M, int64_t N, int64_t R, const double *A, int64_t lda, con
     if (rc != F4J_OK) return rc;
     double *base = (double *)s.lsws.p;
     double *T = base + off[0], *Q = base + off[1], *Qh = base + off[2], *v = base + off[3], *w = base + off[4],
-           *qf = base + off[5], *diag = base + off[6], *Y = base + off[7], *det = base + off[8];
+           *qf = base + off[5], *diag = base + off[6], *Y = base + off[7], *det = base + off[8], *xs = base + off[9];
     F4J_HIP(hipMemcpy2DAsync(T, N * sizeof(double), A, lda * sizeof(double), N * sizeof(double), M,
                              hipMemcpyDeviceToDevice, stream),
             "copy of A");
     F4J_HIP(launch_householder_qr(T, M, N, N, v, w, qf, diag, stream), "Householder QR launch");
-    F4J_HIP(launch_householder_q(T, M, N, N, qf, Q, N, w, stream), "Q launch");
+    F4J_HIP(launch_householder_q(T, M, N, N, qf, Q, N, v, w, stream), "Q launch");
     F4J_HIP(launch_transpose(Q, M, N, N, Qh, M, stream), "transpose launch");
     if (R > 0) {
         rc = gemm_dev(false, F4J_OP_N, F4J_OP_N, N, R, M, Qh, M, B, ldb, Y, R, precision, 0, 0, stream);
         if (rc != F4J_OK) return rc;
     }
-    F4J_HIP(launch_back_solve(T, N, N, Y, R, X, ldx, R, det, stream), "back substitution launch");
+    F4J_HIP(launch_back_solve(T, N, N, Y, R, X, ldx, R, xs, det, stream), "back substitution launch");
     double d = 0.0;
     F4J_HIP(hipMemcpyAsync(&d, det, sizeof(double), hipMemcpyDeviceToHost, stream), "copy of det R");
     F4J_HIP(hipStreamSynchronize(stream), "hipStreamSynchronize");

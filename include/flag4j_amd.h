@@ -207,7 +207,7 @@ int f4j_zgemv_dev(int64_t M, int64_t K, const double *A, int64_t lda, const doub
  * F4J_ERR_SINGULAR: |det R| <= 2^-53 N or det R is NaN, det R being the product of R's diagonal
  * from the last entry up (checkSingular); X then holds the back substitution's values.  The _dev
  * form waits for `stream` before it returns (the check reads det R).  Library workspace: about
- * 3 M N + N R doubles, grown and reused like the staging buffers. */
+ * 3 M N + 2 N R doubles, grown and reused like the staging buffers. */
 int f4j_dlstsq(int64_t M, int64_t N, int64_t R, const double *A, int64_t lda, const double *B, int64_t ldb,
                double *X, int64_t ldx, int precision);
 int f4j_dlstsq_dev(int64_t M, int64_t N, int64_t R, const double *A, int64_t lda, const double *B, int64_t ldb,
