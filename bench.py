@@ -428,7 +428,38 @@ def bench_single_configs(np, torch, f4j, rng, dev, stream, T, cpu=True):
     out["real_16384x4096x16384"] = e
     del dA, dB, dC
     torch.cuda.empty_cache()
+    out["lstsq_4096x1024x16"] = bench_lstsq(np, torch, f4j, rng, dev, stream, cpu)
     return out
+
+
+def bench_lstsq(np, torch, f4j, rng, dev, stream, cpu=True, M=4096, N=1024, R=16):
+    """RealLstsqSolver.solve(A, B), a caller of the product: Householder QR, Q, Q^H B and the
+    back substitution on the device (f4j_dlstsq_dev, AUTO precision), beside the oracle's
+    single-threaded restatement of the same algorithm on this host."""
+    from oracle import oracle as o
+    A, B = rng.uniform(31, M, N), rng.uniform(32, M, R)
+    dA, dB = torch.from_numpy(A).to(dev), torch.from_numpy(B).to(dev)
+    dX = torch.empty((N, R), dtype=torch.float64, device=dev)
+    L = f4j._native.load()
+    sh = stream.cuda_stream
+    ms = time_dev(torch, stream, lambda: f4j._native.check(
+        L.f4j_dlstsq_dev(M, N, R, dA.data_ptr(), N, dB.data_ptr(), R, dX.data_ptr(), R, f4j._native.PREC["auto"],
+                         sh)), 3)
+    # the algorithm's flops: QR and Q each 2 M N^2 - 2 N^3 / 3, Q^H B 2 M N R, back substitution N^2 R
+    fl = 2 * (2.0 * M * N * N - 2.0 * N ** 3 / 3) + 2.0 * M * N * R + 1.0 * N * N * R
+    e = {"workload": f"RealLstsqSolver.solve, A {M}x{N}, B {M}x{R}, entries 2r-1 (seeds 31, 32)",
+         "ms": round(ms, 3), "gflops_algorithmic": round(fl / (ms * 1e-3) / 1e9, 1)}
+    if cpu:
+        X = dX.cpu().numpy()
+        t0 = time.perf_counter()
+        st, OX = o.lstsq(A, B)
+        t = time.perf_counter() - t0
+        e["max_rel_diff_vs_oracle"] = float(np.max(np.abs(X - OX)) / np.max(np.abs(OX)))
+        e["cpu_baseline"] = {"value": round(t * 1e3, 1), "unit": "ms per solve", "cores": 1, "kind": "port",
+                             "sample": "whole solve, once: the oracle's restatement of RealQR, getQ, "
+                                       "Matrix.mult and RealBackSolver (single-threaded C)"}
+    del dA, dB, dX
+    return e
 
 
 def bench_gemv(np, torch, f4j, dA, x, stream, reps=50):
@@ -590,6 +621,24 @@ def main():
                 "frac": round(achieved / peak, 4), "traffic": load_traffic(f"{prof}_{n}_pmc.txt"),
                 "kernel_ms": round(kern_ms, 4)}
 
+    def vendor_int8():
+        """The vendor library on the same work as crt_gemm_kernel's launch: 8192^3 INT8
+        products with INT32 results through torch._int_mm (hipBLASLt on ROCm), B stored
+        N x K as the residue planes are; the launch holds `products` such products."""
+        if "split_int8" not in timings:
+            return None
+        Ai = torch.randint(-128, 128, (n, n), dtype=torch.int8, device=dev)
+        Bi = torch.randint(-128, 128, (n, n), dtype=torch.int8, device=dev)
+        ms_i = time_dev(torch, torch.cuda.current_stream(), lambda: torch._int_mm(Ai, Bi.t()), 10)
+        del Ai, Bi
+        tops = 2.0 * n ** 3 / (ms_i * 1e-3) / 1e12
+        return {"library": "hipBLASLt (torch._int_mm)", "ms_per_product": round(ms_i, 4),
+                "ms_for_launch_products": round(ms_i * timings["split_int8"][1], 3), "achieved": round(tops, 1),
+                "unit": "TOP/s", "frac": round(tops / PEAK_INT8_TOPS, 4)}
+
+    int8_vendor = vendor_int8()
+    log(f"[bench] vendor INT8: {int8_vendor}")
+
     if args.profile_copy:
         src = torch.empty(64 * 1024 * 1024, dtype=torch.float64, device=dev).fill_(1.0)
         dst = torch.empty_like(src)
@@ -659,6 +708,7 @@ def main():
                    "M": n, "N": n, "K": n, "form": best[0], "fp64_kernel": fp64_kernel},
         "roofline": roofline(best[0]),
         "roofline_fp64": roofline("fp64") if "fp64" in timings else None,
+        "int8_vendor_library": int8_vendor,
         "cpu_baseline": cpu,
         "forms": forms,
         "accuracy": {"max_err_over_absAabsB_sampled": forms[best[0]]["max_err"], "samples": 512,
