@@ -37,6 +37,7 @@
 
 #include "amdgcn.h"
 #include "f4j_internal.h"
+#include "gemv_layout.h"
 
 #pragma clang fp contract(off)
 
@@ -69,53 +70,11 @@ constexpr bool a_cplx() { return KIND == GEMV_CPLX || KIND == GEMV_CX; }
 template <int KIND>
 constexpr bool x_cplx() { return KIND == GEMV_CPLX || KIND == GEMV_RX; }
 
-// The blocked form's streaming kernels (gemv_dma_kernel, gemv_kernel): a wave
-// runs 64 chains, (row, 64-block) for 8 rows x 8 blocks; per stage 2 x loads
-// per lane in the register-staged form.
-template <int KIND>
-constexpr int rows_per_wave() { static_assert(KIND == GEMV_BLOCKED, "blocked form"); return 8; }
-template <int KIND>
-constexpr int x_loads() { return 2; }
-
-// Stage s covers k = kstage(s) + [0, 512): 16 elements of each of the 8 blocks;
-// within it, the k a lane loads depends only on the lane and the load index.
-template <int KIND>
-__device__ inline int kstage(int64_t s) { return (int)(512 * (s >> 2) + 16 * (s & 3)); }
-
-// Row (within the wave's slab) and k offset (within the stage) of A load i of
-// lane l.  A stage holds 16 elements of each of the 64 chains.  Load i of lane l
-// lands in LDS at double 64 i + l (VEC 1) or 128 i + 2 l (VEC 2); the element
-// each lane fetches is rotated so that the reading lanes spread over the banks.
-template <int KIND, int VEC>
-__device__ inline void a_slot(int i, int l, int &r, int &kl) {
-    int ch, el;
-    if constexpr (VEC == 2) {
-        const int b = l >> 3;
-        ch = 8 * i + b;                                // reader of chain c finds pair p at (p + c/8 + c%8) & 7
-        el = 2 * (((l & 7) - i - b) & 7);
-    } else {
-        const int b = l >> 4;
-        ch = 4 * i + b;                                // reader of chain c finds element e at (e + c/4 + c%4) & 15
-        el = ((l & 15) - i - b) & 15;
-    }
-    r = ch >> 3;                                       // chain = (row ch / 8, block ch % 8)
-    kl = 64 * (ch & 7) + el;
-}
-
-// k offset (within the stage), component and sign of x load h of lane l.
-template <int KIND>
-__device__ inline void x_slot(int h, int l, int conjx, int &kl, int &part, bool &neg) {
-    part = 0;
-    neg = false;
-    const int j = 4 * h + (l >> 4);                    // block j: element e at lane position (e + j) & 15
-    kl = 64 * j + (((l & 15) - j) & 15);
-}
-
 // Register-staged form of the blocked kind (A not 16-byte aligned, or odd K).
 template <int KIND, int VEC, int D>
 __global__ __launch_bounds__(64) void gemv_kernel(GemvArgs g) {
     static_assert(KIND == GEMV_BLOCKED, "the blocked form (other kinds: gemv_split_kernel)");
-    constexpr int R = rows_per_wave<KIND>(), XN = x_loads<KIND>(), NA = 16 / VEC;
+    constexpr int R = gemv::ROWS, XN = 2, NA = 16 / VEC;
     static_assert(D * (NA + XN) <= 63, "loads in flight must fit the vmcnt counter");
     __shared__ __attribute__((aligned(16))) double lds[1024 + 64 * XN];
     const int l = threadIdx.x;
@@ -134,20 +93,20 @@ __global__ __launch_bounds__(64) void gemv_kernel(GemvArgs g) {
 #pragma unroll
     for (int i = 0; i < NA; ++i) {
         int r, kl;
-        a_slot<KIND, VEC>(i, l, r, kl);
+        gemv::a_slot<VEC>(i, l, r, kl);
         akl[i] = r < rows ? kl : K;
         aoff[i] = (uint32_t)(((int64_t)r * g.lda + (int64_t)kl) * 8);
     }
 #pragma unroll
     for (int h = 0; h < XN; ++h) {
-        int part;
-        x_slot<KIND>(h, l, g.conjx, xkl[h], part, xneg[h]);
-        xoff[h] = (uint32_t)(((int64_t)xkl[h] * incx + part) * 8);
+        xkl[h] = gemv::x_slot(h, l);
+        xneg[h] = false;
+        xoff[h] = (uint32_t)((int64_t)xkl[h] * incx * 8);
     }
     AV va[D][NA];
     double vx[D][XN];
     auto issue = [&](AV (&a)[NA], double (&xv)[XN], int64_t s) {
-        const int ks = kstage<KIND>(s);
+        const int ks = gemv::kstage(s);
         const uint32_t ka = (uint32_t)ks * 8u, kx = (uint32_t)ks * (uint32_t)incx * 8u;
 #pragma unroll
         for (int i = 0; i < NA; ++i) {
@@ -165,23 +124,6 @@ __global__ __launch_bounds__(64) void gemv_kernel(GemvArgs g) {
         }
     };
 
-    // Where this lane's chain finds its elements in LDS.
-    int a_base, a_rot, x_base, x_rot = 0;
-    {
-        if constexpr (VEC == 2) {
-            a_base = 128 * (l >> 3) + 16 * (l & 7);
-            a_rot = (l >> 3) + (l & 7);
-        } else {
-            a_base = 64 * (l >> 2) + 16 * (l & 3);
-            a_rot = (l >> 2) + (l & 3);
-        }
-        {
-            const int j = l & 7;
-            x_base = 1024 + 64 * (j >> 2) + 16 * (j & 3);
-            x_rot = j;
-        }
-    }
-
     double s_acc = 0.0;    // this lane's chain
     double dest = 0.0;     // blocked form: the row's running sum (lanes l % 8 == 0)
 
@@ -194,26 +136,26 @@ __global__ __launch_bounds__(64) void gemv_kernel(GemvArgs g) {
             const int64_t s = s0 + d;
 #pragma unroll
             for (int i = 0; i < NA; ++i) {
-                if constexpr (VEC == 2) *(f64x2 *)&lds[128 * i + 2 * l] = va[d][i];
-                else lds[64 * i + l] = va[d][i];
+                if constexpr (VEC == 2) *(f64x2 *)&lds[gemv::a_store<2>(i, l)] = va[d][i];
+                else lds[gemv::a_store<1>(i, l)] = va[d][i];
             }
 #pragma unroll
-            for (int h = 0; h < XN; ++h) lds[1024 + 64 * h + l] = vx[d][h];
+            for (int h = 0; h < XN; ++h) lds[gemv::x_store(h, l)] = vx[d][h];
             issue(va[d], vx[d], s + D);       // past the last stage the offsets are out of range: zeros
             if constexpr (VEC == 2) {
 #pragma unroll
                 for (int p = 0; p < 8; ++p) {
-                    const f64x2 a = *(const f64x2 *)&lds[a_base + 2 * ((p + a_rot) & 7)];
-                    const double x0 = lds[x_base + ((2 * p + x_rot) & 15)];
-                    const double x1 = lds[x_base + ((2 * p + 1 + x_rot) & 15)];
+                    const f64x2 a = *(const f64x2 *)&lds[gemv::a_read<2>(l, p)];
+                    const double x0 = lds[gemv::x_read(l, 2 * p)];
+                    const double x1 = lds[gemv::x_read(l, 2 * p + 1)];
                     s_acc = __dadd_rn(s_acc, __dmul_rn(a.x, x0));
                     s_acc = __dadd_rn(s_acc, __dmul_rn(a.y, x1));
                 }
             } else {
 #pragma unroll
                 for (int e = 0; e < 16; ++e) {
-                    const double a = lds[a_base + ((e + a_rot) & 15)];
-                    const double xv = lds[x_base + ((e + x_rot) & 15)];
+                    const double a = lds[gemv::a_read<1>(l, e)];
+                    const double xv = lds[gemv::x_read(l, e)];
                     s_acc = __dadd_rn(s_acc, __dmul_rn(a, xv));
                 }
             }
@@ -244,18 +186,11 @@ __global__ __launch_bounds__(64) void gemv_kernel(GemvArgs g) {
 // It serves the blocked form (the other kinds run gemv_split_kernel): NL = 8
 // A loads per stage, 8 rows x 8 blocks of chains per wave.
 
-// Bytes per stage slot: NL KB of A, then 1 KB of x.
-constexpr int slot_bytes(int nl) { return 1024 * nl + 1024; }
-
-// x loads per stage: XV 2 = 16-byte pairs (x aligned, unit stride, K even), XV 1 = dwords.
-template <int KIND, int XV>
-constexpr int x_dma_loads() { return KIND == GEMV_BLOCKED && XV == 1 ? 4 : 1; }
-
 template <int KIND, int XV, int D, int NL>
 __global__ __launch_bounds__(64) void gemv_dma_kernel(GemvArgs g) {
-    static_assert(KIND == GEMV_BLOCKED && NL == 8, "the blocked form (other kinds: gemv_split_kernel)");
-    constexpr int R = rows_per_wave<KIND>() * NL / 8, XL = x_dma_loads<KIND, XV>(), L = NL + XL;
-    constexpr int SLOT = slot_bytes(NL), XOFF = 1024 * NL;
+    static_assert(KIND == GEMV_BLOCKED && NL == gemv::DMA_NL, "the blocked form (other kinds: gemv_split_kernel)");
+    constexpr int R = gemv::ROWS, XL = gemv::x_dma_loads<XV>(), L = gemv::dma_loads<XV>();
+    constexpr int SLOT = gemv::slot_bytes();
     constexpr int ew = a_cplx<KIND>() ? 2 : 1, xw = x_cplx<KIND>() ? 2 : 1;
     static_assert((D - 1) * L <= 63, "loads in flight must fit the vmcnt counter");
     __shared__ __attribute__((aligned(1024))) char lds[D * SLOT];
@@ -275,32 +210,23 @@ __global__ __launch_bounds__(64) void gemv_dma_kernel(GemvArgs g) {
 #pragma unroll
     for (int i = 0; i < NL; ++i) {
         int r, kl;
-        a_slot<KIND, 2>(i, l, r, kl);
+        gemv::a_slot<2>(i, l, r, kl);
         akl[i] = kl;
         arow[i] = (uint32_t)((int64_t)(r < rows ? r : rows - 1) * g.lda * 8);
     }
     int xkl[XL], xadd[XL];
 #pragma unroll
-    for (int h = 0; h < XL; ++h) {
-        if constexpr (XV == 2) {
-            const int j = l >> 3;                        // block j: pair p at lane position (p + j) & 7
-            xkl[h] = 64 * j + 2 * (((l & 7) - j) & 7);
-            xadd[h] = 0;
-        } else {
-            const int m = 32 * h + (l >> 1);             // double m of the stage's x area
-            const int j = m >> 4;                        // block j: element e at (e + j) & 15
-            xkl[h] = 64 * j + (((m & 15) - j) & 15);
-            xadd[h] = 4 * (l & 1);
-        }
-    }
+    for (int h = 0; h < XL; ++h) gemv::dma_x_slot<XV>(h, l, xkl[h], xadd[h]);
 
+    // The LDS destination of a load is the address given for lane 0; lane l's
+    // bytes follow at l times the load size (gemv::dma_a_store, dma_x_store).
     auto issue = [&](int64_t s) {
-        const int ks = kstage<KIND>(s);
+        const int ks = gemv::kstage(s);
         char *slot = lds + (int)(s % D) * SLOT;
 #pragma unroll
         for (int i = 0; i < NL; ++i) {
             const int k = min(ks + akl[i], ka_max);
-            __builtin_amdgcn_raw_ptr_buffer_load_lds(ra, (lds_void *)(slot + 1024 * i), 16,
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(ra, (lds_void *)(slot + gemv::dma_a_store(i, 0)), 16,
                                                      arow[i] + (uint32_t)k * (8u * ew), 0, 0, 0);
         }
 #pragma unroll
@@ -308,33 +234,33 @@ __global__ __launch_bounds__(64) void gemv_dma_kernel(GemvArgs g) {
             const int k = min(ks + xkl[h], kx_max);
             const uint32_t off = (uint32_t)k * (uint32_t)incx * 8u + xadd[h];   // incx in doubles
             if constexpr (XV == 2)
-                __builtin_amdgcn_raw_ptr_buffer_load_lds(rx, (lds_void *)(slot + XOFF), 16, off, 0, 0, 0);
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(rx, (lds_void *)(slot + gemv::dma_x_store<2>(h, 0)), 16, off,
+                                                         0, 0, 0);
             else
-                __builtin_amdgcn_raw_ptr_buffer_load_lds(rx, (lds_void *)(slot + XOFF + 256 * h), 4, off, 0, 0, 0);
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(rx, (lds_void *)(slot + gemv::dma_x_store<1>(h, 0)), 4, off,
+                                                         0, 0, 0);
         }
     };
 
-    // reader geometry (bytes within a slot): lane l runs chain (row l / 8, block l % 8)
-    const int j = l & 7;
-    const int a_base = 1024 * (l >> 3) + 128 * j, a_rot = (l >> 3) + j;
-    const int kb = 64 * j, x_base = XOFF + 128 * j, x_rot = j;
+    // lane l runs chain (row l / 8, block l % 8): k = kstage + kb + 2 p, 2 p + 1
+    const int kb = 64 * (l & 7);
 
     double s_acc = 0.0, dest = 0.0;
     auto compute = [&](int64_t s, bool masked) {
         const char *slot = lds + (int)(s % D) * SLOT;
-        const int ks = kstage<KIND>(s);
+        const int ks = gemv::kstage(s);
         {
 #pragma unroll
             for (int p = 0; p < 8; ++p) {
-                const f64x2 a = *(const f64x2 *)(slot + a_base + 16 * ((p + a_rot) & 7));
+                const f64x2 a = *(const f64x2 *)(slot + gemv::dma_a_read(l, p));
                 double x0, x1;
                 if constexpr (XV == 2) {
-                    const f64x2 w = *(const f64x2 *)(slot + x_base + 16 * ((p + x_rot) & 7));
+                    const f64x2 w = *(const f64x2 *)(slot + gemv::dma_x_read<2>(l, p, 0));
                     x0 = w.x;
                     x1 = w.y;
                 } else {
-                    x0 = *(const double *)(slot + x_base + 8 * ((2 * p + x_rot) & 15));
-                    x1 = *(const double *)(slot + x_base + 8 * ((2 * p + 1 + x_rot) & 15));
+                    x0 = *(const double *)(slot + gemv::dma_x_read<1>(l, p, 0));
+                    x1 = *(const double *)(slot + gemv::dma_x_read<1>(l, p, 1));
                 }
                 const double v = __dadd_rn(__dadd_rn(s_acc, __dmul_rn(a.x, x0)), __dmul_rn(a.y, x1));
                 s_acc = (!masked || ks + kb + 2 * p < K) ? v : s_acc;
@@ -344,15 +270,13 @@ __global__ __launch_bounds__(64) void gemv_dma_kernel(GemvArgs g) {
 
 #pragma unroll
     for (int d = 0; d < D - 1; ++d) issue(d);
-    // a stage is complete when every chain's 16 elements lie below K
-    const int span = 64 * 7 + 16;
     for (int64_t s = 0; s < g.nstage; ++s) {
         __builtin_amdgcn_sched_barrier(0);
-        F4J_WAIT_VMCNT((D - 2) * L);
+        F4J_WAIT_VMCNT(gemv::dma_wait(D, L));
         __builtin_amdgcn_sched_barrier(0);
         issue(s + D - 1);                 // past the last stage: clamped reloads, never read
         __builtin_amdgcn_sched_barrier(0);
-        if (kstage<KIND>(s) + span <= K) compute(s, false);
+        if (gemv::kstage(s) + gemv::SPAN <= K) compute(s, false);
         else compute(s, true);
         if ((s & 3) == 3) {                // the row's 8 block sums, added in block order
             double t = dest;
@@ -368,9 +292,9 @@ __global__ __launch_bounds__(64) void gemv_dma_kernel(GemvArgs g) {
     if ((l & 7) == 0 && r < rows) g.y[(row0 + r) * g.incy] = dest;
 }
 
-template <int KIND, int XV, int D, int NL = 8>
+template <int KIND, int XV, int D, int NL = gemv::DMA_NL>
 void launch_dma(GemvArgs g, int64_t stages, hipStream_t stream) {
-    constexpr int R = rows_per_wave<KIND>() * NL / 8;
+    constexpr int R = gemv::ROWS;
     g.nstage = stages;
     hipLaunchKernelGGL((gemv_dma_kernel<KIND, XV, D, NL>), dim3((unsigned)((g.M + R - 1) / R)), dim3(64), 0, stream,
                        g);
@@ -473,7 +397,7 @@ __global__ __launch_bounds__(256) void gemv_split_kernel(GemvArgs g) {
 
 template <int KIND, int VEC, int D>
 void launch(GemvArgs g, int64_t stages, hipStream_t stream) {
-    constexpr int R = rows_per_wave<KIND>();
+    constexpr int R = gemv::ROWS;
     g.nstage = (stages + D - 1) / D * D;
     hipLaunchKernelGGL((gemv_kernel<KIND, VEC, D>), dim3((unsigned)((g.M + R - 1) / R)), dim3(64), 0, stream, g);
 }
@@ -516,7 +440,7 @@ hipError_t launch_gemv(int kind, int64_t M, int64_t K, const double *A, int64_t 
     const bool a16 = ((uintptr_t)A & 15) == 0 && (a_width(kind) == 2 || (lda % 2 == 0 && K % 2 == 0));
     const bool x16 = ((uintptr_t)x & 15) == 0 && incx == 1 && K % 2 == 0;
     if (kind == GEMV_BLOCKED) {
-        const int64_t stages = 4 * ((K + 511) / 512);
+        const int64_t stages = gemv::stages_for(K);
         if (a16 && x16) launch_dma<GEMV_BLOCKED, 2, 4>(g, stages, stream);
         else if (a16) launch_dma<GEMV_BLOCKED, 1, 4>(g, stages, stream);
         else launch<GEMV_BLOCKED, 1, 3>(g, stages, stream);

@@ -96,3 +96,34 @@ def sched_emu():
 def test_xcd_schedule(sched_emu, nprod, workers, seed):
     r = subprocess.run([sched_emu, str(nprod), str(workers), str(seed)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0 and r.stdout.strip().startswith("OK"), r.stdout
+
+
+GEMV_SRC = os.path.join(ROOT, "tests", "emu", "gemv_emu.cpp")
+GEMV_BIN = os.path.join(ROOT, "_build", "gemv_emu")
+
+
+@pytest.fixture(scope="module")
+def gemv_emu():
+    """Host emulation of the matrix-vector kernels (direct-to-LDS ring with its vmcnt
+    waits, register-staged form, one-chain-per-row split kernel) with the kernels'
+    own index functions."""
+    return build_once(GEMV_SRC, GEMV_BIN, [os.path.join(ROOT, "flag4j_amd", "csrc", "gemv_layout.h")])
+
+
+@pytest.mark.parametrize("mode,M,K,lda,incx", [
+    ("dma2", 8, 512, 512, 1), ("dma2", 13, 1000, 1002, 1), ("dma2", 64, 64, 64, 1), ("dma2", 5, 2, 2, 1),
+    ("dma2", 100, 4094, 4096, 1), ("dma2", 9, 514, 520, 1),
+    ("dma1", 21, 600, 600, 3), ("dma1", 8, 130, 132, 1), ("dma1", 3, 2048, 2050, 2),
+    ("reg", 7, 999, 1001, 2), ("reg", 17, 1, 1, 1), ("reg", 33, 513, 515, 1), ("reg", 8, 64, 64, 5),
+    ("reg", 12, 1536, 1537, 1)])
+def test_gemv_blocked_kernels(gemv_emu, mode, M, K, lda, incx):
+    r = subprocess.run([gemv_emu, mode, str(M), str(K), str(lda), str(incx)], capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 0 and r.stdout.strip().endswith("ok"), r.stdout + r.stderr
+
+
+@pytest.mark.parametrize("kind", ["chain", "cplx", "rx", "cx"])
+@pytest.mark.parametrize("M,K", [(37, 200), (16, 64), (5, 1), (40, 129)])
+def test_gemv_split_kernel(gemv_emu, kind, M, K):
+    r = subprocess.run([gemv_emu, "split", kind, str(M), str(K)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0 and r.stdout.strip().endswith("ok"), r.stdout + r.stderr
