@@ -1181,7 +1181,7 @@ int f4j_shutdown(void) {
         std::lock_guard<std::mutex> sg(st.mu);
         stage_release(st);
     }
-    for (DevBuf *b : {&s.ws, &s.mixws})
+    for (DevBuf *b : {&s.ws, &s.mixws, &s.lsws})
         if (b->p) {
             F4J_HIP(hipFree(b->p), "hipFree");
             b->p = nullptr;
