@@ -52,6 +52,12 @@ struct GroupedGuard {
     unsigned long long above = 0;
 };
 
+// Tile configurations from F4J_DIRECT_BASE on are the direct-load grouped
+// kernels (real operands only): F4J_DIRECT_BASE + 0..3 = one wave per 32 x 32,
+// 64 x 32, 32 x 64, 64 x 64 tile, one-wave workgroups.
+constexpr int F4J_DIRECT_BASE = 100;
+constexpr int F4J_NUM_DIRECT = 4;
+
 // Persistent grouped FP64 GEMM over a work-item list; workgroup g runs items
 // [begin[g], begin[g+1]).  All problems share op kinds and tile config.
 hipError_t launch_gemm_f64_grouped(OpKind ka, OpKind kb, const GemmProblem *probs, const GroupedItem *items,

@@ -67,6 +67,11 @@ struct TileGeom {
 };
 
 TileGeom geom_of(int cfg) {
+    if (cfg >= F4J_DIRECT_BASE) {              // direct-load kernels: one wave per tile, k steps of 4
+        static const int FM[F4J_NUM_DIRECT] = {2, 4, 2, 4}, FN[F4J_NUM_DIRECT] = {2, 2, 4, 4};
+        const int i = cfg - F4J_DIRECT_BASE;
+        return TileGeom{16 * FM[i], 16 * FN[i], 4, 64};
+    }
     switch (cfg) {
 #define F4J_TILE_GEOM(id, BM, BN, BK, WGM, WGN, MINW) \
     case id: return TileGeom{BM, BN, BK, WGM * WGN * 64};
@@ -173,6 +178,19 @@ void free_bucket(f4j_grouped_plan::Bucket &b) {
     b.d_begin = nullptr;
     b.items = 0;
     b.wgs = 0;
+}
+
+// The direct-load kernels read real operands through 32-bit buffer offsets:
+// every problem's A and B must span fewer than 2^31 bytes.
+bool direct_ok(const f4j_grouped_plan &pl) {
+    if (pl.is_complex) return false;
+    for (const GemmProblem &p : pl.probs) {
+        const bool a_rows = pl.ka == R_N, b_rows = pl.kb == R_N;
+        const int64_t ea = a_rows ? (p.M - 1) * p.lda + p.K : (p.K - 1) * p.lda + p.M;
+        const int64_t eb = b_rows ? (p.K - 1) * p.ldb + p.N : (p.N - 1) * p.ldb + p.K;
+        if (ea * 8 >= (int64_t(1) << 31) || eb * 8 >= (int64_t(1) << 31)) return false;
+    }
+    return true;
 }
 
 // Build bucket b's LPT schedule for tile configuration cfg and upload it.
@@ -351,6 +369,7 @@ int setup_fp64(f4j_grouped_plan &pl, const int64_t *M, const int64_t *N, const i
             auto it = g_tune.find(tune_key(arch, pl.ka, pl.kb, b));
             if (it != g_tune.end()) cfg = it->second;
         }
+        if (cfg >= F4J_DIRECT_BASE && !direct_ok(pl)) cfg = DEFAULT_CFG[b];   // real operands below 2^31 bytes only
         if (const int rc = schedule_bucket(pl, b, cfg); rc != F4J_OK) return rc;
     }
     return F4J_OK;
@@ -446,7 +465,10 @@ int f4j_grouped_plan_tune(f4j_grouped_plan *plan, int reps) {
         if (plan->buckets[b].items == 0) continue;
         float best = 1e30f;
         int best_cfg = plan->buckets[b].cfg;
-        for (int cfg : CANDIDATES) {
+        std::vector<int> cands(std::begin(CANDIDATES), std::end(CANDIDATES));
+        if (direct_ok(*plan))                  // the direct-load kernels serve real operands below 2^31 bytes
+            for (int i = 0; i < F4J_NUM_DIRECT; ++i) cands.push_back(F4J_DIRECT_BASE + i);
+        for (int cfg : cands) {
             int rc = schedule_bucket(*plan, b, cfg);
             if (rc != F4J_OK) return rc;
             G_HIP(enqueue_bucket(*plan, b, plan->cap), "tune warmup");
@@ -595,7 +617,7 @@ int f4j_tuner_load(const char *path) {
         std::string arch, kinds;
         int bucket = -1, cfg = -1;
         if (!(ss >> arch >> kinds >> bucket >> cfg) || bucket < 0 || bucket >= NUM_BUCKETS || cfg < 0 ||
-            cfg >= NUM_FP64_TILES)
+            (cfg >= NUM_FP64_TILES && (cfg < F4J_DIRECT_BASE || cfg >= F4J_DIRECT_BASE + F4J_NUM_DIRECT)))
             return fail(F4J_ERR_INVALID_ARG, std::string(path) + ":" + std::to_string(lineno) + ": malformed entry");
         add[arch + " " + kinds + " " + std::to_string(bucket)] = cfg;
     }

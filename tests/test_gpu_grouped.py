@@ -159,3 +159,52 @@ def test_tuner_cache_read_at_init_and_written_by_tune(tmp_path):
     finally:
         L.config_set("tunerCache", "")
         L.check(lib.f4j_tuner_clear())
+
+
+@pytest.mark.parametrize("cfg", [100, 101, 102, 103])
+@pytest.mark.parametrize("opA,opB", [("N", "N"), ("N", "T"), ("T", "N"), ("T", "T")])
+def test_direct_load_configurations_bitwise(tmp_path, cfg, opA, opB):
+    """The direct-load grouped kernels (one wave per tile, fragments from global memory,
+    tile configurations 100-103), forced on every bucket through the tuner cache: every
+    product, ragged edges and K not a multiple of 4 included, equals the single-product
+    FP64 call bit for bit (same MFMA k steps per element, in the same order); products
+    with N = 1, which a single call runs in the reference's matrix-vector order, are
+    held to the FP64 bound."""
+    As, Bs = rand_problems(500 + cfg, 90, 1, 300, opA, opB)
+    kinds = f"{'NT'.index(opA)},{'NT'.index(opB)}"
+    cache = tmp_path / "direct.txt"
+    cache.write_text("".join(f"gfx950 {kinds} {b} {cfg}\n" for b in range(4)))
+    lib = L.load()
+    L.check(lib.f4j_tuner_clear())
+    try:
+        L.check(lib.f4j_tuner_load(str(cache).encode()))
+        shapes = [(op(a, opA).shape[0], op(b, opB).shape[1], op(a, opA).shape[1]) for a, b in zip(As, Bs)]
+        pa = np.concatenate([a.reshape(-1) for a in As])
+        pb = np.concatenate([b.reshape(-1) for b in Bs])
+        oa = np.cumsum([0] + [a.size for a in As])
+        ob = np.cumsum([0] + [b.size for b in Bs])
+        oc = np.cumsum([0] + [m * n for m, n, _ in shapes])
+        with device.DeviceBuffer.from_array(pa) as dA, device.DeviceBuffer.from_array(pb) as dB, \
+                device.DeviceBuffer(int(oc[-1]) * 8) as dC:
+            plan = grouped.GroupedPlan(shapes, [dA.ptr + 8 * int(x) for x in oa[:-1]],
+                                       [dB.ptr + 8 * int(x) for x in ob[:-1]],
+                                       [dC.ptr + 8 * int(x) for x in oc[:-1]],
+                                       [a.shape[1] for a in As], [b.shape[1] for b in Bs],
+                                       [n for _, n, _ in shapes], opA=opA, opB=opB)
+            info = plan.info()
+            assert all(c == cfg for c, t in zip(info["tile_cfg"], info["tiles"]) if t > 0), info
+            plan.run()
+            device.sync()
+            C = dC.download(np.float64, int(oc[-1]))
+            plan.close()
+    finally:
+        L.check(lib.f4j_tuner_clear())
+    for i, (A, B) in enumerate(zip(As, Bs)):
+        M, N, K = shapes[i]
+        if N == 1:              # a single call with N = 1 runs the reference's matrix-vector order instead
+            assert check_real(C[oc[i]:oc[i + 1]].reshape(M, N), op(A, opA), op(B, opB))
+            continue
+        single = np.full((M, N), np.nan)
+        L.check(lib.f4j_dgemm(L.OP[opA], L.OP[opB], M, N, K, A.ctypes.data, A.shape[1], B.ctypes.data, B.shape[1],
+                              single.ctypes.data, max(N, 1), L.PREC["fp64"], 0))
+        assert np.array_equal(C[oc[i]:oc[i + 1]].reshape(M, N), single), (i, M, N, K)
