@@ -258,11 +258,14 @@ int f4j_grouped_plan_split_report(f4j_grouped_plan *plan, double *err, int64_t c
 int f4j_grouped_plan_run(f4j_grouped_plan *plan, void *stream);
 int f4j_grouped_plan_destroy(f4j_grouped_plan *plan);
 /* Work of the plan (flops, classical count) and, per shape bucket (nb <= 4 entries), the tile
- * configuration id and the number of tiles. */
+ * configuration id and the number of tiles.  Ids 0-8: LDS-tiled kernels; 100-103: direct-load
+ * kernels (one wave per 32x32, 64x32, 32x64 or 64x64 tile; real operands spanning < 2^31 bytes). */
 int f4j_grouped_plan_info(const f4j_grouped_plan *plan, double *flops, int *tile_cfg, int64_t *tiles, int nb);
 /* Autotuner: time every candidate tile configuration per bucket on this plan's own problems
  * (overwrites their C), keep the fastest in the plan and in the process-wide cache keyed by
- * (device arch, op kinds, bucket), and recapture the graph.  The cache persists via save/load. */
+ * (device arch, op kinds, bucket), and recapture the graph.  The cache persists via save/load
+ * (text lines "arch kinds bucket id"; a direct-load id cached for a plan it cannot serve falls
+ * back to the bucket's default).  Every configuration gives the same bits. */
 int f4j_grouped_plan_tune(f4j_grouped_plan *plan, int reps);
 int f4j_tuner_save(const char *path);
 int f4j_tuner_load(const char *path);
