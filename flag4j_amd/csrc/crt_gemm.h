@@ -29,7 +29,7 @@ struct CrtTables {
 struct CrtPlan {
     CrtTables tb{};
     int64_t Mp = 0, Np = 0, Kp = 0;      // padded sizes of the residue slices
-    int64_t plane = 0;                   // bytes between the product residues of two moduli (M N, 256-aligned)
+    int64_t plane = 0;                   // bytes of one modulus's product residues (Mp Np, crt::res_block layout)
     size_t off_ra = 0, off_rb = 0, off_res = 0, off_ea = 0, off_eb = 0, off_err = 0, off_part = 0, bytes = 0;
     size_t off_flag = 0;                 // int: set by the exponent kernels when an operand holds NaN or Inf
 };

@@ -444,23 +444,27 @@ __global__ __launch_bounds__(512, 2) void crt_gemm_kernel(CrtGemmArgs a, CrtTabl
     }
     F4J_WAIT_VMCNT(0);                            // no LDS write in flight when the block ends
 
-    // acc mod m (crt::product_residue: in [0, m], m kept for exact multiples)
+    // acc mod m (crt::product_residue: in [0, m], m kept for exact multiples), a
+    // lane's 16 residues of a block as one 16-byte store into the padded plane
+    // (crt::res_block): no per-entry range checks; the reconstruction skips padding.
     const double m = (double)tb.m[l];
     const double minv = tb.minv[l];
-    unsigned char *R = a.R + (int64_t)l * a.plane;
+    unsigned char *R = a.R + (int64_t)l * a.plane + 16 * lane;
+    const int64_t NB = a.Np >> 5;
     const int64_t rw = m0 + crt::wave_row(wave), cw = n0 + crt::wave_col(wave);
 #pragma unroll
     for (int i = 0; i < crt::RB; ++i)
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            const int64_t col = cw + crt::acc_col(j, lane);
-            if (col >= a.N) continue;
+            int rs[16];
 #pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const int64_t row = rw + crt::acc_row(i, r, lane);
-                if (row >= a.M) continue;
-                R[row * a.N + col] = (unsigned char)crt::product_residue(acc[i][j][r], m, minv);
-            }
+            for (int r = 0; r < 16; ++r) rs[r] = crt::product_residue(acc[i][j][r], m, minv);
+            int4 w;
+            w.x = (int)pack4(rs[0], rs[1], rs[2], rs[3]);
+            w.y = (int)pack4(rs[4], rs[5], rs[6], rs[7]);
+            w.z = (int)pack4(rs[8], rs[9], rs[10], rs[11]);
+            w.w = (int)pack4(rs[12], rs[13], rs[14], rs[15]);
+            *(int4 *)(R + crt::res_block(rw + 32 * i, cw + 32 * j, NB)) = w;
         }
 }
 
@@ -468,79 +472,56 @@ __global__ __launch_bounds__(512, 2) void crt_gemm_kernel(CrtGemmArgs a, CrtTabl
 // Reconstruction: C_ij = 2^(eA_i + eB_j - 2t) * CRT(R[.][i][j]).
 // ------------------------------------------------------------------------
 
-constexpr int RV = 4;                             // product entries per thread
-using ResWord = uint32_t;                         // RV residue bytes of one modulus
-
-// Thread -> RV consecutive entries of the flattened M x N product: one 4-byte
-// load per modulus (planes are 256-byte aligned) instead of RV byte loads.
-// Four entries (not eight) keep the kernel at 97 VGPRs, four waves per SIMD,
-// so that one wave's loads overlap another's arithmetic (0.43 ms instead of
-// 0.61 at 8192^2 with 16 moduli).
+// Thread -> one lane's 16 entries of a 32 x 32 block of the product planes
+// (crt::res_block; a wave is one block): one 16-byte load per modulus, all
+// issued before any is used, then four rounds of four entries (rows
+// crt::res_row, column crt::res_col), so only four entries' chunk sums are live.
+// NW = residue words loaded (the moduli rounded up to a multiple of four:
+// words past n re-read plane 0, a cache hit, unused), which bounds the
+// registers by the moduli in use.
 // Output: real C (cplx = 0): C_ij = value; complex C (interleaved):
 // Re C_ij (+)= cre value, Im C_ij (+)= cim value (first: assign).
-__global__ __launch_bounds__(256) void crt_reconstruct_kernel(const unsigned char *R, int64_t plane, int64_t M,
-                                                              int64_t N, const int *eA, const int *eB, double *C,
-                                                              int64_t ldc, CrtTables tb, int cplx, double cre,
-                                                              double cim, int first) {
-    const int64_t g0 = ((int64_t)blockIdx.x * 256 + threadIdx.x) * RV;
-    const int64_t total = M * N;
-    if (g0 >= total) return;
-    const int cnt = (int)min<int64_t>(RV, total - g0);
-    double S[CRT_MAX_CHUNKS][RV];
+template <int NW>
+__global__ __launch_bounds__(256) void crt_reconstruct_kernel(const unsigned char *R, int64_t plane, int64_t NB,
+                                                              int64_t nblocks, int64_t M, int64_t N, const int *eA,
+                                                              const int *eB, double *C, int64_t ldc, CrtTables tb,
+                                                              int cplx, double cre, double cim, int first) {
+    const int64_t b = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (b >= nblocks) return;
+    const int64_t row0 = (b / NB) * 32, col0 = (b % NB) * 32;
+    if (row0 >= M || col0 >= N) return;           // padding block (uniform per wave)
+    int4 word[NW];
 #pragma unroll
-    for (int p = 0; p < CRT_MAX_CHUNKS; ++p)
+    for (int l = 0; l < NW; ++l)
+        word[l] = *(const int4 *)(R + (int64_t)(l < tb.n ? l : 0) * plane + (b << 10) + 16 * lane);
+    const int64_t j = col0 + crt::res_col(lane);
+    const bool col_ok = j < N;
+    const int eb = eB[col_ok ? j : N - 1];
 #pragma unroll
-        for (int e = 0; e < RV; ++e) S[p][e] = 0.0;
-    // All residue words are loaded before any is used, so every load of the
-    // thread is in flight at once (the loop over moduli would otherwise wait
-    // for each 8-byte load in turn).
-    // exponents of every entry's row and column, loaded up front as well
-    int ea_e[RV], eb_e[RV];
-    {
-        int64_t i = g0 / N, j = g0 - i * N;
+    for (int q = 0; q < 4; ++q) {                 // items 4q .. 4q + 3 (word component q)
+        double S[CRT_MAX_CHUNKS][4];
 #pragma unroll
-        for (int e = 0; e < RV; ++e) {
-            const int64_t ii = e < cnt ? i : 0, jj = e < cnt ? j : 0;
-            ea_e[e] = eA[ii];
-            eb_e[e] = eB[jj];
-            if (++j == N) {
-                j = 0;
-                ++i;
+        for (int p = 0; p < CRT_MAX_CHUNKS; ++p)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) S[p][e] = 0.0;
+#pragma unroll
+        for (int l = 0; l < NW; ++l) {
+            if (l < tb.n) {      // uniform; no early exit, so word[] keeps static indices (registers)
+                const unsigned wq = (unsigned)(q == 0 ? word[l].x : q == 1 ? word[l].y : q == 2 ? word[l].z : word[l].w);
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const double r = (double)((wq >> (8 * e)) & 255u);
+#pragma unroll
+                    for (int p = 0; p < CRT_MAX_CHUNKS; ++p) S[p][e] = fma(r, tb.w[l][p], S[p][e]);   // exact integers
+                }
             }
         }
-    }
-    ResWord word[CRT_MAX_MODULI];
-    if (cnt == RV) {
-        // unconditional loads (moduli past n re-read plane 0, a cache hit, unused):
-        // a load issued on only some paths would make the compiler wait for all of them
 #pragma unroll
-        for (int l = 0; l < CRT_MAX_MODULI; ++l)
-            word[l] = *(const ResWord *)(R + (int64_t)(l < tb.n ? l : 0) * plane + g0);
-    } else {
-#pragma unroll
-        for (int l = 0; l < CRT_MAX_MODULI; ++l) {
-            unsigned char r[RV];
-#pragma unroll
-            for (int e = 0; e < RV; ++e) r[e] = (l < tb.n && e < cnt) ? R[(int64_t)l * plane + g0 + e] : 0;
-            __builtin_memcpy(&word[l], r, RV);
-        }
-    }
-#pragma unroll
-    for (int l = 0; l < CRT_MAX_MODULI; ++l) {
-        if (l < tb.n) {          // uniform; no early exit, so word[] keeps static indices (registers)
-            unsigned char r[RV];
-            __builtin_memcpy(r, &word[l], RV);
-#pragma unroll
-            for (int p = 0; p < CRT_MAX_CHUNKS; ++p)
-#pragma unroll
-                for (int e = 0; e < RV; ++e) S[p][e] = fma((double)r[e], tb.w[l][p], S[p][e]);   // exact integers
-        }
-    }
-    int64_t i = g0 / N, j = g0 - i * N;
-#pragma unroll
-    for (int e = 0; e < RV; ++e) {
-        if (e < cnt) {
-            const int ea = ea_e[e], eb = eb_e[e];
+        for (int e = 0; e < 4; ++e) {
+            const int64_t i = row0 + crt::res_row(4 * q + e, lane);
+            if (!col_ok || i >= M) continue;
+            const int ea = eA[i];
             double out = 0.0;
             if (ea != INT_MIN && eb != INT_MIN)
                 out = ldexp(crt_recombine(S[0][e], S[1][e], S[2][e], S[3][e], tb), ea + eb - 2 * tb.t);
@@ -551,10 +532,6 @@ __global__ __launch_bounds__(256) void crt_reconstruct_kernel(const unsigned cha
             } else {
                 C[i * ldc + j] = out;
             }
-        }
-        if (++j == N) {
-            j = 0;
-            ++i;
         }
     }
 }
@@ -622,7 +599,7 @@ bool plan_crt(int64_t M, int64_t N, int64_t K, int n, int check_rows, CrtPlan *p
     auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
     pl.off_ra = 0;
     pl.off_rb = al((size_t)n * pl.Mp * pl.Kp);
-    pl.plane = (int64_t)al((size_t)M * N);
+    pl.plane = (int64_t)(pl.Mp * pl.Np);          // crt::res_block layout, a multiple of 2^16
     pl.off_res = pl.off_rb + al((size_t)n * pl.Np * pl.Kp);
     pl.off_ea = pl.off_res + (size_t)n * pl.plane;
     pl.off_eb = pl.off_ea + al((size_t)M * 4);
@@ -709,9 +686,16 @@ hipError_t launch_crt_gemm(const CrtPlan &pl, const CrtOperand &oa, const CrtOpe
     hipLaunchKernelGGL(crt_gemm_kernel, dim3((unsigned)((pl.Mp / CBM) * (pl.Np / CBN)), (unsigned)pl.tb.n),
                        dim3(64 * crt::WAVES), 0, stream, a, pl.tb);
     if (ev_end) (void)hipEventRecord(ev_end, stream);
-    const int64_t threads = (M * N + RV - 1) / RV;
-    hipLaunchKernelGGL(crt_reconstruct_kernel, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, stream, res,
-                       pl.plane, M, N, ea, eb, C, ldc, pl.tb, (int)out.cplx, out.cre, out.cim, (int)out.first);
+    const int64_t NB = pl.Np / 32, nblocks = (pl.Mp / 32) * NB;
+    const dim3 rgrid((unsigned)((nblocks + 3) / 4));
+#define F4J_RECONSTRUCT(NW)                                                                                        \
+    hipLaunchKernelGGL(crt_reconstruct_kernel<NW>, rgrid, dim3(256), 0, stream, res, pl.plane, NB, nblocks, M, N, ea, \
+                       eb, C, ldc, pl.tb, (int)out.cplx, out.cre, out.cim, (int)out.first)
+    if (pl.tb.n <= 8) F4J_RECONSTRUCT(8);
+    else if (pl.tb.n <= 12) F4J_RECONSTRUCT(12);
+    else if (pl.tb.n <= 16) F4J_RECONSTRUCT(16);
+    else F4J_RECONSTRUCT(CRT_MAX_MODULI);
+#undef F4J_RECONSTRUCT
     return hipGetLastError();
 }
 

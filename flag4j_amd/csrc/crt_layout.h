@@ -82,5 +82,15 @@ F4J_HD int wave_col(int w) { return 128 * (w >> 2); }
 F4J_HD int acc_row(int i, int r, int lane) { return 32 * i + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3); }
 F4J_HD int acc_col(int j, int lane) { return 32 * j + (lane & 31); }
 
+// Product residues of one modulus: a plane of Mp x Np bytes in the MFMA's result
+// order, so the epilogue stores a lane's 16 residues of a 32 x 32 block with one
+// 16-byte store.  Block (row / 32, col / 32) is 1 KB at res_block, blocks ordered
+// with the column block fastest (NB = Np / 32 per block row); lane L's bytes
+// 16 L .. 16 L + 15 are its result items r = 0 .. 15, at row res_row(r, L) and
+// column res_col(L) of the block (acc_row / acc_col of block (0, 0)).
+F4J_HD int64_t res_block(int64_t row0, int64_t col0, int64_t NB) { return ((row0 >> 5) * NB + (col0 >> 5)) << 10; }
+F4J_HD int res_row(int r, int lane) { return acc_row(0, r, lane); }
+F4J_HD int res_col(int lane) { return acc_col(0, lane); }
+
 }  // namespace crt
 }  // namespace f4j

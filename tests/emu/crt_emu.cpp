@@ -8,8 +8,10 @@
 // lane L holds line L % 32, k = 16 (L / 32) + v; result item r of lane L is
 // row 8 (r / 4) + 4 (L / 32) + r % 4, column L % 32.  Checks: every residue
 // byte written once and in bounds, every load in bounds, every fragment read
-// from the LDS stage of the k slice being computed, every product residue
-// written once and equal to the direct integer product modulo m.
+// from the LDS stage of the k slice being computed, every product residue byte
+// of the padded plane written once, and every entry, read back as the
+// reconstruction kernel reads the plane, equal to the direct integer product
+// modulo m.
 //
 // usage: crt_emu <M> <N> <K> [seed]
 #include <cstdio>
@@ -80,7 +82,9 @@ int main(int argc, char **argv) {
     // A through the contiguous-line split, B through the strided one (as for op(B) = N)
     const std::vector<signed char> Ar = split_write(A, M, K, Mp, Kp, false), Br = split_write(B, N, K, Np, Kp, true);
 
-    std::vector<int> R(M * N, -1), Rhits(M * N, 0);
+    // product residues of one modulus, in the padded plane layout (crt::res_block)
+    const int64_t NB = Np / 32;
+    std::vector<int> R(Mp * Np, -1), Rhits(Mp * Np, 0);
     const int64_t tiles_m = Mp / crt::BM, tiles_n = Np / crt::BN;
     const int nt = (int)(Kp / crt::SK);
     std::vector<int> tile_hits(tiles_m * tiles_n, 0);
@@ -149,21 +153,38 @@ int main(int argc, char **argv) {
                                 }
                 }
         }
-        // epilogue
+        // epilogue: lane's 16 residues of block (i, j) as one 16-byte store, byte r = item r
         for (int w = 0; w < crt::WAVES; ++w)
             for (int i = 0; i < crt::RB; ++i)
                 for (int j = 0; j < 4; ++j)
-                    for (int lane = 0; lane < 64; ++lane)
+                    for (int lane = 0; lane < 64; ++lane) {
+                        const int64_t base = crt::res_block(m0 + crt::wave_row(w) + 32 * i,
+                                                            n0 + crt::wave_col(w) + 32 * j, NB) + 16 * lane;
+                        CHECK(base % 16 == 0 && base >= 0 && base + 16 <= Mp * Np, "residue store out of bounds");
+                        if (base < 0 || base + 16 > Mp * Np) continue;
                         for (int r = 0; r < 16; ++r) {
-                            const int64_t row = m0 + crt::wave_row(w) + crt::acc_row(i, r, lane);
-                            const int64_t col = n0 + crt::wave_col(w) + crt::acc_col(j, lane);
-                            if (row >= M || col >= N) continue;
                             long long v = acc[(((w * crt::RB + i) * 4 + j) * 64 + lane) * 16 + r] % m;
                             if (v < 0) v += m;
-                            R[row * N + col] = (int)v;
-                            ++Rhits[row * N + col];
+                            R[base + r] = (int)v;
+                            ++Rhits[base + r];
                         }
+                    }
     }
+    for (int64_t i = 0; i < Mp * Np; ++i) CHECK(Rhits[i] == 1, "residue byte %ld written %d times", (long)i, Rhits[i]);
+    // the reconstruction's reading: block b of the plane, lane L, item r -> entry
+    // (32 (b / NB) + res_row(r, L), 32 (b % NB) + res_col(L)), checked against the
+    // documented MFMA result order (item r of lane L: row 8 (r / 4) + 4 (L / 32) + r % 4, column L % 32)
+    std::vector<int> Rd(M * N, -1), Rdh(M * N, 0);
+    for (int64_t b = 0; b < (Mp / 32) * NB; ++b)
+        for (int lane = 0; lane < 64; ++lane)
+            for (int r = 0; r < 16; ++r) {
+                CHECK(crt::res_row(r, lane) == 8 * (r / 4) + 4 * (lane / 32) + r % 4 && crt::res_col(lane) == lane % 32,
+                      "reconstruction reads item %d of lane %d at the wrong entry", r, lane);
+                const int64_t row = 32 * (b / NB) + crt::res_row(r, lane), col = 32 * (b % NB) + crt::res_col(lane);
+                if (row >= M || col >= N) continue;
+                Rd[row * N + col] = R[(b << 10) + 16 * lane + r];
+                ++Rdh[row * N + col];
+            }
     for (size_t i = 0; i < tile_hits.size(); ++i) CHECK(tile_hits[i] == 1, "tile %zu visited %d times", i, tile_hits[i]);
     for (int64_t x = 0; x < M; ++x)
         for (int64_t y = 0; y < N; ++y) {
@@ -171,8 +192,8 @@ int main(int argc, char **argv) {
             for (int64_t k = 0; k < K; ++k) s += (long long)A[x * K + k] * B[y * K + k];
             s %= m;
             if (s < 0) s += m;
-            CHECK(Rhits[x * N + y] == 1, "R(%ld,%ld) written %d times", (long)x, (long)y, Rhits[x * N + y]);
-            CHECK(R[x * N + y] == s, "R(%ld,%ld) = %d, want %lld", (long)x, (long)y, R[x * N + y], s);
+            CHECK(Rdh[x * N + y] == 1, "R(%ld,%ld) read %d times", (long)x, (long)y, Rdh[x * N + y]);
+            CHECK(Rd[x * N + y] == s, "R(%ld,%ld) = %d, want %lld", (long)x, (long)y, Rd[x * N + y], s);
         }
     if (g_fail) {
         std::printf("FAILED %d checks\n", g_fail);
