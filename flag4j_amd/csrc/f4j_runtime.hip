@@ -106,6 +106,22 @@ struct State {
                                             // modular products (guarded by ws_mu)
     hipEvent_t fork_ev = nullptr, join_ev = nullptr, split_fork_ev = nullptr, split_join_ev = nullptr;
     void *scratch_ptr[NOALLOC_MAX_DEVICES] = {};   // per device: its copy of `scratch` (set at init)
+    // The last complex digit-form product's batch plan (its tables, digit pools and
+    // schedule are fixed by the arguments below), reused by a call with the same
+    // ones instead of being built and freed again (guarded by ws_mu).
+    struct CplxSplitKey {
+        int opA, opB, carrier, slices, use3m, check_rows;
+        int64_t M, N, K, lda, ldb, ldc;
+        const double *A, *B;
+        double *C;
+        bool operator==(const CplxSplitKey &o) const {
+            return opA == o.opA && opB == o.opB && carrier == o.carrier && slices == o.slices && use3m == o.use3m &&
+                   check_rows == o.check_rows && M == o.M && N == o.N && K == o.K && lda == o.lda && ldb == o.ldb &&
+                   ldc == o.ldc && A == o.A && B == o.B && C == o.C;
+        }
+    };
+    SplitBatch *cplx_split = nullptr;
+    CplxSplitKey cplx_split_key{};
 };
 
 State &S() {
@@ -545,13 +561,24 @@ int run_split_cplx(int opA, int opB, int64_t M, int64_t N, int64_t K, const doub
         std::lock_guard<std::mutex> g(S().cfg_mu);
         c = S().cfg;
     }
-    const std::vector<BatchProblem> pr{BatchProblem{M, N, K, A, lda, B, ldb, C, ldc}};
-    std::string err;
-    int status = F4J_OK;
-    SplitBatch *b = split_batch_create(true, opA, opB, carrier, S_, use3m != 0, c.split_check_rows, pr, &err, &status);
-    if (!b) return fail(status, err);
     State &s = S();
     std::lock_guard<std::mutex> g(s.ws_mu);
+    // The plan of the last call with these arguments is idle (every call waits for
+    // its stream before it returns, holding ws_mu): reuse it, else replace it.
+    const State::CplxSplitKey key{opA, opB, carrier, S_, use3m != 0, c.split_check_rows, M, N, K, lda, ldb, ldc,
+                                  A, B, C};
+    if (!s.cplx_split || !(s.cplx_split_key == key)) {
+        split_batch_destroy(s.cplx_split);
+        s.cplx_split = nullptr;
+        const std::vector<BatchProblem> pr{BatchProblem{M, N, K, A, lda, B, ldb, C, ldc}};
+        std::string err;
+        int status = F4J_OK;
+        s.cplx_split = split_batch_create(true, opA, opB, carrier, S_, use3m != 0, c.split_check_rows, pr, &err,
+                                          &status);
+        if (!s.cplx_split) return fail(status, err);
+        s.cplx_split_key = key;
+    }
+    SplitBatch *b = s.cplx_split;
     hipEvent_t *ev = c.split_timing ? s.tev : nullptr;
     hipError_t e = split_batch_enqueue(b, stream, ev ? ev[0] : nullptr, ev ? ev[1] : nullptr);
     f4j_split_report rep{};
@@ -565,8 +592,11 @@ int run_split_cplx(int opA, int opB, int64_t M, int64_t N, int64_t K, const doub
     double perr = -1.0;
     if (e == hipSuccess) e = split_batch_errors(b, &perr, 1, stream);      // synchronises
     if (e == hipSuccess) e = hipStreamSynchronize(stream);
-    split_batch_destroy(b);
-    if (e != hipSuccess) return hip_fail(e, "complex split gemm");
+    if (e != hipSuccess) {
+        split_batch_destroy(s.cplx_split);
+        s.cplx_split = nullptr;
+        return hip_fail(e, "complex split gemm");
+    }
     if (ev) {
         float ms = 0.f;
         F4J_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]), "hipEventElapsedTime");
@@ -1180,6 +1210,11 @@ int f4j_shutdown(void) {
     for (Stage &st : s.stages) {
         std::lock_guard<std::mutex> sg(st.mu);
         stage_release(st);
+    }
+    {
+        std::lock_guard<std::mutex> wg(s.ws_mu);
+        split_batch_destroy(s.cplx_split);
+        s.cplx_split = nullptr;
     }
     for (DevBuf *b : {&s.ws, &s.mixws, &s.lsws})
         if (b->p) {
