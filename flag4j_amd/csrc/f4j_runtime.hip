@@ -122,6 +122,13 @@ struct State {
     };
     SplitBatch *cplx_split = nullptr;
     CplxSplitKey cplx_split_key{};
+    // Pinned host words the split forms read back (non-finite flag, checked error),
+    // so those copies do not stall the host between launches (guarded by ws_mu).
+    struct Readback {
+        int nonfinite;
+        double err;
+    };
+    Readback *rb = nullptr;
 };
 
 State &S() {
@@ -169,6 +176,7 @@ int init_locked(State &s, int device) {
     F4J_HIP(hipEventCreateWithFlags(&s.join_ev, hipEventDisableTiming), "hipEventCreate");
     F4J_HIP(hipEventCreateWithFlags(&s.split_fork_ev, hipEventDisableTiming), "hipEventCreate");
     F4J_HIP(hipEventCreateWithFlags(&s.split_join_ev, hipEventDisableTiming), "hipEventCreate");
+    if (!s.rb) F4J_HIP(hipHostMalloc((void **)&s.rb, sizeof(State::Readback), hipHostMallocDefault), "hipHostMalloc");
     s.device = device;
     s.inited = true;
     return F4J_OK;
@@ -494,17 +502,18 @@ int run_split(int opA, int opB, int64_t M, int64_t N, int64_t K, const double *A
     rep.max_rel_err = -1.0;
     // NaN or Inf among the inputs (flagged by the exponent kernels): the digit
     // slices cannot carry them; FP64 arithmetic propagates them as the reference's does.
-    int nonfinite = 0;
-    F4J_HIP(hipMemcpyAsync(&nonfinite, (char *)s.ws.p + pl.off_flag, sizeof(int), hipMemcpyDeviceToHost, stream),
-            "copy flag");
-    double err = 0.0;
-    if (rep.checked_rows > 0) {
+    // The check's kernels go first, then both read-backs into pinned memory: one wait.
+    if (rep.checked_rows > 0)
         F4J_HIP(launch_split_residual(pl, opA, opB, M, N, K, A, lda, B, ldb, C, ldc, rep.checked_rows, s.ws.p, stream),
                 "split residual launch");
-        F4J_HIP(hipMemcpyAsync(&err, (char *)s.ws.p + pl.off_err, sizeof(double), hipMemcpyDeviceToHost, stream),
+    F4J_HIP(hipMemcpyAsync(&s.rb->nonfinite, (char *)s.ws.p + pl.off_flag, sizeof(int), hipMemcpyDeviceToHost, stream),
+            "copy flag");
+    if (rep.checked_rows > 0)
+        F4J_HIP(hipMemcpyAsync(&s.rb->err, (char *)s.ws.p + pl.off_err, sizeof(double), hipMemcpyDeviceToHost, stream),
                 "copy residual");
-    }
     F4J_HIP(hipStreamSynchronize(stream), "stream synchronize");
+    const int nonfinite = s.rb->nonfinite;
+    const double err = rep.checked_rows > 0 ? s.rb->err : 0.0;
     if (nonfinite) {
         GemmProblem p{M, N, K, A, lda, B, ldb, C, ldc, 0};
         F4J_HIP(launch_gemm_f64(opA ? R_T : R_N, opB ? R_T : R_N, p, stream, pick_tile(M, N)),
@@ -720,17 +729,20 @@ int run_crt(bool cplx, int opA, int opB, int64_t M, int64_t N, int64_t K, const 
     // NaN or Inf among the inputs (flagged by the exponent kernels): the
     // residues cannot carry them, so the product is recomputed in FP64, whose
     // arithmetic propagates them as the reference's does.
-    int nonfinite = 0;
-    F4J_HIP(hipMemcpyAsync(&nonfinite, w + pl.off_flag, sizeof(int), hipMemcpyDeviceToHost, stream), "copy flag");
-    double err = 0.0;
+    // The check's kernels go first, then both read-backs into pinned memory: one wait.
     if (rep.checked_rows > 0) {
         unsigned long long *eb = (unsigned long long *)(w + pl.off_err);
         F4J_HIP(hipStreamWaitEvent(stream, s.join_ev, 0), "hipStreamWaitEvent");
         F4J_HIP(launch_residual_combine(cplx, M, N, K, C, ldc, rep.checked_rows, part, eb, stream),
                 "crt residual launch");
-        F4J_HIP(hipMemcpyAsync(&err, w + pl.off_err, sizeof(double), hipMemcpyDeviceToHost, stream), "copy residual");
     }
+    F4J_HIP(hipMemcpyAsync(&s.rb->nonfinite, w + pl.off_flag, sizeof(int), hipMemcpyDeviceToHost, stream), "copy flag");
+    if (rep.checked_rows > 0)
+        F4J_HIP(hipMemcpyAsync(&s.rb->err, w + pl.off_err, sizeof(double), hipMemcpyDeviceToHost, stream),
+                "copy residual");
     F4J_HIP(hipStreamSynchronize(stream), "stream synchronize");
+    const int nonfinite = s.rb->nonfinite;
+    const double err = rep.checked_rows > 0 ? s.rb->err : 0.0;
     if (nonfinite) {
         F4J_HIP(launch_fp64(cplx, opA, opB, M, N, K, A, lda, B, ldb, C, ldc, stream), "fp64 for non-finite inputs");
         rep.precision = F4J_PREC_FP64;
@@ -1215,6 +1227,10 @@ int f4j_shutdown(void) {
         std::lock_guard<std::mutex> wg(s.ws_mu);
         split_batch_destroy(s.cplx_split);
         s.cplx_split = nullptr;
+    }
+    if (s.rb) {
+        F4J_HIP(hipHostFree(s.rb), "hipHostFree");
+        s.rb = nullptr;
     }
     for (DevBuf *b : {&s.ws, &s.mixws, &s.lsws})
         if (b->p) {
