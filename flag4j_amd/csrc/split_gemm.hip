@@ -40,7 +40,10 @@
 #include <cstdint>
 #include <vector>
 
+#include "amdgcn.h"
 #include "f4j_internal.h"
+#include "gemm_tile.h"
+#include "slice_frag.h"
 #include "split_gemm.h"
 #include "split_tile.h"
 
@@ -48,6 +51,8 @@ namespace f4j {
 namespace {
 
 using slice::digits;
+using slice::f32x16;
+using slice::mfma32;
 using slice::u32x4;
 
 
@@ -106,9 +111,16 @@ __global__ __launch_bounds__(256) void exp_cols_kernel(const double *X, int64_t 
 }
 
 // ------------------------------------------------------------------------
-// Digit split.  Output: S slices, slice s at out + s*Xp*Kp, row-major Xp x Kp
-// (k contiguous), zero outside the X x K matrix.
+// Digit split.  Output: S slices, slice s at out + s*Xp*Kp, zero outside the
+// X x K matrix; each slice fragment-major (dfrag::frag_offset with k counted in
+// bytes: a thread's 8 consecutive digits of one line, k a multiple of 8, are one
+// MFMA lane's 16 bytes).
 // ------------------------------------------------------------------------
+
+template <class T>
+__device__ inline T *digits_at(T *out, int s, int64_t Xp, int64_t Kp, int64_t x, int64_t k0) {
+    return (T *)((char *)out + (int64_t)s * Xp * Kp * 2 + dfrag::frag_offset(x, 2 * k0, Kp / 16));
+}
 
 // Lines contiguous: thread handles 8 consecutive k of one line.
 template <int R, int S, class T>
@@ -131,7 +143,7 @@ __global__ __launch_bounds__(256) void split_lines_kernel(const double *X, int64
         T w[8];
 #pragma unroll
         for (int u = 0; u < 8; ++u) w[u] = d[u][s];
-        *(u32x4 *)(out + (int64_t)s * Xp * Kp + x * Kp + k0) = __builtin_bit_cast(u32x4, w);
+        *(u32x4 *)digits_at(out, s, Xp, Kp, x, k0) = __builtin_bit_cast(u32x4, w);
     }
 }
 
@@ -162,7 +174,7 @@ __global__ __launch_bounds__(256) void split_cols_kernel(const double *X, int64_
             T w[8];
 #pragma unroll
             for (int u = 0; u < 8; ++u) w[u] = d[u][s];
-            *(u32x4 *)(out + (int64_t)s * Xp * Kp + x * Kp + k0 + kg) = __builtin_bit_cast(u32x4, w);
+            *(u32x4 *)digits_at(out, s, Xp, Kp, x, k0 + kg) = __builtin_bit_cast(u32x4, w);
         }
     }
 }
@@ -174,37 +186,133 @@ __global__ __launch_bounds__(256) void split_cols_kernel(const double *X, int64_
 // result item r of lane l is row 8(r/4) + 4(l/32) + r%4, column l%32.
 // ------------------------------------------------------------------------
 
-constexpr int SBM = slice::BM, SBN = slice::BN;
+constexpr int SBM = dfrag::BM, SBN = dfrag::BN;
 
-// Digit pairs of one launch, read from the kernel arguments.
-struct ArgSegs {
-    const SliceGemmArgs &args;
-    __device__ int a(int g) const { return args.seg_a[g]; }
-    __device__ int b(int g) const { return args.seg_b[g]; }
-    __device__ int shift(int g) const { return args.seg_shift[g]; }
-};
-
+// Fragment-major digit-slice products (slice_frag.h): a block computes a
+// 256 x 128 tile with 8 waves of 64 x 64 over the launch's digit pairs, as one
+// stream of 128-byte k steps (tps per pair); both operands' fragments go from
+// the slices straight into a two-stage LDS ring by direct-to-LDS loads, each
+// fragment loaded once per block, one barrier per step.  The FP32 accumulators
+// are exact over FLUSH steps (BF16: 4 = 256 digits, FP16: 1 = 64) and are added,
+// times 2^shift of their pair, into FP64 ones at least every FLUSH steps and at
+// the end of each pair (the sums are exact integers, so where the flushes fall
+// does not change the result).  At the barrier of step t every wave has read stage t % 2 and
+// has its own loads of step t + 1 retired; after it step t + 2 goes into stage
+// t % 2 (past the last step the last one again, so every step issues the same
+// loads and the wait counts stay exact).
 template <class T, int FLUSH>
-__global__ __launch_bounds__(256, 2) void slice_gemm_kernel(SliceGemmArgs a) {
-    __shared__ __attribute__((aligned(16))) T lds[2 * slice::STAGE];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int64_t tiles_n = a.Np / SBN;
-    const int64_t m0 = (blockIdx.x / tiles_n) * SBM, n0 = (blockIdx.x % tiles_n) * SBN;
-    const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
+__global__ __launch_bounds__(512, 2) void slice_frag_kernel(SliceGemmArgs a) {
+    __shared__ __attribute__((aligned(1024))) unsigned char lds[dfrag::STAGES * dfrag::STAGE];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);   // uniform: keeps descriptors scalar
+    int64_t tm, tn;
+    tile_of(blockIdx.x, a.Mp / SBM, a.Np / SBN, tm, tn);
+    const int64_t m0 = (int64_t)__builtin_amdgcn_readfirstlane((int)tm) * SBM;
+    const int64_t n0 = (int64_t)__builtin_amdgcn_readfirstlane((int)tn) * SBN;
+
+    f32x16 acc[2][2];
     double acc64[2][2][16];
-    slice::tile_loop<T, FLUSH, 2, 2>(a.A, a.B, a.Mp, a.Np, a.Kp, a.nseg, ArgSegs{a}, m0, n0, lds, acc64);
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                acc[i][j][r] = 0.f;
+                acc64[i][j][r] = 0.0;
+            }
+
+    const int64_t Kpb = 2 * a.Kp, KC = Kpb >> 5;
+    const int tps = (int)(Kpb / dfrag::SK);          // steps per digit pair
+    const int nt = a.nseg * tps;
+    const bool is_a = dfrag::loads_a(wave);
+    const unsigned char *op = (const unsigned char *)(is_a ? a.A : a.B);
+    const int64_t slice_bytes = (is_a ? a.Mp : a.Np) * Kpb;
+    const int64_t x0 = is_a ? m0 : n0;
+
+    auto issue = [&](int slot, int t) {              // step t into stage slot % STAGES
+        const int seg = t / tps, kt = t - seg * tps;
+        const int sl = is_a ? a.seg_a[seg] : a.seg_b[seg];
+        const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+            (void *)(op + (int64_t)sl * slice_bytes + dfrag::load_src(x0, wave, 0, kt, KC)), (short)0, 0x7FFFFFF0,
+            0x00020000);
+        auto loads = [&](auto count) {
+#pragma unroll
+            for (int p = 0; p < decltype(count)::value; ++p)
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(
+                    rs, (lds_void *)(lds + dfrag::load_lds(slot, wave, p)), 16, dfrag::load_voff(lane),
+                    (int)(dfrag::load_src(x0, wave, p, kt, KC) - dfrag::load_src(x0, wave, 0, kt, KC)), 0, 0);
+        };
+        if (is_a)                                    // uniform: A waves load LOADS_A fragments, B waves LOADS_B
+            loads(std::integral_constant<int, dfrag::LOADS_A>{});
+        else
+            loads(std::integral_constant<int, dfrag::LOADS_B>{});
+    };
+    static_assert(dfrag::STAGES == 2, "the waits below assume two stages");
+    issue(0, 0);
+    F4J_WAIT_VMCNT(0);
+    __builtin_amdgcn_s_barrier();
+    issue(1, min(1, nt - 1));
+    u32x4 fa[2][2], fb[2][2];
+    // The two waves of a SIMD (w and w + 4) flush half an interval apart, so one's
+    // FP64 work runs beside the other's MFMAs; every interval stays <= FLUSH steps.
+    int since = dfrag::loads_a(wave) ? 0 : FLUSH / 2;
+    for (int t = 0; t < nt; ++t) {
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int i = 0; i < 2; ++i) fa[0][i] = *(const u32x4 *)(lds + dfrag::a_frag(t, wave, i, 0, lane));
+#pragma unroll
+        for (int j = 0; j < 2; ++j) fb[0][j] = *(const u32x4 *)(lds + dfrag::b_frag(t, wave, j, 0, lane));
+#pragma unroll
+        for (int ks = 0; ks < dfrag::KB; ++ks) {
+            if (ks + 1 < dfrag::KB) {
+#pragma unroll
+                for (int i = 0; i < 2; ++i)
+                    fa[(ks + 1) & 1][i] = *(const u32x4 *)(lds + dfrag::a_frag(t, wave, i, ks + 1, lane));
+#pragma unroll
+                for (int j = 0; j < 2; ++j)
+                    fb[(ks + 1) & 1][j] = *(const u32x4 *)(lds + dfrag::b_frag(t, wave, j, ks + 1, lane));
+            }
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j) acc[i][j] = mfma32<T>(fa[ks & 1][i], fb[ks & 1][j], acc[i][j]);
+        }
+        const int seg = t / tps;
+        if (++since == FLUSH || t + 1 == (seg + 1) * tps) {   // uniform
+            const double sc = __builtin_bit_cast(double, (uint64_t)(1023 + a.seg_shift[seg]) << 52);
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j)
+#pragma unroll
+                    for (int r = 0; r < 16; ++r) {
+                        acc64[i][j][r] = fma((double)acc[i][j][r], sc, acc64[i][j][r]);
+                        acc[i][j][r] = 0.f;
+                    }
+            since = 0;
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        F4J_WAIT_VMCNT(0);                           // this wave's loads of step t + 1 are in LDS
+        __builtin_amdgcn_s_waitcnt(0xC07F);          // lgkmcnt(0): its reads of stage t % 2 are done
+        __builtin_amdgcn_s_barrier();
+        __builtin_amdgcn_sched_barrier(0);
+        issue(t, min(t + 2, nt - 1));                // step t + 2 into stage t % 2
+    }
+    F4J_WAIT_VMCNT(0);                               // no LDS write in flight when the block ends
 
     // Epilogue: C = beta C + 2^(eA+eB+base) acc64.
+    const int64_t rw = m0 + dfrag::wave_row(wave), cw = n0 + dfrag::wave_col(wave);
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
-            const int64_t col = n0 + slice::acc_col(wn, j, lane);
+            const int64_t col = cw + dfrag::acc_col(j, lane);
             if (col >= a.N) continue;
             const int eb = a.eB[col];
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
-                const int64_t row = m0 + slice::acc_row(wm, i, r, lane);
+                const int64_t row = rw + dfrag::acc_row(i, r, lane);
                 if (row >= a.M) continue;
                 const int ea = a.eA[row];
                 double v = 0.0;
@@ -494,10 +602,13 @@ hipError_t launch_split_gemm(const SplitPlan &pl, int opA, int opB, int64_t M, i
             a.seg_shift[q] = g.seg_shift[q];
         }
         const unsigned blocks = (unsigned)((pl.Mp / SBM) * (pl.Np / SBN));
+        // flush every 64 digits (FP16: 1 step) or 256 (BF16: 4 steps), split_flush()
         if (fp16)
-            hipLaunchKernelGGL((slice_gemm_kernel<_Float16, 64>), dim3(blocks), dim3(256), 0, stream, a);
+            hipLaunchKernelGGL((slice_frag_kernel<_Float16, 64 / (dfrag::SK / 2)>), dim3(blocks), dim3(64 * dfrag::WAVES),
+                               0, stream, a);
         else
-            hipLaunchKernelGGL((slice_gemm_kernel<__bf16, 256>), dim3(blocks), dim3(256), 0, stream, a);
+            hipLaunchKernelGGL((slice_frag_kernel<__bf16, 256 / (dfrag::SK / 2)>), dim3(blocks),
+                               dim3(64 * dfrag::WAVES), 0, stream, a);
         first = false;
     }
     if (ev_end) (void)hipEventRecord(ev_end, stream);

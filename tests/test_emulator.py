@@ -68,6 +68,27 @@ def test_crt_kernel_index_math(crt_emu, shape):
     assert r.returncode == 0 and r.stdout.strip().startswith("OK"), r.stdout
 
 
+SLICE_SRC = os.path.join(ROOT, "tests", "emu", "slice_emu.cpp")
+SLICE_BIN = os.path.join(ROOT, "_build", "slice_emu")
+
+
+@pytest.fixture(scope="module")
+def slice_emu():
+    """Host emulation of the fragment-major digit-slice product (digit split writes,
+    LDS ring, fragment reads, FP32 flushes into FP64, epilogue) with the kernel's own
+    offset functions."""
+    deps = [os.path.join(ROOT, "flag4j_amd", "csrc", h) for h in ("gemm_tile.h", "slice_frag.h")]
+    return build_once(SLICE_SRC, SLICE_BIN, deps)
+
+
+@pytest.mark.parametrize("shape", [(1, 1, 1, 1, 4), (37, 45, 29, 2, 4), (300, 200, 500, 3, 4), (257, 129, 384, 4, 1),
+                                   (260, 140, 300, 3, 1), (513, 257, 256, 2, 4)])
+def test_digit_slice_kernel_index_math(slice_emu, shape):
+    """shape = (M, N, K, digits, flush steps): BF16 flushes every 4 steps, FP16 every step."""
+    r = subprocess.run([slice_emu, *map(str, shape)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0 and r.stdout.strip().startswith("OK"), r.stdout
+
+
 RES_SRC = os.path.join(ROOT, "tests", "emu", "residue_emu.cpp")
 RES_BIN = os.path.join(ROOT, "_build", "residue_emu")
 

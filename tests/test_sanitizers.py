@@ -53,6 +53,7 @@ EMU_CASES = {
     "crt_emu": [(7, 300, 13), (130, 70, 257), (257, 129, 384)],
     "sched_emu": [(5, 3, 2), (200, 512, 3), (1000, 100, 6)],
     "residue_emu": [()],
+    "slice_emu": [(37, 45, 29, 2, 4), (257, 129, 300, 3, 1)],
 }
 
 
