@@ -26,7 +26,7 @@ struct SplitGroup {
 
 struct SplitPlan {
     int S = 0, carrier = SPLIT_BF16;
-    int64_t Mp = 0, Np = 0, Kp = 0;     // padded sizes of the slice arrays
+    int64_t Mp = 0, Np = 0, Kp = 0;     // padded sizes of the slice arrays (256 rows, 128 columns, 256 digits)
     size_t off_slices_a = 0, off_slices_b = 0, off_exp_a = 0, off_exp_b = 0, off_err = 0, off_residual = 0, bytes = 0;
     size_t off_flag = 0;                 // int: set by the exponent kernels when an operand holds NaN or Inf
     std::vector<SplitGroup> groups;     // in launch order (smallest weights first)
@@ -34,7 +34,7 @@ struct SplitPlan {
 };
 
 struct SliceGemmArgs {
-    const void *A, *B;                  // slice arrays [S][Xp][Kp], k contiguous
+    const void *A, *B;                  // digit slices [S][Xp x Kp], fragment-major (slice_frag.h)
     int64_t Mp, Np, Kp, M, N;
     const int *eA, *eB;                 // row / column exponents
     double *C;

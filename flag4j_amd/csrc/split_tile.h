@@ -1,7 +1,8 @@
-// Device building blocks of the split-precision (Ozaki-style) GEMM, shared by
-// the single-product kernel (split_gemm.hip) and the batched kernel
-// (split_batch.hip): the digit split of one FP64 value and the k loop of one
-// output tile over a list of digit pairs (segments).
+// Device building blocks of the split-precision (Ozaki-style) GEMM: the digit
+// split of one FP64 value and the MFMA wrapper (shared by the single-product
+// kernels in split_gemm.hip and the batched ones in split_batch.hip), and the k
+// loop of one output tile over a list of digit pairs (segments) of the batched
+// slice-product kernel.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -26,8 +27,6 @@ struct Tile {
     static constexpr int BM = 64 * BI, BN = 64 * BJ;
     static constexpr int STAGE = (BM + BN) * LD;   // elements per LDS stage
 };
-constexpr int BM = Tile<2, 2>::BM, BN = Tile<2, 2>::BN;   // single-product tile
-constexpr int STAGE = Tile<2, 2>::STAGE;
 
 // Signed radix-2^R digits of v 2^-e (|v| < 2^e): v 2^-e = 2^-(R-1) (d_0 + 2^-R (d_1 + ...)).
 // Every step is exact in FP64; |d_s| <= 2^(R-1) fits the carrier T exactly.
