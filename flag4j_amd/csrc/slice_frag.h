@@ -33,7 +33,7 @@ constexpr int WAVES = 8;
 constexpr int SK = 128;                       // k bytes (64 digits) per step
 constexpr int KB = SK / 32;                   // 32-byte MFMA k steps (k blocks) per step
 constexpr int A_LB = BM / 32, B_LB = BN / 32; // line blocks of A and B in a tile
-constexpr int STAGES = 2;
+constexpr int STAGES = 3;
 constexpr int LOADS_A = 2 * KB;               // fragment loads per step of an A wave (waves 0-3)
 constexpr int LOADS_B = KB;                   // and of a B wave (waves 4-7)
 constexpr int STAGE = (A_LB + B_LB) * KB * 1024;   // LDS bytes of one step: A's fragments, then B's

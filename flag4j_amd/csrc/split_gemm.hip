@@ -191,15 +191,16 @@ constexpr int SBM = dfrag::BM, SBN = dfrag::BN;
 // Fragment-major digit-slice products (slice_frag.h): a block computes a
 // 256 x 128 tile with 8 waves of 64 x 64 over the launch's digit pairs, as one
 // stream of 128-byte k steps (tps per pair); both operands' fragments go from
-// the slices straight into a two-stage LDS ring by direct-to-LDS loads, each
-// fragment loaded once per block, one barrier per step.  The FP32 accumulators
+// the slices straight into a three-stage LDS ring (144 KB) by direct-to-LDS
+// loads, each fragment loaded once per block, one barrier per step.  The FP32 accumulators
 // are exact over FLUSH steps (BF16: 4 = 256 digits, FP16: 1 = 64) and are added,
 // times 2^shift of their pair, into FP64 ones at least every FLUSH steps and at
 // the end of each pair (the sums are exact integers, so where the flushes fall
-// does not change the result).  At the barrier of step t every wave has read stage t % 2 and
-// has its own loads of step t + 1 retired; after it step t + 2 goes into stage
-// t % 2 (past the last step the last one again, so every step issues the same
-// loads and the wait counts stay exact).
+// does not change the result).  At the barrier of step t every wave has read
+// stage t % STAGES and has its own loads of step t + 1 retired (those of the
+// STAGES - 2 later steps may still be in flight); after it step t + STAGES goes
+// into stage t % STAGES (past the last step the last one again, so every step
+// issues the same loads and the wait counts stay exact).
 template <class T, int FLUSH>
 __global__ __launch_bounds__(512, 2) void slice_frag_kernel(SliceGemmArgs a) {
     __shared__ __attribute__((aligned(1024))) unsigned char lds[dfrag::STAGES * dfrag::STAGE];
@@ -248,11 +249,20 @@ __global__ __launch_bounds__(512, 2) void slice_frag_kernel(SliceGemmArgs a) {
         else
             loads(std::integral_constant<int, dfrag::LOADS_B>{});
     };
-    static_assert(dfrag::STAGES == 2, "the waits below assume two stages");
-    issue(0, 0);
-    F4J_WAIT_VMCNT(0);
+    // Wait until this wave's loads of the oldest outstanding step are in LDS (the
+    // STAGES - 2 newer steps' loads may stay in flight).
+    auto wait_oldest = [&]() {
+        static_assert(dfrag::STAGES >= 2 && (dfrag::STAGES - 2) * dfrag::LOADS_A < 64, "vmcnt holds 0..63");
+        if (is_a)
+            F4J_WAIT_VMCNT((dfrag::STAGES - 2) * dfrag::LOADS_A);
+        else
+            F4J_WAIT_VMCNT((dfrag::STAGES - 2) * dfrag::LOADS_B);
+    };
+#pragma unroll
+    for (int s = 0; s + 1 < dfrag::STAGES; ++s) issue(s, min(s, nt - 1));
+    wait_oldest();
     __builtin_amdgcn_s_barrier();
-    issue(1, min(1, nt - 1));
+    issue(dfrag::STAGES - 1, min(dfrag::STAGES - 1, nt - 1));
     u32x4 fa[2][2], fb[2][2];
     // The two waves of a SIMD (w and w + 4) flush half an interval apart, so one's
     // FP64 work runs beside the other's MFMAs; every interval stays <= FLUSH steps.
@@ -293,11 +303,11 @@ __global__ __launch_bounds__(512, 2) void slice_frag_kernel(SliceGemmArgs a) {
             since = 0;
         }
         __builtin_amdgcn_sched_barrier(0);
-        F4J_WAIT_VMCNT(0);                           // this wave's loads of step t + 1 are in LDS
-        __builtin_amdgcn_s_waitcnt(0xC07F);          // lgkmcnt(0): its reads of stage t % 2 are done
+        wait_oldest();                               // this wave's loads of step t + 1 are in LDS
+        __builtin_amdgcn_s_waitcnt(0xC07F);          // lgkmcnt(0): its reads of stage t % STAGES are done
         __builtin_amdgcn_s_barrier();
         __builtin_amdgcn_sched_barrier(0);
-        issue(t, min(t + 2, nt - 1));                // step t + 2 into stage t % 2
+        issue(t, min(t + dfrag::STAGES, nt - 1));    // step t + STAGES into stage t % STAGES
     }
     F4J_WAIT_VMCNT(0);                               // no LDS write in flight when the block ends
 
