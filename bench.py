@@ -501,13 +501,14 @@ def bench_gemv(np, torch, f4j, dA, x, stream, reps=50):
                         "us": round(usc, 2), "gb_s": round(gbc, 1), "frac": round(gbc / PEAK_HBM_GBS, 4)}}
 
 
-def load_traffic(name):
-    """HBM bytes per launch from the committed rocprofv3 --pmc summary, if present."""
+def load_traffic(name, key="traffic_bytes_per_launch"):
+    """HBM bytes per launch (or per product, as `key` names) from the committed
+    rocprofv3 --pmc summary, if present."""
     path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None
     for line in open(path):
-        if line.startswith("traffic_bytes_per_launch:"):
+        if line.startswith(key + ":"):
             return float(line.split(":")[1])
     return None
 
@@ -609,16 +610,18 @@ def main():
 
     def roofline(prec):
         kern_ms, products = timings[prec]
+        key = "traffic_bytes_per_launch"
         if prec == "fp64":
             achieved, peak, unit, prof = flops / (kern_ms * 1e-3) / 1e12, PEAK_FP64_TFLOPS, "TFLOP/s", "dgemm_f64"
         elif prec == "split_int8":
             achieved, peak, unit, prof = (products * flops / (kern_ms * 1e-3) / 1e12, PEAK_INT8_TOPS, "TOP/s",
                                           "crt_gemm_i8")
-        else:
+        else:   # digit forms: kern_ms covers both level-group launches of a product
             achieved, peak, unit, prof = (products * flops / (kern_ms * 1e-3) / 1e12, PEAK_BF16_TFLOPS, "TFLOP/s",
-                                          "slice_gemm_" + prec[6:])
+                                          "slice_frag")
+            key = "traffic_bytes_per_product_" + prec[6:]
         return {"bound": "mfma", "achieved": round(achieved, 3), "peak": peak, "unit": unit,
-                "frac": round(achieved / peak, 4), "traffic": load_traffic(f"{prof}_{n}_pmc.txt"),
+                "frac": round(achieved / peak, 4), "traffic": load_traffic(f"{prof}_{n}_pmc.txt", key),
                 "kernel_ms": round(kern_ms, 4)}
 
     def vendor_int8():
