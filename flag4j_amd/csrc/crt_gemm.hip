@@ -146,7 +146,6 @@ int t_for(int n, int64_t K) {
 // ------------------------------------------------------------------------
 
 using crt::pack4;
-using crt::scaled_int;
 
 __device__ inline double opnd_value(const CrtOperand &o, int64_t x, int64_t k) {
     const double *p = o.X + x * o.sx + k * o.sk;
@@ -249,31 +248,34 @@ __global__ __launch_bounds__(256) void crt_split_kernel(CrtOperand o, int64_t nx
     const int64_t x = x0 + r;
     const int ex = x < nx ? e[x] : INT_MIN;
     // scaled integers, split once (v = vh 2^26 + vl) for the per-modulus residues
+    const int sh = crt::scaled_shift(ex, tb.t);
     double vh[16], vl[16];
 #pragma unroll
     for (int u = 0; u < 16; ++u) {
-        const double v = scaled_int(tile[r][kb + u], ex, tb.t);
+        const double v = crt::scaled_int_sh(tile[r][kb + u], sh);
         if (WIDE) {
             vh[u] = v;               // kept whole: the wide step reduces it per modulus
             vl[u] = 0.0;
         } else {
             crt::residue_split(v, vh[u], vl[u]);
+            vl[u] = crt::residue_bias(vl[u]);     // the biased form (crt_residue.h)
         }
     }
     const int64_t KC = Kp >> 5;
     int4 *dst = (int4 *)(out + crt::frag_offset(x, k0 + kb, KC));
     const int64_t lstride = Xp * Kp / 16;
     for (int l = 0; l < tb.n; ++l) {
-        const double m = (double)tb.m[l], mi = tb.minv[l], c26 = tb.c26[l];
-        int rs[16];
+        const double m = (double)tb.m[l], mi = tb.minv[l], c26 = tb.c26[l], cq = crt::residue_cq(mi);
+        unsigned rs[16];
 #pragma unroll
         for (int u = 0; u < 16; ++u)
-            rs[u] = WIDE ? crt::residue_wide(vh[u], m, mi, c26) : crt::residue_of(vh[u], vl[u], m, mi, c26);
+            rs[u] = WIDE ? crt::residue_wide_biased(vh[u], m, mi, cq, c26)
+                         : crt::residue_biased(vh[u], vl[u], m, mi, cq, c26);
         int4 w;
-        w.x = (int)pack4(rs[0], rs[1], rs[2], rs[3]);
-        w.y = (int)pack4(rs[4], rs[5], rs[6], rs[7]);
-        w.z = (int)pack4(rs[8], rs[9], rs[10], rs[11]);
-        w.w = (int)pack4(rs[12], rs[13], rs[14], rs[15]);
+        w.x = (int)crt::pack4_biased(rs[0], rs[1], rs[2], rs[3]);
+        w.y = (int)crt::pack4_biased(rs[4], rs[5], rs[6], rs[7]);
+        w.z = (int)crt::pack4_biased(rs[8], rs[9], rs[10], rs[11]);
+        w.w = (int)crt::pack4_biased(rs[12], rs[13], rs[14], rs[15]);
         dst[l * lstride] = w;
     }
 }
@@ -298,31 +300,34 @@ __global__ __launch_bounds__(256) void crt_split_cols_kernel(CrtOperand o, int64
     const int xl = crt::split_cols_line(t), kb = crt::split_cols_k(t);
     const int64_t x = x0 + xl;
     const int ex = x < nx ? e[x] : INT_MIN;
+    const int sh = crt::scaled_shift(ex, tb.t);
     double vh[16], vl[16];
 #pragma unroll
     for (int u = 0; u < 16; ++u) {
-        const double v = scaled_int(tile[xl][kb + u], ex, tb.t);
+        const double v = crt::scaled_int_sh(tile[xl][kb + u], sh);
         if (WIDE) {
             vh[u] = v;
             vl[u] = 0.0;
         } else {
             crt::residue_split(v, vh[u], vl[u]);
+            vl[u] = crt::residue_bias(vl[u]);     // the biased form (crt_residue.h)
         }
     }
     const int64_t KC = Kp >> 5;
     int4 *dst = (int4 *)(out + crt::frag_offset(x, k0 + kb, KC));
     const int64_t lstride = Xp * Kp / 16;
     for (int l = 0; l < tb.n; ++l) {
-        const double m = (double)tb.m[l], mi = tb.minv[l], c26 = tb.c26[l];
-        int rs[16];
+        const double m = (double)tb.m[l], mi = tb.minv[l], c26 = tb.c26[l], cq = crt::residue_cq(mi);
+        unsigned rs[16];
 #pragma unroll
         for (int u = 0; u < 16; ++u)
-            rs[u] = WIDE ? crt::residue_wide(vh[u], m, mi, c26) : crt::residue_of(vh[u], vl[u], m, mi, c26);
+            rs[u] = WIDE ? crt::residue_wide_biased(vh[u], m, mi, cq, c26)
+                         : crt::residue_biased(vh[u], vl[u], m, mi, cq, c26);
         int4 w;
-        w.x = (int)pack4(rs[0], rs[1], rs[2], rs[3]);
-        w.y = (int)pack4(rs[4], rs[5], rs[6], rs[7]);
-        w.z = (int)pack4(rs[8], rs[9], rs[10], rs[11]);
-        w.w = (int)pack4(rs[12], rs[13], rs[14], rs[15]);
+        w.x = (int)crt::pack4_biased(rs[0], rs[1], rs[2], rs[3]);
+        w.y = (int)crt::pack4_biased(rs[4], rs[5], rs[6], rs[7]);
+        w.z = (int)crt::pack4_biased(rs[8], rs[9], rs[10], rs[11]);
+        w.w = (int)crt::pack4_biased(rs[12], rs[13], rs[14], rs[15]);
         dst[l * lstride] = w;
     }
 }

@@ -17,6 +17,14 @@
 // 128 is the byte of -128: every residue fits a signed byte with no correction.
 // Wider v (2^58 <= |v| <= 2^75) is first reduced to |v| < 2^24 by one exact step
 // v - rint(v / m) m (the computed quotient is off by at most 2^16).
+// The split kernels evaluate the same steps biased by 2^52 + 128 (residue_biased):
+//   U = vh c26 + (vl + 2^52 + 128)   exact, an integer in (2^52 - 2^41, 2^52 + 2^41);
+//   q = floor(U minv - 2^52 minv)    one fma: 2^52 minv is exact, so this is the
+//                                     same rounding of (u + 128) / m as above, the same q;
+//   R = U - q m = 2^52 + r + 128     exact, in [2^52, 2^52 + 256], so the low word of
+//                                     R's bits is r + 128 with no conversion;
+// and four such words pack into the four signed bytes r with one XOR of 0x80 per
+// byte ((r + 128) mod 256 XOR 128 is the byte of r for r in [-128, 128]).
 //
 // Product residues.  For an int32 accumulator a, r = a - floor(a / m) m with the
 // quotient from one multiply (error below 2^-28, fractional parts multiples of
@@ -53,6 +61,32 @@ F4J_HD int residue_wide(double v, double m, double minv, double c26) {
     return residue_of(0.0, v1, m, minv, c26);
 }
 
+// Biased form (header comment): vb = residue_bias(vl) once per entry, cq =
+// residue_cq(minv) once per modulus; the result is r + 128, r = residue_of(vh, vl, ...).
+constexpr double BIAS = 4503599627370496.0 + 128.0;         // 2^52 + 128
+F4J_HD double residue_bias(double vl) { return vl + BIAS; }
+F4J_HD double residue_cq(double minv) { return -4503599627370496.0 * minv; }
+F4J_HD unsigned residue_biased(double vh, double vb, double m, double minv, double cq, double c26) {
+    const double U = fma(vh, c26, vb);
+    const double q = floor(fma(U, minv, cq));
+    return (unsigned)__builtin_bit_cast(uint64_t, fma(-q, m, U));
+}
+F4J_HD unsigned residue_wide_biased(double v, double m, double minv, double cq, double c26) {
+    const double v1 = fma(-rint(v * minv), m, v);
+    return residue_biased(0.0, residue_bias(v1), m, minv, cq, c26);
+}
+// Four biased residues (r + 128, first in the lowest byte) as the signed bytes r:
+// three byte permutes and one XOR on the device.
+F4J_HD unsigned pack4_biased(unsigned a, unsigned b, unsigned c, unsigned d) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const unsigned ab = __builtin_amdgcn_perm(b, a, 0x0c0c0400u);     // bytes a0 b0 0 0
+    const unsigned cd = __builtin_amdgcn_perm(d, c, 0x0c0c0400u);     // bytes c0 d0 0 0
+    return __builtin_amdgcn_perm(cd, ab, 0x05040100u) ^ 0x80808080u;
+#else
+    return ((a & 255u) | ((b & 255u) << 8) | ((c & 255u) << 16) | (d << 24)) ^ 0x80808080u;
+#endif
+}
+
 // Product residue in [0, m], as a double (exact integer) and as an int.
 F4J_HD double product_residue_d(int a, double m, double minv) {
     const double d = (double)a;
@@ -63,6 +97,12 @@ F4J_HD int product_residue(int a, double m, double minv) { return (int)product_r
 // Entry v of a line with exponent e (|v| < 2^e; INT_MIN for an all-zero line)
 // as a t-bit integer: rint(v 2^(t - e)), |result| <= 2^t.
 F4J_HD double scaled_int(double v, int e, int t) { return e == INT_MIN ? 0.0 : rint(ldexp(v, t - e)); }
+// The same with the shift taken once per line: scaled_shift(e, t) is t - e, with e
+// raised to -4096 for an all-zero line (INT_MIN; its entries are zeros, which any
+// shift keeps, and every finite nonzero double has e >= -1073).  A max rather
+// than a test for INT_MIN, so that no select is applied per entry.
+F4J_HD int scaled_shift(int e, int t) { return t - (e > -4096 ? e : -4096); }
+F4J_HD double scaled_int_sh(double v, int sh) { return rint(ldexp(v, sh)); }
 
 // Four residues as the bytes of one 32-bit word (first in the lowest byte).
 F4J_HD unsigned pack4(int a, int b, int c, int d) {

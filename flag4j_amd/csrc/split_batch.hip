@@ -362,30 +362,33 @@ __global__ __launch_bounds__(256) void bcrt_split_kernel(const Side *__restrict_
     const int r = lane & 31, kb = 16 * (lane >> 5);
     const int64_t x = x0 + r;
     const int ex = epool[s.eoff + x];
+    const int sh = crt::scaled_shift(ex, s.t);
     double vh[16], vl[16];
 #pragma unroll
     for (int u = 0; u < 16; ++u) {
-        const double v = crt::scaled_int(tile[wave][r][kb + u], ex, s.t);
+        const double v = crt::scaled_int_sh(tile[wave][r][kb + u], sh);
         if (WIDE) {
             vh[u] = v;                             // kept whole: the wide step reduces it per modulus
             vl[u] = 0.0;
         } else {
             crt::residue_split(v, vh[u], vl[u]);
+            vl[u] = crt::residue_bias(vl[u]);     // the biased form (crt_residue.h)
         }
     }
     int4 *dst = (int4 *)(rpool + s.dig + crt::frag_offset(x, k0 + kb, s.Kp >> 5));
     const int64_t lstride = s.Xp * s.Kp / 16;
     for (int l = 0; l < tb.n; ++l) {
-        const double m = (double)tb.m[l], mi = tb.minv[l], c26 = tb.c26[l];
-        int rs[16];
+        const double m = (double)tb.m[l], mi = tb.minv[l], c26 = tb.c26[l], cq = crt::residue_cq(mi);
+        unsigned rs[16];
 #pragma unroll
         for (int u = 0; u < 16; ++u)
-            rs[u] = WIDE ? crt::residue_wide(vh[u], m, mi, c26) : crt::residue_of(vh[u], vl[u], m, mi, c26);
+            rs[u] = WIDE ? crt::residue_wide_biased(vh[u], m, mi, cq, c26)
+                         : crt::residue_biased(vh[u], vl[u], m, mi, cq, c26);
         int4 v;
-        v.x = (int)crt::pack4(rs[0], rs[1], rs[2], rs[3]);
-        v.y = (int)crt::pack4(rs[4], rs[5], rs[6], rs[7]);
-        v.z = (int)crt::pack4(rs[8], rs[9], rs[10], rs[11]);
-        v.w = (int)crt::pack4(rs[12], rs[13], rs[14], rs[15]);
+        v.x = (int)crt::pack4_biased(rs[0], rs[1], rs[2], rs[3]);
+        v.y = (int)crt::pack4_biased(rs[4], rs[5], rs[6], rs[7]);
+        v.z = (int)crt::pack4_biased(rs[8], rs[9], rs[10], rs[11]);
+        v.w = (int)crt::pack4_biased(rs[12], rs[13], rs[14], rs[15]);
         dst[l * lstride] = v;
     }
 }

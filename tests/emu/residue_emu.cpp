@@ -7,6 +7,9 @@
 // magnitudes at every t, the extremes +-2^t, and the cases where the quotient
 // (u + 128) / m is an exact integer (v = k m - 128), the only ones in which the
 // kernel's floor may land one below.
+// The biased form of the split kernels (residue_biased, pack4_biased) must give
+// the word r + 128 and pack to the same bytes, and their per-line shift
+// (scaled_shift, scaled_int_sh) the same scaled integers as scaled_int.
 // Product residues: random int32 accumulators, multiples of m and the int32
 // extremes; the result must be congruent and lie in [0, m].
 // Prints "OK <count>" or the first mismatches and exits non-zero.
@@ -54,6 +57,17 @@ static void check_operand(double v, int t) {
         const bool ok = ((r % m) + m) % m == want && r >= -128 && (r <= 127 || (m == 256 && r == 128));
         ++total;
         if (!ok && bad++ < 10) std::printf("operand t=%d v=%.17g m=%d got %d want %d (mod m)\n", t, v, m, r, want);
+        // the split kernels' biased form: word r + 128, and the packed byte equals the byte of r
+        const double cq = f4j::crt::residue_cq(mi);
+        const unsigned rb = wide ? f4j::crt::residue_wide_biased(v, m, mi, cq, c26)
+                                 : f4j::crt::residue_biased(vh, f4j::crt::residue_bias(vl), m, mi, cq, c26);
+        const unsigned word = f4j::crt::pack4_biased(rb, 0u, rb, 128u);
+        const bool okb = rb == (unsigned)(r + 128) && (word & 255u) == ((unsigned)r & 255u) &&
+                         ((word >> 16) & 255u) == ((unsigned)r & 255u) && ((word >> 8) & 255u) == 128u &&
+                         (word >> 24) == 0u;
+        ++total;
+        if (!okb && bad++ < 10)
+            std::printf("biased t=%d v=%.17g m=%d got word %u (r %d) packed %08x\n", t, v, m, rb, r, word);
     }
 }
 
@@ -79,6 +93,21 @@ int main() {
                     check_operand(-v - 256.0, t);
                 }
             }
+        }
+    }
+    // per-line shift of the split kernels: the same scaled integers, zeros for an all-zero line
+    for (int it = 0; it < 200000; ++it) {
+        const int t = 1 + (int)(g() % 75);
+        const double x = std::ldexp((double)(g() >> 11), -53) * 2 - 1;
+        int e;
+        (void)std::frexp(x * std::ldexp(1.0, (int)(g() % 2000) - 1000), &e);
+        const double v = std::ldexp(x, e - 1);                 // |v| < 2^e
+        const int em = e + (int)(g() % 4);                     // the line's exponent, >= e
+        const double a = f4j::crt::scaled_int(v, em, t), b = f4j::crt::scaled_int_sh(v, f4j::crt::scaled_shift(em, t));
+        const double z = f4j::crt::scaled_int_sh(0.0, f4j::crt::scaled_shift(INT_MIN, t));
+        ++total;
+        if (!(a == b || (std::isnan(a) && std::isnan(b))) || z != 0.0 || std::signbit(z)) {
+            if (bad++ < 10) std::printf("shift t=%d v=%.17g e=%d: %.17g vs %.17g, zero line %.17g\n", t, v, em, a, b, z);
         }
     }
     // product residues
