@@ -277,8 +277,11 @@ __global__ __launch_bounds__(64) void gemm_f64_direct_kernel(const GemmProblem *
 #pragma unroll
                 for (int j = 0; j < FN; ++j) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(x[i], y[j], acc[i][j], 0, 0, 0);
         };
+        // K = 0: no loads at all (the operands may hold no element); C is zero
+        if (nks > 0) {
 #pragma unroll
-        for (int d = 0; d < D; ++d) fetch(d, ra[d], rb[d]);
+            for (int d = 0; d < D; ++d) fetch(d, ra[d], rb[d]);
+        }
         int64_t s = 0;
         for (; s + D <= nks; s += D) {
 #pragma unroll

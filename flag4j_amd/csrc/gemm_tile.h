@@ -103,10 +103,12 @@ struct TileCopy {
     F4J_HD static int lane_col(int tid) { return V * (tid % SCV); }
     F4J_HD static int lane_row(int tid) { return tid / SCV; }
     F4J_HD static int lds_index(int tid, int pass) { return (lane_row(tid) + pass * ROWS_PER_PASS) * O::LD + lane_col(tid); }
-    // num_records (bytes) for a block at (r0, c0) of a rows x cols matrix.
+    // num_records (bytes) for a block at (r0, c0) of a rows x cols matrix; 0 for a
+    // block that starts past the matrix's last element (e.g. K = 0: no rows at all),
+    // so that every load of it returns zero.
     F4J_HD static uint32_t num_records(int64_t r0, int64_t c0, int64_t rows, int64_t cols, int64_t ld) {
         const int64_t n = ((rows - 1 - r0) * ld + (cols - c0)) * 8;
-        return n > (int64_t)0x7FFFFFF0 ? 0x7FFFFFF0u : (uint32_t)n;
+        return n <= 0 ? 0u : n > (int64_t)0x7FFFFFF0 ? 0x7FFFFFF0u : (uint32_t)n;
     }
     // Byte offset of pass `pass` for lane `tid`, or OOB if its columns are past the edge.
     F4J_HD static uint32_t offset(int tid, int pass, int64_t c0, int64_t cols, int64_t ld) {

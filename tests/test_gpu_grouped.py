@@ -161,18 +161,24 @@ def test_tuner_cache_read_at_init_and_written_by_tune(tmp_path):
         L.check(lib.f4j_tuner_clear())
 
 
-@pytest.mark.parametrize("cfg", [100, 101, 102, 103])
+@pytest.mark.parametrize("cfg", list(range(9)) + [100, 101, 102, 103])
 @pytest.mark.parametrize("opA,opB", [("N", "N"), ("N", "T"), ("T", "N"), ("T", "T")])
-def test_direct_load_configurations_bitwise(tmp_path, cfg, opA, opB):
-    """The direct-load grouped kernels (one wave per tile, fragments from global memory,
-    tile configurations 100-103), forced on every bucket through the tuner cache: every
-    product, ragged edges and K not a multiple of 4 included, equals the single-product
-    FP64 call bit for bit (same MFMA k steps per element, in the same order); products
-    with N = 1, which a single call runs in the reference's matrix-vector order, are
-    held to the FP64 bound."""
+def test_tile_configurations_bitwise(tmp_path, cfg, opA, opB):
+    """Every grouped tile configuration -- the LDS kernels 0-8 (8- and 16-byte loads mixed
+    within a workgroup) and the direct-load kernels 100-103 (one wave per tile, fragments
+    from global memory) -- forced on every bucket through the tuner cache: every
+    product, ragged edges, K not a multiple of 4 and K = 0 (no operand element to read:
+    every load of its tiles lies outside the buffer records, or is not issued) included,
+    equals the single-product FP64 call bit for bit
+    (same MFMA k steps per element, in the same order); products with N = 1, which a
+    single call runs in the reference's matrix-vector order, are held to the FP64 bound."""
     As, Bs = rand_problems(500 + cfg, 90, 1, 300, opA, opB)
+    for i in (5, 41, 42, 89):                          # K = 0 items between the others
+        M, N = op(As[i], opA).shape[0], op(Bs[i], opB).shape[1]
+        As[i] = np.zeros((M, 0) if opA == "N" else (0, M))
+        Bs[i] = np.zeros((0, N) if opB == "N" else (N, 0))
     kinds = f"{'NT'.index(opA)},{'NT'.index(opB)}"
-    cache = tmp_path / "direct.txt"
+    cache = tmp_path / "tiles.txt"
     cache.write_text("".join(f"gfx950 {kinds} {b} {cfg}\n" for b in range(4)))
     lib = L.load()
     L.check(lib.f4j_tuner_clear())
@@ -189,7 +195,7 @@ def test_direct_load_configurations_bitwise(tmp_path, cfg, opA, opB):
             plan = grouped.GroupedPlan(shapes, [dA.ptr + 8 * int(x) for x in oa[:-1]],
                                        [dB.ptr + 8 * int(x) for x in ob[:-1]],
                                        [dC.ptr + 8 * int(x) for x in oc[:-1]],
-                                       [a.shape[1] for a in As], [b.shape[1] for b in Bs],
+                                       [max(a.shape[1], 1) for a in As], [max(b.shape[1], 1) for b in Bs],
                                        [n for _, n, _ in shapes], opA=opA, opB=opB)
             info = plan.info()
             assert all(c == cfg for c, t in zip(info["tile_cfg"], info["tiles"]) if t > 0), info
@@ -205,6 +211,6 @@ def test_direct_load_configurations_bitwise(tmp_path, cfg, opA, opB):
             assert check_real(C[oc[i]:oc[i + 1]].reshape(M, N), op(A, opA), op(B, opB))
             continue
         single = np.full((M, N), np.nan)
-        L.check(lib.f4j_dgemm(L.OP[opA], L.OP[opB], M, N, K, A.ctypes.data, A.shape[1], B.ctypes.data, B.shape[1],
-                              single.ctypes.data, max(N, 1), L.PREC["fp64"], 0))
+        L.check(lib.f4j_dgemm(L.OP[opA], L.OP[opB], M, N, K, A.ctypes.data, max(A.shape[1], 1), B.ctypes.data,
+                              max(B.shape[1], 1), single.ctypes.data, max(N, 1), L.PREC["fp64"], 0))
         assert np.array_equal(C[oc[i]:oc[i + 1]].reshape(M, N), single), (i, M, N, K)
