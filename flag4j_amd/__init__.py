@@ -2,16 +2,16 @@
 
 The GPU work is done by libflag4j_amd.so (HIP kernels for gfx950 behind the C
 ABI in include/flag4j_amd.h).  This package mirrors the reference's Java API
-for the multiply path (Matrix, CMatrix, Vector, CVector, Shape) and its least-squares caller and exposes the C ABI to
-Python.  Importing it loads the native library and fails if it is missing.
+for the multiply path (Matrix, CMatrix, Vector, CVector, Shape), its least-squares and pseudo-inverse callers, and
+exposes the C ABI to Python.  Importing it loads the native library and fails if it is missing.
 """
 from . import _native
-from .arrays import (CMatrix, CVector, IllegalArgumentException, LinearAlgebraException, Matrix, RealLstsqSolver, Shape,
-                     SingularMatrixException, Vector)
+from .arrays import (CMatrix, CVector, IllegalArgumentException, Invert, LinearAlgebraException, Matrix, RealLstsqSolver,
+                     Shape, SingularMatrixException, Vector)
 from .device import DeviceBuffer, dgemm_dev, sync, zgemm_dev
 
 _native.load()
 
 __all__ = ["Matrix", "CMatrix", "Vector", "CVector", "Shape", "LinearAlgebraException", "IllegalArgumentException",
-           "RealLstsqSolver", "SingularMatrixException",
+           "RealLstsqSolver", "Invert", "SingularMatrixException",
            "DeviceBuffer", "dgemm_dev", "zgemm_dev", "sync"]

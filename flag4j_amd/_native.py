@@ -70,6 +70,8 @@ SIGNATURES = {
     "f4j_zgemv_dev": ([_i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _i, _vp], _i),
     "f4j_dlstsq": ([_i64, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _i], _i),
     "f4j_dlstsq_dev": ([_i64, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _i, _vp], _i),
+    "f4j_dpinv": ([_i64, _i64, _vp, _i64, _vp, _i64], _i),
+    "f4j_dpinv_dev": ([_i64, _i64, _vp, _i64, _vp, _i64, _vp], _i),
     "f4j_grouped_plan_create": ([_i, _i, _i, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i,
                                  ctypes.POINTER(_vp)], _i),
     "f4j_grouped_plan_create_split": ([_i, _i, _i, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i,

@@ -443,3 +443,27 @@ class RealLstsqSolver:
             raise SingularMatrixException("Could not solve system.")
         _native.check(st)
         return Vector(x.reshape(-1)) if vector else Matrix(x)
+
+
+class Invert:
+    """Invert.pInv (M:linalg/Invert.java:292-298): the Moore-Penrose pseudo-inverse, on the GPU
+    (f4j_dpinv).  The reference forms V S^-1 U^T from RealSVD; for the full-rank matrices it
+    accepts (Invert.invDiag raises SingularMatrixException when |prod s_i| <= 2^-53 min(m, n))
+    the pseudo-inverse is unique, and this computes it as R^-1 Q^T from the reduced Householder
+    QR (a wide matrix through its transpose), in the reference's least-squares arithmetic:
+    bit for bit RealLstsqSolver.solve(A, I).  Singular: |det R| = prod s_i, the same criterion."""
+
+    @staticmethod
+    def pInv(src):
+        if not isinstance(src, Matrix):
+            raise TypeError("pInv takes a Matrix")
+        M, N = src.numRows, src.numCols
+        if M == 0 or N == 0:       # the C ABI has no empty pseudo-inverse (F4J_ERR_SHAPE)
+            raise IllegalArgumentException(f"pInv of an empty matrix of shape {src.shape}.")
+        a = np.ascontiguousarray(src.data, dtype=np.float64).reshape(M, N)
+        x = np.empty((N, M))
+        st = _native.load().f4j_dpinv(M, N, _ptr(a), N, _ptr(x), M)
+        if st == _native.ERR_SINGULAR:
+            raise SingularMatrixException("Could not invert.")
+        _native.check(st)
+        return Matrix(x)

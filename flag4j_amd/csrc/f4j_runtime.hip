@@ -1191,6 +1191,82 @@ int lstsq_host(int64_t M, int64_t N, int64_t R, const double *A, int64_t lda, co
     return rc;
 }
 
+int validate_pinv(int64_t M, int64_t N, const void *A, int64_t lda, const void *X, int64_t ldx) {
+    if (M < 0 || N < 0) return fail(F4J_ERR_INVALID_ARG, "negative dimension");
+    if (M == 0 || N == 0) return fail(F4J_ERR_SHAPE, "pseudo-inverse of an empty matrix (S has no entries)");
+    if (!A || !X) return fail(F4J_ERR_INVALID_ARG, "A or X is null");
+    if (lda < N) return fail(F4J_ERR_INVALID_ARG, "lda < N");
+    if (ldx < M) return fail(F4J_ERR_INVALID_ARG, "ldx < M");
+    if (std::max(M, N) > (int64_t)65535 * 32) return fail(F4J_ERR_UNSUPPORTED, "pseudo-inverse supports at most 2097120 rows or columns");
+    return F4J_OK;
+}
+
+// Pseudo-inverse on device operands (Invert.pInv): for a tall A (M >= N)
+// X = R^-1 Q^T, the least-squares solve with B = I_M (Q^T I = Q^T exactly, so
+// the product is skipped); a wide A goes through its transpose, X = ((A^T)^+)^T.
+// Waits for `stream` to check det R.
+int pinv_dev(int64_t M, int64_t N, const double *A, int64_t lda, double *X, int64_t ldx, hipStream_t stream) {
+    State &s = S();
+    std::lock_guard<std::mutex> lk(s.ls_mu);
+    const bool tall = M >= N;
+    const int64_t m = tall ? M : N, n = tall ? N : M;       // the tall matrix T0 (m x n) that is factored
+    // T, Q, Qh, v, w, qf, diag, P (n x m: A^+ of T0, for a wide A), det, xs
+    const int64_t parts[10] = {m * n, m * n, n * m, m, n, n, n, tall ? 0 : n * m, 1, n * m};
+    int64_t off[10], total = 0;
+    for (int i = 0; i < 10; ++i) {
+        off[i] = total;
+        total += (parts[i] + 31) & ~int64_t(31);
+    }
+    int rc = grow(s.lsws, (size_t)total * sizeof(double));
+    if (rc != F4J_OK) return rc;
+    double *base = (double *)s.lsws.p;
+    double *T = base + off[0], *Q = base + off[1], *Qh = base + off[2], *v = base + off[3], *w = base + off[4],
+           *qf = base + off[5], *diag = base + off[6], *P = base + off[7], *det = base + off[8], *xs = base + off[9];
+    if (tall)
+        F4J_HIP(hipMemcpy2DAsync(T, n * sizeof(double), A, lda * sizeof(double), n * sizeof(double), m,
+                                 hipMemcpyDeviceToDevice, stream),
+                "copy of A");
+    else
+        F4J_HIP(launch_transpose(A, M, N, lda, T, n, stream), "transpose of A");
+    F4J_HIP(launch_householder_qr(T, m, n, n, v, w, qf, diag, stream), "Householder QR launch");
+    F4J_HIP(launch_householder_q(T, m, n, n, qf, Q, n, v, w, stream), "Q launch");
+    F4J_HIP(launch_transpose(Q, m, n, n, Qh, m, stream), "transpose launch");
+    // R Y = Qh, m right-hand sides: Y = T0^+ (n x m)
+    F4J_HIP(launch_back_solve(T, n, n, Qh, m, tall ? X : P, tall ? ldx : m, m, xs, det, stream),
+            "back substitution launch");
+    if (!tall) F4J_HIP(launch_transpose(P, n, m, m, X, ldx, stream), "transpose of (A^T)^+");
+    double d = 0.0;
+    F4J_HIP(hipMemcpyAsync(&d, det, sizeof(double), hipMemcpyDeviceToHost, stream), "copy of det R");
+    F4J_HIP(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    const double ad = std::fabs(d);
+    // Invert.invDiag: |prod s_i| <= EPS_F64 k (|det R| = prod s_i), or NaN
+    if (ad <= 0x1.0p-53 * (double)n || std::isnan(ad)) return fail(F4J_ERR_SINGULAR, "Could not invert.");
+    return F4J_OK;
+}
+
+// Host-array pseudo-inverse: A moves in through a staging context, X out.
+int pinv_host(int64_t M, int64_t N, const double *A, int64_t lda, double *X, int64_t ldx) {
+    int rc = validate_pinv(M, N, A, lda, X, ldx);
+    if (rc != F4J_OK) return rc;
+    if ((rc = ensure_ready()) != F4J_OK) return rc;
+    std::unique_lock<std::mutex> lk;
+    Stage &g = stage_acquire(lk);
+    if ((rc = stage_setup(g)) != F4J_OK) return rc;
+    if ((rc = grow(g.A, (size_t)(M * N) * sizeof(double))) != F4J_OK) return rc;
+    if ((rc = grow(g.C, (size_t)(N * M) * sizeof(double))) != F4J_OK) return rc;
+    if ((rc = stage_in(g, g.A.p, A, M, N * sizeof(double), lda * sizeof(double))) != F4J_OK) return rc;
+    rc = pinv_dev(M, N, (const double *)g.A.p, N, (double *)g.C.p, M, g.st);
+    if (rc != F4J_OK && rc != F4J_ERR_SINGULAR) {
+        (void)hipStreamSynchronize(g.st);
+        return rc;
+    }
+    const std::string msg = t_err;
+    const int rc_out = stage_out(g, X, ldx * sizeof(double), g.C.p, N, M * sizeof(double));
+    if (rc_out != F4J_OK) return rc_out;
+    t_err = msg;
+    return rc;
+}
+
 int runtime_ready() { return ensure_ready(); }
 
 hipStream_t runtime_stream(void *stream) { return pick_stream(stream); }
@@ -1421,6 +1497,17 @@ int f4j_dlstsq_dev(int64_t M, int64_t N, int64_t R, const double *A, int64_t lda
     if (rc != F4J_OK) return rc;
     if ((rc = ensure_ready()) != F4J_OK) return rc;
     return lstsq_dev(M, N, R, A, lda, B, ldb, X, ldx, precision, pick_stream(stream));
+}
+
+int f4j_dpinv(int64_t M, int64_t N, const double *A, int64_t lda, double *X, int64_t ldx) {
+    return pinv_host(M, N, A, lda, X, ldx);
+}
+
+int f4j_dpinv_dev(int64_t M, int64_t N, const double *A, int64_t lda, double *X, int64_t ldx, void *stream) {
+    int rc = validate_pinv(M, N, A, lda, X, ldx);
+    if (rc != F4J_OK) return rc;
+    if ((rc = ensure_ready()) != F4J_OK) return rc;
+    return pinv_dev(M, N, A, lda, X, ldx, pick_stream(stream));
 }
 
 int f4j_zgemv(int64_t M, int64_t K, const double *A, int64_t lda, const double *x, int64_t incx, double *y,

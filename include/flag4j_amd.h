@@ -213,6 +213,21 @@ int f4j_dlstsq(int64_t M, int64_t N, int64_t R, const double *A, int64_t lda, co
 int f4j_dlstsq_dev(int64_t M, int64_t N, int64_t R, const double *A, int64_t lda, const double *B, int64_t ldb,
                    double *X, int64_t ldx, int precision, void *stream);
 
+/* ---- Pseudo-inverse: a caller of the product ----
+ * Replaces Invert.pInv(Matrix) M:linalg/Invert.java:292-298 (V S^-1 U^T from RealSVD, S inverted by
+ *   Invert.invDiag :126-145, which raises SingularMatrixException when |prod s_i| <= EPS_F64 k,
+ *   EPS_F64 = 2^-53 (M:util/Flag4jConstants.java:52), k = min(M, N)).
+ * X (N x M, ldx) = A^+ for A (M x N, lda), M, N >= 1.  For M >= N, A^+ = R^-1 Q^T from the reduced
+ * Householder QR of A (the least-squares solve of f4j_dlstsq with B = I_M, whose product Q^T I is
+ * exact and is skipped); for M < N, A^+ = ((A^T)^+)^T.  The pseudo-inverse of a full-rank matrix is
+ * unique, so this equals the reference's V S^-1 U^T up to rounding; bit for bit it equals the
+ * reference's own least-squares arithmetic (RealLstsqSolver.solve(A, I)).  F4J_ERR_SINGULAR when
+ * |det R| <= 2^-53 min(M, N) or det R is NaN: |det R| = prod s_i, the reference's criterion; X then
+ * holds the back substitution's values.  The _dev form waits for `stream` before it returns.
+ * Library workspace: about 3 M N + 2 min(M, N) max(M, N) doubles, grown and reused. */
+int f4j_dpinv(int64_t M, int64_t N, const double *A, int64_t lda, double *X, int64_t ldx);
+int f4j_dpinv_dev(int64_t M, int64_t N, const double *A, int64_t lda, double *X, int64_t ldx, void *stream);
+
 /* ---- grouped (batched) products ----
  * No reference counterpart: Flag4j multiplies one pair per call; a batch is a Java loop of
  * Matrix.mult (M:arrays/dense/Matrix.java:638-640) or A.H().mult(B) calls, each paying a

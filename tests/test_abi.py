@@ -72,6 +72,25 @@ def test_lstsq_arguments_rejected_before_device_use():
     assert _native.load().f4j_dlstsq_dev(3, 4, 1, p, 4, p, 1, p, 1, -1, None) == L.ERR_SHAPE
 
 
+def test_pinv_arguments_rejected_before_device_use():
+    """f4j_dpinv validates before it touches the device: non-empty shape, pointers present,
+    leading dimensions large enough (X is N x M: ldx >= M)."""
+    a = np.zeros(64)
+    p = ctypes.c_void_p(a.ctypes.data)
+    L = _native
+    f = _native.load().f4j_dpinv
+    assert f(0, 3, p, 3, p, 1) == L.ERR_SHAPE
+    assert f(3, 0, p, 1, p, 3) == L.ERR_SHAPE
+    assert f(-1, 2, p, 2, p, 1) == L.ERR_INVALID_ARG
+    assert f(4, 2, None, 2, p, 4) == L.ERR_INVALID_ARG                    # null A
+    assert f(4, 2, p, 2, None, 4) == L.ERR_INVALID_ARG                    # null X
+    assert f(4, 2, p, 1, p, 4) == L.ERR_INVALID_ARG                       # lda < N
+    assert f(4, 2, p, 2, p, 3) == L.ERR_INVALID_ARG                       # ldx < M
+    assert _native.load().f4j_dpinv_dev(0, 3, p, 3, p, 1, None) == L.ERR_SHAPE
+    with pytest.raises(f4j.IllegalArgumentException):
+        f4j.Invert.pInv(f4j.Matrix(np.ones((0, 3))))
+
+
 def test_lstsq_solver_errors_match_reference_exceptions():
     """RealLstsqSolver.solve: Qh.mult(b) with b's rows != A's rows -> LinearAlgebraException;
     a wide A leaves R non-square -> IllegalArgumentException (ValidateParameters.ensureSquare)."""

@@ -1,6 +1,7 @@
 """GPU parity of the least-squares solve (f4j_dlstsq / f4j_dlstsq_dev, RealLstsqSolver.solve)
 against the oracle's restatement (Householder QR, Q, Q^H B, back substitution) and the
-reference's golden vectors (T:linalg/solvers/RealLstsqSolverTests.java:17-98).
+reference's golden vectors (T:linalg/solvers/RealLstsqSolverTests.java:17-98), and of the
+pseudo-inverse built on it (f4j_dpinv, Invert.pInv).
 
 Bars:
   * STRICT precision: X and the status equal the oracle's bit for bit (the QR, Q and the
@@ -173,3 +174,92 @@ def test_device_entry_with_leading_dimensions():
         X = dX.download(np.float64, N * ldx).reshape(N, ldx)
     assert same_bits(np.ascontiguousarray(X[:, :R]), o.lstsq(A, B)[1])
     assert np.all(X[:, R:] == 7.0)
+
+
+# ---- pseudo-inverse (f4j_dpinv, Invert.pInv) ----
+
+def gpu_pinv(A):
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    M, N = A.shape
+    X = np.full((N, M), np.nan)
+    st = _native.load().f4j_dpinv(M, N, p(A), N, p(X), M)
+    return st, X
+
+
+def penrose_residuals(A, X):
+    """The four Moore-Penrose conditions, each relative to the size of its terms."""
+    nA, nX = np.linalg.norm(A, 2), np.linalg.norm(X, 2)
+    AX, XA = A @ X, X @ A
+    return (np.linalg.norm(AX @ A - A, 2) / nA, np.linalg.norm(XA @ X - X, 2) / nX,
+            np.linalg.norm(AX - AX.T, 2), np.linalg.norm(XA - XA.T, 2))
+
+
+@pytest.mark.parametrize("M,N", [(1, 1), (7, 7), (130, 40), (40, 130), (300, 299), (257, 512), (1000, 64)])
+def test_pinv_equals_the_reference_least_squares_arithmetic(M, N):
+    """Bit for bit RealLstsqSolver.solve(A, I) of the oracle (its QR, Q, Q^H I and back
+    substitution; Q^H I = Q^H exactly) for a tall A, and its transpose through A^T for a wide A."""
+    rng = np.random.default_rng(100 + M + N)
+    A, _ = system(rng, M, N, 1)
+    st, X = gpu_pinv(A)
+    if M >= N:
+        ost, OX = o.lstsq(A, np.eye(M))
+    else:
+        ost, OXt = o.lstsq(np.ascontiguousarray(A.T), np.eye(N))
+        OX = np.ascontiguousarray(OXt.T)
+    assert st == _native.OK and ost == 0
+    assert same_bits(X, OX)
+
+
+@pytest.mark.parametrize("M,N", [(64, 64), (500, 120), (120, 500), (1024, 512)])
+def test_pinv_moore_penrose_conditions(M, N):
+    """What the mathematics fixes: A^+ is the unique X with A X A = A, X A X = X and A X, X A
+    symmetric.  On matrices with condition number 1e2 every residual is a small multiple of
+    the rounding unit; numpy's SVD-based pinv (LAPACK), an independent computation of the same
+    unique matrix, agrees within the conditioning's bound."""
+    rng = np.random.default_rng(7 * M + N)
+    U, _ = np.linalg.qr(rng.standard_normal((M, min(M, N))))
+    V, _ = np.linalg.qr(rng.standard_normal((N, min(M, N))))
+    # condition number 100, prod s_i = 1: the reference's singularity test is on |det R| =
+    # prod s_i <= 2^-53 min(M, N), which values spread below 1 alone would trip
+    s = np.logspace(1, -1, min(M, N))
+    A = (U * s) @ V.T
+    st, X = gpu_pinv(A)
+    assert st == _native.OK
+    u = 2.0**-53
+    bound = 64 * max(M, N) * u * 100.0
+    r = penrose_residuals(A, X)
+    assert max(r[0], r[2], r[3]) <= bound and r[1] <= bound * 100.0, r
+    ref = np.linalg.pinv(A)
+    assert np.linalg.norm(X - ref, 2) <= bound * 100.0 * np.linalg.norm(ref, 2)
+    Y = f4j.Invert.pInv(f4j.Matrix(A)).numpy()
+    assert same_bits(Y, X)
+
+
+def test_pinv_singular_matrices():
+    """A zero column (tall) or a zero row (wide) makes det R = 0: SingularMatrixException, as
+    Invert.invDiag raises it for a zero singular value (|prod s_i| = |det R|)."""
+    rng = np.random.default_rng(41)
+    A, _ = system(rng, 90, 30, 1, spread=0)
+    Z = A.copy()
+    Z[:, 11] = 0.0
+    assert gpu_pinv(Z)[0] == _native.ERR_SINGULAR
+    assert gpu_pinv(np.ascontiguousarray(Z.T))[0] == _native.ERR_SINGULAR
+    assert b"Could not invert" in _native.load().f4j_last_error()
+    with pytest.raises(f4j.SingularMatrixException):
+        f4j.Invert.pInv(f4j.Matrix(Z))
+
+
+def test_pinv_device_entry_with_leading_dimensions():
+    rng = np.random.default_rng(43)
+    for M, N in ((300, 70), (70, 300)):
+        lda, ldx = N + 3, M + 5
+        A, _ = system(rng, M, N, 1)
+        Ap = np.zeros((M, lda))
+        Ap[:, :N] = A
+        L = _native.load()
+        with device.DeviceBuffer.from_array(Ap) as dA, device.DeviceBuffer(N * ldx * 8) as dX:
+            dX.upload(np.full(N * ldx, 7.0))
+            _native.check(L.f4j_dpinv_dev(M, N, dA.ptr, lda, dX.ptr, ldx, None))
+            X = dX.download(np.float64, N * ldx).reshape(N, ldx)
+        assert same_bits(np.ascontiguousarray(X[:, :M]), gpu_pinv(A)[1])
+        assert np.all(X[:, M:] == 7.0)
