@@ -295,11 +295,13 @@ def bench_complex_batched(np, torch, f4j, rng, dev, T, cpu=True, count=2000, rep
     plan = grouped.GroupedPlan(*args)
     plan.tune(1)
     ms_d = timed_plan(torch, plan, st, reps)
+    info = plan.info()
     plan.close()
     out = {"workload": "2000 complex128 A^H B, M,N,K log-uniform on [32,1024], parts (2r-1) 10^(4(2r'-1)) "
                        "(java.util.Random seeds 5, 51, 52)",
            "fp64_grouped_ms": round(ms_d, 3), "fp64_grouped_tflops": round(flops / ms_d / 1e9, 2),
-           "per_problem_roofline_ms": round(roof * 1e3, 4), "roofline_frac": round(roof * 1e3 / ms_d, 4)}
+           "per_problem_roofline_ms": round(roof * 1e3, 4), "roofline_frac": round(roof * 1e3 / ms_d, 4),
+           "tile_cfg_per_bucket": info["tile_cfg"], "tiles_per_bucket": info["tiles"]}
     f4j._native.config_set("splitCheckRows", 1)
     for key, prec in (("split_fp16", "split_fp16"), ("split_int8", "split_int8")):
         plan = grouped.GroupedPlan(*args, precision=prec, use3m=True)

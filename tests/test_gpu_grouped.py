@@ -214,3 +214,44 @@ def test_tile_configurations_bitwise(tmp_path, cfg, opA, opB):
         L.check(lib.f4j_dgemm(L.OP[opA], L.OP[opB], M, N, K, A.ctypes.data, max(A.shape[1], 1), B.ctypes.data,
                               max(B.shape[1], 1), single.ctypes.data, max(N, 1), L.PREC["fp64"], 0))
         assert np.array_equal(C[oc[i]:oc[i + 1]].reshape(M, N), single), (i, M, N, K)
+
+
+@pytest.mark.parametrize("cfg", list(range(9)))
+@pytest.mark.parametrize("opA,opB", [("C", "N"), ("N", "N"), ("T", "C"), ("N", "T")])
+def test_complex_tile_configurations_bitwise(tmp_path, cfg, opA, opB):
+    """Complex grouped products (real-expanded operands: the LDS kernels swap and negate in
+    their fragment addressing) with every LDS tile configuration forced on every bucket: each
+    product, K = 0 items included, equals the single complex FP64 call bit for bit; N = 1
+    products (a matrix-vector order in the single call) are held to the FP64 bound."""
+    As, Bs = rand_problems(700 + cfg, 40, 1, 160, opA, opB, cplx=True)
+    for i in (3, 21):
+        M, N = op(As[i], opA).shape[0], op(Bs[i], opB).shape[1]
+        As[i] = np.zeros((M, 0) if opA == "N" else (0, M), dtype=np.complex128)
+        Bs[i] = np.zeros((0, N) if opB == "N" else (N, 0), dtype=np.complex128)
+    kz = {"N": 2, "T": 3, "C": 4}
+    cache = tmp_path / "ctiles.txt"
+    cache.write_text("".join(f"gfx950 {kz[opA]},{kz[opB]} {b} {cfg}\n" for b in range(4)))
+    lib = L.load()
+    L.check(lib.f4j_tuner_clear())
+    try:
+        L.check(lib.f4j_tuner_load(str(cache).encode()))
+        Cs = grouped.mult_all(As, Bs, opA, opB)
+    finally:
+        L.check(lib.f4j_tuner_clear())
+    for i, (A, B, C) in enumerate(zip(As, Bs, Cs)):
+        oA, oB = op(A, opA), op(B, opB)
+        M, K = oA.shape
+        N = oB.shape[1]
+        if N == 1:
+            ref = o.cplx_gemm(A, B, opA, opB) if K else np.zeros((M, N), dtype=np.complex128)
+            nA = np.abs(oA.real) + np.abs(oA.imag)
+            nB = np.abs(oB.real) + np.abs(oB.imag)
+            lim = 2 * 2 * max(K, 1) * U * (nA @ nB)
+            assert np.all(np.abs(C.real - ref.real) <= lim) and np.all(np.abs(C.imag - ref.imag) <= lim)
+            continue
+        a = np.ascontiguousarray(A)
+        b = np.ascontiguousarray(B)
+        single = np.full((M, N), np.nan, dtype=np.complex128)
+        L.check(lib.f4j_zgemm(L.OP[opA], L.OP[opB], M, N, K, a.ctypes.data, max(a.shape[1], 1), b.ctypes.data,
+                              max(b.shape[1], 1), single.ctypes.data, max(N, 1), L.PREC["fp64"], 0, 0))
+        assert np.array_equal(C, single), (i, M, N, K)
