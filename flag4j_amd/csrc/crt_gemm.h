@@ -9,6 +9,7 @@ namespace f4j {
 
 constexpr int CRT_MAX_MODULI = 20;
 constexpr int CRT_MAX_CHUNKS = 4;
+constexpr int CRT_MAX_JOBS = 4;          // real products of one call: 1 (real), 3 (complex 3M) or 4 (4M)
 
 // Reconstruction constants.  With M = prod m_l and W_l = (M/m_l) * ((M/m_l)^-1 mod m_l),
 // X = sum_l r_l W_l = A'B' (mod M).  Every W_l (and M) is cut into P chunks of 40 bits on a
@@ -105,6 +106,18 @@ struct CrtSide {
     hipEvent_t fork, join;
 };
 
+// The real products of a complex call, all on the storage of the same A and B:
+// job j multiplies the views a_ca[j] re + a_cb[j] im of op(A) and b_ca[j] re +
+// b_cb[j] im of op(B) and contributes cre[j] and cim[j] times its product to
+// Re C and Im C.
+struct CrtJobs {
+    int n = 0;
+    double a_ca[CRT_MAX_JOBS] = {}, a_cb[CRT_MAX_JOBS] = {}, b_ca[CRT_MAX_JOBS] = {}, b_cb[CRT_MAX_JOBS] = {};
+    double cre[CRT_MAX_JOBS] = {}, cim[CRT_MAX_JOBS] = {};
+};
+// Bytes between consecutive jobs' workspaces (plan.bytes rounded up to 4 KiB).
+size_t crt_job_stride(const CrtPlan &pl);
+
 // Enqueue exponents, residue slices, the n residue products and the FP64
 // reconstruction into C.  `ws` holds plan.bytes bytes.
 // ev_begin / ev_end (optional) are recorded on `stream` around the product launch.
@@ -113,5 +126,14 @@ hipError_t launch_crt_gemm(const CrtPlan &pl, const CrtOperand &oa, const CrtOpe
                            int64_t K, double *C, int64_t ldc, const CrtOutput &out, void *ws, hipStream_t stream,
                            hipEvent_t ev_begin = nullptr, hipEvent_t ev_end = nullptr,
                            const CrtSide *side = nullptr);
+// All jobs (3 or 4) of a complex product in one sequence: each operand pass (exponents,
+// residues) reads the operand once for every job's view, one product launch covers
+// every job's moduli and one reconstruction writes the interleaved complex C once.
+// oa and ob give the storage (their ca / cb are ignored); `ws` holds
+// jobs.n * crt_job_stride(plan) bytes, job j's workspace at j * crt_job_stride(plan).
+hipError_t launch_crt_gemm_jobs(const CrtPlan &pl, const CrtOperand &oa, const CrtOperand &ob, int64_t M, int64_t N,
+                                int64_t K, double *C, int64_t ldc, const CrtJobs &jobs, void *ws, hipStream_t stream,
+                                hipEvent_t ev_begin = nullptr, hipEvent_t ev_end = nullptr,
+                                const CrtSide *side = nullptr);
 
 }  // namespace f4j

@@ -147,10 +147,6 @@ int t_for(int n, int64_t K) {
 
 using crt::pack4;
 
-__device__ inline double opnd_value(const CrtOperand &o, int64_t x, int64_t k) {
-    const double *p = o.X + x * o.sx + k * o.sk;
-    return o.cplx ? fma(o.ca, p[0], o.cb * p[1]) : p[0];
-}
 
 __device__ inline int exp_of(double v) {
     if (v == 0.0) return INT_MIN;
@@ -163,172 +159,224 @@ __global__ __launch_bounds__(256) void crt_fill_kernel(int *p, int64_t n, int v)
     for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) p[i] = v;
 }
 
+// Views of one operand's storage for the J jobs of a call: view j of element
+// (re, im) is ca[j] re + cb[j] im (real data: the value itself, J = 1).
+struct CrtCombos {
+    double ca[CRT_MAX_JOBS], cb[CRT_MAX_JOBS];
+};
+__device__ inline double combo_value(const CrtOperand &o, const CrtCombos &cc, int j, double re, double im) {
+    return o.cplx ? fma(cc.ca[j], re, cc.cb[j] * im) : re;
+}
+// The stored element (re, im) of line x, index k (im = 0 for real data).
+__device__ inline void opnd_pair(const CrtOperand &o, int64_t x, int64_t k, double &re, double &im) {
+    const double *p = o.X + x * o.sx + k * o.sk;
+    re = p[0];
+    im = o.cplx ? p[1] : 0.0;
+}
+
 // Line exponents (smallest E with |value| < 2^E over the line; INT_MIN for an
-// all-zero line).  Contiguous lines: one block per line, lanes along k.  A NaN
-// or infinite entry sets *nonfinite (the residues cannot represent it).
-// Eight loads per lane are issued before any is used (one at a time kept too
-// few bytes in flight: 3.9 TB/s).
+// all-zero line) of the J views, e[j estride + x].  Contiguous lines: one block
+// per line, lanes along k.  A NaN or infinite entry sets *nonfinite (the
+// residues cannot represent it).  Eight loads per lane are issued before any is
+// used (one at a time kept too few bytes in flight: 3.9 TB/s).
 constexpr int EXP_UNROLL = 8;
-__global__ __launch_bounds__(256) void crt_exp_lines_kernel(CrtOperand o, int64_t K, int *e, int *nonfinite) {
+template <int J>
+__global__ __launch_bounds__(256) void crt_exp_lines_kernel(CrtOperand o, CrtCombos cc, int64_t K, int *e,
+                                                            int64_t estride, int *nonfinite) {
     const int64_t x = blockIdx.x;
-    int m = INT_MIN;
+    int m[J];
+#pragma unroll
+    for (int j = 0; j < J; ++j) m[j] = INT_MIN;
     bool bad = false;
+    auto take = [&](double re, double im) {
+#pragma unroll
+        for (int j = 0; j < J; ++j) {
+            const double v = combo_value(o, cc, j, re, im);
+            bad |= !__builtin_isfinite(v);
+            m[j] = max(m[j], exp_of(v));
+        }
+    };
     int64_t k = threadIdx.x;
     for (; k + 256 * (EXP_UNROLL - 1) < K; k += 256 * EXP_UNROLL) {
-        double v[EXP_UNROLL];
+        double re[EXP_UNROLL], im[EXP_UNROLL];
 #pragma unroll
-        for (int u = 0; u < EXP_UNROLL; ++u) v[u] = opnd_value(o, x, k + 256 * u);
+        for (int u = 0; u < EXP_UNROLL; ++u) opnd_pair(o, x, k + 256 * u, re[u], im[u]);
 #pragma unroll
-        for (int u = 0; u < EXP_UNROLL; ++u) {
-            bad |= !__builtin_isfinite(v[u]);
-            m = max(m, exp_of(v[u]));
-        }
+        for (int u = 0; u < EXP_UNROLL; ++u) take(re[u], im[u]);
     }
     for (; k < K; k += 256) {
-        const double v = opnd_value(o, x, k);
-        bad |= !__builtin_isfinite(v);
-        m = max(m, exp_of(v));
+        double re, im;
+        opnd_pair(o, x, k, re, im);
+        take(re, im);
     }
     if (bad) *nonfinite = 1;
-    for (int off = 32; off > 0; off >>= 1) m = max(m, __shfl_xor(m, off));
-    __shared__ int red[4];
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __shared__ int red[J][4];
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+        int mj = m[j];
+        for (int off = 32; off > 0; off >>= 1) mj = max(mj, __shfl_xor(mj, off));
+        if ((threadIdx.x & 63) == 0) red[j][threadIdx.x >> 6] = mj;
+    }
     __syncthreads();
-    if (threadIdx.x == 0) e[x] = max(max(red[0], red[1]), max(red[2], red[3]));
+    if (threadIdx.x < J)
+        e[threadIdx.x * estride + x] =
+            max(max(red[threadIdx.x][0], red[threadIdx.x][1]), max(red[threadIdx.x][2], red[threadIdx.x][3]));
 }
 
 // Strided lines: thread per line (lanes along the lines), one k-chunk per
 // block row, merged with atomicMax into e (filled with INT_MIN first).
-__global__ __launch_bounds__(256) void crt_exp_cols_kernel(CrtOperand o, int64_t nx, int64_t K, int64_t kchunk,
-                                                           int *e, int *nonfinite) {
+template <int J>
+__global__ __launch_bounds__(256) void crt_exp_cols_kernel(CrtOperand o, CrtCombos cc, int64_t nx, int64_t K,
+                                                           int64_t kchunk, int *e, int64_t estride, int *nonfinite) {
     const int64_t x = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (x >= nx) return;
     const int64_t k0 = (int64_t)blockIdx.y * kchunk, k1 = min(K, k0 + kchunk);
-    int m = INT_MIN;
+    int m[J];
+#pragma unroll
+    for (int j = 0; j < J; ++j) m[j] = INT_MIN;
     bool bad = false;
+    auto take = [&](double re, double im) {
+#pragma unroll
+        for (int j = 0; j < J; ++j) {
+            const double v = combo_value(o, cc, j, re, im);
+            bad |= !__builtin_isfinite(v);
+            m[j] = max(m[j], exp_of(v));
+        }
+    };
     int64_t k = k0;
     for (; k + EXP_UNROLL <= k1; k += EXP_UNROLL) {
-        double v[EXP_UNROLL];
+        double re[EXP_UNROLL], im[EXP_UNROLL];
 #pragma unroll
-        for (int u = 0; u < EXP_UNROLL; ++u) v[u] = opnd_value(o, x, k + u);
+        for (int u = 0; u < EXP_UNROLL; ++u) opnd_pair(o, x, k + u, re[u], im[u]);
 #pragma unroll
-        for (int u = 0; u < EXP_UNROLL; ++u) {
-            bad |= !__builtin_isfinite(v[u]);
-            m = max(m, exp_of(v[u]));
-        }
+        for (int u = 0; u < EXP_UNROLL; ++u) take(re[u], im[u]);
     }
     for (; k < k1; ++k) {
-        const double v = opnd_value(o, x, k);
-        bad |= !__builtin_isfinite(v);
-        m = max(m, exp_of(v));
+        double re, im;
+        opnd_pair(o, x, k, re, im);
+        take(re, im);
     }
     if (bad) *nonfinite = 1;
-    if (m != INT_MIN) atomicMax(e + x, m);
+#pragma unroll
+    for (int j = 0; j < J; ++j)
+        if (m[j] != INT_MIN) atomicMax(e + j * estride + x, m[j]);
 }
 
 // Contiguous lines (a line is a row of the stored matrix): one block is 32
 // lines x 128 k, staged through LDS with coalesced reads along k; thread t then
 // takes line t % 32, 16 k at 16 ((t / 32) % 2) + 32 (t / 64) and writes one
 // 16-byte fragment piece per modulus (a wave writes 1 KB).
-template <bool WIDE>
-__global__ __launch_bounds__(256) void crt_split_kernel(CrtOperand o, int64_t nx, int64_t K, const int *e,
-                                                        signed char *out, int64_t Xp, int64_t Kp, CrtTables tb) {
+template <bool WIDE, int J>
+__global__ __launch_bounds__(256) void crt_split_kernel(CrtOperand o, CrtCombos cc, int64_t nx, int64_t K, const int *e,
+                                                        int64_t estride, signed char *out, int64_t ostride, int64_t Xp,
+                                                        int64_t Kp, CrtTables tb) {
     __shared__ double tile[32][129];
     const int64_t k0 = (int64_t)blockIdx.x * 128, x0 = (int64_t)blockIdx.y * 32;
     const int t = threadIdx.x;
-    {
-        const int c = t & 127;
-        for (int r = t >> 7; r < 32; r += 2) {
-            const int64_t x = x0 + r, k = k0 + c;
-            tile[r][c] = (x < nx && k < K) ? opnd_value(o, x, k) : 0.0;
+    for (int j = 0; j < J; ++j) {      // view j of the operand (one job)
+        {
+            const int c = t & 127;
+            for (int r = t >> 7; r < 32; r += 2) {
+                const int64_t x = x0 + r, k = k0 + c;
+                double re = 0.0, im = 0.0;
+                if (x < nx && k < K) opnd_pair(o, x, k, re, im);
+                tile[r][c] = combo_value(o, cc, j, re, im);
+            }
         }
-    }
-    __syncthreads();
-    const int r = crt::split_line(t), kb = crt::split_k(t);
-    const int64_t x = x0 + r;
-    const int ex = x < nx ? e[x] : INT_MIN;
-    // scaled integers, split once (v = vh 2^26 + vl) for the per-modulus residues
-    const int sh = crt::scaled_shift(ex, tb.t);
-    double vh[16], vl[16];
+        __syncthreads();
+        const int r = crt::split_line(t), kb = crt::split_k(t);
+        const int64_t x = x0 + r;
+        const int ex = x < nx ? e[j * estride + x] : INT_MIN;
+        // scaled integers, split once (v = vh 2^26 + vl) for the per-modulus residues
+        const int sh = crt::scaled_shift(ex, tb.t);
+        double vh[16], vl[16];
 #pragma unroll
-    for (int u = 0; u < 16; ++u) {
-        const double v = crt::scaled_int_sh(tile[r][kb + u], sh);
-        if (WIDE) {
-            vh[u] = v;               // kept whole: the wide step reduces it per modulus
-            vl[u] = 0.0;
-        } else {
-            crt::residue_split(v, vh[u], vl[u]);
-            vl[u] = crt::residue_bias(vl[u]);     // the biased form (crt_residue.h)
+        for (int u = 0; u < 16; ++u) {
+            const double v = crt::scaled_int_sh(tile[r][kb + u], sh);
+            if (WIDE) {
+                vh[u] = v;               // kept whole: the wide step reduces it per modulus
+                vl[u] = 0.0;
+            } else {
+                crt::residue_split(v, vh[u], vl[u]);
+                vl[u] = crt::residue_bias(vl[u]);     // the biased form (crt_residue.h)
+            }
         }
-    }
-    const int64_t KC = Kp >> 5;
-    int4 *dst = (int4 *)(out + crt::frag_offset(x, k0 + kb, KC));
-    const int64_t lstride = Xp * Kp / 16;
-    for (int l = 0; l < tb.n; ++l) {
-        const double m = (double)tb.m[l], mi = tb.minv[l], c26 = tb.c26[l], cq = crt::residue_cq(mi);
-        unsigned rs[16];
+        const int64_t KC = Kp >> 5;
+        int4 *dst = (int4 *)(out + j * ostride + crt::frag_offset(x, k0 + kb, KC));
+        const int64_t lstride = Xp * Kp / 16;
+        for (int l = 0; l < tb.n; ++l) {
+            const double m = (double)tb.m[l], mi = tb.minv[l], c26 = tb.c26[l], cq = crt::residue_cq(mi);
+            unsigned rs[16];
 #pragma unroll
-        for (int u = 0; u < 16; ++u)
-            rs[u] = WIDE ? crt::residue_wide_biased(vh[u], m, mi, cq, c26)
-                         : crt::residue_biased(vh[u], vl[u], m, mi, cq, c26);
-        int4 w;
-        w.x = (int)crt::pack4_biased(rs[0], rs[1], rs[2], rs[3]);
-        w.y = (int)crt::pack4_biased(rs[4], rs[5], rs[6], rs[7]);
-        w.z = (int)crt::pack4_biased(rs[8], rs[9], rs[10], rs[11]);
-        w.w = (int)crt::pack4_biased(rs[12], rs[13], rs[14], rs[15]);
-        dst[l * lstride] = w;
+            for (int u = 0; u < 16; ++u)
+                rs[u] = WIDE ? crt::residue_wide_biased(vh[u], m, mi, cq, c26)
+                             : crt::residue_biased(vh[u], vl[u], m, mi, cq, c26);
+            int4 w;
+            w.x = (int)crt::pack4_biased(rs[0], rs[1], rs[2], rs[3]);
+            w.y = (int)crt::pack4_biased(rs[4], rs[5], rs[6], rs[7]);
+            w.z = (int)crt::pack4_biased(rs[8], rs[9], rs[10], rs[11]);
+            w.w = (int)crt::pack4_biased(rs[12], rs[13], rs[14], rs[15]);
+            dst[l * lstride] = w;
+        }
+        if (j + 1 < J) __syncthreads();    // the tile is reloaded for the next view
     }
 }
 
 // Strided lines (a line is a column of the stored matrix): one block is 128
 // lines x 32 k, so every k row of the block is read as 1 KB of contiguous
 // memory; thread t then takes line t % 128 and 16 k at 16 (t / 128).
-template <bool WIDE>
-__global__ __launch_bounds__(256) void crt_split_cols_kernel(CrtOperand o, int64_t nx, int64_t K, const int *e,
-                                                             signed char *out, int64_t Xp, int64_t Kp, CrtTables tb) {
+template <bool WIDE, int J>
+__global__ __launch_bounds__(256) void crt_split_cols_kernel(CrtOperand o, CrtCombos cc, int64_t nx, int64_t K,
+                                                             const int *e, int64_t estride, signed char *out,
+                                                             int64_t ostride, int64_t Xp, int64_t Kp, CrtTables tb) {
     __shared__ double tile[128][33];
     const int64_t k0 = (int64_t)blockIdx.x * 32, x0 = (int64_t)blockIdx.y * 128;
     const int t = threadIdx.x;
-    {
-        const int c = t & 127;
-        for (int r = t >> 7; r < 32; r += 2) {
-            const int64_t x = x0 + c, k = k0 + r;
-            tile[c][r] = (x < nx && k < K) ? opnd_value(o, x, k) : 0.0;
+    for (int j = 0; j < J; ++j) {      // view j of the operand (one job)
+        {
+            const int c = t & 127;
+            for (int r = t >> 7; r < 32; r += 2) {
+                const int64_t x = x0 + c, k = k0 + r;
+                double re = 0.0, im = 0.0;
+                if (x < nx && k < K) opnd_pair(o, x, k, re, im);
+                tile[c][r] = combo_value(o, cc, j, re, im);
+            }
         }
-    }
-    __syncthreads();
-    const int xl = crt::split_cols_line(t), kb = crt::split_cols_k(t);
-    const int64_t x = x0 + xl;
-    const int ex = x < nx ? e[x] : INT_MIN;
-    const int sh = crt::scaled_shift(ex, tb.t);
-    double vh[16], vl[16];
+        __syncthreads();
+        const int xl = crt::split_cols_line(t), kb = crt::split_cols_k(t);
+        const int64_t x = x0 + xl;
+        const int ex = x < nx ? e[j * estride + x] : INT_MIN;
+        const int sh = crt::scaled_shift(ex, tb.t);
+        double vh[16], vl[16];
 #pragma unroll
-    for (int u = 0; u < 16; ++u) {
-        const double v = crt::scaled_int_sh(tile[xl][kb + u], sh);
-        if (WIDE) {
-            vh[u] = v;
-            vl[u] = 0.0;
-        } else {
-            crt::residue_split(v, vh[u], vl[u]);
-            vl[u] = crt::residue_bias(vl[u]);     // the biased form (crt_residue.h)
+        for (int u = 0; u < 16; ++u) {
+            const double v = crt::scaled_int_sh(tile[xl][kb + u], sh);
+            if (WIDE) {
+                vh[u] = v;
+                vl[u] = 0.0;
+            } else {
+                crt::residue_split(v, vh[u], vl[u]);
+                vl[u] = crt::residue_bias(vl[u]);     // the biased form (crt_residue.h)
+            }
         }
-    }
-    const int64_t KC = Kp >> 5;
-    int4 *dst = (int4 *)(out + crt::frag_offset(x, k0 + kb, KC));
-    const int64_t lstride = Xp * Kp / 16;
-    for (int l = 0; l < tb.n; ++l) {
-        const double m = (double)tb.m[l], mi = tb.minv[l], c26 = tb.c26[l], cq = crt::residue_cq(mi);
-        unsigned rs[16];
+        const int64_t KC = Kp >> 5;
+        int4 *dst = (int4 *)(out + j * ostride + crt::frag_offset(x, k0 + kb, KC));
+        const int64_t lstride = Xp * Kp / 16;
+        for (int l = 0; l < tb.n; ++l) {
+            const double m = (double)tb.m[l], mi = tb.minv[l], c26 = tb.c26[l], cq = crt::residue_cq(mi);
+            unsigned rs[16];
 #pragma unroll
-        for (int u = 0; u < 16; ++u)
-            rs[u] = WIDE ? crt::residue_wide_biased(vh[u], m, mi, cq, c26)
-                         : crt::residue_biased(vh[u], vl[u], m, mi, cq, c26);
-        int4 w;
-        w.x = (int)crt::pack4_biased(rs[0], rs[1], rs[2], rs[3]);
-        w.y = (int)crt::pack4_biased(rs[4], rs[5], rs[6], rs[7]);
-        w.z = (int)crt::pack4_biased(rs[8], rs[9], rs[10], rs[11]);
-        w.w = (int)crt::pack4_biased(rs[12], rs[13], rs[14], rs[15]);
-        dst[l * lstride] = w;
+            for (int u = 0; u < 16; ++u)
+                rs[u] = WIDE ? crt::residue_wide_biased(vh[u], m, mi, cq, c26)
+                             : crt::residue_biased(vh[u], vl[u], m, mi, cq, c26);
+            int4 w;
+            w.x = (int)crt::pack4_biased(rs[0], rs[1], rs[2], rs[3]);
+            w.y = (int)crt::pack4_biased(rs[4], rs[5], rs[6], rs[7]);
+            w.z = (int)crt::pack4_biased(rs[8], rs[9], rs[10], rs[11]);
+            w.w = (int)crt::pack4_biased(rs[12], rs[13], rs[14], rs[15]);
+            dst[l * lstride] = w;
+        }
+        if (j + 1 < J) __syncthreads();    // the tile is reloaded for the next view
     }
 }
 
@@ -346,6 +394,7 @@ struct CrtGemmArgs {
     int64_t Mp, Np, Kp, M, N;
     unsigned char *R;                             // [n][plane], each plane M x N
     int64_t plane;
+    int64_t job_stride;                           // bytes between the jobs' workspaces (blockIdx.y = job n + l)
 };
 
 // Block tile BM x 256 (BM = 128 RB), eight waves of 32 RB x 128 (RB x 4 MFMA
@@ -363,7 +412,8 @@ __global__ __launch_bounds__(512, 2) void crt_gemm_kernel(CrtGemmArgs a, CrtTabl
     __shared__ __attribute__((aligned(1024))) signed char lds[crt::STAGES * crt::STAGE];
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);   // uniform: keeps descriptors scalar
-    const int l = blockIdx.y;
+    const int l = blockIdx.y % tb.n;
+    const int64_t jo = (int64_t)(blockIdx.y / tb.n) * a.job_stride;
     int64_t tm, tn;
     tile_of(blockIdx.x, a.Mp / CBM, a.Np / CBN, tm, tn);
     const int64_t m0 = (int64_t)__builtin_amdgcn_readfirstlane((int)tm) * CBM;
@@ -380,7 +430,7 @@ __global__ __launch_bounds__(512, 2) void crt_gemm_kernel(CrtGemmArgs a, CrtTabl
     const int KC = (int)(a.Kp >> 5);
     // this wave's operand: A's slice (waves 0-3) or B's (waves 4-7) of modulus l
     const bool is_a = crt::loads_a(wave);
-    const signed char *src = is_a ? a.A + (int64_t)l * a.Mp * a.Kp : a.B + (int64_t)l * a.Np * a.Kp;
+    const signed char *src = is_a ? a.A + jo + (int64_t)l * a.Mp * a.Kp : a.B + jo + (int64_t)l * a.Np * a.Kp;
     const int64_t x0 = is_a ? m0 : n0;
     const int nt = (int)(a.Kp / crt::SK);
 
@@ -479,7 +529,7 @@ __global__ __launch_bounds__(512, 2) void crt_gemm_kernel(CrtGemmArgs a, CrtTabl
     // (crt::res_block): no per-entry range checks; the reconstruction skips padding.
     const double m = (double)tb.m[l];
     const double minv = tb.minv[l];
-    unsigned char *R = a.R + (int64_t)l * a.plane + 16 * lane;
+    unsigned char *R = a.R + jo + (int64_t)l * a.plane + 16 * lane;
     const int64_t NB = a.Np >> 5;
     const int64_t rw = m0 + crt::wave_row(wave), cw = n0 + crt::wave_col(wave);
 #pragma unroll
@@ -562,6 +612,86 @@ __global__ __launch_bounds__(256) void crt_reconstruct_kernel(const unsigned cha
             } else {
                 C[i * ldc + j] = out;
             }
+        }
+    }
+}
+
+// All J jobs of a complex product in one pass (jobs in consecutive workspaces
+// `jstride` bytes apart, with their own residue planes and exponents): for each
+// entry, job j's value v_j is reconstructed as above, and Re C = sum_j cre_j v_j,
+// Im C = sum_j cim_j v_j are summed in job order from the first job's terms --
+// the order in which successive one-job launches would apply them -- and C is
+// written once.
+struct CrtJobCoef {
+    double cre[CRT_MAX_JOBS], cim[CRT_MAX_JOBS];
+};
+template <int NW, int J>
+__global__ __launch_bounds__(256) void crt_reconstruct_jobs_kernel(const unsigned char *R, int64_t plane, int64_t NB,
+                                                                   int64_t nblocks, int64_t M, int64_t N,
+                                                                   const int *eA, const int *eB, int64_t jstride,
+                                                                   double *C, int64_t ldc, CrtTables tb,
+                                                                   CrtJobCoef jc) {
+    const int64_t b = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (b >= nblocks) return;
+    const int64_t row0 = (b / NB) * 32, col0 = (b % NB) * 32;
+    if (row0 >= M || col0 >= N) return;           // padding block (uniform per wave)
+    const int64_t j = col0 + crt::res_col(lane);
+    const bool col_ok = j < N;
+    double re[16], im[16];                        // the lane's 16 entries (result items r = 0 .. 15)
+#pragma unroll
+    for (int jb = 0; jb < J; ++jb) {
+        const unsigned char *Rj = R + jb * jstride;
+        const int *eAj = (const int *)((const char *)eA + jb * jstride);
+        const int *eBj = (const int *)((const char *)eB + jb * jstride);
+        int4 word[NW];
+#pragma unroll
+        for (int l = 0; l < NW; ++l)
+            word[l] = *(const int4 *)(Rj + (int64_t)(l < tb.n ? l : 0) * plane + (b << 10) + 16 * lane);
+        const int eb = eBj[col_ok ? j : N - 1];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {             // items 4q .. 4q + 3 (word component q)
+            double S[CRT_MAX_CHUNKS][4];
+#pragma unroll
+            for (int p = 0; p < CRT_MAX_CHUNKS; ++p)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) S[p][e] = 0.0;
+#pragma unroll
+            for (int l = 0; l < NW; ++l) {
+                if (l < tb.n) {  // uniform; no early exit, so word[] keeps static indices (registers)
+                    const unsigned wq =
+                        (unsigned)(q == 0 ? word[l].x : q == 1 ? word[l].y : q == 2 ? word[l].z : word[l].w);
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        const double r = (double)((wq >> (8 * e)) & 255u);
+#pragma unroll
+                        for (int p = 0; p < CRT_MAX_CHUNKS; ++p) S[p][e] = fma(r, tb.w[l][p], S[p][e]);
+                    }
+                }
+            }
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int64_t i = row0 + crt::res_row(4 * q + e, lane);
+                double out = 0.0;
+                if (col_ok && i < M) {
+                    const int ea = eAj[i];
+                    if (ea != INT_MIN && eb != INT_MIN)
+                        out = ldexp(crt_recombine(S[0][e], S[1][e], S[2][e], S[3][e], tb), ea + eb - 2 * tb.t);
+                }
+                const int it = 4 * q + e;
+                re[it] = jb == 0 ? jc.cre[0] * out : re[it] + jc.cre[jb] * out;
+                im[it] = jb == 0 ? jc.cim[0] * out : im[it] + jc.cim[jb] * out;
+            }
+        }
+    }
+    if (!col_ok) return;
+#pragma unroll
+    for (int it = 0; it < 16; ++it) {
+        const int64_t i = row0 + crt::res_row(it, lane);
+        if (i < M) {
+            double *c = C + 2 * (i * ldc + j);
+            c[0] = re[it];
+            c[1] = im[it];
         }
     }
 }
@@ -654,49 +784,63 @@ CrtOperand crt_operand(const double *X, int64_t ld, bool lines, bool cplx, doubl
     return o;
 }
 
-hipError_t launch_crt_gemm(const CrtPlan &pl, const CrtOperand &oa, const CrtOperand &ob, int64_t M, int64_t N,
-                           int64_t K, double *C, int64_t ldc, const CrtOutput &out, void *ws, hipStream_t stream,
-                           hipEvent_t ev_begin, hipEvent_t ev_end, const CrtSide *side) {
+namespace {
+
+// Enqueue J jobs on the storage of A and B (job j's operand views ccA/ccB, its
+// workspace crt_job_stride(pl) bytes after job j - 1's): exponents and residues of
+// every view from one kernel per operand and pass (each pass reads the operand
+// once, its later views from L2), one product launch over all jobs' moduli, one
+// reconstruction (J = 1: the one-job kernel with `out`; J > 1: complex C written
+// once from the jobs' coefficients `jc`).
+template <int J>
+hipError_t enqueue_crt(const CrtPlan &pl, const CrtOperand &oa, const CrtOperand &ob, const CrtCombos &ccA,
+                       const CrtCombos &ccB, int64_t M, int64_t N, int64_t K, double *C, int64_t ldc,
+                       const CrtOutput &out, const CrtJobCoef &jc, void *ws, hipStream_t stream, hipEvent_t ev_begin,
+                       hipEvent_t ev_end, const CrtSide *side) {
+    const int64_t js = (int64_t)crt_job_stride(pl);
     char *w = (char *)ws;
     signed char *ra = (signed char *)(w + pl.off_ra), *rb = (signed char *)(w + pl.off_rb);
     unsigned char *res = (unsigned char *)(w + pl.off_res);
     int *ea = (int *)(w + pl.off_ea), *eb = (int *)(w + pl.off_eb);
     int *nonfinite = (int *)(w + pl.off_flag);
-    auto exps = [&](const CrtOperand &o, int64_t nx, int *e, hipStream_t stream) {
+    auto exps = [&](const CrtOperand &o, const CrtCombos &cc, int64_t nx, int *e, hipStream_t stream) {
         if (nx <= 0) return;
         if (o.lines) {
-            hipLaunchKernelGGL(crt_exp_lines_kernel, dim3((unsigned)nx), dim3(256), 0, stream, o, K, e, nonfinite);
+            hipLaunchKernelGGL((crt_exp_lines_kernel<J>), dim3((unsigned)nx), dim3(256), 0, stream, o, cc, K, e,
+                               js / 4, nonfinite);
         } else {
             // k chunks sized for at least ~2048 workgroups (a fixed 256 gave 48
             // workgroups for 2048 columns and K = 1536: 130 us instead of ~10)
             const int64_t col_blocks = (nx + 255) / 256;
             const int64_t kchunk =
                 std::max<int64_t>(EXP_UNROLL, std::min<int64_t>(256, (K * col_blocks / 2048) / EXP_UNROLL * EXP_UNROLL));
-            hipLaunchKernelGGL(crt_fill_kernel, dim3(64), dim3(256), 0, stream, e, nx, INT_MIN);
-            hipLaunchKernelGGL(crt_exp_cols_kernel, dim3((unsigned)((nx + 255) / 256), (unsigned)((K + kchunk - 1) / kchunk)),
-                               dim3(256), 0, stream, o, nx, K, kchunk, e, nonfinite);
+            for (int j = 0; j < J; ++j)
+                hipLaunchKernelGGL(crt_fill_kernel, dim3(64), dim3(256), 0, stream, e + j * (js / 4), nx, INT_MIN);
+            hipLaunchKernelGGL((crt_exp_cols_kernel<J>),
+                               dim3((unsigned)((nx + 255) / 256), (unsigned)((K + kchunk - 1) / kchunk)), dim3(256), 0,
+                               stream, o, cc, nx, K, kchunk, e, js / 4, nonfinite);
         }
     };
     if (out.first) hipLaunchKernelGGL(crt_fill_kernel, dim3(1), dim3(256), 0, stream, nonfinite, 1, 0);
     const bool wide = pl.tb.t > 58;
-    auto split = [&](const CrtOperand &o, int64_t nx, int64_t Xp, const int *ex, signed char *dst,
-                     hipStream_t stream) {
-        const dim3 grid((unsigned)(pl.Kp / 128), (unsigned)(Xp / 32));
+    auto split = [&](const CrtOperand &o, const CrtCombos &cc, int64_t nx, int64_t Xp, const int *ex,
+                     signed char *dst, hipStream_t stream) {
         if (o.lines) {
+            const dim3 grid((unsigned)(pl.Kp / 128), (unsigned)(Xp / 32));
             if (wide)
-                hipLaunchKernelGGL((crt_split_kernel<true>), grid, dim3(256), 0, stream, o, nx, K, ex, dst, Xp, pl.Kp,
-                                   pl.tb);
+                hipLaunchKernelGGL((crt_split_kernel<true, J>), grid, dim3(256), 0, stream, o, cc, nx, K, ex, js / 4,
+                                   dst, js, Xp, pl.Kp, pl.tb);
             else
-                hipLaunchKernelGGL((crt_split_kernel<false>), grid, dim3(256), 0, stream, o, nx, K, ex, dst, Xp,
-                                   pl.Kp, pl.tb);
+                hipLaunchKernelGGL((crt_split_kernel<false, J>), grid, dim3(256), 0, stream, o, cc, nx, K, ex, js / 4,
+                                   dst, js, Xp, pl.Kp, pl.tb);
         } else {
             const dim3 gc((unsigned)(pl.Kp / 32), (unsigned)(Xp / 128));
             if (wide)
-                hipLaunchKernelGGL((crt_split_cols_kernel<true>), gc, dim3(256), 0, stream, o, nx, K, ex, dst, Xp,
-                                   pl.Kp, pl.tb);
+                hipLaunchKernelGGL((crt_split_cols_kernel<true, J>), gc, dim3(256), 0, stream, o, cc, nx, K, ex,
+                                   js / 4, dst, js, Xp, pl.Kp, pl.tb);
             else
-                hipLaunchKernelGGL((crt_split_cols_kernel<false>), gc, dim3(256), 0, stream, o, nx, K, ex, dst, Xp,
-                                   pl.Kp, pl.tb);
+                hipLaunchKernelGGL((crt_split_cols_kernel<false, J>), gc, dim3(256), 0, stream, o, cc, nx, K, ex,
+                                   js / 4, dst, js, Xp, pl.Kp, pl.tb);
         }
     };
     hipStream_t sb = stream;
@@ -705,32 +849,79 @@ hipError_t launch_crt_gemm(const CrtPlan &pl, const CrtOperand &oa, const CrtOpe
         if (hipError_t e = hipEventRecord(side->fork, stream); e != hipSuccess) return e;
         if (hipError_t e = hipStreamWaitEvent(sb, side->fork, 0); e != hipSuccess) return e;
     }
-    exps(ob, N, eb, sb);
-    split(ob, N, pl.Np, eb, rb, sb);
+    exps(ob, ccB, N, eb, sb);
+    split(ob, ccB, N, pl.Np, eb, rb, sb);
     if (side) {
         if (hipError_t e = hipEventRecord(side->join, sb); e != hipSuccess) return e;
     }
-    exps(oa, M, ea, stream);
-    split(oa, M, pl.Mp, ea, ra, stream);
+    exps(oa, ccA, M, ea, stream);
+    split(oa, ccA, M, pl.Mp, ea, ra, stream);
     if (side) {
         if (hipError_t e = hipStreamWaitEvent(stream, side->join, 0); e != hipSuccess) return e;
     }
-    CrtGemmArgs a{ra, rb, pl.Mp, pl.Np, pl.Kp, M, N, res, pl.plane};
+    CrtGemmArgs a{ra, rb, pl.Mp, pl.Np, pl.Kp, M, N, res, pl.plane, js};
     if (ev_begin) (void)hipEventRecord(ev_begin, stream);
-    hipLaunchKernelGGL(crt_gemm_kernel, dim3((unsigned)((pl.Mp / CBM) * (pl.Np / CBN)), (unsigned)pl.tb.n),
+    hipLaunchKernelGGL(crt_gemm_kernel, dim3((unsigned)((pl.Mp / CBM) * (pl.Np / CBN)), (unsigned)(pl.tb.n * J)),
                        dim3(64 * crt::WAVES), 0, stream, a, pl.tb);
     if (ev_end) (void)hipEventRecord(ev_end, stream);
     const int64_t NB = pl.Np / 32, nblocks = (pl.Mp / 32) * NB;
     const dim3 rgrid((unsigned)((nblocks + 3) / 4));
+    if constexpr (J == 1) {
 #define F4J_RECONSTRUCT(NW)                                                                                        \
     hipLaunchKernelGGL(crt_reconstruct_kernel<NW>, rgrid, dim3(256), 0, stream, res, pl.plane, NB, nblocks, M, N, ea, \
                        eb, C, ldc, pl.tb, (int)out.cplx, out.cre, out.cim, (int)out.first)
-    if (pl.tb.n <= 8) F4J_RECONSTRUCT(8);
-    else if (pl.tb.n <= 12) F4J_RECONSTRUCT(12);
-    else if (pl.tb.n <= 16) F4J_RECONSTRUCT(16);
-    else F4J_RECONSTRUCT(CRT_MAX_MODULI);
+        if (pl.tb.n <= 8) F4J_RECONSTRUCT(8);
+        else if (pl.tb.n <= 12) F4J_RECONSTRUCT(12);
+        else if (pl.tb.n <= 16) F4J_RECONSTRUCT(16);
+        else F4J_RECONSTRUCT(CRT_MAX_MODULI);
 #undef F4J_RECONSTRUCT
+    } else {
+#define F4J_RECONSTRUCT(NW)                                                                                          \
+    hipLaunchKernelGGL((crt_reconstruct_jobs_kernel<NW, J>), rgrid, dim3(256), 0, stream, res, pl.plane, NB, nblocks, M, \
+                       N, ea, eb, js, C, ldc, pl.tb, jc)
+        if (pl.tb.n <= 8) F4J_RECONSTRUCT(8);
+        else if (pl.tb.n <= 12) F4J_RECONSTRUCT(12);
+        else if (pl.tb.n <= 16) F4J_RECONSTRUCT(16);
+        else F4J_RECONSTRUCT(CRT_MAX_MODULI);
+#undef F4J_RECONSTRUCT
+    }
     return hipGetLastError();
+}
+
+}  // namespace
+
+size_t crt_job_stride(const CrtPlan &pl) { return (pl.bytes + 4095) & ~size_t(4095); }
+
+hipError_t launch_crt_gemm(const CrtPlan &pl, const CrtOperand &oa, const CrtOperand &ob, int64_t M, int64_t N,
+                           int64_t K, double *C, int64_t ldc, const CrtOutput &out, void *ws, hipStream_t stream,
+                           hipEvent_t ev_begin, hipEvent_t ev_end, const CrtSide *side) {
+    CrtCombos ccA{}, ccB{};
+    ccA.ca[0] = oa.ca, ccA.cb[0] = oa.cb, ccB.ca[0] = ob.ca, ccB.cb[0] = ob.cb;
+    return enqueue_crt<1>(pl, oa, ob, ccA, ccB, M, N, K, C, ldc, out, CrtJobCoef{}, ws, stream, ev_begin, ev_end,
+                          side);
+}
+
+hipError_t launch_crt_gemm_jobs(const CrtPlan &pl, const CrtOperand &oa, const CrtOperand &ob, int64_t M, int64_t N,
+                                int64_t K, double *C, int64_t ldc, const CrtJobs &jobs, void *ws, hipStream_t stream,
+                                hipEvent_t ev_begin, hipEvent_t ev_end, const CrtSide *side) {
+    CrtCombos ccA{}, ccB{};
+    CrtJobCoef jc{};
+    for (int j = 0; j < jobs.n && j < CRT_MAX_JOBS; ++j) {
+        ccA.ca[j] = jobs.a_ca[j], ccA.cb[j] = jobs.a_cb[j];
+        ccB.ca[j] = jobs.b_ca[j], ccB.cb[j] = jobs.b_cb[j];
+        jc.cre[j] = jobs.cre[j], jc.cim[j] = jobs.cim[j];
+    }
+    CrtOutput out;
+    out.cplx = true;
+    out.first = true;
+    switch (jobs.n) {
+        case 3:
+            return enqueue_crt<3>(pl, oa, ob, ccA, ccB, M, N, K, C, ldc, out, jc, ws, stream, ev_begin, ev_end, side);
+        case 4:
+            return enqueue_crt<4>(pl, oa, ob, ccA, ccB, M, N, K, C, ldc, out, jc, ws, stream, ev_begin, ev_end, side);
+        default:
+            return hipErrorInvalidValue;
+    }
 }
 
 }  // namespace f4j

@@ -671,14 +671,16 @@ int run_crt(bool cplx, int opA, int opB, int64_t M, int64_t N, int64_t K, const 
     if (!cplx) jobs = {{1, 0, 1, 0, 1, 0}};
     else if (use3m) jobs = {{1, 0, 1, 0, 1, -1}, {0, sa, 0, sb, -1, -1}, {1, sa, 1, sb, 0, 1}};
     else jobs = {{1, 0, 1, 0, 1, 0}, {0, sa, 0, sb, -1, 0}, {1, 0, 0, sb, 0, 1}, {0, sa, 1, 0, 0, 1}};
-    if (fixed && pl.bytes > fixed->cap)
+    // complex: every job's residues and product residues at once (one workspace per job)
+    const size_t need = cplx ? jobs.size() * crt_job_stride(pl) : pl.bytes;
+    if (fixed && need > fixed->cap)
         return fail(F4J_ERR_INVALID_ARG, "split_int8 without allocation: this product needs " +
-                                             std::to_string(pl.bytes) + " bytes of scratch, the limit is " +
+                                             std::to_string(need) + " bytes of scratch, the limit is " +
                                              std::to_string(fixed->cap) + " (about 1024^3 with 16 moduli)");
     State &s = S();
     std::lock_guard<std::mutex> g(s.ws_mu);
     if (!fixed) {
-        const int rc = grow(s.ws, pl.bytes);
+        const int rc = grow(s.ws, need);
         if (rc != F4J_OK) return rc;
     }
     void *wsp = fixed ? fixed->p : s.ws.p;
@@ -695,21 +697,31 @@ int run_crt(bool cplx, int opA, int opB, int64_t M, int64_t N, int64_t K, const 
         }
     } side_idle;
     side_idle.side = s.side;
-    for (size_t jb = 0; jb < jobs.size(); ++jb) {
-        const Job &j = jobs[jb];
-        const CrtOperand oa = crt_operand(A, lda, opA == F4J_OP_N, cplx, j.ca_a, j.cb_a);
-        const CrtOperand ob = crt_operand(B, ldb, opB != F4J_OP_N, cplx, j.ca_b, j.cb_b);
-        CrtOutput out;
-        out.cplx = cplx;
-        out.cre = j.cre;
-        out.cim = j.cim;
-        out.first = jb == 0;
+    {
         // B's exponents and residues on the side stream, beside A's on this one
         const CrtSide side{s.side, s.split_fork_ev, s.split_join_ev};
-        F4J_HIP(launch_crt_gemm(pl, oa, ob, M, N, K, C, ldc, out, wsp, stream, timing ? s.tev[2 * jb] : nullptr,
-                                timing ? s.tev[2 * jb + 1] : nullptr, &side),
-                "crt gemm launch");
-        if (jb == 0 && check_rows > 0) {
+        if (!cplx) {
+            const CrtOperand oa = crt_operand(A, lda, opA == F4J_OP_N, false);
+            const CrtOperand ob = crt_operand(B, ldb, opB != F4J_OP_N, false);
+            F4J_HIP(launch_crt_gemm(pl, oa, ob, M, N, K, C, ldc, CrtOutput{}, wsp, stream,
+                                    timing ? s.tev[0] : nullptr, timing ? s.tev[1] : nullptr, &side),
+                    "crt gemm launch");
+        } else {
+            // all jobs at once: each operand read once per pass for every job's view
+            CrtJobs cj;
+            cj.n = (int)jobs.size();
+            for (size_t jb = 0; jb < jobs.size(); ++jb) {
+                cj.a_ca[jb] = jobs[jb].ca_a, cj.a_cb[jb] = jobs[jb].cb_a;
+                cj.b_ca[jb] = jobs[jb].ca_b, cj.b_cb[jb] = jobs[jb].cb_b;
+                cj.cre[jb] = jobs[jb].cre, cj.cim[jb] = jobs[jb].cim;
+            }
+            const CrtOperand oa = crt_operand(A, lda, opA == F4J_OP_N, true);
+            const CrtOperand ob = crt_operand(B, ldb, opB != F4J_OP_N, true);
+            F4J_HIP(launch_crt_gemm_jobs(pl, oa, ob, M, N, K, C, ldc, cj, wsp, stream, timing ? s.tev[0] : nullptr,
+                                         timing ? s.tev[1] : nullptr, &side),
+                    "crt gemm launch");
+        }
+        if (check_rows > 0) {
             // The check's compensated sums need only A and B: they follow B's
             // split on the side stream (which has waited for the inputs) and run
             // beside the residue products.
@@ -723,7 +735,7 @@ int run_crt(bool cplx, int opA, int opB, int64_t M, int64_t N, int64_t K, const 
     rep.precision = F4J_PREC_SPLIT_INT8;
     rep.slices = n;
     rep.products = n * (int)jobs.size();
-    rep.launches = (int)jobs.size();
+    rep.launches = 1;                        // one product launch covers every job's moduli
     rep.checked_rows = check_rows;
     rep.max_rel_err = -1.0;
     // NaN or Inf among the inputs (flagged by the exponent kernels): the
@@ -770,12 +782,9 @@ int run_crt(bool cplx, int opA, int opB, int64_t M, int64_t N, int64_t K, const 
     }
     F4J_HIP(hipStreamSynchronize(stream), "stream synchronize");
     if (timing) {
-        rep.product_ms = 0.0;
-        for (size_t jb = 0; jb < jobs.size(); ++jb) {
-            float ms = 0.f;
-            F4J_HIP(hipEventElapsedTime(&ms, s.tev[2 * jb], s.tev[2 * jb + 1]), "hipEventElapsedTime");
-            rep.product_ms += ms;
-        }
+        float ms = 0.f;
+        F4J_HIP(hipEventElapsedTime(&ms, s.tev[0], s.tev[1]), "hipEventElapsedTime");
+        rep.product_ms = ms;
     }
     t_split = rep;
     return F4J_OK;

@@ -294,7 +294,7 @@ typedef struct f4j_split_report {
     int precision;        /* precision delivered: the SPLIT_* asked for, or FP64 after a fallback */
     int slices;           /* digit slices per operand */
     int products;         /* slice products computed (S (S + 1) / 2) */
-    int launches;         /* slice-product kernel launches */
+    int launches;         /* slice-product kernel launches (the modular form: one for all real products) */
     int checked_rows;     /* rows verified by the residual check (0 = check off) */
     double max_rel_err;   /* max over the checked rows of |C - AB| / (|A||B|); -1 if not checked */
     double product_ms;    /* time of the slice-product launches, from HIP events on the call's stream,
